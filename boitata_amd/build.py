@@ -1,0 +1,55 @@
+"""Builds libboitata_gpu.so, the HIP library behind include/boitata_gpu.h.
+
+hipcc for gfx950 only. Every translation unit is compiled with
+-ffp-contract=off (the FP64 path reproduces the reference's unfused
+arithmetic; FP32 code opts into contraction locally) and with packed FP32
+instructions disabled. Run as ``python -m boitata_amd.build``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+LIB = HERE / "libboitata_gpu.so"
+SOURCES = ["bk_search.hip", "bk_solve.hip", "bk_api.hip"]
+ARCH = os.environ.get("BK_OFFLOAD_ARCH", "gfx950")
+FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def build(verbose: bool = False) -> Path:
+    """Compiles every source and links the shared library; returns its path."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    objs = []
+    out_dir = CSRC / "_obj"
+    out_dir.mkdir(exist_ok=True)
+    for src in SOURCES:
+        obj = out_dir / (Path(src).stem + ".o")
+        cmd = [hipcc, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

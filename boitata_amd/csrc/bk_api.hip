@@ -1,0 +1,959 @@
+// bk_api.hip — the C ABI of include/boitata_gpu.h: context, device buffers and
+// the per-pass pipeline (grid build, search, bucketing by neighbour count,
+// batched solves, deferred systems).
+//
+// Host-side parameter arithmetic restates crates/core/src/rotation.rs:8-43 and
+// crates/variogram/src/aniso.rs:88-94 in the reference's order so that the
+// frames handed to the kernels equal the reference's to the bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/boitata_gpu.h"
+#include "bk_search.h"
+#include "bk_solve.h"
+
+namespace {
+
+struct M3 {
+  double m[9];
+};
+
+M3 mm(const M3& a, const M3& b) {
+  M3 c;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t0 = a.m[3 * i] * b.m[j];
+      double t1 = a.m[3 * i + 1] * b.m[3 + j];
+      double t2 = a.m[3 * i + 2] * b.m[6 + j];
+      c.m[3 * i + j] = (t0 + t1) + t2;
+    }
+  return c;
+}
+
+const double kPi = 3.14159265358979323846;
+
+// rotation.rs:8-16
+M3 rotation(double az, double dip, double rake) {
+  double a = az * (kPi / 180.0), d = dip * (kPi / 180.0), r = rake * (kPi / 180.0);
+  double sa = std::sin(a), ca = std::cos(a), sd = std::sin(d), cd = std::cos(d), sr = std::sin(r), cr = std::cos(r);
+  M3 r1{{sa, ca, 0.0, -ca, sa, 0.0, 0.0, 0.0, 1.0}};
+  M3 r2{{cd, 0.0, -sd, 0.0, 1.0, 0.0, sd, 0.0, cd}};
+  M3 r3{{1.0, 0.0, 0.0, 0.0, cr, sr, 0.0, -sr, cr}};
+  return mm(mm(r3, r2), r1);
+}
+
+// rotation.rs:40-43
+M3 block_frame(double az, double dip, double rake) {
+  M3 r = rotation(az, dip, rake), b;
+  for (int j = 0; j < 3; ++j) {
+    b.m[j] = -r.m[3 + j];
+    b.m[3 + j] = r.m[j];
+    b.m[6 + j] = r.m[6 + j];
+  }
+  return b;
+}
+
+// aniso.rs:88-94 with major = 1.
+M3 aniso_matrix(double az, double dip, double rake, double semi, double minor) {
+  M3 d{{1.0 / 1.0, 0.0, 0.0, 0.0, 1.0 / semi, 0.0, 0.0, 0.0, 1.0 / minor}};
+  return mm(d, rotation(az, dip, rake));
+}
+
+const M3 kIdentity{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct bk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // Samples.
+  int64_t n = 0;
+  Buf x, y, z, v, e, hole;
+  bool has_err = false, has_hole = false;
+  bool has_origin = false;
+  double origin[3] = {0, 0, 0};
+  bool planar = true;
+  // Grid cache, keyed by the search frame.
+  bk::GridBuffers gb;
+  GridDev grid{};
+  bool grid_valid = false;
+  double grid_F[9] = {0};
+  double cell_factor = 1.0;
+  ShellTable tab{};
+  void* tab_storage = nullptr;
+  // Scratch (grow-only).
+  Buf tx, ty, tz, left, nidx, ncount, sstatus, lists, counts, status, value, variance, lagrange, support, n_used,
+      guard, redo, offs, which, iters;
+  // Tuning.
+  int edges[3] = {16, 32, 64};
+  int max_iter = 8;
+  double tol = 1e-9;
+  // LOO-CV state.
+  bool loo_ready = false;
+  int64_t loo_m = 0;
+  int loo_stride = 0;
+  Buf loo_idx, loo_count, loo_status;
+};
+
+namespace {
+
+int fail(bk_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPOK(c, x)                                                        \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) return fail(c, BK_E_HIP, hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(bk_ctx* c, Buf& b, size_t bytes) {
+  if (bytes <= b.cap) return 0;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  if (bytes == 0) return 0;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) return fail(c, BK_E_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+  b.cap = bytes;
+  return 0;
+}
+
+void release(Buf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b = Buf();
+}
+
+bool is_device(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t at;
+  hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// An input array on the device: itself when device memory, else a copy.
+template <class T>
+int stage_in(bk_ctx* c, const T* src, int64_t n, Buf& b, const T** out) {
+  if (!src) {
+    *out = nullptr;
+    return 0;
+  }
+  if (is_device(src)) {
+    *out = src;
+    return 0;
+  }
+  if (int r = ensure(c, b, std::max<int64_t>(n, 1) * sizeof(T))) return r;
+  if (n > 0) HIPOK(c, hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  *out = (const T*)b.p;
+  return 0;
+}
+
+// An output array on the device: itself when device memory, else scratch
+// copied back by stage_out.
+template <class T>
+int out_ptr(bk_ctx* c, T* dst, int64_t n, Buf& b, T** out) {
+  if (!dst) {
+    *out = nullptr;
+    return 0;
+  }
+  if (is_device(dst)) {
+    *out = dst;
+    return 0;
+  }
+  if (int r = ensure(c, b, std::max<int64_t>(n, 1) * sizeof(T))) return r;
+  *out = (T*)b.p;
+  return 0;
+}
+
+template <class T>
+int stage_out(bk_ctx* c, T* dst, const T* dev, int64_t n) {
+  if (!dst || dst == dev || n == 0) return 0;
+  HIPOK(c, hipMemcpyAsync(dst, dev, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+// Search::check (search.rs:214-264).
+const char* check_search(const bk_search* s) {
+  if (std::isnan(s->radius) || s->radius <= 0.0 || s->max_samples <= 0 || s->min_samples > s->max_samples ||
+      s->min_samples < 0)
+    return "need radius > 0 and 1 <= min_samples <= max_samples";
+  int64_t count = 0, ring = 0;
+  if (s->sectors > 0) {
+    if (s->octant) return "give octant or sectors, not both";
+    if (s->sectors < 2 || s->sectors > 360 || s->max_per_sector <= 0)
+      return "sectors need 2 <= count <= 360 and max_per_sector >= 1";
+    count = ring = s->sectors;
+  } else if (s->octant) {
+    count = 8;
+    ring = 4;
+  }
+  if (s->max_per_octant >= 0) {
+    if (!s->octant) return "max_per_octant needs octant";
+    if (s->max_per_octant == 0) return "max_per_sector must be >= 1";
+  }
+  bool rules = s->min_sectors >= 0 || s->max_empty_sectors >= 0;
+  if (rules && !(s->octant || s->sectors > 0)) return "min_sectors and max_empty_sectors need octant or sectors";
+  if (s->min_sectors >= 0 && (s->min_sectors == 0 || s->min_sectors > count || s->min_sectors > s->max_samples))
+    return "min_sectors must be at least 1 and at most the number of sectors (8 octants) and max_samples";
+  if (s->max_empty_sectors >= 0 && s->max_empty_sectors >= ring)
+    return "max_empty_sectors must be below the number of sectors (4 octants per half)";
+  if (s->min_holes >= 0 && (s->min_holes == 0 || s->min_holes > s->max_samples))
+    return "min_holes must be at least 1 and at most max_samples";
+  if (s->has_anisotropy &&
+      !(s->ratio_semi > 0 && std::isfinite(s->ratio_semi) && s->ratio_minor > 0 && std::isfinite(s->ratio_minor)))
+    return "all ranges must be finite and > 0";
+  if (s->has_anisotropy && !(std::isfinite(s->azimuth) && std::isfinite(s->dip) && std::isfinite(s->rake)))
+    return "azimuth, dip and rake must be finite";
+  return nullptr;
+}
+
+// Variogram::validate (composite.rs:37-48, model.rs:44-57, aniso.rs:29-47).
+const char* check_variogram(const bk_variogram* v) {
+  if (!(v->nugget >= 0.0 && std::isfinite(v->nugget))) return "nugget must be finite and >= 0";
+  if (v->n_structures < 0 || v->n_structures > BK_MAX_STRUCTURES) return "at most 4 structures";
+  for (int k = 0; k < v->n_structures; ++k) {
+    const bk_structure& s = v->structures[k];
+    if (s.model < BK_SPHERICAL || s.model > BK_POWER) return "unknown model";
+    if (!(s.sill >= 0.0 && std::isfinite(s.sill) && s.range > 0.0 && std::isfinite(s.range)))
+      return "structure sill must be finite and >= 0, range finite and > 0";
+    bool own = s.ratio_semi > 0.0 || s.ratio_minor > 0.0;
+    if (own && !(s.ratio_semi > 0 && std::isfinite(s.ratio_semi) && s.ratio_minor > 0 && std::isfinite(s.ratio_minor)))
+      return "all ranges must be finite and > 0";
+  }
+  if (v->has_anisotropy) {
+    if (!(v->ratio_semi > 0 && std::isfinite(v->ratio_semi) && v->ratio_minor > 0 && std::isfinite(v->ratio_minor)))
+      return "all ranges must be finite and > 0";
+    if (!(std::isfinite(v->azimuth) && std::isfinite(v->dip) && std::isfinite(v->rake)))
+      return "azimuth, dip and rake must be finite";
+  }
+  return nullptr;
+}
+
+// The variogram as the kernels read it.
+void dev_variogram(const bk_variogram* v, DevVariogram& d, DevVariogram32& f) {
+  std::memset(&d, 0, sizeof(d));
+  std::memset(&f, 0, sizeof(f));
+  d.S = v->n_structures;
+  d.nugget = v->nugget;
+  double s = -0.0;
+  for (int k = 0; k < d.S; ++k) s += v->structures[k].sill;
+  d.c0 = v->nugget + s;  // composite.rs:50-52
+  d.has_aniso = v->has_anisotropy;
+  double az = v->has_anisotropy ? v->azimuth : 0.0, dip = v->has_anisotropy ? v->dip : 0.0,
+         rk = v->has_anisotropy ? v->rake : 0.0;
+  M3 rot = rotation(az, dip, rk);
+  std::memcpy(d.rot, rot.m, sizeof(d.rot));
+  // Frame 0: the shared anisotropy (or none); own ratios get frames of their own.
+  bool shared_used = d.S == 0;
+  for (int k = 0; k < d.S; ++k) {
+    const bk_structure& st = v->structures[k];
+    bool own = st.ratio_semi > 0.0 && st.ratio_minor > 0.0;
+    if (!own) shared_used = true;
+  }
+  int nf = 0;
+  int shared_frame = -1;
+  if (shared_used) {
+    shared_frame = nf++;
+    M3 F = v->has_anisotropy ? aniso_matrix(az, dip, rk, v->ratio_semi, v->ratio_minor) : kIdentity;
+    std::memcpy(d.F[shared_frame], F.m, sizeof(F.m));
+    d.ident[shared_frame] = !v->has_anisotropy;
+    d.lag_major[shared_frame] = 1.0;
+    d.lag_semi[shared_frame] = v->has_anisotropy ? v->ratio_semi : 1.0;
+    d.lag_minor[shared_frame] = v->has_anisotropy ? v->ratio_minor : 1.0;
+  }
+  d.stationary = 1;
+  for (int k = 0; k < d.S; ++k) {
+    const bk_structure& st = v->structures[k];
+    bool own = st.ratio_semi > 0.0 && st.ratio_minor > 0.0;
+    if (own) {
+      int fr = nf++;
+      M3 F = aniso_matrix(az, dip, rk, st.ratio_semi, st.ratio_minor);
+      std::memcpy(d.F[fr], F.m, sizeof(F.m));
+      d.ident[fr] = 0;
+      d.lag_major[fr] = 1.0;
+      d.lag_semi[fr] = st.ratio_semi;
+      d.lag_minor[fr] = st.ratio_minor;
+      d.frame[k] = fr;
+    } else {
+      d.frame[k] = shared_frame;
+    }
+    d.model[k] = st.model;
+    d.param[k] = st.param;
+    d.sill[k] = st.sill;
+    d.range[k] = std::max(st.range, std::numeric_limits<double>::min());
+    d.inv_range[k] = 1.0 / d.range[k];
+    if (st.model == BK_POWER) d.stationary = 0;
+  }
+  d.nf = nf;
+  d.nug_frame = d.S > 0 ? d.frame[0] : 0;
+  f.S = d.S;
+  f.nf = d.nf;
+  f.nug_frame = d.nug_frame;
+  f.nugget = (float)d.nugget;
+  f.c0 = (float)d.c0;
+  for (int k = 0; k < d.S; ++k) {
+    f.model[k] = d.model[k];
+    f.frame[k] = d.frame[k];
+    f.param[k] = (float)d.param[k];
+    f.sill[k] = (float)d.sill[k];
+    f.inv_range[k] = (float)d.inv_range[k];
+  }
+}
+
+__global__ void k_first_finite(int64_t n, const double* x, const double* y, const double* z, unsigned long long* out,
+                               unsigned long long* nonplanar) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (isfinite(x[i]) && isfinite(y[i]) && isfinite(z[i])) atomicMin(out, (unsigned long long)i);
+  // search.rs:470-473: every point at the first point's elevation.
+  if (!(z[i] == z[0])) atomicAdd(nonplanar, 1ull);
+}
+
+__global__ void k_copy_int(int64_t n, const int32_t* src, int32_t* dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src ? src[i] : -1;
+}
+
+__global__ void k_fill_nan(int64_t m, const int32_t* status, double* value, double* variance, double* lagrange,
+                           double* support, int32_t* n_used) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  if (status[t] != BK_OK) {
+    const double nan = __longlong_as_double(0x7ff8000000000000LL);
+    value[t] = nan;
+    variance[t] = nan;
+    if (lagrange) lagrange[t] = nan;
+    if (support) support[t] = nan;
+    if (n_used) n_used[t] = 0;
+  }
+}
+
+// The leave-one-out targets: sample locations, the sample left out.
+__global__ void k_loo_targets(int64_t m, const int64_t* which, const double* x, const double* y, const double* z,
+                              double* tx, double* ty, double* tz, int64_t* left) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  int64_t i = which ? which[k] : k;
+  tx[k] = x[i];
+  ty[k] = y[i];
+  tz[k] = z[i];
+  left[k] = i;
+}
+
+// batch.rs:177-182: an empty selection is no estimate.
+__global__ void k_loo_nonempty(int64_t m, const int32_t* count, int32_t* sstatus) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m && sstatus[k] == BK_OK && count[k] == 0) sstatus[k] = BK_SEARCH_FAILED;
+}
+
+// A target whose candidate pool overflowed is not searched by a wider kernel
+// yet: it is reported unsupported.
+__global__ void k_map_overflow(int64_t m, int32_t* status) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m && status[t] == BK_ST_OVERFLOW) status[t] = BK_UNSUPPORTED;
+}
+
+unsigned blocks_of(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>((n + t - 1) / t, 1); }
+
+// Search parameters for the kernels, and the grid for the search's frame.
+int prepare_search(bk_ctx* c, const bk_search* s, const bk_variogram* vg, SearchArgs& a, int64_t& stride) {
+  if (const char* m = check_search(s)) return fail(c, BK_E_INVALID, m);
+  std::memset(&a, 0, sizeof(a));
+  // search.rs:343-348: the search's ellipsoid, else the variogram's.
+  bool has = false;
+  double az = 0, dip = 0, rk = 0, semi = 1, minor = 1;
+  if (s->has_anisotropy) {
+    has = true;
+    az = s->azimuth;
+    dip = s->dip;
+    rk = s->rake;
+    semi = s->ratio_semi;
+    minor = s->ratio_minor;
+  } else if (vg && vg->has_anisotropy) {
+    has = true;
+    az = vg->azimuth;
+    dip = vg->dip;
+    rk = vg->rake;
+    semi = vg->ratio_semi;
+    minor = vg->ratio_minor;
+  }
+  M3 F = has ? aniso_matrix(az, dip, rk, semi, minor) : kIdentity;
+  M3 axes = has ? block_frame(az, dip, rk) : kIdentity;
+  std::memcpy(a.F, F.m, sizeof(F.m));
+  std::memcpy(a.axes, axes.m, sizeof(axes.m));
+  for (int d = 0; d < 3; ++d) a.origin[d] = c->has_origin ? c->origin[d] : 0.0;
+  a.radius2 = s->radius * s->radius;
+  // A selection never holds more than the samples.
+  int64_t maxs = std::min<int64_t>(s->max_samples, std::max<int64_t>(c->n, 1));
+  if (maxs > BK_MAX_SEARCH)
+    return fail(c, BK_E_INVALID, "max_samples above 256 (with as many samples) is beyond the device search");
+  a.max_samples = (int)maxs;
+  a.min_samples = (int)std::min<int64_t>(s->min_samples, std::numeric_limits<int>::max());
+  a.balances = s->octant || s->sectors > 0;
+  a.planar = c->planar;
+  a.plane = s->sectors > 0 ? (int)s->sectors : 0;
+  a.n_sectors = a.plane ? a.plane : (a.planar ? 4 : 8);
+  if (s->sectors > 0)
+    a.sector_cap = (int)std::min<int64_t>(s->max_per_sector, std::numeric_limits<int>::max());
+  else if (s->octant)
+    a.sector_cap = (int)std::min<int64_t>(
+        s->max_per_octant >= 0 ? s->max_per_octant : (s->max_samples + a.n_sectors - 1) / a.n_sectors,
+        std::numeric_limits<int>::max());
+  else
+    a.sector_cap = std::numeric_limits<int>::max();
+  a.max_per_hole = s->max_per_hole >= 0 ? (int)std::min<int64_t>(s->max_per_hole, 1 << 30) : -1;
+  a.octant_rank = s->octant && a.max_per_hole < 0;
+  a.min_sectors = (int)s->min_sectors;
+  a.max_empty_sectors = (int)s->max_empty_sectors;
+  a.min_holes = (int)std::min<int64_t>(s->min_holes, 1 << 30);
+  a.sx = (const double*)c->x.p;
+  a.sy = (const double*)c->y.p;
+  a.sz = (const double*)c->z.p;
+  a.hole = c->has_hole ? (const int32_t*)c->hole.p : nullptr;
+  a.tab = c->tab;
+  // Grid over the projected samples, rebuilt when the frame changes.
+  if (c->n > 0) {
+    bool same = c->grid_valid && std::memcmp(c->grid_F, F.m, sizeof(F.m)) == 0;
+    if (!same) {
+      int r = bk::build_grid(c->stream, c->n, a.sx, a.sy, a.sz, F.m, a.origin, c->cell_factor, c->gb, c->grid);
+      if (r) return fail(c, BK_E_HIP, "grid build failed");
+      std::memcpy(c->grid_F, F.m, sizeof(F.m));
+      c->grid_valid = true;
+    }
+    a.g = c->grid;
+  }
+  stride = a.max_samples;
+  a.stride = (int)stride;
+  return 0;
+}
+
+// Runs the search for m targets (device arrays) into ctx scratch.
+int run_search(bk_ctx* c, SearchArgs& a, int64_t m, const double* tx, const double* ty, const double* tz,
+               const int64_t* left) {
+  if (int r = ensure(c, c->nidx, std::max<int64_t>(m * a.stride, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->ncount, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->sstatus, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  a.tx = tx;
+  a.ty = ty;
+  a.tz = tz;
+  a.left_out = left;
+  a.m = m;
+  a.list = nullptr;
+  a.idx = (int32_t*)c->nidx.p;
+  a.count = (int32_t*)c->ncount.p;
+  a.status = (int32_t*)c->sstatus.p;
+  if (m == 0) return 0;
+  if (bk::launch_search(c->stream, a, m)) return fail(c, BK_E_HIP, "search launch failed");
+  k_map_overflow<<<blocks_of(m), 256, 0, c->stream>>>(m, a.status);
+  return 0;
+}
+
+// Solves every target with status OK after the search; outputs per target.
+int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int32_t precision, int64_t m,
+              const double* tx, const double* ty, const double* tz, int stride, const bk_block* blk, double* value,
+              double* variance, double* lagrange, double* support, int32_t* n_used, int32_t* status) {
+  SolveArgs a;
+  std::memset(&a, 0, sizeof(a));
+  dev_variogram(vg, a.v, a.v32);
+  a.kind = kind;
+  a.param = param;
+  a.sx = (const double*)c->x.p;
+  a.sy = (const double*)c->y.p;
+  a.sz = (const double*)c->z.p;
+  a.sv = (const double*)c->v.p;
+  a.serr = c->has_err ? (const double*)c->e.p : nullptr;
+  a.tx = tx;
+  a.ty = ty;
+  a.tz = tz;
+  a.nidx = (const int32_t*)c->nidx.p;
+  a.ncount = (const int32_t*)c->ncount.p;
+  a.stride = stride;
+  a.value = value;
+  a.variance = variance;
+  a.lagrange = lagrange;
+  a.support = support;
+  a.n_used = n_used;
+  a.status = status;
+  a.max_iter = c->max_iter;
+  a.tol = c->tol;
+  if (blk) {
+    // block.rs:40-70 and py/estimation.rs:947-958: offsets along the block frame.
+    int npts = blk->nx * blk->ny * blk->nz;
+    std::vector<double> off(3 * npts);
+    M3 f = block_frame(blk->rotation[0], blk->rotation[1], blk->rotation[2]);
+    double dx = blk->size[0] / blk->nx, dy = blk->size[1] / blk->ny, dz = blk->size[2] / blk->nz;
+    double x0 = 0.0 - blk->size[0] / 2.0 + dx / 2.0, y0 = 0.0 - blk->size[1] / 2.0 + dy / 2.0,
+           z0 = 0.0 - blk->size[2] / 2.0 + dz / 2.0;
+    int k = 0;
+    for (int i = 0; i < blk->nx; ++i)
+      for (int j = 0; j < blk->ny; ++j)
+        for (int l = 0; l < blk->nz; ++l, ++k) {
+          double p0 = x0 + i * dx, p1 = y0 + j * dy, p2 = z0 + l * dz;
+          // frame' * p, nalgebra order
+          for (int d = 0; d < 3; ++d) {
+            double t0 = f.m[d] * p0, t1 = f.m[3 + d] * p1, t2 = f.m[6 + d] * p2;
+            off[3 * k + d] = (t0 + t1) + t2;
+          }
+        }
+    if (int r = ensure(c, c->offs, off.size() * sizeof(double))) return r;
+    HIPOK(c, hipMemcpyAsync(c->offs.p, off.data(), off.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    a.block = 1;
+    a.npts = npts;
+    a.offs = (const double*)c->offs.p;
+    a.cbb = 0.0;  // set per system: C(B,B) does not depend on the centre, computed once on the host below
+    // composite.rs:120-129 on the discretisation centred at the origin.
+    {
+      // Evaluate on the host with the same arithmetic as the device helpers.
+      auto lag = [&](double px, double py, double pz, double qx, double qy, double qz, double* h) {
+        double vx = qx - px, vy = qy - py, vz = qz - pz;
+        double r0 = 0, r1 = 0, r2 = 0;
+        bool rot = false;
+        for (int fr = 0; fr < a.v.nf; ++fr) {
+          if (a.v.ident[fr]) {
+            h[fr] = std::sqrt((vx * vx + vy * vy) + vz * vz);
+            continue;
+          }
+          if (!rot) {
+            r0 = (a.v.rot[0] * vx + a.v.rot[1] * vy) + a.v.rot[2] * vz;
+            r1 = (a.v.rot[3] * vx + a.v.rot[4] * vy) + a.v.rot[5] * vz;
+            r2 = (a.v.rot[6] * vx + a.v.rot[7] * vy) + a.v.rot[8] * vz;
+            rot = true;
+          }
+          double s0 = r0 / a.v.lag_major[fr], s1 = r1 / a.v.lag_semi[fr], s2 = r2 / a.v.lag_minor[fr];
+          h[fr] = std::sqrt((s0 * s0 + s1 * s1) + s2 * s2);
+        }
+      };
+      auto shape = [&](int model, double prm, double h, double rr) -> double {
+        if (!std::isfinite(h) || h <= 0.0) return 0.0;
+        double t = h / rr;
+        switch (model) {
+          case BK_SPHERICAL:
+            return t >= 1.0 ? 1.0 : 1.5 * t - 0.5 * (t * (t * t));
+          case BK_EXPONENTIAL:
+            return 1.0 - std::exp(-3.0 * t);
+          case BK_GAUSSIAN:
+            return 1.0 - std::exp(-3.0 * t * t);
+          case BK_CUBIC: {
+            if (t >= 1.0) return 1.0;
+            double t2 = t * t;
+            return 7.0 * t2 - (35.0 / 4.0) * (t * t2) + (7.0 / 2.0) * (t * (t2 * t2)) -
+                   (3.0 / 4.0) * ((t * t2) * (t2 * t2));
+          }
+          case BK_PENTASPHERICAL: {
+            if (t >= 1.0) return 1.0;
+            double t2 = t * t;
+            return (15.0 / 8.0) * t - (5.0 / 4.0) * (t * t2) + (3.0 / 8.0) * (t * (t2 * t2));
+          }
+          case BK_CIRCULAR:
+            if (t >= 1.0) return 1.0;
+            return 1.0 - (2.0 / kPi) * std::acos(t) + (2.0 * t / kPi) * std::sqrt(1.0 - t * t);
+          case BK_SINEHOLE: {
+            double cc = kPi * t;
+            return 1.0 - std::sin(cc) / cc;
+          }
+          case BK_MATERN: {
+            double nu = prm < 1.0 ? 0.5 : (prm < 2.0 ? 1.5 : 2.5);
+            double delta = (std::sqrt(2.0 * nu) * 3.0) * t;
+            double ee = std::exp(-delta);
+            if (nu == 0.5) return 1.0 - ee;
+            if (nu == 1.5) return 1.0 - (1.0 + delta) * ee;
+            return 1.0 - (1.0 + delta + delta * delta / 3.0) * ee;
+          }
+          case BK_POWER:
+            return std::pow(t, prm);
+        }
+        return 0.0;
+      };
+      auto bcov = [&](const double* h) {
+        if (h[a.v.nug_frame] <= 0.0) return a.v.c0 - a.v.nugget;
+        double ss = -0.0;
+        for (int kk = 0; kk < a.v.S; ++kk)
+          ss += a.v.sill[kk] * shape(a.v.model[kk], a.v.param[kk], h[a.v.frame[kk]], a.v.range[kk]);
+        return a.v.c0 - (a.v.nugget + ss);
+      };
+      if (npts == 1) {
+        a.cbb = a.v.c0;
+      } else {
+        double total = -0.0;
+        for (int p = 0; p < npts; ++p) {
+          double row = -0.0;
+          for (int q = 0; q < npts; ++q) {
+            double h[BK_MAX_FRAMES];
+            lag(off[3 * p], off[3 * p + 1], off[3 * p + 2], off[3 * q], off[3 * q + 1], off[3 * q + 2], h);
+            row += bcov(h);
+          }
+          total += row;
+        }
+        a.cbb = total / (double)((int64_t)npts * npts);
+      }
+    }
+  }
+  // Bucket the searched targets by neighbour count.
+  const int nb = 3;
+  if (int r = ensure(c, c->lists, std::max<int64_t>(m, 1) * (nb + 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
+  HIPOK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(int32_t), c->stream));
+  int32_t* lists = (int32_t*)c->lists.p;
+  int32_t* counts = (int32_t*)c->counts.p;
+  if (bk::launch_bucket(c->stream, m, (const int32_t*)c->ncount.p, (const int32_t*)c->sstatus.p, status, value,
+                        variance, lists, counts, m, c->edges, nb, a.v.stationary))
+    return fail(c, BK_E_HIP, "bucket launch failed");
+  int32_t hc[8];
+  HIPOK(c, hipMemcpyAsync(hc, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->redo, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  a.guard = (int32_t*)c->guard.p;
+  a.redo = (int32_t*)c->redo.p;
+  a.guard_count = counts + 4;
+  a.redo_count = counts + 5;
+  for (int b = 0; b < nb; ++b) {
+    if (hc[b] == 0) continue;
+    a.list = lists + (int64_t)b * m;
+    a.n_list = hc[b];
+    int r = precision == BK_FP32_IR ? bk::launch_krige32(c->stream, a, c->edges[b])
+                                    : bk::launch_krige64(c->stream, a, c->edges[b]);
+    if (r) return fail(c, BK_E_HIP, "solve launch failed");
+  }
+  // Systems the FP32 form handed back: FP64 over the redo list.
+  if (precision == BK_FP32_IR) {
+    int32_t h2[8];
+    HIPOK(c, hipMemcpyAsync(h2, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPOK(c, hipStreamSynchronize(c->stream));
+    if (h2[5] > 0) {
+      SolveArgs b = a;
+      b.list = a.redo;
+      b.n_list = h2[5];
+      if (bk::launch_krige64(c->stream, b, 64)) return fail(c, BK_E_HIP, "solve launch failed");
+    }
+  }
+  // The LU list (n > 64, power models) joins the guarded systems.
+  int32_t h3[8];
+  HIPOK(c, hipMemcpyAsync(h3, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  if (hc[nb] > 0) {
+    SolveArgs b = a;
+    b.guard = lists + (int64_t)nb * m;
+    b.guard_count = counts + nb;
+    b.n_list = hc[nb];
+    if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
+  }
+  if (h3[4] > 0) {
+    SolveArgs b = a;
+    b.n_list = h3[4];
+    if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
+  }
+  return 0;
+}
+
+int finalize(bk_ctx* c, int64_t m, int32_t* status, double* value, double* variance, double* lagrange,
+             double* support, int32_t* n_used) {
+  if (m == 0) return 0;
+  k_fill_nan<<<blocks_of(m), 256, 0, c->stream>>>(m, status, value, variance, lagrange, support, n_used);
+  HIPOK(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bk_version(void) { return "boitata-mi355x 0.1.0 (gfx950)"; }
+
+int bk_create(bk_ctx** out, int device) {
+  if (!out) return BK_E_INVALID;
+  *out = nullptr;
+  bk_ctx* c = new bk_ctx();
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete c;
+    return BK_E_HIP;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BK_E_HIP;
+  }
+  if (bk::upload_shell_table(c->tab, &c->tab_storage)) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BK_E_HIP;
+  }
+  *out = c;
+  return 0;
+}
+
+void bk_destroy(bk_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (Buf* b : {&c->x, &c->y, &c->z, &c->v, &c->e, &c->hole, &c->tx, &c->ty, &c->tz, &c->left, &c->nidx, &c->ncount,
+                 &c->sstatus, &c->lists, &c->counts, &c->status, &c->value, &c->variance, &c->lagrange, &c->support,
+                 &c->n_used, &c->guard, &c->redo, &c->offs, &c->which, &c->iters, &c->loo_idx, &c->loo_count,
+                 &c->loo_status})
+    release(*b);
+  bk::free_grid(c->gb);
+  if (c->tab_storage) (void)hipFree(c->tab_storage);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* bk_last_error(const bk_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int bk_check_search(bk_ctx* c, const bk_search* s) {
+  if (!s) return fail(c, BK_E_INVALID, "no search");
+  if (const char* m = check_search(s)) return fail(c, BK_E_INVALID, m);
+  return 0;
+}
+
+int bk_check_variogram(bk_ctx* c, const bk_variogram* v) {
+  if (!v) return fail(c, BK_E_INVALID, "no variogram");
+  if (const char* m = check_variogram(v)) return fail(c, BK_E_INVALID, m);
+  return 0;
+}
+
+int bk_set_samples(bk_ctx* c, int64_t n, const double* x, const double* y, const double* z, const double* value,
+                   const double* error_var, const int32_t* hole) {
+  if (!c) return BK_E_INVALID;
+  if (n < 0 || n > (int64_t)2147483647) return fail(c, BK_E_INVALID, "need 0 <= n < 2^31 samples");
+  if (n > 0 && (!x || !y || !z || !value)) return fail(c, BK_E_INVALID, "coordinates and values are required");
+  HIPOK(c, hipSetDevice(c->device));
+  size_t bytes = std::max<int64_t>(n, 1) * sizeof(double);
+  for (Buf* b : {&c->x, &c->y, &c->z, &c->v, &c->e})
+    if (int r = ensure(c, *b, bytes)) return r;
+  if (int r = ensure(c, c->hole, std::max<int64_t>(n, 1) * sizeof(int32_t))) return r;
+  auto copy = [&](Buf& b, const void* src, size_t sz) -> int {
+    if (n == 0) return 0;
+    HIPOK(c, hipMemcpyAsync(b.p, src, n * sz, is_device(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            c->stream));
+    return 0;
+  };
+  if (int r = copy(c->x, x, 8)) return r;
+  if (int r = copy(c->y, y, 8)) return r;
+  if (int r = copy(c->z, z, 8)) return r;
+  if (int r = copy(c->v, value, 8)) return r;
+  c->has_err = error_var != nullptr;
+  if (c->has_err)
+    if (int r = copy(c->e, error_var, 8)) return r;
+  c->has_hole = hole != nullptr;
+  if (c->has_hole)
+    if (int r = copy(c->hole, hole, 4)) return r;
+  c->n = n;
+  c->grid_valid = false;
+  c->loo_ready = false;
+  // Origin: the first sample with finite coordinates; planar: one elevation.
+  c->has_origin = false;
+  c->planar = true;
+  if (n > 0) {
+    if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
+    unsigned long long init[2] = {~0ull, 0ull};
+    unsigned long long* d = (unsigned long long*)c->counts.p;
+    HIPOK(c, hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    k_first_finite<<<blocks_of(n), 256, 0, c->stream>>>(n, (const double*)c->x.p, (const double*)c->y.p,
+                                                        (const double*)c->z.p, d, d + 1);
+    unsigned long long res[2];
+    HIPOK(c, hipMemcpyAsync(res, d, sizeof(res), hipMemcpyDeviceToHost, c->stream));
+    HIPOK(c, hipStreamSynchronize(c->stream));
+    c->planar = res[1] == 0;
+    if (res[0] != ~0ull) {
+      c->has_origin = true;
+      double o[3];
+      HIPOK(c, hipMemcpy(&o[0], (const double*)c->x.p + res[0], 8, hipMemcpyDeviceToHost));
+      HIPOK(c, hipMemcpy(&o[1], (const double*)c->y.p + res[0], 8, hipMemcpyDeviceToHost));
+      HIPOK(c, hipMemcpy(&o[2], (const double*)c->z.p + res[0], 8, hipMemcpyDeviceToHost));
+      std::memcpy(c->origin, o, sizeof(o));
+    }
+  }
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bk_neighbors(bk_ctx* c, const bk_search* s, const bk_variogram* vg, int64_t m, const double* tx, const double* ty,
+                 const double* tz, const int64_t* left_out, int32_t* idx, int32_t* count, int32_t* status) {
+  if (!c || !s) return BK_E_INVALID;
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  HIPOK(c, hipSetDevice(c->device));
+  SearchArgs a;
+  int64_t stride;
+  if (int r = prepare_search(c, s, vg, a, stride)) return r;
+  const double *dx, *dy, *dz;
+  const int64_t* dl;
+  if (int r = stage_in(c, tx, m, c->tx, &dx)) return r;
+  if (int r = stage_in(c, ty, m, c->ty, &dy)) return r;
+  if (int r = stage_in(c, tz, m, c->tz, &dz)) return r;
+  if (int r = stage_in(c, left_out, m, c->left, &dl)) return r;
+  if (int r = run_search(c, a, m, dx, dy, dz, dl)) return r;
+  // The caller's layout has stride max_samples (not clamped to the sample count).
+  int64_t user_stride = s->max_samples;
+  if (user_stride == stride) {
+    if (is_device(idx))
+      HIPOK(c, hipMemcpyAsync(idx, c->nidx.p, m * stride * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+    else
+      HIPOK(c, hipMemcpyAsync(idx, c->nidx.p, m * stride * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    HIPOK(c, hipMemcpy2DAsync(idx, user_stride * sizeof(int32_t), c->nidx.p, stride * sizeof(int32_t),
+                              stride * sizeof(int32_t), m, is_device(idx) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              c->stream));
+  }
+  HIPOK(c, hipMemcpyAsync(count, c->ncount.p, m * sizeof(int32_t),
+                          is_device(count) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+  HIPOK(c, hipMemcpyAsync(status, c->sstatus.p, m * sizeof(int32_t),
+                          is_device(status) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  // A pool overflow is not yet searched by a wider kernel.
+  return 0;
+}
+
+static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                        const bk_block* blk, int32_t precision, int64_t m, const double* tx, const double* ty,
+                        const double* tz, const int64_t* which_loo, bool loo, double* value, double* variance,
+                        double* lagrange, double* support, int32_t* n_used, int32_t* status) {
+  if (!c || !vg || !s) return BK_E_INVALID;
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  if (kind < BK_ORDINARY || kind > BK_INDICATOR) return fail(c, BK_E_INVALID, "unknown kind");
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
+  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
+  bool stationary = true;
+  for (int k = 0; k < vg->n_structures; ++k) stationary = stationary && vg->structures[k].model != BK_POWER;
+  if (kind == BK_SIMPLE && !stationary)
+    return fail(c, BK_E_INVALID,
+                "simple kriging needs a variogram with a sill: every structure must be stationary (not power); use "
+                "ordinary kriging");
+  if (blk) {
+    if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
+      return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
+    for (int d = 0; d < 3; ++d)
+      if (!(std::isfinite(blk->size[d]) && blk->size[d] >= 0.0))
+        return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
+  }
+  HIPOK(c, hipSetDevice(c->device));
+  SearchArgs a;
+  int64_t stride;
+  if (int r = prepare_search(c, s, vg, a, stride)) return r;
+  const double *dx, *dy, *dz;
+  const int64_t* dl = nullptr;
+  if (loo) {
+    const int64_t* dw = nullptr;
+    if (which_loo) {
+      if (int r = stage_in(c, which_loo, m, c->which, &dw)) return r;
+    } else {
+      m = c->n;
+    }
+    for (Buf* b : {&c->tx, &c->ty, &c->tz})
+      if (int r = ensure(c, *b, std::max<int64_t>(m, 1) * sizeof(double))) return r;
+    if (int r = ensure(c, c->left, std::max<int64_t>(m, 1) * sizeof(int64_t))) return r;
+    if (m > 0)
+      k_loo_targets<<<blocks_of(m), 256, 0, c->stream>>>(m, dw, (const double*)c->x.p, (const double*)c->y.p,
+                                                         (const double*)c->z.p, (double*)c->tx.p, (double*)c->ty.p,
+                                                         (double*)c->tz.p, (int64_t*)c->left.p);
+    dx = (const double*)c->tx.p;
+    dy = (const double*)c->ty.p;
+    dz = (const double*)c->tz.p;
+    dl = (const int64_t*)c->left.p;
+  } else {
+    if (int r = stage_in(c, tx, m, c->tx, &dx)) return r;
+    if (int r = stage_in(c, ty, m, c->ty, &dy)) return r;
+    if (int r = stage_in(c, tz, m, c->tz, &dz)) return r;
+  }
+  double *ov, *ovar, *olag = nullptr, *osup = nullptr;
+  int32_t *onu = nullptr, *ost;
+  if (int r = out_ptr(c, value, m, c->value, &ov)) return r;
+  if (int r = out_ptr(c, variance, m, c->variance, &ovar)) return r;
+  if (int r = out_ptr(c, lagrange, m, c->lagrange, &olag)) return r;
+  if (int r = out_ptr(c, support, m, c->support, &osup)) return r;
+  if (int r = out_ptr(c, n_used, m, c->n_used, &onu)) return r;
+  if (status) {
+    if (int r = out_ptr(c, status, m, c->status, &ost)) return r;
+  } else {
+    if (int r = ensure(c, c->status, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+    ost = (int32_t*)c->status.p;
+  }
+  if (int r = run_search(c, a, m, dx, dy, dz, dl)) return r;
+  if (loo && m > 0) k_loo_nonempty<<<blocks_of(m), 256, 0, c->stream>>>(m, (const int32_t*)c->ncount.p,
+                                                                       (int32_t*)c->sstatus.p);
+  if (int r = run_solve(c, vg, kind, param, precision, m, dx, dy, dz, (int)stride, blk, ov, ovar, olag, osup, onu,
+                        ost))
+    return r;
+  if (int r = finalize(c, m, ost, ov, ovar, olag, osup, onu)) return r;
+  if (int r = stage_out(c, value, ov, m)) return r;
+  if (int r = stage_out(c, variance, ovar, m)) return r;
+  if (int r = stage_out(c, lagrange, olag, m)) return r;
+  if (int r = stage_out(c, support, osup, m)) return r;
+  if (int r = stage_out(c, n_used, onu, m)) return r;
+  if (status)
+    if (int r = stage_out(c, status, ost, m)) return r;
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bk_krige(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param, int32_t precision,
+             int64_t m, const double* tx, const double* ty, const double* tz, double* value, double* variance,
+             double* lagrange, int32_t* n_used, int32_t* status) {
+  return krige_common(c, vg, s, kind, param, nullptr, precision, m, tx, ty, tz, nullptr, false, value, variance,
+                      lagrange, nullptr, n_used, status);
+}
+
+int bk_block_krige(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                   const bk_block* block, int32_t precision, int64_t m, const double* tx, const double* ty,
+                   const double* tz, double* value, double* variance, double* lagrange, double* support,
+                   int32_t* n_used, int32_t* status) {
+  if (!block) return fail(c, BK_E_INVALID, "no block");
+  return krige_common(c, vg, s, kind, param, block, precision, m, tx, ty, tz, nullptr, false, value, variance,
+                      lagrange, support, n_used, status);
+}
+
+int bk_loo(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param, int32_t precision,
+           int64_t m, const int64_t* which, double* value, double* variance, int32_t* status) {
+  return krige_common(c, vg, s, kind, param, nullptr, precision, which ? m : (c ? c->n : 0), nullptr, nullptr, nullptr,
+                      which, true, value, variance, nullptr, nullptr, nullptr, status);
+}
+
+int bk_loo_prepare(bk_ctx* c, const bk_search* s, const bk_variogram* vg) {
+  return fail(c, BK_E_STATE, "not built yet");
+}
+
+int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, double* loss, double* grad,
+                     int64_t* n_used) {
+  return fail(c, BK_E_STATE, "not built yet");
+}
+
+// Tuning knobs (not part of the reference interface): bucket edges, FP32+IR
+// sweeps and tolerance, grid cell factor.
+int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor) {
+  if (!c) return BK_E_INVALID;
+  if (edges3) {
+    for (int k = 0; k < 3; ++k)
+      if (edges3[k] != 16 && edges3[k] != 32 && edges3[k] != 64)
+        return fail(c, BK_E_INVALID, "bucket edges are 16, 32 or 64");
+    if (!(edges3[0] <= edges3[1] && edges3[1] <= edges3[2] && edges3[2] == 64))
+      return fail(c, BK_E_INVALID, "bucket edges must rise to 64");
+    for (int k = 0; k < 3; ++k) c->edges[k] = edges3[k];
+  }
+  if (max_iter > 0) c->max_iter = max_iter;
+  if (tol > 0) c->tol = tol;
+  if (cell_factor > 0 && cell_factor != c->cell_factor) {
+    c->cell_factor = cell_factor;
+    c->grid_valid = false;
+  }
+  return 0;
+}
+
+}  // extern "C"

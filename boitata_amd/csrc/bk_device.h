@@ -1,0 +1,256 @@
+// bk_device.h — device-side covariance arithmetic shared by the search and
+// solve kernels of the MI355X kriging engine (gfx950, wave64).
+//
+// FP64 functions follow the reference's operation order so the plain FP64
+// path reproduces boitatá's rounding wherever the math library agrees:
+//   crates/variogram/src/model.rs:76-126       shape
+//   crates/variogram/src/composite.rs:50-115   total_sill, gamma, cov, block_cov_points
+//   crates/variogram/src/aniso.rs:101-119      Anisotropy::lag, euclidean
+//   crates/estimation/src/kernel.rs:36-56      the frame and the packed covariance fill
+// The translation units that include this header are compiled with
+// -ffp-contract=off; FP32 code opts into contraction locally.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/boitata_gpu.h"
+
+#define BK_WAVE 64
+#define BK_MAX_FRAMES (BK_MAX_STRUCTURES + 1)
+
+// Wave-local ordering of LDS traffic between lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double bcast(double v, int lane) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ float bcastf(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ int bcasti(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Sum over the wave (butterfly; every lane gets the total).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, BK_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, BK_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, BK_WAVE));
+  return v;
+}
+
+// The variogram as the kernels read it. Frames: the linear maps to the
+// isotropic space of each structure (kernel.rs:36-42 matrix()); a structure on
+// the shared anisotropy uses the shared frame, so with shared ratios there is
+// one frame and the arithmetic is the reference's.
+struct DevVariogram {
+  int S;              // structures
+  int nf;             // frames in use
+  int nug_frame;      // frame of the nugget's h > 0 test (that of structure 0)
+  int stationary;
+  double nugget;
+  double c0;          // total_sill in the reference's order
+  int model[BK_MAX_STRUCTURES];
+  int frame[BK_MAX_STRUCTURES];
+  double param[BK_MAX_STRUCTURES];
+  double sill[BK_MAX_STRUCTURES];
+  double range[BK_MAX_STRUCTURES];   // max(range, MIN_POSITIVE)
+  double inv_range[BK_MAX_STRUCTURES];
+  double F[BK_MAX_FRAMES][9];        // row-major frames (matrix() of each anisotropy)
+  int ident[BK_MAX_FRAMES];          // the shared frame without anisotropy: no map, euclidean lags
+  // Anisotropy::lag form (aniso.rs:101-111): shared rotation, per-frame axes.
+  int has_aniso;
+  double rot[9];
+  double lag_major[BK_MAX_FRAMES], lag_semi[BK_MAX_FRAMES], lag_minor[BK_MAX_FRAMES];
+};
+
+// r_i = (m_i0 v0 + m_i1 v1) + m_i2 v2, unfused.
+__device__ __forceinline__ void mv3(const double* m, double v0, double v1, double v2, double& r0, double& r1,
+                                    double& r2) {
+  r0 = (m[0] * v0 + m[1] * v1) + m[2] * v2;
+  r1 = (m[3] * v0 + m[4] * v1) + m[5] * v2;
+  r2 = (m[6] * v0 + m[7] * v1) + m[8] * v2;
+}
+
+// Rust powi lowered by binary exponentiation.
+__device__ __forceinline__ double pw3(double t) { return t * (t * t); }
+__device__ __forceinline__ double pw5(double t) {
+  double t2 = t * t;
+  return t * (t2 * t2);
+}
+__device__ __forceinline__ double pw7(double t) {
+  double t2 = t * t;
+  return (t * t2) * (t2 * t2);
+}
+
+// model.rs:76-126, FP64, t = h / range with a true division.
+__device__ __forceinline__ double shape64(int model, double param, double h, double r) {
+  if (!isfinite(h) || h <= 0.0) return 0.0;
+  double t = h / r;
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0 ? 1.0 : 1.5 * t - 0.5 * pw3(t);
+    case BK_EXPONENTIAL:
+      return 1.0 - exp(-3.0 * t);
+    case BK_GAUSSIAN:
+      return 1.0 - exp(-3.0 * t * t);
+    case BK_CUBIC:
+      if (t >= 1.0) return 1.0;
+      return 7.0 * (t * t) - (35.0 / 4.0) * pw3(t) + (7.0 / 2.0) * pw5(t) - (3.0 / 4.0) * pw7(t);
+    case BK_PENTASPHERICAL:
+      if (t >= 1.0) return 1.0;
+      return (15.0 / 8.0) * t - (5.0 / 4.0) * pw3(t) + (3.0 / 8.0) * pw5(t);
+    case BK_CIRCULAR:
+      if (t >= 1.0) return 1.0;
+      return 1.0 - (2.0 / 3.14159265358979323846) * acos(t) +
+             (2.0 * t / 3.14159265358979323846) * sqrt(1.0 - t * t);
+    case BK_SINEHOLE: {
+      double c = 3.14159265358979323846 * t;
+      return 1.0 - sin(c) / c;
+    }
+    case BK_MATERN: {
+      double nu = param < 1.0 ? 0.5 : (param < 2.0 ? 1.5 : 2.5);
+      double delta = (sqrt(2.0 * nu) * 3.0) * t;
+      double e = exp(-delta);
+      if (nu == 0.5) return 1.0 - e;
+      if (nu == 1.5) return 1.0 - (1.0 + delta) * e;
+      return 1.0 - (1.0 + delta + delta * delta / 3.0) * e;
+    }
+    case BK_POWER:
+      return pow(t, param);
+  }
+  return 0.0;
+}
+
+// composite.rs:69-85 with a lag per frame.
+__device__ __forceinline__ double cov64(const DevVariogram& v, const double* h) {
+  double hn = h[v.nug_frame];
+  if (hn <= 0.0) return v.c0;
+  double s = -0.0;
+  for (int k = 0; k < v.S; ++k) s += v.sill[k] * shape64(v.model[k], v.param[k], h[v.frame[k]], v.range[k]);
+  return v.c0 - (v.nugget + s);
+}
+
+// composite.rs:108-115: nugget left out where the points coincide.
+__device__ __forceinline__ double block_cov64(const DevVariogram& v, const double* h) {
+  if (h[v.nug_frame] <= 0.0) return v.c0 - v.nugget;
+  return cov64(v, h);
+}
+
+__device__ __forceinline__ double norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
+
+// Anisotropy::lag per frame between world points p and q (aniso.rs:101-111),
+// or euclidean (aniso.rs:114-119) without anisotropy.
+__device__ __forceinline__ void lags_world(const DevVariogram& v, double px, double py, double pz, double qx,
+                                           double qy, double qz, double* h) {
+  double vx = qx - px, vy = qy - py, vz = qz - pz;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  bool rotated = false;
+  for (int f = 0; f < v.nf; ++f) {
+    if (v.ident[f]) {
+      h[f] = norm3(vx, vy, vz);
+      continue;
+    }
+    if (!rotated) {
+      mv3(v.rot, vx, vy, vz, r0, r1, r2);
+      rotated = true;
+    }
+    double s0 = r0 / v.lag_major[f], s1 = r1 / v.lag_semi[f], s2 = r2 / v.lag_minor[f];
+    h[f] = norm3(s0, s1, s2);
+  }
+}
+
+__device__ __forceinline__ double cov_points64(const DevVariogram& v, double px, double py, double pz, double qx,
+                                               double qy, double qz) {
+  double h[BK_MAX_FRAMES];
+  lags_world(v, px, py, pz, qx, qy, qz, h);
+  return cov64(v, h);
+}
+
+__device__ __forceinline__ double block_cov_points64(const DevVariogram& v, double px, double py, double pz,
+                                                     double qx, double qy, double qz) {
+  double h[BK_MAX_FRAMES];
+  lags_world(v, px, py, pz, qx, qy, qz, h);
+  return block_cov64(v, h);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 forms for the refined path (contraction allowed, native transcendentals).
+struct DevVariogram32 {
+  int S, nf, nug_frame;
+  float nugget, c0;
+  int model[BK_MAX_STRUCTURES];
+  int frame[BK_MAX_STRUCTURES];
+  float param[BK_MAX_STRUCTURES];
+  float sill[BK_MAX_STRUCTURES];
+  float inv_range[BK_MAX_STRUCTURES];
+};
+
+__device__ __forceinline__ float shape32(int model, float param, float h, float inv_r) {
+#pragma clang fp contract(fast)
+  if (!(h > 0.0f)) return 0.0f;
+  float t = h * inv_r;
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0f ? 1.0f : t * (1.5f - 0.5f * t * t);
+    case BK_EXPONENTIAL:
+      return 1.0f - __expf(-3.0f * t);
+    case BK_GAUSSIAN:
+      return 1.0f - __expf(-3.0f * t * t);
+    case BK_CUBIC: {
+      if (t >= 1.0f) return 1.0f;
+      float t2 = t * t;
+      return t2 * (7.0f + t * (-8.75f + t2 * (3.5f - 0.75f * t2)));
+    }
+    case BK_PENTASPHERICAL: {
+      if (t >= 1.0f) return 1.0f;
+      float t2 = t * t;
+      return t * (1.875f + t2 * (-1.25f + 0.375f * t2));
+    }
+    case BK_CIRCULAR:
+      if (t >= 1.0f) return 1.0f;
+      return 1.0f - 0.63661977f * acosf(t) + 0.63661977f * t * sqrtf(1.0f - t * t);
+    case BK_SINEHOLE: {
+      float c = 3.14159265f * t;
+      return 1.0f - __sinf(c) / c;
+    }
+    case BK_MATERN: {
+      float nu = param < 1.0f ? 0.5f : (param < 2.0f ? 1.5f : 2.5f);
+      float delta = sqrtf(2.0f * nu) * 3.0f * t;
+      float e = __expf(-delta);
+      if (nu == 0.5f) return 1.0f - e;
+      if (nu == 1.5f) return 1.0f - (1.0f + delta) * e;
+      return 1.0f - (1.0f + delta + delta * delta * (1.0f / 3.0f)) * e;
+    }
+    case BK_POWER:
+      return powf(t, param);
+  }
+  return 0.0f;
+}
+
+__device__ __forceinline__ float cov32(const DevVariogram32& v, const float* h) {
+#pragma clang fp contract(fast)
+  if (!(h[v.nug_frame] > 0.0f)) return v.c0;
+  float s = 0.0f;
+  for (int k = 0; k < v.S; ++k) s += v.sill[k] * shape32(v.model[k], v.param[k], h[v.frame[k]], v.inv_range[k]);
+  return v.c0 - (v.nugget + s);
+}

@@ -1,0 +1,90 @@
+// bk_search.h — neighbour search of the MI355X kriging engine.
+//
+// Replaces SearchTree (crates/estimation/src/search.rs:692-1006) and its cell
+// grid (crates/estimation/src/search/grid.rs). The result is defined by the
+// reference alone: candidates within the radius in increasing
+// (IEEE bits of the FP64 squared distance in the search frame, sample index),
+// offered greedily to the selector of search.rs:476-577. The device grid is an
+// acceleration structure only; its cell size is free.
+#pragma once
+
+#include <stdint.h>
+
+#include "bk_device.h"
+
+#define BK_POOL 512       // candidates held per target between releases
+#define BK_MAX_SEARCH 256 // most samples one search selects on the device
+#define BK_REACH 12
+
+// Internal per-target status: the candidate pool overflowed; the target is
+// searched again by the windowed kernel.
+#define BK_ST_OVERFLOW 100
+
+struct GridDev {
+  double lo[3];
+  double h;
+  int dims[3];
+  const uint32_t* start;      // cells + 1
+  const double4* entries;     // projected x, y, z; w holds the sample index bits
+};
+
+struct ShellTable {
+  const char4* offsets;       // sorted by m = sum max(0, |o| - 1)^2
+  const uint32_t* end;        // end of each shell in offsets
+  const uint32_t* m;          // m of each shell
+  int n;
+};
+
+struct SearchArgs {
+  GridDev g;
+  ShellTable tab;
+  double F[9];                // search frame (Anisotropy::matrix() or identity)
+  double origin[3];           // first finite sample (search.rs:714-734)
+  double radius2;
+  int max_samples, min_samples;
+  int balances;               // octant or plane sectors
+  int octant_rank;            // octants without a hole cap: prune by per-octant rank
+  int n_sectors;              // 8, 4 (planar octants) or the plane sector count
+  int plane;                  // plane sectors (atan2 in the major/semi plane)
+  int planar;
+  int sector_cap;
+  int max_per_hole;           // -1 none
+  int min_sectors, max_empty_sectors, min_holes;  // -1 none
+  double axes[9];             // block_frame of the metric's angles (search.rs:404-417)
+  const double* sx;
+  const double* sy;
+  const double* sz;
+  const int32_t* hole;        // may be null
+  const double* tx;
+  const double* ty;
+  const double* tz;
+  const int64_t* left_out;    // may be null
+  const int32_t* list;        // targets to search (null: 0..m)
+  int64_t m;
+  int32_t* idx;               // m * stride
+  int32_t* count;
+  int32_t* status;
+  int stride;
+};
+
+// Launchers (bk_search.hip).
+namespace bk {
+// Projects samples, bounds them and counting-sorts them into cells of
+// side cell_factor * (the reference's side rule), on the device.
+struct GridBuffers {
+  double4* entries = nullptr;
+  uint32_t* start = nullptr;
+  uint32_t* cell_of = nullptr;
+  uint32_t* fill = nullptr;
+  double* px = nullptr;       // projected samples (SoA, reused by the scratch of the build)
+  double* bounds = nullptr;   // partial min / max
+  int64_t cap_n = 0, cap_cells = 0, cap_blocks = 0;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const double* z, const double F[9],
+               const double origin[3], double cell_factor, GridBuffers& gb, GridDev& out);
+void free_grid(GridBuffers& gb);
+int upload_shell_table(ShellTable& tab, void** storage);
+int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets);
+}  // namespace bk

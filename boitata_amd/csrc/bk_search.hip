@@ -1,0 +1,843 @@
+// bk_search.hip — device grid and nearest-first neighbour search.
+//
+// The selection restates crates/estimation/src/search.rs:476-577 (Selector)
+// and :951-1005 (SearchTree::neighbors_around / offer) over candidates ordered
+// by (FP64 squared distance bits, sample index), the order the reference's grid
+// releases them in (crates/estimation/src/search/grid.rs:13-15,115-127,209-255).
+// Distances are computed exactly as the reference computes them: the frame
+// product in nalgebra's order and d2 = ((a*a + b*b) + c*c), unfused (this file
+// is compiled with -ffp-contract=off), so the neighbour sets are bit-identical.
+//
+// One wavefront per target. Cells are visited shell by shell (offsets within
+// 12 cells sorted by the squared lower bound of their distance, then rings);
+// before a shell is scanned every pooled candidate below its bound is released
+// in order to the selector. Candidates travel in batches of at most 64, one per
+// lane: sorted across the wave, merged into a sorted LDS pool, and pruned where
+// the reference could never select them (beyond max_samples, a full octant or
+// hole, or ranked past what is left of an octant's cap).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "bk_search.h"
+
+namespace {
+
+__device__ __forceinline__ uint64_t dbits(double d) { return (uint64_t)__double_as_longlong(d); }
+
+__device__ __forceinline__ bool key_less(uint64_t ka, uint32_t ia, uint64_t kb, uint32_t ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = __shfl_xor((unsigned)(v & 0xffffffffu), m, BK_WAVE);
+  uint32_t hi = __shfl_xor((unsigned)(v >> 32), m, BK_WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Ascending bitonic sort of one candidate per lane, by (key, idx).
+__device__ __forceinline__ void wave_sort(uint64_t& k, uint32_t& i, int& sec, int& hl, int lane) {
+#pragma unroll
+  for (int size = 2; size <= BK_WAVE; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      uint64_t ok = shfl_xor_u64(k, stride);
+      uint32_t oi = __shfl_xor(i, stride, BK_WAVE);
+      int os = __shfl_xor(sec, stride, BK_WAVE);
+      int oh = __shfl_xor(hl, stride, BK_WAVE);
+      bool ascending = (lane & size) == 0;
+      bool lower = (lane & stride) == 0;
+      bool other_less = key_less(ok, oi, k, i);
+      bool take = (lower == ascending) ? other_less : key_less(k, i, ok, oi);
+      if (take) {
+        k = ok;
+        i = oi;
+        sec = os;
+        hl = oh;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int o = 1; o < BK_WAVE; o <<= 1) {
+    int u = __shfl_up(v, o, BK_WAVE);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int clamp_cell(double v, int dim) {
+  // Rust `as i64` saturates (NaN -> 0) before the clamp (grid.rs:183-188).
+  double f = floor(v);
+  f = fmax(f, -1.0);
+  f = fmin(f, (double)dim);
+  int c = (int)f;
+  return c < 0 ? 0 : (c > dim - 1 ? dim - 1 : c);
+}
+
+struct SearchLds {
+  uint64_t pk[BK_POOL];
+  uint32_t pi[BK_POOL];
+  int ps[BK_POOL];
+  int ph[BK_POOL];
+  uint64_t bk[BK_WAVE];
+  uint32_t bi[BK_WAVE];
+  int cs[BK_WAVE];
+  int ex[BK_WAVE];
+  int chosen[BK_MAX_SEARCH];
+  int hole_id[BK_MAX_SEARCH];
+  int hole_cnt[BK_MAX_SEARCH];
+  int per_sector[360];
+  uint64_t thr_sec[8];
+  int scalar[4];
+};
+
+struct State {
+  int P;          // pooled candidates
+  int n;          // chosen
+  int n_holes;    // distinct holes in hole_id (with a hole cap)
+  int untagged;
+  int overflow;
+  uint64_t thr;   // with a pool full of sure picks: keys above are useless
+  uint32_t thr_i;
+};
+
+// Number of pool entries with (key, idx) < (k, i).
+__device__ __forceinline__ int pool_lower(const SearchLds& L, int P, uint64_t k, uint32_t i) {
+  int lo = 0, hi = P;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (key_less(L.pk[mid], L.pi[mid], k, i))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int batch_lower(const SearchLds& L, int nb, uint64_t k, uint32_t i) {
+  int lo = 0, hi = nb;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (key_less(L.bk[mid], L.bi[mid], k, i))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Keeps the pool entries whose flag is set, in order (two phases: read, write).
+template <class Keep>
+__device__ __forceinline__ int pool_compact(SearchLds& L, int P, int lane, Keep keep) {
+  constexpr int R = BK_POOL / BK_WAVE;
+  uint64_t k[R];
+  uint32_t ii[R];
+  int s[R], h[R], pos[R];
+  int base = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int j = r * BK_WAVE + lane;
+    bool kp = false;
+    if (j < P) {
+      k[r] = L.pk[j];
+      ii[r] = L.pi[j];
+      s[r] = L.ps[j];
+      h[r] = L.ph[j];
+      kp = keep(r, j);
+    }
+    uint64_t mask = __ballot(kp);
+    pos[r] = kp ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+    base += __popcll(mask);
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (pos[r] >= 0) {
+      L.pk[pos[r]] = k[r];
+      L.pi[pos[r]] = ii[r];
+      L.ps[pos[r]] = s[r];
+      L.ph[pos[r]] = h[r];
+    }
+  }
+  wave_sync();
+  return base;
+}
+
+struct Ctx {
+  const SearchArgs* a;
+  SearchLds* L;
+  State st;
+  int lane;
+  double q0, q1, q2;
+  double t0, t1, t2;
+  int64_t left_out;
+  int capped;
+  __device__ bool full() const { return st.n >= a->max_samples; }
+
+  __device__ int sector_of(uint32_t i) const {
+    double d0, d1, d2v;
+    mv3(a->axes, a->sx[i] - t0, a->sy[i] - t1, a->sz[i] - t2, d0, d1, d2v);
+    if (a->plane) {
+      // search.rs:430-434
+      const double tau = 6.283185307179586;
+      double turn = fmod(atan2(-d0, d1), tau);
+      if (turn < 0.0) turn = turn + tau;
+      double f = turn / tau * (double)a->plane;
+      int k = f >= 0.0 ? (f < 2147483647.0 ? (int)f : 2147483647) : 0;
+      return k < a->plane - 1 ? k : a->plane - 1;
+    }
+    int bx = d0 >= 0.0, by = d1 >= 0.0, bz = d2v >= 0.0;
+    return a->planar ? (bx << 1) | by : (bx << 2) | (by << 1) | bz;
+  }
+
+  __device__ int hole_count(int h) const {
+    for (int k = 0; k < st.n_holes; ++k)
+      if (L->hole_id[k] == h) return L->hole_cnt[k];
+    return 0;
+  }
+
+  // After a merge: drop what can never be selected, and set the thresholds.
+  __device__ void prune() {
+    int rem = a->max_samples - st.n;
+    if (!capped) {
+      if (st.P > rem) st.P = rem;
+      if (st.P == rem && rem > 0) {
+        st.thr = L->pk[st.P - 1];
+        st.thr_i = L->pi[st.P - 1];
+      }
+      return;
+    }
+    if (a->octant_rank) {
+      // Per-octant rank among pooled candidates: rank >= what is left of the
+      // octant's cap is never selected (no hole cap, so an earlier candidate
+      // of the octant is selected unless the selection is full first).
+      int nsec = a->n_sectors;
+      int running[8];
+      for (int o = 0; o < 8; ++o) running[o] = 0;
+      int sure = 0;
+      constexpr int R = BK_POOL / BK_WAVE;
+      int flags[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int j = r * BK_WAVE + lane;
+        int s = j < st.P ? L->ps[j] : -1;
+        int rank = 0;
+        for (int o = 0; o < nsec; ++o) {
+          uint64_t mask = __ballot(s == o);
+          if (s == o) rank = running[o] + __popcll(mask & ((1ull << lane) - 1ull));
+          running[o] += __popcll(mask);
+        }
+        bool k = s >= 0 && rank < a->sector_cap - L->per_sector[s];
+        // Sure picks in order; beyond what is left of max_samples none is taken.
+        uint64_t km = __ballot(k);
+        int before = sure + __popcll(km & ((1ull << lane) - 1ull));
+        sure += __popcll(km);
+        flags[r] = k && before < rem;
+      }
+      int P = st.P;
+      st.P = pool_compact(*L, P, lane, [&](int r, int) { return flags[r] != 0; });
+      // Thresholds: an octant holding all it may still take, or a pool of
+      // `rem` sure picks, makes larger keys useless.
+      if (lane == 0) {
+        int cnt[8];
+        for (int o = 0; o < 8; ++o) {
+          cnt[o] = 0;
+          L->thr_sec[o] = ~0ull;
+        }
+        for (int j = 0; j < st.P; ++j) {
+          int s = L->ps[j];
+          cnt[s]++;
+          if (cnt[s] == a->sector_cap - L->per_sector[s]) L->thr_sec[s] = L->pk[j];
+        }
+      }
+      wave_sync();
+      if (st.P == rem && rem > 0) {
+        st.thr = L->pk[st.P - 1];
+        st.thr_i = L->pi[st.P - 1];
+      }
+    }
+  }
+
+  // Merges this lane's candidate (have) into the pool.
+  __device__ void merge(bool have, uint64_t key, uint32_t idx, int sec, int hl) {
+    uint64_t mask = __ballot(have);
+    if (mask == 0) return;
+    int nb = __popcll(mask);
+    if (!have) {
+      key = ~0ull;
+      idx = 0xffffffffu;
+      sec = 0;
+      hl = -1;
+    }
+    wave_sort(key, idx, sec, hl, lane);
+    if (st.P + nb > BK_POOL) {
+      st.overflow = 1;
+      return;
+    }
+    L->bk[lane] = key;
+    L->bi[lane] = idx;
+    wave_sync();
+    constexpr int R = BK_POOL / BK_WAVE;
+    uint64_t k[R];
+    uint32_t ii[R];
+    int s[R], h[R], pos[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      int j = r * BK_WAVE + lane;
+      pos[r] = -1;
+      if (j < st.P) {
+        k[r] = L->pk[j];
+        ii[r] = L->pi[j];
+        s[r] = L->ps[j];
+        h[r] = L->ph[j];
+        pos[r] = j + batch_lower(*L, nb, k[r], ii[r]);
+      }
+    }
+    int bpos = lane < nb ? lane + pool_lower(*L, st.P, key, idx) : -1;
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (pos[r] >= 0) {
+        L->pk[pos[r]] = k[r];
+        L->pi[pos[r]] = ii[r];
+        L->ps[pos[r]] = s[r];
+        L->ph[pos[r]] = h[r];
+      }
+    }
+    if (bpos >= 0) {
+      L->pk[bpos] = key;
+      L->pi[bpos] = idx;
+      L->ps[bpos] = sec;
+      L->ph[bpos] = hl;
+    }
+    wave_sync();
+    st.P += nb;
+    prune();
+  }
+
+  // Scans the cell `id` of this lane (if valid); candidates go to the pool.
+  __device__ void scan_cells(bool valid, int id) {
+    int cs = 0, cnt = 0;
+    if (valid) {
+      cs = (int)a->g.start[id];
+      cnt = (int)a->g.start[id + 1] - cs;
+    }
+    int inc = wave_incl_scan(cnt, lane);
+    int total = bcasti(inc, BK_WAVE - 1);
+    if (total == 0) return;
+    L->cs[lane] = cs;
+    L->ex[lane] = inc - cnt;
+    wave_sync();
+    for (int base = 0; base < total; base += BK_WAVE) {
+      int f = base + lane;
+      bool have = false;
+      uint64_t key = 0;
+      uint32_t idx = 0;
+      int sec = 0, hl = -1;
+      if (f < total) {
+        int lo = 0, hi = BK_WAVE - 1;
+        while (lo < hi) {  // largest k with ex[k] <= f
+          int mid = (lo + hi + 1) >> 1;
+          if (L->ex[mid] <= f)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        int e = L->cs[lo] + (f - L->ex[lo]);
+        double4 E = a->g.entries[e];
+        double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
+        double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
+        idx = (uint32_t)__double_as_longlong(E.w);
+        if (d2 <= a->radius2 && (int64_t)idx != left_out) {
+          key = dbits(d2);
+          have = key_less(key, idx, st.thr, st.thr_i);
+          if (have && a->balances) {
+            sec = sector_of(idx);
+            if (L->per_sector[sec] >= a->sector_cap) have = false;
+            if (have && a->octant_rank && key > L->thr_sec[sec]) have = false;
+          }
+          if (have && a->hole) {
+            hl = a->hole[idx];
+            if (a->max_per_hole >= 0 && hl >= 0 && hole_count(hl) >= a->max_per_hole) have = false;
+          }
+        }
+      }
+      merge(have, key, idx, sec, hl);
+      if (st.overflow) return;
+    }
+    wave_sync();
+  }
+
+  // Offers every pooled candidate with d2 < bound, in order.
+  __device__ void release(double bound) {
+    if (st.P == 0) return;
+    uint64_t b = dbits(bound);
+    int r = pool_lower(*L, st.P, b, 0);
+    if (r == 0) return;
+    if (!capped) {
+      int take = min(r, a->max_samples - st.n);
+      for (int k = lane; k < take; k += BK_WAVE) L->chosen[st.n + k] = (int)L->pi[k];
+      st.n += take;
+    } else {
+      if (lane == 0) {
+        int n = st.n, nh = st.n_holes, un = st.untagged;
+        for (int e = 0; e < r && n < a->max_samples; ++e) {
+          int hl = L->ph[e];
+          int slot = -1;
+          if (hl >= 0)
+            for (int k = 0; k < nh; ++k)
+              if (L->hole_id[k] == hl) {
+                slot = k;
+                break;
+              }
+          if (a->max_per_hole >= 0 && slot >= 0 && L->hole_cnt[slot] >= a->max_per_hole) continue;
+          if (a->balances) {
+            int s = L->ps[e];
+            if (L->per_sector[s] >= a->sector_cap) continue;
+            L->per_sector[s] += 1;
+          }
+          if (hl >= 0) {
+            if (slot >= 0)
+              L->hole_cnt[slot] += 1;
+            else {
+              L->hole_id[nh] = hl;
+              L->hole_cnt[nh] = 1;
+              nh++;
+            }
+          } else {
+            un++;
+          }
+          L->chosen[n++] = (int)L->pi[e];
+        }
+        L->scalar[0] = n;
+        L->scalar[1] = nh;
+        L->scalar[2] = un;
+      }
+      wave_sync();
+      st.n = L->scalar[0];
+      st.n_holes = L->scalar[1];
+      st.untagged = L->scalar[2];
+    }
+    st.P = pool_compact(*L, st.P, lane, [&](int, int j) { return j >= r; });
+    st.thr = ~0ull;
+    st.thr_i = 0xffffffffu;
+    if (!full()) prune();
+  }
+};
+
+__global__ void __launch_bounds__(BK_WAVE) k_search(const SearchArgs a) {
+  __shared__ SearchLds L;
+  const int lane = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  const int64_t t = a.list ? a.list[b] : b;
+  Ctx c;
+  c.a = &a;
+  c.L = &L;
+  c.lane = lane;
+  c.st = State{0, 0, 0, 0, 0, ~0ull, 0xffffffffu};
+  c.capped = a.balances || a.max_per_hole >= 0;
+  c.left_out = a.left_out ? a.left_out[t] : -1;
+  c.t0 = a.tx[t];
+  c.t1 = a.ty[t];
+  c.t2 = a.tz[t];
+  for (int k = lane; k < 360; k += BK_WAVE) L.per_sector[k] = 0;
+  if (lane < 8) L.thr_sec[lane] = ~0ull;
+  wave_sync();
+  // search.rs:738-740: the target in the search frame.
+  mv3(a.F, c.t0 - a.origin[0], c.t1 - a.origin[1], c.t2 - a.origin[2], c.q0, c.q1, c.q2);
+  const int n_samples_nonempty = a.g.start != nullptr;
+  if (n_samples_nonempty && a.max_samples > 0) {
+    const GridDev& g = a.g;
+    int c0 = clamp_cell((c.q0 - g.lo[0]) / g.h, g.dims[0]);
+    int c1 = clamp_cell((c.q1 - g.lo[1]) / g.h, g.dims[1]);
+    int c2 = clamp_cell((c.q2 - g.lo[2]) / g.h, g.dims[2]);
+    double h2 = g.h * g.h;
+    double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
+    const double slack = 1.0 - 1e-9;
+    bool done = false;
+    uint32_t begin = 0;
+    for (int s = 0; s < a.tab.n && !done; ++s) {
+      double bound = fmin(h2 * (double)a.tab.m[s], beyond) * slack;
+      if (bound > a.radius2) break;
+      c.release(bound);
+      if (c.full()) {
+        done = true;
+        break;
+      }
+      uint32_t end = a.tab.end[s];
+      for (uint32_t chunk = begin; chunk < end; chunk += BK_WAVE) {
+        uint32_t o = chunk + lane;
+        bool valid = o < end;
+        int id = 0;
+        if (valid) {
+          char4 off = a.tab.offsets[o];
+          int x = c0 + off.x, y = c1 + off.y, z = c2 + off.z;
+          valid = x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2];
+          id = x + g.dims[0] * (y + g.dims[1] * z);
+        }
+        c.scan_cells(valid, id);
+        if (c.st.overflow) break;
+      }
+      if (c.st.overflow) done = true;
+      begin = end;
+    }
+    if (!done) {
+      int far = max(max(max(c0, g.dims[0] - 1 - c0), max(c1, g.dims[1] - 1 - c1)), max(c2, g.dims[2] - 1 - c2));
+      for (int r = BK_REACH + 1; r <= far && !done; ++r) {
+        double x = (double)(r - 1) * g.h;
+        double bound = x * x * slack;
+        if (bound > a.radius2) break;
+        c.release(bound);
+        if (c.full()) {
+          done = true;
+          break;
+        }
+        int xlo = max(-r, -c0), xhi = min(r, g.dims[0] - 1 - c0);
+        int ylo = max(-r, -c1), yhi = min(r, g.dims[1] - 1 - c1);
+        int zlo = max(-r, -c2), zhi = min(r, g.dims[2] - 1 - c2);
+        int nx = xhi - xlo + 1, ny = yhi - ylo + 1, nz = zhi - zlo + 1;
+        int pairs = nx * ny;
+        for (int pb = 0; pb < pairs && !c.st.overflow; pb += BK_WAVE) {
+          int p = pb + lane;
+          bool valid = p < pairs;
+          int x0 = xlo + (valid ? p % nx : 0), y0 = ylo + (valid ? p / nx : 0);
+          bool side = valid && (abs(x0) == r || abs(y0) == r);
+          int zc = !valid ? 0 : (side ? nz : (int)(zlo <= -r) + (int)(r <= zhi));
+          int zmax = zc;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) zmax = max(zmax, __shfl_xor(zmax, o, BK_WAVE));
+          for (int k = 0; k < zmax && !c.st.overflow; ++k) {
+            bool v = k < zc;
+            int z = side ? zlo + k : ((zlo <= -r && k == 0) ? -r : r);
+            int id = (x0 + c0) + g.dims[0] * ((y0 + c1) + g.dims[1] * (z + c2));
+            c.scan_cells(v, v ? id : 0);
+          }
+        }
+        if (c.st.overflow) done = true;
+      }
+    }
+    if (!done) c.release(INFINITY);
+  }
+  wave_sync();
+  // Selector::finish (search.rs:547-576).
+  int n = c.st.n;
+  int* out = a.idx + (int64_t)t * a.stride;
+  for (int k = lane; k < n; k += BK_WAVE) out[k] = L.chosen[k];
+  if (lane == 0) {
+    int status = BK_OK;
+    if (c.st.overflow) {
+      status = BK_ST_OVERFLOW;
+    } else if (n < a.min_samples) {
+      status = BK_SEARCH_FAILED;
+    } else {
+      if (a.min_sectors >= 0) {
+        int held = 0;
+        for (int s = 0; s < a.n_sectors; ++s) held += L.per_sector[s] > 0;
+        if (held < a.min_sectors) status = BK_SEARCH_FAILED;
+      }
+      if (status == BK_OK && a.max_empty_sectors >= 0) {
+        // search.rs:445-467: longest run of empty sectors around each ring.
+        int best = 0;
+        int rings = a.plane ? 1 : (a.planar ? 1 : 2);
+        for (int ri = 0; ri < rings; ++ri) {
+          int len = a.plane ? a.plane : 4;
+          auto at = [&](int k) {
+            if (a.plane) return k;
+            if (a.planar) {
+              const int r0[4] = {3, 1, 0, 2};
+              return r0[k];
+            }
+            const int r1[4] = {7, 3, 1, 5};
+            const int r2[4] = {6, 2, 0, 4};
+            return ri == 0 ? r1[k] : r2[k];
+          };
+          bool all = true;
+          for (int k = 0; k < len; ++k) all = all && L.per_sector[at(k)] == 0;
+          if (all) {
+            best = max(best, len);
+            continue;
+          }
+          int run = 0, longest = 0;
+          for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < len; ++k) {
+              run = L.per_sector[at(k)] == 0 ? run + 1 : 0;
+              longest = max(longest, run);
+            }
+          best = max(best, longest);
+        }
+        if (best > a.max_empty_sectors) status = BK_SEARCH_FAILED;
+      }
+      if (status == BK_OK && a.min_holes >= 0) {
+        int distinct = 0;
+        for (int k = 0; k < n; ++k) {
+          int h = a.hole ? a.hole[L.chosen[k]] : -1;
+          if (h < 0) {
+            distinct++;
+            continue;
+          }
+          bool seen = false;
+          for (int j = 0; j < k && !seen; ++j) seen = a.hole[L.chosen[j]] == h;
+          distinct += !seen;
+        }
+        if (distinct < a.min_holes) status = BK_SEARCH_FAILED;
+      }
+    }
+    a.count[t] = n;
+    a.status[t] = status;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grid build.
+__global__ void k_project(int64_t n, const double* x, const double* y, const double* z, const double* F,
+                          double o0, double o1, double o2, double* px, double* py, double* pz) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r0, r1, r2;
+  mv3(F, x[i] - o0, y[i] - o1, z[i] - o2, r0, r1, r2);
+  px[i] = r0;
+  py[i] = r1;
+  pz[i] = r2;
+}
+
+// Partial min / max of the finite projected points, one pair per block.
+__global__ void __launch_bounds__(256) k_bounds(int64_t n, const double* px, const double* py, const double* pz,
+                                                double* part) {
+  __shared__ double red[6][256];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double p[3] = {px[i], py[i], pz[i]};
+    if (isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = fmin(lo[d], p[d]);
+        hi[d] = fmax(hi[d], p[d]);
+      }
+  }
+  for (int d = 0; d < 3; ++d) {
+    red[d][threadIdx.x] = lo[d];
+    red[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+        red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_cell_count(int64_t n, const double* px, const double* py, const double* pz, double lo0,
+                             double lo1, double lo2, double h, int d0, int d1, int d2, uint32_t* cell_of,
+                             uint32_t* count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c0 = clamp_cell((px[i] - lo0) / h, d0);
+  int c1 = clamp_cell((py[i] - lo1) / h, d1);
+  int c2 = clamp_cell((pz[i] - lo2) / h, d2);
+  uint32_t id = (uint32_t)(c0 + d0 * (c1 + d1 * c2));
+  cell_of[i] = id;
+  atomicAdd(&count[id], 1u);
+}
+
+__global__ void k_scatter(int64_t n, const double* px, const double* py, const double* pz, const uint32_t* cell_of,
+                          const uint32_t* start, uint32_t* fill, double4* entries) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = cell_of[i];
+  uint32_t pos = start[c] + atomicAdd(&fill[c], 1u);
+  entries[pos] = make_double4(px[i], py[i], pz[i], __longlong_as_double((long long)i));
+}
+
+}  // namespace
+
+namespace bk {
+
+#define HIPCHK(x)                      \
+  do {                                 \
+    if ((x) != hipSuccess) return -2;  \
+  } while (0)
+
+// grid.rs:83-106: side for about one point per cell over the spanned axes,
+// doubled until there are at most 4 n + 64 cells.
+static double reference_side(const double lo[3], const double hi[3], int64_t n) {
+  double ext[3];
+  for (int d = 0; d < 3; ++d) ext[d] = hi[d] - lo[d];
+  double longest = 0.0;
+  for (double e : ext) longest = std::fmax(longest, e);
+  if (longest <= 0.0) return 1.0;
+  double prod = 1.0;
+  int cnt = 0;
+  for (double e : ext)
+    if (e > longest * 1e-9) {
+      prod *= e;
+      cnt++;
+    }
+  double h = std::pow(prod / (double)std::max<int64_t>(n, 1), 1.0 / (double)cnt);
+  auto cells = [&](double hh) {
+    double c = 1.0;
+    for (double e : ext) c *= std::floor(e / hh) + 1.0;
+    return c;
+  };
+  while (cells(h) > 4.0 * (double)n + 64.0) h *= 2.0;
+  return h;
+}
+
+void free_grid(GridBuffers& gb) {
+  (void)hipFree(gb.entries);
+  (void)hipFree(gb.start);
+  (void)hipFree(gb.cell_of);
+  (void)hipFree(gb.fill);
+  (void)hipFree(gb.px);
+  (void)hipFree(gb.bounds);
+  (void)hipFree(gb.scan_tmp);
+  gb = GridBuffers();
+}
+
+int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const double* z, const double F[9],
+               const double origin[3], double cell_factor, GridBuffers& gb, GridDev& out) {
+  const int T = 256;
+  int64_t blocks = (n + T - 1) / T;
+  int64_t red_blocks = std::min<int64_t>(std::max<int64_t>(blocks, 1), 1024);
+  if (n > gb.cap_n) {
+    (void)hipFree(gb.entries);
+    (void)hipFree(gb.cell_of);
+    (void)hipFree(gb.px);
+    HIPCHK(hipMalloc(&gb.entries, n * sizeof(double4)));
+    HIPCHK(hipMalloc(&gb.cell_of, n * sizeof(uint32_t)));
+    HIPCHK(hipMalloc(&gb.px, 3 * n * sizeof(double)));
+    gb.cap_n = n;
+  }
+  // Partial bounds of up to 1024 blocks, then the frame.
+  if (!gb.bounds) HIPCHK(hipMalloc(&gb.bounds, (1024 * 6 + 16) * sizeof(double)));
+  double* dF = gb.bounds + 1024 * 6;
+  double* px = gb.px;
+  double* py = gb.px + n;
+  double* pz = gb.px + 2 * n;
+  HIPCHK(hipMemcpyAsync(dF, F, 9 * sizeof(double), hipMemcpyHostToDevice, s));
+  k_project<<<dim3((unsigned)blocks), dim3(T), 0, s>>>(n, x, y, z, dF, origin[0], origin[1], origin[2], px, py, pz);
+  k_bounds<<<dim3((unsigned)red_blocks), dim3(256), 0, s>>>(n, px, py, pz, gb.bounds);
+  std::vector<double> part(red_blocks * 6);
+  HIPCHK(hipMemcpyAsync(part.data(), gb.bounds, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t b = 0; b < red_blocks; ++b)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::fmin(lo[d], part[b * 6 + d]);
+      hi[d] = std::fmax(hi[d], part[b * 6 + 3 + d]);
+    }
+  if (!(lo[0] <= hi[0])) {
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = 0.0;
+  }
+  double h = reference_side(lo, hi, n) * cell_factor;
+  int dims[3];
+  int64_t cells = 1;
+  for (int d = 0; d < 3; ++d) {
+    double k = std::floor((hi[d] - lo[d]) / h);
+    dims[d] = (int)std::min(k, 1e9) + 1;
+    cells *= dims[d];
+  }
+  if (cells > gb.cap_cells) {
+    (void)hipFree(gb.start);
+    (void)hipFree(gb.fill);
+    HIPCHK(hipMalloc(&gb.start, (cells + 1) * sizeof(uint32_t)));
+    HIPCHK(hipMalloc(&gb.fill, (cells + 1) * sizeof(uint32_t)));
+    gb.cap_cells = cells;
+  }
+  HIPCHK(hipMemsetAsync(gb.fill, 0, (cells + 1) * sizeof(uint32_t), s));
+  k_cell_count<<<dim3((unsigned)blocks), dim3(T), 0, s>>>(n, px, py, pz, lo[0], lo[1], lo[2], h, dims[0], dims[1],
+                                                         dims[2], gb.cell_of, gb.fill);
+  size_t need = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, need, gb.fill, gb.start, (int)(cells + 1), s);
+  if (need > gb.scan_tmp_bytes) {
+    (void)hipFree(gb.scan_tmp);
+    HIPCHK(hipMalloc(&gb.scan_tmp, need));
+    gb.scan_tmp_bytes = need;
+  }
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(gb.scan_tmp, need, gb.fill, gb.start, (int)(cells + 1), s));
+  HIPCHK(hipMemsetAsync(gb.fill, 0, (cells + 1) * sizeof(uint32_t), s));
+  k_scatter<<<dim3((unsigned)blocks), dim3(T), 0, s>>>(n, px, py, pz, gb.cell_of, gb.start, gb.fill, gb.entries);
+  HIPCHK(hipGetLastError());
+  for (int d = 0; d < 3; ++d) {
+    out.lo[d] = lo[d];
+    out.dims[d] = dims[d];
+  }
+  out.h = h;
+  out.start = gb.start;
+  out.entries = gb.entries;
+  return 0;
+}
+
+// grid.rs:39-64: offsets within REACH sorted by m = sum max(0, |o| - 1)^2.
+int upload_shell_table(ShellTable& tab, void** storage) {
+  struct K {
+    uint32_t m;
+    int x, y, z;
+  };
+  std::vector<K> keyed;
+  for (int x = -BK_REACH; x <= BK_REACH; ++x)
+    for (int y = -BK_REACH; y <= BK_REACH; ++y)
+      for (int z = -BK_REACH; z <= BK_REACH; ++z) {
+        auto f = [](int v) {
+          int a = std::max(std::abs(v) - 1, 0);
+          return (uint32_t)(a * a);
+        };
+        keyed.push_back({f(x) + f(y) + f(z), x, y, z});
+      }
+  std::sort(keyed.begin(), keyed.end(), [](const K& a, const K& b) {
+    if (a.m != b.m) return a.m < b.m;
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y < b.y;
+    return a.z < b.z;
+  });
+  std::vector<char4> off;
+  std::vector<uint32_t> ends, ms;
+  for (size_t k = 0; k < keyed.size(); ++k) {
+    off.push_back(make_char4((signed char)keyed[k].x, (signed char)keyed[k].y, (signed char)keyed[k].z, 0));
+    if (!ms.empty() && ms.back() == keyed[k].m)
+      ends.back() = (uint32_t)(k + 1);
+    else {
+      ms.push_back(keyed[k].m);
+      ends.push_back((uint32_t)(k + 1));
+    }
+  }
+  size_t bytes = off.size() * sizeof(char4) + 2 * ends.size() * sizeof(uint32_t);
+  char* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, bytes));
+  HIPCHK(hipMemcpy(d, off.data(), off.size() * sizeof(char4), hipMemcpyHostToDevice));
+  uint32_t* de = (uint32_t*)(d + off.size() * sizeof(char4));
+  uint32_t* dm = de + ends.size();
+  HIPCHK(hipMemcpy(de, ends.data(), ends.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dm, ms.data(), ms.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  tab.offsets = (const char4*)d;
+  tab.end = de;
+  tab.m = dm;
+  tab.n = (int)ms.size();
+  *storage = d;
+  return 0;
+}
+
+int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets) {
+  for (int64_t b = 0; b < n_targets; b += 2147483647LL) {
+    int64_t nb = std::min<int64_t>(n_targets - b, 2147483647LL);
+    SearchArgs c = a;
+    if (b) {
+      if (c.list)
+        c.list += b;
+      else
+        return -1;  // more than 2^31 targets need a list
+    }
+    k_search<<<dim3((unsigned)nb), dim3(BK_WAVE), 0, s>>>(c);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace bk

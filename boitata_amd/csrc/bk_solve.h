@@ -1,0 +1,71 @@
+// bk_solve.h — batched kriging systems on the device.
+//
+// One system of n <= NMAX samples per group of NMAX lanes (64 / NMAX systems
+// per wavefront); lane r owns row r of the covariance matrix. The FP64 kernel
+// restates crates/estimation/src/kernel.rs:24-123 (frame, packed covariance,
+// left-looking Cholesky with the 1e-12 * sill pivot test, two triangular solve
+// pairs, the ordinary-kriging constraint by the Schur scalar 1'C^-1 1); the
+// block form restates crates/estimation/src/block.rs:100-198 with the bordered
+// system solved through the same factor.
+#pragma once
+
+#include <stdint.h>
+
+#include "bk_device.h"
+
+// Internal per-target status: the system has coincident samples and goes to
+// the guarded kernel; or the FP32 factor failed / refinement did not converge
+// and the system goes to the FP64 kernel.
+#define BK_ST_GUARD 101
+#define BK_ST_REDO64 102
+#define BK_ST_LU 103
+
+struct SolveArgs {
+  DevVariogram v;
+  DevVariogram32 v32;
+  int kind;
+  double param;
+  int block;            // block support: RHS averaged over the discretisation
+  int npts;
+  const double* offs;   // npts * 3 offsets from the block centre (world axes)
+  double cbb;           // C(B, B) of the discretisation (block.rs:100-118)
+  const double* sx;
+  const double* sy;
+  const double* sz;
+  const double* sv;
+  const double* serr;   // may be null
+  const double* tx;
+  const double* ty;
+  const double* tz;
+  const int32_t* nidx;  // neighbour lists, stride per target
+  const int32_t* ncount;
+  int stride;
+  const int32_t* list;  // systems of this launch: target ids
+  int64_t n_list;
+  double* value;
+  double* variance;
+  double* lagrange;     // may be null
+  double* support;      // may be null
+  int32_t* n_used;      // may be null
+  int32_t* status;
+  int32_t* guard;       // targets handed to the guarded (merge + LU) kernel
+  int32_t* guard_count;
+  int32_t* redo;        // FP32 systems handed to the FP64 kernel
+  int32_t* redo_count;
+  int max_iter;         // FP32+IR refinement sweeps
+  double tol;           // FP32+IR: stop when the correction is below tol * |x|
+  // Per-system diagnostics of the refined form (may be null): sweeps taken.
+  int32_t* iters;
+};
+
+namespace bk {
+int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax);
+int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax);
+int launch_guarded(hipStream_t s, const SolveArgs& a);
+int launch_lu(hipStream_t s, const SolveArgs& a);
+// Buckets targets by neighbour count after the search (status OK): lists[b]
+// holds targets with n in (edge[b-1], edge[b]]; others get their status.
+int launch_bucket(hipStream_t s, int64_t m, const int32_t* count, const int32_t* sstatus, int32_t* status,
+                  double* value, double* variance, int32_t* lists, int32_t* counts, int64_t list_stride,
+                  const int* edges, int n_buckets, int stationary);
+}  // namespace bk

@@ -1,0 +1,828 @@
+// bk_solve.hip — batched kriging solves (FP64 reference order; FP32 + FP64
+// iterative refinement; guarded systems with coincident samples; bordered LU).
+//
+// Compiled with -ffp-contract=off: the FP64 kernel reproduces the reference's
+// unfused arithmetic (crates/estimation/src/kernel.rs:47-113). Refinement
+// code opts into contraction locally.
+#include <algorithm>
+
+#include "bk_solve.h"
+
+namespace {
+
+constexpr double PIVOT = 1e-12;  // kernel.rs:17
+constexpr double EPS2 = 1e-6 * 1e-6;  // coincident.rs:11 (EPS * EPS)
+
+__device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
+
+// Pair q of the strict lower triangle in row order: (i, j), j < i.
+__device__ __forceinline__ void pair_of(int q, int& i, int& j) {
+  int r = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)q)) * 0.5f);
+  while (r * (r - 1) / 2 > q) --r;
+  while ((r + 1) * r / 2 <= q) ++r;
+  i = r;
+  j = q - r * (r - 1) / 2;
+}
+
+// Sum over the NMAX lanes of one group.
+template <int NMAX>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = NMAX / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, BK_WAVE);
+  return v;
+}
+
+template <int NMAX>
+__device__ __forceinline__ float group_sumf(float v) {
+#pragma unroll
+  for (int o = NMAX / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, BK_WAVE);
+  return v;
+}
+
+template <int NMAX>
+__device__ __forceinline__ double group_max(double v) {
+#pragma unroll
+  for (int o = NMAX / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, BK_WAVE));
+  return v;
+}
+
+template <int NMAX>
+__device__ __forceinline__ bool group_any(bool p, int g) {
+  uint64_t m = __ballot(p);
+  uint64_t gm = NMAX == 64 ? ~0ull : (((1ull << NMAX) - 1ull) << (g * NMAX));
+  return (m & gm) != 0;
+}
+
+// LDS doubles per system.
+__host__ __device__ constexpr int sys_doubles(int nmax, int nf) { return nmax * (nmax + 1) / 2 + 5 * nmax + nf * 3 * nmax + 8; }
+
+// The estimate from weights w (lane r < n holds w, z, c): krige.rs:98-113,
+// kernel.rs:98-122. Writes the target's outputs from the group's lane 0.
+template <int NMAX>
+__device__ __forceinline__ void finish_outputs(const SolveArgs& a, int t, int r, int n, bool ok, int st_fail,
+                                               double w, double z, double c, double mu, double c0v, double sup) {
+  bool in = ok && r < n;
+  double f = 0.0;
+  if (in) {
+    if (a.kind == BK_SIMPLE)
+      f = w * (z - a.param);
+    else if (a.kind == BK_INDICATOR)
+      f = w * (z <= a.param ? 1.0 : 0.0);
+    else
+      f = w * z;
+  }
+  double sum = group_sum<NMAX>(f);
+  double swc = group_sum<NMAX>(in ? w * c : 0.0);
+  if (r == 0) {
+    if (ok) {
+      double val = sum;
+      if (a.kind == BK_SIMPLE) val = a.param + sum;
+      if (a.kind == BK_INDICATOR) val = fmin(fmax(sum, 0.0), 1.0);
+      a.value[t] = val;
+      a.variance[t] = fmax(c0v - swc - mu, 0.0);
+      if (a.lagrange) a.lagrange[t] = mu;
+      if (a.support) a.support[t] = sup;
+      if (a.n_used) a.n_used[t] = n;
+      a.status[t] = BK_OK;
+    } else {
+      a.value[t] = __longlong_as_double(0x7ff8000000000000LL);
+      a.variance[t] = __longlong_as_double(0x7ff8000000000000LL);
+      if (a.lagrange) a.lagrange[t] = __longlong_as_double(0x7ff8000000000000LL);
+      if (a.support) a.support[t] = __longlong_as_double(0x7ff8000000000000LL);
+      if (a.n_used) a.n_used[t] = 0;
+      a.status[t] = st_fail;
+    }
+  }
+}
+
+// Loads the system's samples into LDS (world coordinates, values, error
+// variances, frame coordinates relative to the target: kernel.rs:36-42) and
+// reports whether two coincide (coincident.rs:13-15, 27-35).
+template <int NMAX>
+__device__ __forceinline__ bool load_system(const SolveArgs& a, int t, int r, int n, int g, double* W, double* V,
+                                            double* E, double* PF, double& z, double& e) {
+  const int nf = a.v.nf;
+  double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+  if (r < n) {
+    int id = a.nidx[(int64_t)t * a.stride + r];
+    double x = a.sx[id], y = a.sy[id], zz = a.sz[id];
+    W[r] = x;
+    W[NMAX + r] = y;
+    W[2 * NMAX + r] = zz;
+    z = a.sv[id];
+    e = a.serr ? a.serr[id] : 0.0;
+    V[r] = z;
+    E[r] = e;
+    double d0 = x - t0, d1 = y - t1, d2 = zz - t2;
+    for (int f = 0; f < nf; ++f) {
+      double p0 = d0, p1 = d1, p2 = d2;
+      if (!a.v.ident[f]) mv3(a.v.F[f], d0, d1, d2, p0, p1, p2);
+      PF[(f * 3 + 0) * NMAX + r] = p0;
+      PF[(f * 3 + 1) * NMAX + r] = p1;
+      PF[(f * 3 + 2) * NMAX + r] = p2;
+    }
+  }
+  wave_sync();
+  bool close = false;
+  if (r < n) {
+    double x = W[r], y = W[NMAX + r], zz = W[2 * NMAX + r];
+    for (int j = 0; j < r && !close; ++j) {
+      double u = x - W[j], v = y - W[NMAX + j], w = zz - W[2 * NMAX + j];
+      close = (u * u + v * v) + w * w <= EPS2;
+    }
+  }
+  return group_any<NMAX>(close, g);
+}
+
+// Covariance between samples i and j in the kernel's frame (kernel.rs:43-53),
+// or between world points for block support (block.rs:152-158 cov_points).
+__device__ __forceinline__ double pair_cov64(const SolveArgs& a, const double* W, const double* PF, int nmax, int i,
+                                             int j) {
+  if (a.block)
+    return cov_points64(a.v, W[i], W[nmax + i], W[2 * nmax + i], W[j], W[nmax + j], W[2 * nmax + j]);
+  double h[BK_MAX_FRAMES];
+  for (int f = 0; f < a.v.nf; ++f) {
+    const double* P = PF + f * 3 * nmax;
+    h[f] = norm3(P[i] - P[j], P[nmax + i] - P[nmax + j], P[2 * nmax + i] - P[2 * nmax + j]);
+  }
+  return cov64(a.v, h);
+}
+
+// Right-hand side of sample r: C(s_r, target) (kernel.rs:55), or the mean
+// point-to-block covariance (block.rs:100-118).
+__device__ __forceinline__ double rhs64(const SolveArgs& a, int t, const double* W, const double* PF, int nmax,
+                                        int r) {
+  if (a.block) {
+    double x = W[r], y = W[nmax + r], z = W[2 * nmax + r];
+    double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+    double acc = -0.0;
+    for (int p = 0; p < a.npts; ++p) {
+      double qx = t0 + a.offs[3 * p], qy = t1 + a.offs[3 * p + 1], qz = t2 + a.offs[3 * p + 2];
+      acc += a.npts == 1 ? cov_points64(a.v, x, y, z, qx, qy, qz) : block_cov_points64(a.v, x, y, z, qx, qy, qz);
+    }
+    return acc / (double)a.npts;
+  }
+  double h[BK_MAX_FRAMES];
+  for (int f = 0; f < a.v.nf; ++f) {
+    const double* P = PF + f * 3 * nmax;
+    h[f] = norm3(P[r] - 0.0, P[nmax + r] - 0.0, P[2 * nmax + r] - 0.0);
+  }
+  return cov64(a.v, h);
+}
+
+// ---------------------------------------------------------------------------
+// FP64, the reference's operation order.
+template <int NMAX>
+__global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
+  constexpr int G = BK_WAVE / NMAX;
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int per = sys_doubles(NMAX, nf);
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int g = lane / NMAX, r = lane % NMAX;
+  double* S = dyn + (wave * G + g) * per;
+  double* L = S;
+  double* W = L + NMAX * (NMAX + 1) / 2;
+  double* V = W + 3 * NMAX;
+  double* E = V + NMAX;
+  double* PF = E + NMAX;
+  double* misc = PF + nf * 3 * NMAX;
+  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
+  const bool active = sys < a.n_list;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z = 0.0, e = 0.0;
+  bool guard = load_system<NMAX>(a, t, r, n, g, W, V, E, PF, z, e);
+  if (guard && r == 0 && active) {
+    a.status[t] = BK_ST_GUARD;
+    a.guard[atomicAdd(a.guard_count, 1)] = t;
+  }
+  const bool go = active && !guard;
+  const double c0 = a.v.c0;
+  // Packed lower triangle (kernel.rs:47-56), pairs spread over the group.
+  const int npairs = n * (n - 1) / 2;
+  if (go)
+    for (int q = r; q < npairs; q += NMAX) {
+      int i, j;
+      pair_of(q, i, j);
+      L[tri(i) + j] = pair_cov64(a, W, PF, NMAX, i, j);
+    }
+  double c = 0.0;
+  if (go && r < n) {
+    L[tri(r) + r] = c0 + e;
+    c = rhs64(a, t, W, PF, NMAX, r);
+  }
+  wave_sync();
+  double reg[NMAX];
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) reg[k] = (go && k <= r && r < n) ? L[tri(r) + k] : 0.0;
+  // Left-looking Cholesky (kernel.rs:58-72): row r of the factor stays in
+  // lane r; the pivot row is read from LDS by the whole group.
+  bool sing = false;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    const bool act = go && j < n;
+    double acc = -0.0;
+#pragma unroll
+    for (int k = 0; k < j; ++k) acc += reg[k] * L[tri(j) + k];
+    double s = reg[j] - acc;
+    if (act && r == j) misc[0] = s;
+    wave_sync();
+    double d = misc[0];
+    if (act && (isnan(d) || d <= PIVOT * c0)) sing = true;
+    if (act && !sing) {
+      d = sqrt(d);
+      if (r == j) {
+        reg[j] = d;
+        L[tri(j) + j] = d;
+      } else if (r > j && r < n) {
+        double x = s / d;
+        reg[j] = x;
+        L[tri(r) + j] = x;
+      }
+    }
+    wave_sync();
+  }
+  // Forward solves L y = c and L y = 1 (kernel.rs:73-78), column by column.
+  const bool ordinary = a.kind != BK_SIMPLE;
+  double ya = c, yb = 1.0, acca = -0.0, accb = -0.0;
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) {
+    const bool act = go && !sing && k < n;
+    if (act && r == k) {
+      ya = (ya - acca) / reg[k];
+      yb = (yb - accb) / reg[k];
+      misc[1] = ya;
+      misc[2] = yb;
+    }
+    wave_sync();
+    if (act && r > k && r < n) {
+      acca += reg[k] * misc[1];
+      accb += reg[k] * misc[2];
+    }
+    wave_sync();
+  }
+  // Back solves L' x = y (kernel.rs:79-84), row k of the factor from LDS.
+  double xa = ya, xb = yb;
+#pragma unroll
+  for (int k = NMAX - 1; k >= 0; --k) {
+    const bool act = go && !sing && k < n;
+    if (act && r == k) {
+      xa = xa / reg[k];
+      xb = xb / reg[k];
+      misc[1] = xa;
+      misc[2] = xb;
+    }
+    wave_sync();
+    if (act && r < k) {
+      double lk = L[tri(k) + r];
+      xa -= lk * misc[1];
+      xb -= lk * misc[2];
+    }
+    wave_sync();
+  }
+  // Schur scalar and weights (kernel.rs:86-97).
+  const bool in = go && !sing && r < n;
+  double sa = group_sum<NMAX>(in ? xa : 0.0);
+  double sb = group_sum<NMAX>(in ? xb : 0.0);
+  double mu = ordinary ? (sa - 1.0) / sb : 0.0;
+  double w = xa - (ordinary ? mu * xb : 0.0);
+  if (active && !guard)
+    finish_outputs<NMAX>(a, t, r, n, !sing, BK_SINGULAR, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 factor, FP64 residual refinement and FP64 Schur scalar.
+template <int NMAX>
+__global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
+  constexpr int G = BK_WAVE / NMAX;
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  // Per system: C64 packed, L32 packed (as floats in half the doubles), W, V, E, PF, misc.
+  const int per = sys_doubles(NMAX, nf) + (NMAX * (NMAX + 1) / 2 + 1) / 2 + 2;
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int g = lane / NMAX, r = lane % NMAX;
+  double* S = dyn + (wave * G + g) * per;
+  double* C = S;
+  double* W = C + NMAX * (NMAX + 1) / 2;
+  double* V = W + 3 * NMAX;
+  double* E = V + NMAX;
+  double* PF = E + NMAX;
+  double* misc = PF + nf * 3 * NMAX;
+  float* L = (float*)(misc + 8);
+  float* miscf = (float*)(misc + 4);
+  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
+  const bool active = sys < a.n_list;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z = 0.0, e = 0.0;
+  bool guard = load_system<NMAX>(a, t, r, n, g, W, V, E, PF, z, e);
+  if (guard && r == 0 && active) {
+    a.status[t] = BK_ST_GUARD;
+    a.guard[atomicAdd(a.guard_count, 1)] = t;
+  }
+  const bool go = active && !guard;
+  const double c0 = a.v.c0;
+  const int npairs = n * (n - 1) / 2;
+  if (go)
+    for (int q = r; q < npairs; q += NMAX) {
+      int i, j;
+      pair_of(q, i, j);
+      double cij = pair_cov64(a, W, PF, NMAX, i, j);
+      C[tri(i) + j] = cij;
+      L[tri(i) + j] = (float)cij;
+    }
+  double c = 0.0;
+  if (go && r < n) {
+    C[tri(r) + r] = c0 + e;
+    L[tri(r) + r] = (float)(c0 + e);
+    c = rhs64(a, t, W, PF, NMAX, r);
+  }
+  wave_sync();
+  float reg[NMAX];
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) reg[k] = (go && k <= r && r < n) ? L[tri(r) + k] : 0.0f;
+  // FP32 Cholesky. A pivot that loses most of its bits sends the system to FP64.
+  bool bad = false;
+  const float c0f = (float)c0;
+  {
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      const bool act = go && j < n;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < j; ++k) acc += reg[k] * L[tri(j) + k];
+      float s = reg[j] - acc;
+      if (act && r == j) miscf[0] = s;
+      wave_sync();
+      float d = miscf[0];
+      if (act && !(d > 1e-5f * c0f)) bad = true;
+      if (act && !bad) {
+        float inv = rsqrtf(d);
+        if (r == j) {
+          reg[j] = d * inv;
+          L[tri(j) + j] = reg[j];
+        } else if (r > j && r < n) {
+          float x = s * inv;
+          reg[j] = x;
+          L[tri(r) + j] = x;
+        }
+      }
+      wave_sync();
+    }
+  }
+  // Solve with the FP32 factor: (L L') dx = res, on both right-hand sides.
+  auto solve32 = [&](float& va, float& vb) {
+#pragma clang fp contract(fast)
+    float acca = 0.0f, accb = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+      const bool act = go && !bad && k < n;
+      if (act && r == k) {
+        float inv = 1.0f / reg[k];
+        va = (va - acca) * inv;
+        vb = (vb - accb) * inv;
+        miscf[1] = va;
+        miscf[2] = vb;
+      }
+      wave_sync();
+      if (act && r > k && r < n) {
+        acca += reg[k] * miscf[1];
+        accb += reg[k] * miscf[2];
+      }
+      wave_sync();
+    }
+#pragma unroll
+    for (int k = NMAX - 1; k >= 0; --k) {
+      const bool act = go && !bad && k < n;
+      if (act && r == k) {
+        float inv = 1.0f / reg[k];
+        va = va * inv;
+        vb = vb * inv;
+        miscf[1] = va;
+        miscf[2] = vb;
+      }
+      wave_sync();
+      if (act && r < k) {
+        float lk = L[tri(k) + r];
+        va -= lk * miscf[1];
+        vb -= lk * miscf[2];
+      }
+      wave_sync();
+    }
+  };
+  const bool ordinary = a.kind != BK_SIMPLE;
+  float fa = (float)c, fb = 1.0f;
+  solve32(fa, fb);
+  double xa = fa, xb = ordinary ? (double)fb : 0.0;
+  // Refinement: residuals against the FP64 matrix, corrections through the
+  // FP32 factor, iterates kept in FP64.
+  int it = 0;
+  bool conv = false;
+  for (; it < a.max_iter; ++it) {
+    // Share the iterates through LDS (V and E are free now: z, e live in registers).
+    if (go && r < n) {
+      V[r] = xa;
+      E[r] = xb;
+    }
+    wave_sync();
+    double ra = c, rb = 1.0;
+    if (go && !bad && r < n) {
+      double sa = -0.0, sb = -0.0;
+      for (int j = 0; j < n; ++j) {
+        double cij = j <= r ? C[tri(r) + j] : C[tri(j) + r];
+        sa = fma(cij, V[j], sa);
+        sb = fma(cij, E[j], sb);
+      }
+      ra = c - sa;
+      rb = 1.0 - sb;
+    }
+    wave_sync();
+    float da = (float)ra, db = ordinary ? (float)rb : 0.0f;
+    solve32(da, db);
+    const bool live = go && !bad && r < n;
+    double na = group_max<NMAX>(live ? fabs((double)da) : 0.0), nx = group_max<NMAX>(live ? fabs(xa) : 0.0);
+    double nb = group_max<NMAX>(live ? fabs((double)db) : 0.0), nxb = group_max<NMAX>(live ? fabs(xb) : 0.0);
+    xa += (double)da;
+    if (ordinary) xb += (double)db;
+    if (na <= a.tol * nx && nb <= a.tol * nxb) {
+      conv = true;
+      ++it;
+      break;
+    }
+  }
+  if (active && !guard && (bad || !conv)) {
+    if (r == 0) {
+      a.status[t] = BK_ST_REDO64;
+      a.redo[atomicAdd(a.redo_count, 1)] = t;
+    }
+  }
+  const bool ok = go && !bad && conv;
+  // The Schur scalar 1'C^-1 1 and the weights in FP64.
+  const bool in = ok && r < n;
+  double sa = group_sum<NMAX>(in ? xa : 0.0);
+  double sb = group_sum<NMAX>(in ? xb : 0.0);
+  double mu = ordinary ? (sa - 1.0) / sb : 0.0;
+  double w = xa - (ordinary ? mu * xb : 0.0);
+  if (ok) {
+    finish_outputs<NMAX>(a, t, r, n, true, BK_OK, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
+    if (a.iters && r == 0) a.iters[t] = it;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Systems with coincident samples (coincident.rs:27-112): merged on lane 0,
+// solved in FP64 by LU on the bordered system of the merged samples (the
+// merged system is small and rare), weights spread back to the members.
+// Also takes systems beyond the Cholesky kernels (n > 64, power models).
+constexpr int LU_MAX = BK_MAX_LU + 1;
+
+__device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, double* ws, int* cl, int lane) {
+  // Runs on one wavefront. A: LU_MAX * LU_MAX, b: LU_MAX, ws: scratch.
+  const int n0 = a.ncount[t];
+  double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+  double* X = ws;                 // merged samples
+  double* Y = X + LU_MAX;
+  double* Z = Y + LU_MAX;
+  double* Vv = Z + LU_MAX;
+  double* Ev = Vv + LU_MAX;
+  double* Vo = Ev + LU_MAX;       // original values
+  double* rhs = Vo + LU_MAX;
+  int* members = cl + LU_MAX;     // cluster sizes
+  __shared__ int sh_n, sh_fail;
+  const double c0 = a.v.c0;
+  if (lane == 0) {
+    // Clusters::find + merge (coincident.rs:27-82), serial.
+    int nc = 0;
+    int lead[LU_MAX];
+    for (int i = 0; i < n0; ++i) {
+      int id = a.nidx[(int64_t)t * a.stride + i];
+      double x = a.sx[id], y = a.sy[id], z = a.sz[id];
+      Vo[i] = a.sv[id];
+      int c = -1;
+      for (int k = 0; k < nc; ++k) {
+        int l = lead[k];
+        double u = a.sx[l] - x, v = a.sy[l] - y, w = a.sz[l] - z;
+        if ((u * u + v * v) + w * w <= EPS2) {
+          c = k;
+          break;
+        }
+      }
+      if (c < 0) {
+        c = nc++;
+        lead[c] = id;
+        members[c] = 0;
+      }
+      cl[i] = c;
+      members[c] += 1;
+    }
+    for (int k = 0; k < nc; ++k) {
+      int l = lead[k];
+      X[k] = a.sx[l];
+      Y[k] = a.sy[l];
+      Z[k] = a.sz[l];
+    }
+    for (int k = 0; k < nc; ++k) {
+      double var = 0.0, sum = -0.0;
+      int first = -1;
+      for (int i = 0; i < n0; ++i)
+        if (cl[i] == k) {
+          if (first < 0) first = i;
+          int id = a.nidx[(int64_t)t * a.stride + i];
+          sum += Vo[i];
+          var += a.serr ? a.serr[id] : 0.0;
+          for (int j = 0; j < n0; ++j)
+            if (cl[j] == k) {
+              int jd = a.nidx[(int64_t)t * a.stride + j];
+              var += cov_points64(a.v, a.sx[id], a.sy[id], a.sz[id], a.sx[jd], a.sy[jd], a.sz[jd]);
+            }
+        }
+      double kk = (double)members[k];
+      int fid = a.nidx[(int64_t)t * a.stride + first];
+      if (members[k] == 1) {
+        Vv[k] = a.sv[fid];
+        Ev[k] = a.serr ? a.serr[fid] : 0.0;
+      } else {
+        Vv[k] = sum / kk;
+        Ev[k] = var / (kk * kk) - c0;
+      }
+    }
+    sh_n = nc;
+  }
+  wave_sync();
+  const int n = sh_n;
+  const bool ordinary = a.kind != BK_SIMPLE;
+  const int dim = ordinary ? n + 1 : n;
+  // Bordered system (krige.rs:151-171, block.rs:140-165) with cov_points.
+  for (int q = lane; q < dim * dim; q += BK_WAVE) {
+    int i = q / dim, j = q % dim;
+    double v;
+    if (i < n && j < n) {
+      v = cov_points64(a.v, X[i], Y[i], Z[i], X[j], Y[j], Z[j]);
+      if (i == j) v += Ev[i];
+    } else if (i == n && j == n) {
+      v = 0.0;
+    } else {
+      v = 1.0;
+    }
+    A[i * LU_MAX + j] = v;
+  }
+  for (int i = lane; i < dim; i += BK_WAVE) {
+    double v = 1.0;
+    if (i < n) {
+      if (a.block) {
+        double acc = -0.0;
+        for (int p = 0; p < a.npts; ++p) {
+          double qx = t0 + a.offs[3 * p], qy = t1 + a.offs[3 * p + 1], qz = t2 + a.offs[3 * p + 2];
+          acc += a.npts == 1 ? cov_points64(a.v, X[i], Y[i], Z[i], qx, qy, qz)
+                             : block_cov_points64(a.v, X[i], Y[i], Z[i], qx, qy, qz);
+        }
+        v = acc / (double)a.npts;
+      } else {
+        v = cov_points64(a.v, X[i], Y[i], Z[i], t0, t1, t2);
+      }
+      rhs[i] = v;
+    }
+    b[i] = v;
+  }
+  wave_sync();
+  // LU with partial pivoting, reciprocal-scaled columns (nalgebra LU::new).
+  if (lane == 0) sh_fail = 0;
+  wave_sync();
+  for (int i = 0; i < dim; ++i) {
+    if (lane == 0) {
+      int piv = i;
+      double best = fabs(A[i * LU_MAX + i]);
+      for (int q = i + 1; q < dim; ++q) {
+        double v = fabs(A[q * LU_MAX + i]);
+        if (v > best) {
+          best = v;
+          piv = q;
+        }
+      }
+      cl[2 * LU_MAX] = piv;
+    }
+    wave_sync();
+    int piv = cl[2 * LU_MAX];
+    double diag = A[piv * LU_MAX + i];
+    if (diag == 0.0) {
+      wave_sync();
+      continue;
+    }
+    if (piv != i) {
+      for (int k = lane; k < dim; k += BK_WAVE) {
+        double u = A[i * LU_MAX + k];
+        A[i * LU_MAX + k] = A[piv * LU_MAX + k];
+        A[piv * LU_MAX + k] = u;
+      }
+      if (lane == 0) {
+        double u = b[i];
+        b[i] = b[piv];
+        b[piv] = u;
+      }
+    }
+    wave_sync();
+    double inv = 1.0 / diag;
+    for (int q = i + 1 + lane; q < dim; q += BK_WAVE) A[q * LU_MAX + i] *= inv;
+    wave_sync();
+    for (int q = lane; q < (dim - i - 1) * (dim - i - 1); q += BK_WAVE) {
+      int rr = i + 1 + q / (dim - i - 1), k = i + 1 + q % (dim - i - 1);
+      A[rr * LU_MAX + k] = A[rr * LU_MAX + k] + (-A[i * LU_MAX + k]) * A[rr * LU_MAX + i];
+    }
+    wave_sync();
+  }
+  if (lane == 0) {
+    for (int i = 0; i < dim; ++i) {
+      double co = b[i];
+      for (int q = i + 1; q < dim; ++q) b[q] = b[q] + (-co) * A[q * LU_MAX + i];
+    }
+    for (int i = dim - 1; i >= 0; --i) {
+      double d = A[i * LU_MAX + i];
+      if (d == 0.0) {
+        sh_fail = 1;
+        break;
+      }
+      b[i] = b[i] / d;
+      double co = b[i];
+      for (int q = 0; q < i; ++q) b[q] = b[q] + (-co) * A[q * LU_MAX + i];
+    }
+    double mu = ordinary ? b[n] : 0.0;
+    double sup = a.block ? a.cbb : c0;
+    if (sh_fail) {
+      a.value[t] = __longlong_as_double(0x7ff8000000000000LL);
+      a.variance[t] = __longlong_as_double(0x7ff8000000000000LL);
+      if (a.lagrange) a.lagrange[t] = __longlong_as_double(0x7ff8000000000000LL);
+      if (a.support) a.support[t] = __longlong_as_double(0x7ff8000000000000LL);
+      if (a.n_used) a.n_used[t] = 0;
+      a.status[t] = BK_SINGULAR;
+    } else {
+      double swc = -0.0;
+      for (int i = 0; i < n; ++i) swc += b[i] * rhs[i];
+      // Spread (coincident.rs:102-112): each member gets its cluster's share,
+      // the value is recomputed from the members.
+      double sum = -0.0;
+      for (int i = 0; i < n0; ++i) {
+        double wi = b[cl[i]] / (double)members[cl[i]];
+        double zz = Vo[i];
+        double f = a.kind == BK_SIMPLE ? zz - a.param : (a.kind == BK_INDICATOR ? (zz <= a.param ? 1.0 : 0.0) : zz);
+        sum += wi * f;
+      }
+      double val = sum;
+      if (a.kind == BK_SIMPLE) val = a.param + sum;
+      if (a.kind == BK_INDICATOR) val = fmin(fmax(sum, 0.0), 1.0);
+      a.value[t] = val;
+      a.variance[t] = fmax(sup - swc - mu, 0.0);
+      if (a.lagrange) a.lagrange[t] = mu;
+      if (a.support) a.support[t] = sup;
+      if (a.n_used) a.n_used[t] = n0;
+      a.status[t] = BK_OK;
+    }
+  }
+  wave_sync();
+}
+
+__global__ void __launch_bounds__(64) k_guarded(const SolveArgs a) {
+  extern __shared__ double dyn[];
+  double* A = dyn;
+  double* b = A + LU_MAX * LU_MAX;
+  double* ws = b + LU_MAX;
+  int* cl = (int*)(ws + 7 * LU_MAX);
+  const int64_t sys = blockIdx.x;
+  if (sys >= *a.guard_count) return;
+  const int t = a.guard[sys];
+  if (a.ncount[t] > BK_MAX_LU) {
+    if (threadIdx.x == 0) {
+      const double nan = __longlong_as_double(0x7ff8000000000000LL);
+      a.value[t] = nan;
+      a.variance[t] = nan;
+      a.status[t] = BK_UNSUPPORTED;
+    }
+    return;
+  }
+  lu_system(a, t, A, b, ws, cl, threadIdx.x);
+}
+
+__global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus, int32_t* status, double* value,
+                         double* variance, int32_t* lists, int32_t* counts, int64_t list_stride, int e0, int e1,
+                         int e2, int e3, int n_buckets, int stationary) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  int st = sstatus[t];
+  int n = count[t];
+  int b = -1;
+  if (st == BK_OK) {
+    if (n == 0) {
+      st = BK_INSUFFICIENT_DATA;
+    } else if (!stationary || n > BK_MAX_CHOL) {
+      b = n_buckets;  // the LU list
+    } else {
+      int edges[4] = {e0, e1, e2, e3};
+      for (int k = 0; k < n_buckets; ++k)
+        if (n <= edges[k]) {
+          b = k;
+          break;
+        }
+    }
+  }
+  if (b >= 0) {
+    int pos = atomicAdd(&counts[b], 1);
+    lists[(int64_t)b * list_stride + pos] = (int32_t)t;
+    status[t] = BK_OK;
+  } else {
+    status[t] = st;
+    value[t] = __longlong_as_double(0x7ff8000000000000LL);
+    variance[t] = __longlong_as_double(0x7ff8000000000000LL);
+  }
+}
+
+}  // namespace
+
+namespace bk {
+
+static size_t shmem64(int nmax, int nf) { return (size_t)2 * (BK_WAVE / nmax) * sys_doubles(nmax, nf) * sizeof(double); }
+static size_t shmem32(int nmax, int nf) {
+  return (size_t)2 * (BK_WAVE / nmax) * (sys_doubles(nmax, nf) + (nmax * (nmax + 1) / 2 + 1) / 2 + 2) * sizeof(double);
+}
+
+template <int NMAX>
+static int run64(hipStream_t s, const SolveArgs& a) {
+  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  int64_t blocks = (a.n_list + per_block - 1) / per_block;
+  if (blocks == 0) return 0;
+  size_t sh = shmem64(NMAX, a.v.nf);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_krige64<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  k_krige64<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <int NMAX>
+static int run32(hipStream_t s, const SolveArgs& a) {
+  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  int64_t blocks = (a.n_list + per_block - 1) / per_block;
+  if (blocks == 0) return 0;
+  size_t sh = shmem32(NMAX, a.v.nf);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_krige32<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  k_krige32<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax) {
+  switch (nmax) {
+    case 16:
+      return run64<16>(s, a);
+    case 32:
+      return run64<32>(s, a);
+    case 64:
+      return run64<64>(s, a);
+  }
+  return -1;
+}
+
+int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax) {
+  switch (nmax) {
+    case 16:
+      return run32<16>(s, a);
+    case 32:
+      return run32<32>(s, a);
+    case 64:
+      return run32<64>(s, a);
+  }
+  return -1;
+}
+
+// Handles a.guard[0 .. *a.guard_count) with a grid of a.n_list blocks (an
+// upper bound on that count).
+int launch_guarded(hipStream_t s, const SolveArgs& a) {
+  if (a.n_list <= 0) return 0;
+  size_t sh = (size_t)(LU_MAX * LU_MAX + LU_MAX + 7 * LU_MAX) * sizeof(double) + (2 * LU_MAX + 4) * sizeof(int);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_guarded, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  k_guarded<<<dim3((unsigned)a.n_list), dim3(64), sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_bucket(hipStream_t s, int64_t m, const int32_t* count, const int32_t* sstatus, int32_t* status,
+                  double* value, double* variance, int32_t* lists, int32_t* counts, int64_t list_stride,
+                  const int* edges, int n_buckets, int stationary) {
+  if (m <= 0) return 0;
+  int e[4] = {0, 0, 0, 0};
+  for (int k = 0; k < n_buckets && k < 4; ++k) e[k] = edges[k];
+  k_bucket<<<dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s>>>(m, count, sstatus, status, value, variance, lists,
+                                                                     counts, list_stride, e[0], e[1], e[2], e[3],
+                                                                     n_buckets, stationary);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace bk

@@ -1,0 +1,159 @@
+"""Engine: one HIP context of the kriging library, driven through the C ABI.
+
+Arrays may be NumPy arrays (host memory, staged by the library) or torch
+tensors on the GPU (passed as device pointers, nothing staged).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._lib import load
+
+
+class EngineError(RuntimeError):
+    """A call of the C ABI returned an error."""
+
+
+def _ptr(a):
+    """Data pointer of a contiguous NumPy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr())
+    if not a.flags.c_contiguous:
+        raise ValueError("array must be C-contiguous")
+    return C.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        import torch
+
+        return a.to(torch.float64).contiguous()
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        import torch
+
+        return a.to(torch.int32).contiguous()
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        import torch
+
+        return a.to(torch.int64).contiguous()
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class Engine:
+    """A context on HIP device `device` holding the fitted samples."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = C.c_void_p()
+        rc = self._lib.bk_create(C.byref(h), int(device))
+        if rc != 0:
+            raise EngineError(f"bk_create failed ({rc}): no usable HIP device {device}")
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.bk_last_error(self._h).decode()
+            raise EngineError(f"{msg} ({rc})")
+
+    def set_samples(self, x, y, z, value, error_variance=None, hole=None):
+        x, y, z, value = _f64(x), _f64(y), _f64(z), _f64(value)
+        err = _f64(error_variance)
+        hole = _i32(hole)
+        self._keep = (x, y, z, value, err, hole)
+        self.n = int(len(x))
+        self._check(self._lib.bk_set_samples(self._h, self.n, _ptr(x), _ptr(y), _ptr(z), _ptr(value), _ptr(err),
+                                             _ptr(hole)))
+
+    def tuning(self, edges=None, max_iter=0, tol=0.0, cell_factor=0.0):
+        e = None if edges is None else (C.c_int32 * 3)(*edges)
+        self._check(self._lib.bk_set_tuning(self._h, e, int(max_iter), float(tol), float(cell_factor)))
+
+    def neighbors(self, search: _abi.Search, variogram: _abi.Variogram | None, tx, ty, tz, left_out=None):
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        left_out = _i64(left_out)
+        m = int(len(tx))
+        idx = np.full((m, int(search.max_samples)), -1, dtype=np.int32)
+        count = np.zeros(m, dtype=np.int32)
+        status = np.zeros(m, dtype=np.int32)
+        vg = C.byref(variogram) if variogram is not None else None
+        self._check(self._lib.bk_neighbors(self._h, C.byref(search), vg, m, _ptr(tx), _ptr(ty), _ptr(tz),
+                                           _ptr(left_out), _ptr(idx), _ptr(count), _ptr(status)))
+        return idx, count, status
+
+    def _outputs(self, m, like):
+        if like is not None and hasattr(like, "data_ptr") and like.is_cuda:
+            import torch
+
+            dev = like.device
+            f = lambda: torch.empty(m, dtype=torch.float64, device=dev)  # noqa: E731
+            i = lambda: torch.empty(m, dtype=torch.int32, device=dev)  # noqa: E731
+        else:
+            f = lambda: np.empty(m, dtype=np.float64)  # noqa: E731
+            i = lambda: np.empty(m, dtype=np.int32)  # noqa: E731
+        return f, i
+
+    def krige(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64):
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        f, i = self._outputs(m, tx)
+        value, variance, lagrange, n_used, status = f(), f(), f(), i(), i()
+        self._check(self._lib.bk_krige(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                       int(precision), m, _ptr(tx), _ptr(ty), _ptr(tz), _ptr(value), _ptr(variance),
+                                       _ptr(lagrange), _ptr(n_used), _ptr(status)))
+        return dict(value=value, variance=variance, lagrange=lagrange, n_used=n_used, status=status)
+
+    def block_krige(self, variogram, search, block, tx, ty, tz, kind=_abi.ORDINARY, param=0.0,
+                    precision=_abi.FP64):
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        f, i = self._outputs(m, tx)
+        value, variance, lagrange, support, n_used, status = f(), f(), f(), f(), i(), i()
+        self._check(self._lib.bk_block_krige(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                             C.byref(block), int(precision), m, _ptr(tx), _ptr(ty), _ptr(tz),
+                                             _ptr(value), _ptr(variance), _ptr(lagrange), _ptr(support),
+                                             _ptr(n_used), _ptr(status)))
+        return dict(value=value, variance=variance, lagrange=lagrange, support=support, n_used=n_used,
+                    status=status)
+
+    def loo(self, variogram, search, which=None, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64):
+        which = _i64(which)
+        m = self.n if which is None else int(len(which))
+        value = np.empty(m, dtype=np.float64)
+        variance = np.empty(m, dtype=np.float64)
+        status = np.empty(m, dtype=np.int32)
+        self._check(self._lib.bk_loo(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                     int(precision), m, _ptr(which), _ptr(value), _ptr(variance), _ptr(status)))
+        return dict(value=value, variance=variance, status=status)

@@ -1,0 +1,209 @@
+/*
+ * boitata_gpu.h — C ABI of the MI355X kriging engine.
+ *
+ * The engine replaces the per-target estimator closure and the search tree of
+ * boitatá's estimation hot path:
+ *   - crates/estimation/src/batch.rs:15-38      estimate_many (search + krige per target)
+ *   - crates/estimation/src/batch.rs:165-190    leave_one_out_at
+ *   - crates/estimation/src/krige.rs:83-95      krige (OK / SK / IK, point support)
+ *   - crates/estimation/src/block.rs:122-198    block_krige_points
+ *   - crates/estimation/src/search.rs:951-982   SearchTree::neighbors_around
+ * Parameters mirror crates/variogram/src/composite.rs:13-18 (Variogram),
+ * crates/variogram/src/aniso.rs:18-26 (Angles), crates/estimation/src/search.rs:31-81
+ * (Search) and the Python constructors crates/py/src/variogram.rs:330-362 and
+ * crates/py/src/estimation.rs:415-531.
+ *
+ * Conventions
+ *   - Every array argument may be host memory or device memory of the context's
+ *     device (detected per pointer). Results are written in place; the library
+ *     keeps nothing but the state given to bk_set_samples.
+ *   - Functions return 0 on success or a negative BK_E* code; the message is in
+ *     bk_last_error(ctx). Per-target failures are not errors: they are reported in
+ *     the status array (the reference's EstimError per target, crates/estimation/src/error.rs:4-16)
+ *     and the target's value and variance are NaN.
+ *   - A context owns one HIP stream on one device and is not thread-safe; calls are
+ *     synchronous. Separate contexts may be used from separate threads.
+ */
+#ifndef BOITATA_GPU_H
+#define BOITATA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-target status, crates/estimation/src/error.rs:4-16. */
+enum {
+  BK_OK = 0,
+  BK_SEARCH_FAILED = 1,      /* too few samples, or a sector / hole rule broken */
+  BK_SINGULAR = 2,           /* pivot <= 1e-12 * sill (kernel.rs:58-65) or LU breakdown */
+  BK_INSUFFICIENT_DATA = 3,  /* no samples */
+  BK_INVALID_PARAMETERS = 4,
+  BK_UNSUPPORTED = 5         /* a system beyond what the device path holds (n > BK_MAX_LU) */
+};
+
+/* Return codes of the API calls. */
+enum {
+  BK_E_INVALID = -1,   /* invalid parameters (Search::check, Variogram::validate) */
+  BK_E_HIP = -2,       /* a HIP runtime error */
+  BK_E_STATE = -3,     /* call order: no samples set, no factor saved */
+  BK_E_NOMEM = -4
+};
+
+/* Kriging kind, crates/estimation/src/krige.rs:20-28. */
+enum { BK_ORDINARY = 0, BK_SIMPLE = 1, BK_INDICATOR = 2 };
+
+/* Arithmetic of the solve. */
+enum {
+  BK_FP64 = 0,     /* plain FP64, the reference's operation order */
+  BK_FP32_IR = 1   /* FP32 covariance and Cholesky, FP64 residual refinement and Schur scalar */
+};
+
+/* Variogram model, crates/variogram/src/model.rs:11-30. */
+enum {
+  BK_SPHERICAL = 0,
+  BK_EXPONENTIAL = 1,
+  BK_GAUSSIAN = 2,
+  BK_CUBIC = 3,
+  BK_PENTASPHERICAL = 4,
+  BK_CIRCULAR = 5,
+  BK_SINEHOLE = 6,
+  BK_MATERN = 7,   /* param = order (snaps to 0.5, 1.5, 2.5) */
+  BK_POWER = 8     /* param = exponent; not stationary */
+};
+
+#define BK_MAX_STRUCTURES 4
+#define BK_MAX_CHOL 64     /* systems solved by Cholesky on the device (kernel.rs:11 MAX) */
+#define BK_MAX_LU 79       /* larger systems and power models: bordered LU on the device */
+
+/* One nested structure, crates/variogram/src/model.rs:34-41. `range` runs along
+ * the major axis. ratio_semi / ratio_minor > 0 give this structure its own
+ * anisotropy ratios under the variogram's rotation (an extension: the reference
+ * shares one anisotropy, composite.rs:17); values <= 0 take the variogram's. */
+typedef struct {
+  int32_t model;
+  int32_t reserved;
+  double param;
+  double sill;
+  double range;
+  double ratio_semi;
+  double ratio_minor;
+} bk_structure;
+
+/* Nugget plus nested structures with an optional anisotropy of major = 1
+ * (crates/py/src/variogram.rs:164-177: rotation (azimuth, dip, rake) in degrees,
+ * ratios (semi, minor) of the major range). */
+typedef struct {
+  double nugget;
+  int32_t n_structures;
+  int32_t has_anisotropy;
+  bk_structure structures[BK_MAX_STRUCTURES];
+  double azimuth, dip, rake;
+  double ratio_semi, ratio_minor;
+} bk_variogram;
+
+/* Search neighbourhood, crates/estimation/src/search.rs:31-81. Optional counts
+ * are -1 when absent. Without an anisotropy of its own the search measures
+ * distance in the variogram's (search.rs:343-348). */
+typedef struct {
+  int64_t min_samples;
+  int64_t max_samples;
+  double radius;              /* in the ellipsoid metric: metres along its major axis */
+  int32_t has_anisotropy;
+  int32_t octant;
+  double azimuth, dip, rake;
+  double ratio_semi, ratio_minor;
+  int64_t max_per_hole;       /* -1: none */
+  int64_t max_per_octant;     /* -1: max_samples / count, rounded up */
+  int64_t sectors;            /* 0: none; else the number of plane sectors */
+  int64_t max_per_sector;
+  int64_t min_sectors;        /* -1: none */
+  int64_t max_empty_sectors;  /* -1: none */
+  int64_t min_holes;          /* -1: none */
+} bk_search;
+
+/* Block support, crates/estimation/src/block.rs:18-71 and
+ * crates/py/src/estimation.rs:947-958: a block of `size` along the axes of
+ * block_frame(rotation), discretised nx * ny * nz. */
+typedef struct {
+  double size[3];
+  int32_t nx, ny, nz;
+  int32_t reserved;
+  double rotation[3];
+} bk_block;
+
+typedef struct bk_ctx bk_ctx;
+
+/* Context on HIP device `device`. */
+int bk_create(bk_ctx** out, int device);
+void bk_destroy(bk_ctx* ctx);
+const char* bk_last_error(const bk_ctx* ctx);
+/* Library version string; never fails. */
+const char* bk_version(void);
+
+/* Validation, as crates/estimation/src/search.rs:214-264 (Search::check) and
+ * crates/variogram/src/composite.rs:37-48 (Variogram::validate). */
+int bk_check_search(bk_ctx* ctx, const bk_search* search);
+int bk_check_variogram(bk_ctx* ctx, const bk_variogram* vg);
+
+/* The fitted samples (crates/estimation/src/lib.rs:70-82, crates/py/src/estimation.rs:1244-1310).
+ * error_var may be NULL (all 0); hole may be NULL (no holes), -1 marks a sample
+ * without a hole. Copied to the device. */
+int bk_set_samples(bk_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                   const double* value, const double* error_var, const int32_t* hole);
+
+/* Neighbour selection of SearchTree::neighbors_around (search.rs:951-982) for m
+ * targets: idx[t * max_samples + k], k < count[t], in selection order.
+ * left_out (may be NULL) holds per target a sample index taken out of the
+ * search, or -1. status[t] is BK_OK or BK_SEARCH_FAILED. */
+int bk_neighbors(bk_ctx* ctx, const bk_search* search, const bk_variogram* vg, int64_t m,
+                 const double* tx, const double* ty, const double* tz, const int64_t* left_out,
+                 int32_t* idx, int32_t* count, int32_t* status);
+
+/* estimate_many with the krige estimator (batch.rs:15-38, krige.rs:83-95).
+ * param is the SK mean or the IK threshold. lagrange, n_used, status may be NULL. */
+int bk_krige(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind,
+             double param, int32_t precision, int64_t m, const double* tx, const double* ty,
+             const double* tz, double* value, double* variance, double* lagrange,
+             int32_t* n_used, int32_t* status);
+
+/* estimate_many with block_krige_points (block.rs:122-198) on the discretisation
+ * of `block` centred on each target (py/estimation.rs:947-958). support_variance
+ * (C(B,B)) may be NULL. */
+int bk_block_krige(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind,
+                   double param, const bk_block* block, int32_t precision, int64_t m,
+                   const double* tx, const double* ty, const double* tz, double* value,
+                   double* variance, double* lagrange, double* support_variance,
+                   int32_t* n_used, int32_t* status);
+
+/* leave_one_out_at (batch.rs:165-190) over the samples at `which` (m of them;
+ * NULL = all samples, m ignored). */
+int bk_loo(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind,
+           double param, int32_t precision, int64_t m, const int64_t* which, double* value,
+           double* variance, int32_t* status);
+
+/* Leave-one-out cross-validation loss and its gradient (no counterpart in the
+ * reference). The neighbour sets are those of bk_loo with `search`, fixed before
+ * the call by bk_loo_prepare, so they do not move with the parameters.
+ *   loss = mean_i (z_i - zhat_-i)^2 + mean_i log sigma2_-i
+ * over the held-out samples whose system solved. grad packs the derivatives in
+ * this order: nugget, sill_s (s < S), then per structure its three ranges in
+ * metres (major = range, semi = range * ratio_semi, minor = range * ratio_minor),
+ * then azimuth, dip, rake in degrees: 4 + 4 S values (12 for two structures). */
+int bk_loo_prepare(bk_ctx* ctx, const bk_search* search, const bk_variogram* vg);
+int bk_loo_loss_grad(bk_ctx* ctx, const bk_variogram* vg, int32_t precision, double* loss,
+                     double* grad, int64_t* n_used);
+
+/* Tuning of the device path (no counterpart in the reference): bucket edges of
+ * the neighbour counts (three of 16, 32, 64, rising to 64; NULL keeps them),
+ * FP32+IR refinement sweeps and relative tolerance, and the grid cell size as
+ * a factor of the reference's side rule (search/grid.rs:83-106). Values <= 0
+ * keep the current setting. Results do not depend on the cell factor. */
+int bk_set_tuning(bk_ctx* ctx, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOITATA_GPU_H */

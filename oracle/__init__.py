@@ -1,0 +1,264 @@
+"""The oracle: a CPU restatement of boitatá's kriging hot path.
+
+TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may load it, and only as the checker or as the
+timed CPU baseline. The engine (boitata_amd) never imports it.
+
+oracle.cpp holds the restatement (each function cites the reference file and
+line it follows); this module binds liboracle.so with ctypes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from boitata_amd import _abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+class Samples(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("x", C.c_void_p),
+        ("y", C.c_void_p),
+        ("z", C.c_void_p),
+        ("value", C.c_void_p),
+        ("err", C.c_void_p),
+        ("hole", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        P, D, I64, I32 = C.c_void_p, C.c_double, C.c_int64, C.c_int
+        SP, VP, SR, BP = C.POINTER(Samples), C.POINTER(_abi.Variogram), C.POINTER(_abi.Search), C.POINTER(_abi.Block)
+        sig = {
+            "or_rotation_matrix": (None, [D, D, D, P]),
+            "or_block_frame": (None, [P, P]),
+            "or_aniso_matrix": (None, [D, D, D, D, D, D, P]),
+            "or_shape": (D, [I32, D, D, D]),
+            "or_d_shape": (None, [I32, D, D, P]),
+            "or_cov": (D, [VP, D]),
+            "or_gamma": (D, [VP, D]),
+            "or_total_sill": (D, [VP]),
+            "or_cov_points": (D, [VP, P, P]),
+            "or_block_cov_points": (D, [VP, P, P]),
+            "or_check_search": (I32, [SR]),
+            "or_block_points": (None, [P, BP, P]),
+            "or_krige_system": (I32, [SP, VP, I32, D, P, I32, P, P, P, P, P]),
+            "or_block_krige_system": (I32, [SP, VP, I32, D, P, I64, P, P, P, P, P, P]),
+            "or_neighbors": (I32, [SP, SR, VP, I64, P, P, P, P, I32, P, P, P, I32]),
+            "or_krige": (I32, [SP, VP, SR, I32, D, I64, P, P, P, P, P, P, P, P, I32]),
+            "or_block_krige": (I32, [SP, VP, SR, I32, D, BP, I64, P, P, P, P, P, P, P, P, P, I32]),
+            "or_loo": (I32, [SP, VP, SR, I32, D, I64, P, P, P, P, I32]),
+            "or_krige_lists": (I32, [SP, VP, I32, D, I64, P, P, P, P, P, I64, P, P, P, P, I32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Data:
+    """Samples held for the oracle (SoA, float64 / int32)."""
+
+    def __init__(self, x, y, z, value=None, err=None, hole=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self.value = np.ascontiguousarray(value if value is not None else np.zeros(len(self.x)), dtype=np.float64)
+        self.err = None if err is None else np.ascontiguousarray(err, dtype=np.float64)
+        self.hole = None if hole is None else np.ascontiguousarray(hole, dtype=np.int32)
+        self.s = Samples(len(self.x), _p(self.x), _p(self.y), _p(self.z), _p(self.value), _p(self.err), _p(self.hole))
+
+    @property
+    def n(self):
+        return len(self.x)
+
+
+def rotation_matrix(az, dip, rake):
+    out = np.zeros(9)
+    lib().or_rotation_matrix(az, dip, rake, _p(out))
+    return out.reshape(3, 3)
+
+
+def block_frame(rot):
+    r = np.ascontiguousarray(rot, dtype=np.float64)
+    out = np.zeros(9)
+    lib().or_block_frame(_p(r), _p(out))
+    return out.reshape(3, 3)
+
+
+def aniso_matrix(az, dip, rake, major, semi, minor):
+    out = np.zeros(9)
+    lib().or_aniso_matrix(az, dip, rake, major, semi, minor, _p(out))
+    return out.reshape(3, 3)
+
+
+def shape(model, h, rng, param=0.0):
+    m = _abi.MODELS[model] if isinstance(model, str) else model
+    return lib().or_shape(m, param, h, rng)
+
+
+def d_shape(model, t, param=0.0):
+    m = _abi.MODELS[model] if isinstance(model, str) else model
+    out = np.zeros(2)
+    lib().or_d_shape(m, param, t, _p(out))
+    return out
+
+
+def cov(vg, h):
+    return lib().or_cov(C.byref(vg), h)
+
+
+def gamma(vg, h):
+    return lib().or_gamma(C.byref(vg), h)
+
+
+def total_sill(vg):
+    return lib().or_total_sill(C.byref(vg))
+
+
+def cov_points(vg, p, q):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    return lib().or_cov_points(C.byref(vg), _p(p), _p(q))
+
+
+def block_cov_points(vg, p, q):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    return lib().or_block_cov_points(C.byref(vg), _p(p), _p(q))
+
+
+def check_search(s):
+    return lib().or_check_search(C.byref(s)) == 0
+
+
+def block_points(t, blk):
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros((blk.nx * blk.ny * blk.nz, 3))
+    lib().or_block_points(_p(t), C.byref(blk), _p(out))
+    return out
+
+
+def krige_system(data: Data, vg, target, kind=_abi.ORDINARY, param=0.0, solver=0):
+    """One system: solver 0 = krige (guard, kernel, LU), 1 = kernel only, 2 = LU only."""
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    v, var, lag = C.c_double(), C.c_double(), C.c_double()
+    nu = C.c_int32()
+    w = np.zeros(max(data.n, 1))
+    st = lib().or_krige_system(C.byref(data.s), C.byref(vg), kind, param, _p(t), solver, C.byref(v), C.byref(var),
+                               C.byref(lag), _p(w), C.byref(nu))
+    return dict(status=st, value=v.value, variance=var.value, lagrange=lag.value, weights=w[: nu.value],
+                n_used=nu.value)
+
+
+def block_krige_system(data: Data, vg, pts, kind=_abi.ORDINARY, param=0.0):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    v, var, lag, sup = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    nu = C.c_int32()
+    w = np.zeros(max(data.n, 1))
+    st = lib().or_block_krige_system(C.byref(data.s), C.byref(vg), kind, param, _p(pts), len(pts), C.byref(v),
+                                     C.byref(var), C.byref(lag), C.byref(sup), _p(w), C.byref(nu))
+    return dict(status=st, value=v.value, variance=var.value, lagrange=lag.value, support=sup.value,
+                weights=w[: nu.value], n_used=nu.value)
+
+
+def neighbors(data: Data, search, vg, targets, left_out=None, tree=True, threads=0):
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    m = len(t)
+    lo = None if left_out is None else np.ascontiguousarray(left_out, dtype=np.int64)
+    idx = np.full((m, int(search.max_samples) if search.max_samples < 1 << 20 else max(data.n, 1)), -1, np.int32)
+    stride_search = search
+    if search.max_samples >= 1 << 20:
+        raise ValueError("oracle neighbour lists need max_samples < 2^20")
+    count = np.zeros(m, np.int32)
+    status = np.zeros(m, np.int32)
+    vgp = C.byref(vg) if vg is not None else None
+    rc = lib().or_neighbors(C.byref(data.s), C.byref(stride_search), vgp, m, _p(tx), _p(ty), _p(tz), _p(lo),
+                            1 if tree else 0, _p(idx), _p(count), _p(status), threads)
+    if rc != 0:
+        raise ValueError("invalid search")
+    return idx, count, status
+
+
+def krige(data: Data, vg, search, targets, kind=_abi.ORDINARY, param=0.0, threads=0):
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    m = len(t)
+    out = {k: np.zeros(m) for k in ("value", "variance", "lagrange")}
+    n_used = np.zeros(m, np.int32)
+    status = np.zeros(m, np.int32)
+    rc = lib().or_krige(C.byref(data.s), C.byref(vg), C.byref(search), kind, param, m, _p(tx), _p(ty), _p(tz),
+                        _p(out["value"]), _p(out["variance"]), _p(out["lagrange"]), _p(n_used), _p(status), threads)
+    if rc != 0:
+        raise ValueError("invalid search")
+    out.update(n_used=n_used, status=status)
+    return out
+
+
+def block_krige(data: Data, vg, search, blk, targets, kind=_abi.ORDINARY, param=0.0, threads=0):
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    m = len(t)
+    out = {k: np.zeros(m) for k in ("value", "variance", "lagrange", "support")}
+    n_used = np.zeros(m, np.int32)
+    status = np.zeros(m, np.int32)
+    rc = lib().or_block_krige(C.byref(data.s), C.byref(vg), C.byref(search), kind, param, C.byref(blk), m, _p(tx),
+                              _p(ty), _p(tz), _p(out["value"]), _p(out["variance"]), _p(out["lagrange"]),
+                              _p(out["support"]), _p(n_used), _p(status), threads)
+    if rc != 0:
+        raise ValueError("invalid search or block")
+    out.update(n_used=n_used, status=status)
+    return out
+
+
+def loo(data: Data, vg, search, which=None, kind=_abi.ORDINARY, param=0.0, threads=0):
+    w = None if which is None else np.ascontiguousarray(which, dtype=np.int64)
+    m = data.n if w is None else len(w)
+    value = np.zeros(m)
+    variance = np.zeros(m)
+    status = np.zeros(m, np.int32)
+    rc = lib().or_loo(C.byref(data.s), C.byref(vg), C.byref(search), kind, param, m, _p(w), _p(value), _p(variance),
+                      _p(status), threads)
+    if rc != 0:
+        raise ValueError("invalid search")
+    return dict(value=value, variance=variance, status=status)
+
+
+def krige_lists(data: Data, vg, targets, idx, count, kind=_abi.ORDINARY, param=0.0, threads=0):
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    count = np.ascontiguousarray(count, dtype=np.int32)
+    m = len(t)
+    value, variance, lagrange = np.zeros(m), np.zeros(m), np.zeros(m)
+    status = np.zeros(m, np.int32)
+    lib().or_krige_lists(C.byref(data.s), C.byref(vg), kind, param, m, _p(tx), _p(ty), _p(tz), _p(idx), _p(count),
+                         idx.shape[1], _p(value), _p(variance), _p(lagrange), _p(status), threads)
+    return dict(value=value, variance=variance, lagrange=lagrange, status=status)

@@ -1,0 +1,1477 @@
+// oracle.cpp — CPU restatement of boitatá's kriging hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the correctness oracle and the timed
+// CPU baseline of the engine under boitata_amd/; the engine never links, loads
+// or calls it. It restates, in FP64 and in the reference's operation order
+// (compiled with -ffp-contract=off so no product is fused into an FMA):
+//   crates/core/src/rotation.rs:8-43           rotation_matrix, block_frame
+//   crates/variogram/src/aniso.rs:75-119       Anisotropy::{new, matrix, lag}, euclidean
+//   crates/variogram/src/model.rs:76-126,181-311 shape (+ the derivatives the adjoint uses)
+//   crates/variogram/src/composite.rs:50-129   Variogram::{total_sill, gamma, cov, lag, ...}
+//   crates/estimation/src/search.rs:214-264    Search::check
+//   crates/estimation/src/search.rs:396-595    Sectors, Selector, select
+//   crates/estimation/src/search.rs:647-687    neighbors_in (the scan)
+//   crates/estimation/src/search.rs:692-1006   SearchTree, Space
+//   crates/estimation/src/search/grid.rs:11-381 Grid (nearest-first cells)
+//   crates/estimation/src/coincident.rs:11-112 Clusters
+//   crates/estimation/src/kernel.rs:24-123     the stack Cholesky kernel
+//   crates/estimation/src/krige.rs:83-201      krige, value, check_simple, krige_lu
+//   crates/estimation/src/block.rs:18-198      Discretization, block_covariances, block_krige_points
+//   crates/estimation/src/batch.rs:15-38,165-190 estimate_many, leave_one_out_at
+//   crates/py/src/estimation.rs:947-958        block_points
+// Third-party arithmetic the reference delegates to nalgebra 0.33.3 (absent from
+// the reference tree): 3x3 products are restated in nalgebra's column-axpy order,
+// r_i = (m_i0 v0 + m_i1 v1) + m_i2 v2; DMatrix::lu().solve is restated as
+// partial-pivoting LU with reciprocal-scaled columns and column-oriented
+// triangular solves. Below about 1e-9 relative the LU outputs are parity
+// unpinned (no nalgebra here to compare against).
+//
+// Extension (not in the reference): a structure may carry its own anisotropy
+// ratios under the variogram's rotation; with every structure on the shared
+// ratios each function reduces to the reference's arithmetic exactly.
+
+#include "../include/boitata_gpu.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <queue>
+#include <utility>
+#include <vector>
+
+namespace orc {
+
+using P3 = std::array<double, 3>;
+struct M3 {
+  double m[9];  // row-major
+  double operator()(int i, int j) const { return m[3 * i + j]; }
+};
+
+static const double kPi = 3.14159265358979323846;
+
+// nalgebra Matrix3 * Vector3 (gemv, column-axpy order).
+static inline P3 mv(const M3& a, const P3& v) {
+  P3 r;
+  for (int i = 0; i < 3; ++i) {
+    double t0 = a(i, 0) * v[0];
+    double t1 = a(i, 1) * v[1];
+    double t2 = a(i, 2) * v[2];
+    r[i] = (t0 + t1) + t2;
+  }
+  return r;
+}
+
+// nalgebra Matrix3 * Matrix3 (gemm as column-wise gemv).
+static inline M3 mm(const M3& a, const M3& b) {
+  M3 c;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t0 = a(i, 0) * b(0, j);
+      double t1 = a(i, 1) * b(1, j);
+      double t2 = a(i, 2) * b(2, j);
+      c.m[3 * i + j] = (t0 + t1) + t2;
+    }
+  return c;
+}
+
+static inline M3 identity() { return M3{{1, 0, 0, 0, 1, 0, 0, 0, 1}}; }
+
+// f64::to_radians: self * (PI / 180).
+static inline double to_radians(double d) { return d * (kPi / 180.0); }
+
+// crates/core/src/rotation.rs:8-16
+M3 rotation_matrix(double azimuth, double dip, double rake) {
+  double a = to_radians(azimuth), d = to_radians(dip), r = to_radians(rake);
+  double sa = std::sin(a), ca = std::cos(a);
+  double sd = std::sin(d), cd = std::cos(d);
+  double sr = std::sin(r), cr = std::cos(r);
+  M3 r1{{sa, ca, 0.0, -ca, sa, 0.0, 0.0, 0.0, 1.0}};
+  M3 r2{{cd, 0.0, -sd, 0.0, 1.0, 0.0, sd, 0.0, cd}};
+  M3 r3{{1.0, 0.0, 0.0, 0.0, cr, sr, 0.0, -sr, cr}};
+  return mm(mm(r3, r2), r1);
+}
+
+// crates/core/src/rotation.rs:40-43: rows (-R1, R0, R2).
+M3 block_frame(const double rot[3]) {
+  M3 r = rotation_matrix(rot[0], rot[1], rot[2]);
+  M3 b;
+  for (int j = 0; j < 3; ++j) {
+    b.m[j] = -r(1, j);
+    b.m[3 + j] = r(0, j);
+    b.m[6 + j] = r(2, j);
+  }
+  return b;
+}
+
+// Rust `powi` lowered by LLVM's binary exponentiation (x^3 = x * x^2, ...).
+static inline double powi(double x, int n) {
+  double res = 0.0;
+  bool have = false;
+  double cur = x;
+  while (n) {
+    if (n & 1) {
+      res = have ? res * cur : cur;
+      have = true;
+    }
+    n >>= 1;
+    if (n) cur = cur * cur;
+  }
+  return have ? res : 1.0;
+}
+
+// crates/variogram/src/aniso.rs:18-111 (major, semi, minor ranges).
+struct Aniso {
+  double azimuth = 0, dip = 0, rake = 0, major = 1, semi = 1, minor = 1;
+  M3 rot = identity();
+  void init() { rot = rotation_matrix(azimuth, dip, rake); }
+  // aniso.rs:88-94: diag(1/major, 1/semi, 1/minor) * R
+  M3 matrix() const {
+    M3 d{{1.0 / major, 0.0, 0.0, 0.0, 1.0 / semi, 0.0, 0.0, 0.0, 1.0 / minor}};
+    return mm(d, rot);
+  }
+  // aniso.rs:101-111
+  double lag(const P3& p, const P3& q) const {
+    P3 v{q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+    P3 r = mv(rot, v);
+    double s0 = r[0] / major, s1 = r[1] / semi, s2 = r[2] / minor;
+    return std::sqrt((s0 * s0 + s1 * s1) + s2 * s2);
+  }
+};
+
+// aniso.rs:114-119
+static inline double euclidean(const P3& p, const P3& q) {
+  double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+  return std::sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+// crates/variogram/src/model.rs:76-126
+double shape(int model, double param, double h, double range) {
+  if (!std::isfinite(h)) return 0.0;
+  if (h <= 0.0) return 0.0;
+  double r = std::max(range, std::numeric_limits<double>::min());
+  double t = h / r;
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0 ? 1.0 : 1.5 * t - 0.5 * powi(t, 3);
+    case BK_EXPONENTIAL:
+      return 1.0 - std::exp(-3.0 * t);
+    case BK_GAUSSIAN:
+      return 1.0 - std::exp(-3.0 * t * t);
+    case BK_CUBIC:
+      if (t >= 1.0) return 1.0;
+      return 7.0 * powi(t, 2) - (35.0 / 4.0) * powi(t, 3) + (7.0 / 2.0) * powi(t, 5) -
+             (3.0 / 4.0) * powi(t, 7);
+    case BK_PENTASPHERICAL:
+      if (t >= 1.0) return 1.0;
+      return (15.0 / 8.0) * t - (5.0 / 4.0) * powi(t, 3) + (3.0 / 8.0) * powi(t, 5);
+    case BK_CIRCULAR:
+      if (t >= 1.0) return 1.0;
+      return 1.0 - (2.0 / kPi) * std::acos(t) + (2.0 * t / kPi) * std::sqrt(1.0 - t * t);
+    case BK_SINEHOLE: {
+      double c = kPi * t;
+      return 1.0 - std::sin(c) / c;
+    }
+    case BK_MATERN: {
+      // model.rs:136-179: order snaps to 0.5, 1.5, 2.5; delta = sqrt(2 nu) * 3 * t
+      double nu = param < 1.0 ? 0.5 : (param < 2.0 ? 1.5 : 2.5);
+      double delta = (std::sqrt(2.0 * nu) * 3.0) * t;
+      double e = std::exp(-delta);
+      if (nu == 0.5) return 1.0 - e;
+      if (nu == 1.5) return 1.0 - (1.0 + delta) * e;
+      return 1.0 - (1.0 + delta + delta * delta / 3.0) * e;
+    }
+    case BK_POWER:
+      return std::pow(t, param);
+  }
+  return 0.0;
+}
+
+// model.rs:207-311: (dg/dt, d2g/dt2) of the normalised shape; used to check the
+// adjoint's closed forms.
+std::pair<double, double> d_shape(int model, double param, double t) {
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0 ? std::make_pair(0.0, 0.0) : std::make_pair(1.5 - 1.5 * t * t, -3.0 * t);
+    case BK_EXPONENTIAL: {
+      double e = std::exp(-3.0 * t);
+      return {3.0 * e, -9.0 * e};
+    }
+    case BK_GAUSSIAN: {
+      double e = std::exp(-3.0 * t * t);
+      return {6.0 * t * e, (6.0 - 36.0 * t * t) * e};
+    }
+    default:
+      return {std::nan(""), std::nan("")};
+  }
+}
+
+struct Structure {
+  int model = BK_SPHERICAL;
+  double param = 0, sill = 0, range = 1;
+  bool own = false;    // carries its own anisotropy ratios (extension)
+  Aniso aniso;         // its own anisotropy when `own`
+  bool stationary() const { return model != BK_POWER; }
+};
+
+// crates/variogram/src/composite.rs:13-129
+struct Variogram {
+  double nugget = 0;
+  std::vector<Structure> st;
+  bool has_aniso = false;
+  Aniso aniso;  // the shared anisotropy (major = 1)
+
+  double total_sill() const {
+    double s = -0.0;
+    for (auto& x : st) s += x.sill;
+    return nugget + s;
+  }
+  bool is_stationary() const {
+    for (auto& x : st)
+      if (!x.stationary()) return false;
+    return true;
+  }
+  bool any_own() const {
+    for (auto& x : st)
+      if (x.own) return true;
+    return false;
+  }
+  // gamma with per-structure lags hs[s]; the nugget's h > 0 test uses h0.
+  double gamma_lags(double h0, const double* hs) const {
+    double s = -0.0;
+    for (size_t k = 0; k < st.size(); ++k) s += st[k].sill * shape(st[k].model, st[k].param, hs[k], st[k].range);
+    return h0 > 0.0 ? nugget + s : 0.0;
+  }
+  double cov_lags(double h0, const double* hs) const {
+    if (h0 <= 0.0) return total_sill();
+    return total_sill() - gamma_lags(h0, hs);
+  }
+  // composite.rs:69-85 (scalar lag, shared anisotropy)
+  double gamma(double h) const {
+    double hs[BK_MAX_STRUCTURES];
+    for (size_t k = 0; k < st.size(); ++k) hs[k] = h;
+    return gamma_lags(h, hs);
+  }
+  double cov(double h) const { return h <= 0.0 ? total_sill() : total_sill() - gamma(h); }
+  double lag(const P3& p, const P3& q) const { return has_aniso ? aniso.lag(p, q) : euclidean(p, q); }
+  // Lags of every structure between p and q; returns the nugget test lag.
+  double lags(const P3& p, const P3& q, double* hs) const {
+    double h = lag(p, q);
+    for (size_t k = 0; k < st.size(); ++k) hs[k] = st[k].own ? st[k].aniso.lag(p, q) : h;
+    return st.empty() ? h : hs[0];
+  }
+  // composite.rs:101-103
+  double cov_points(const P3& p, const P3& q) const {
+    double hs[BK_MAX_STRUCTURES];
+    double h0 = lags(p, q, hs);
+    return cov_lags(h0, hs);
+  }
+  // composite.rs:108-115
+  double block_cov_points(const P3& p, const P3& q) const {
+    double hs[BK_MAX_STRUCTURES];
+    double h0 = lags(p, q, hs);
+    if (h0 <= 0.0) return total_sill() - nugget;
+    return cov_lags(h0, hs);
+  }
+  // composite.rs:120-129
+  double block_mean_cov(const std::vector<P3>& pts) const {
+    if (pts.size() == 1) return total_sill();
+    double total = -0.0;
+    for (auto& p : pts) {
+      double row = -0.0;
+      for (auto& q : pts) row += block_cov_points(p, q);
+      total += row;
+    }
+    return total / (double)(pts.size() * pts.size());
+  }
+};
+
+static Variogram make_variogram(const bk_variogram* v) {
+  Variogram g;
+  g.nugget = v->nugget;
+  g.has_aniso = v->has_anisotropy != 0;
+  if (g.has_aniso) {
+    g.aniso.azimuth = v->azimuth;
+    g.aniso.dip = v->dip;
+    g.aniso.rake = v->rake;
+    g.aniso.major = 1.0;
+    g.aniso.semi = v->ratio_semi;
+    g.aniso.minor = v->ratio_minor;
+    g.aniso.init();
+  }
+  for (int k = 0; k < v->n_structures && k < BK_MAX_STRUCTURES; ++k) {
+    const bk_structure& s = v->structures[k];
+    Structure x;
+    x.model = s.model;
+    x.param = s.param;
+    x.sill = s.sill;
+    x.range = s.range;
+    x.own = s.ratio_semi > 0.0 && s.ratio_minor > 0.0;
+    if (x.own) {
+      x.aniso.azimuth = v->has_anisotropy ? v->azimuth : 0.0;
+      x.aniso.dip = v->has_anisotropy ? v->dip : 0.0;
+      x.aniso.rake = v->has_anisotropy ? v->rake : 0.0;
+      x.aniso.major = 1.0;
+      x.aniso.semi = s.ratio_semi;
+      x.aniso.minor = s.ratio_minor;
+      x.aniso.init();
+    }
+    g.st.push_back(x);
+  }
+  return g;
+}
+
+struct Sample {
+  P3 loc;
+  double value;
+  int32_t hole;  // -1: none
+  double err;
+};
+
+enum Kind { ORDINARY = 0, SIMPLE = 1, INDICATOR = 2 };
+
+struct Estimate {
+  double value = NAN, variance = NAN, lagrange = NAN, support = NAN;
+  int n_used = 0;
+  std::vector<double> weights;
+};
+
+// ---------------------------------------------------------------------------
+// Coincident guard, crates/estimation/src/coincident.rs:11-112
+static const double EPS = 1e-6;
+static inline bool close(const P3& p, const P3& q) {
+  double a = p[0] - q[0], b = p[1] - q[1], c = p[2] - q[2];
+  return (a * a + b * b) + c * c <= EPS * EPS;
+}
+
+struct Clusters {
+  std::vector<int> of;
+  std::vector<std::vector<int>> members;
+  static bool find(const std::vector<Sample>& s, Clusters& out) {
+    bool any = false;
+    for (size_t i = 0; i < s.size() && !any; ++i)
+      for (size_t j = 0; j < i; ++j)
+        if (close(s[i].loc, s[j].loc)) {
+          any = true;
+          break;
+        }
+    if (!any) return false;
+    out.of.clear();
+    out.members.clear();
+    for (size_t i = 0; i < s.size(); ++i) {
+      int c = -1;
+      for (size_t k = 0; k < out.members.size(); ++k)
+        if (close(s[out.members[k][0]].loc, s[i].loc)) {
+          c = (int)k;
+          break;
+        }
+      if (c >= 0) {
+        out.of.push_back(c);
+        out.members[c].push_back((int)i);
+      } else {
+        out.of.push_back((int)out.members.size());
+        out.members.push_back({(int)i});
+      }
+    }
+    return true;
+  }
+  std::vector<Sample> merge(const std::vector<Sample>& s, const Variogram& vg) const {
+    double c0 = vg.total_sill();
+    std::vector<Sample> out;
+    for (auto& m : members) {
+      const Sample& first = s[m[0]];
+      if (m.size() == 1) {
+        out.push_back(first);
+        continue;
+      }
+      double k = (double)m.size();
+      double var = 0.0;
+      for (int i : m) {
+        var += s[i].err;
+        for (int j : m) var += vg.cov_points(s[i].loc, s[j].loc);
+      }
+      double sum = -0.0;
+      for (int i : m) sum += s[i].value;
+      Sample x = first;
+      x.value = sum / k;
+      x.err = var / (k * k) - c0;
+      out.push_back(x);
+    }
+    return out;
+  }
+};
+
+// krige.rs:98-113
+static double kind_value(int kind, double param, const std::vector<double>& w, const std::vector<Sample>& s) {
+  double sum = -0.0;
+  for (size_t i = 0; i < w.size(); ++i) {
+    double z = s[i].value;
+    double f = kind == SIMPLE ? z - param : (kind == INDICATOR ? (z <= param ? 1.0 : 0.0) : z);
+    sum += w[i] * f;
+  }
+  if (kind == SIMPLE) return param + sum;
+  if (kind == INDICATOR) return std::min(std::max(sum, 0.0), 1.0);
+  return sum;
+}
+
+static void spread(const Clusters& c, Estimate& e, int kind, double param, const std::vector<Sample>& s) {
+  std::vector<double> w(c.of.size());
+  for (size_t i = 0; i < c.of.size(); ++i) w[i] = e.weights[c.of[i]] / (double)c.members[c.of[i]].size();
+  e.weights = w;
+  e.value = kind_value(kind, param, e.weights, s);
+  e.n_used = (int)c.of.size();
+}
+
+// ---------------------------------------------------------------------------
+// The stack Cholesky kernel, crates/estimation/src/kernel.rs:24-123.
+// Returns BK_OK / BK_SINGULAR, or -1 when the system is beyond it (LU takes it).
+static const int MAXK = 64;
+static const double PIVOT = 1e-12;
+
+int krige_kernel(int kind, double param, const P3& target, const std::vector<Sample>& s, const Variogram& vg,
+                 Estimate& out) {
+  int n = (int)s.size();
+  if (n == 0 || n > MAXK || !vg.is_stationary()) return -1;
+  // Frames: the shared one, and each structure's own.
+  int S = (int)vg.st.size();
+  M3 shared = vg.has_aniso ? vg.aniso.matrix() : identity();
+  std::vector<M3> fr(S);
+  for (int k = 0; k < S; ++k) fr[k] = vg.st[k].own ? vg.st[k].aniso.matrix() : shared;
+  std::vector<P3> p(n);                    // shared frame
+  std::vector<std::vector<P3>> ps(S, std::vector<P3>(n));
+  for (int i = 0; i < n; ++i) {
+    P3 d{s[i].loc[0] - target[0], s[i].loc[1] - target[1], s[i].loc[2] - target[2]};
+    p[i] = vg.has_aniso ? mv(shared, d) : d;
+    for (int k = 0; k < S; ++k) ps[k][i] = vg.st[k].own ? mv(fr[k], d) : p[i];
+  }
+  auto norm = [](const P3& a, const P3& b) {
+    double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    return std::sqrt((x * x + y * y) + z * z);
+  };
+  auto covij = [&](int i, int j) {
+    double hs[BK_MAX_STRUCTURES];
+    double h = norm(p[i], p[j]);
+    for (int k = 0; k < S; ++k) hs[k] = vg.st[k].own ? norm(ps[k][i], ps[k][j]) : h;
+    return vg.cov_lags(S ? hs[0] : h, hs);
+  };
+  auto cov0 = [&](int i) {
+    P3 o{0.0, 0.0, 0.0};
+    double hs[BK_MAX_STRUCTURES];
+    double h = norm(p[i], o);
+    for (int k = 0; k < S; ++k) hs[k] = vg.st[k].own ? norm(ps[k][i], o) : h;
+    return vg.cov_lags(S ? hs[0] : h, hs);
+  };
+  double c0 = vg.total_sill();
+  std::vector<double> l(n * (n + 1) / 2), c(n);
+  for (int i = 0; i < n; ++i) {
+    int row = i * (i + 1) / 2;
+    for (int j = 0; j < i; ++j) l[row + j] = covij(i, j);
+    l[row + i] = c0 + s[i].err;
+    c[i] = cov0(i);
+  }
+  auto dot = [](const double* a, const double* b, int len) {
+    double acc = -0.0;
+    for (int k = 0; k < len; ++k) acc += a[k] * b[k];
+    return acc;
+  };
+  for (int j = 0; j < n; ++j) {
+    int rj = j * (j + 1) / 2;
+    double d = l[rj + j] - dot(&l[rj], &l[rj], j);
+    if (std::isnan(d) || d <= PIVOT * c0) return BK_SINGULAR;
+    d = std::sqrt(d);
+    l[rj + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      int ri = i * (i + 1) / 2;
+      l[ri + j] = (l[ri + j] - dot(&l[ri], &l[rj], j)) / d;
+    }
+  }
+  auto solve = [&](std::vector<double>& v) {
+    for (int i = 0; i < n; ++i) {
+      int ri = i * (i + 1) / 2;
+      v[i] = (v[i] - dot(&l[ri], v.data(), i)) / l[ri + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double acc = v[i];
+      for (int k = i + 1; k < n; ++k) acc -= l[k * (k + 1) / 2 + i] * v[k];
+      v[i] = acc / l[i * (i + 1) / 2 + i];
+    }
+  };
+  std::vector<double> a = c;
+  solve(a);
+  bool ordinary = kind != SIMPLE;
+  std::vector<double> b(n, 0.0);
+  double mu = 0.0;
+  if (ordinary) {
+    std::fill(b.begin(), b.end(), 1.0);
+    solve(b);
+    double sa = -0.0, sb = -0.0;
+    for (int i = 0; i < n; ++i) sa += a[i];
+    for (int i = 0; i < n; ++i) sb += b[i];
+    mu = (sa - 1.0) / sb;
+  }
+  out.weights.resize(n);
+  for (int i = 0; i < n; ++i) out.weights[i] = a[i] - mu * b[i];
+  out.value = kind_value(kind, param, out.weights, s);
+  double swc = -0.0;
+  for (int i = 0; i < n; ++i) swc += out.weights[i] * c[i];
+  out.variance = std::max(c0 - swc - mu, 0.0);
+  out.n_used = n;
+  out.lagrange = mu;
+  out.support = c0;
+  return BK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Dense LU with partial pivoting: nalgebra's LU::new + solve (restated).
+// a is row-major dim x dim, overwritten; b overwritten with the solution.
+// Returns false when U has a zero on its diagonal (nalgebra's solve -> None).
+static bool lu_solve(std::vector<double>& a, std::vector<double>& b, int dim) {
+  std::vector<int> perm(dim);
+  for (int i = 0; i < dim; ++i) perm[i] = i;
+  for (int i = 0; i < dim; ++i) {
+    int piv = i;
+    double best = std::fabs(a[i * dim + i]);
+    for (int r = i + 1; r < dim; ++r) {
+      double v = std::fabs(a[r * dim + i]);
+      if (v > best) {
+        best = v;
+        piv = r;
+      }
+    }
+    double diag = a[piv * dim + i];
+    if (diag == 0.0) continue;  // nalgebra leaves the column; solve fails later
+    if (piv != i) {
+      for (int k = 0; k < dim; ++k) std::swap(a[i * dim + k], a[piv * dim + k]);
+      std::swap(b[i], b[piv]);
+    }
+    double inv = 1.0 / diag;
+    for (int r = i + 1; r < dim; ++r) a[r * dim + i] *= inv;
+    for (int k = i + 1; k < dim; ++k) {
+      double y = a[i * dim + k];
+      for (int r = i + 1; r < dim; ++r) a[r * dim + k] = a[r * dim + k] + (-y) * a[r * dim + i];
+    }
+  }
+  // L y = P b (unit diagonal), column oriented.
+  for (int i = 0; i < dim; ++i) {
+    double coeff = b[i];
+    for (int r = i + 1; r < dim; ++r) b[r] = b[r] + (-coeff) * a[r * dim + i];
+  }
+  // U x = y, column oriented.
+  for (int i = dim - 1; i >= 0; --i) {
+    double d = a[i * dim + i];
+    if (d == 0.0) return false;
+    b[i] = b[i] / d;
+    double coeff = b[i];
+    for (int r = 0; r < i; ++r) b[r] = b[r] + (-coeff) * a[r * dim + i];
+  }
+  return true;
+}
+
+// krige.rs:117-124
+static bool check_simple(int kind, const Variogram& vg) { return !(kind == SIMPLE && !vg.is_stationary()); }
+
+// krige.rs:127-201
+int krige_lu(int kind, double param, const P3& target, const std::vector<Sample>& s, const Variogram& vg,
+             Estimate& out) {
+  int n = (int)s.size();
+  if (n == 0) return BK_INSUFFICIENT_DATA;
+  if (!check_simple(kind, vg)) return BK_INVALID_PARAMETERS;
+  std::vector<double> z(n);
+  for (int i = 0; i < n; ++i) z[i] = kind == INDICATOR ? (s[i].value <= param ? 1.0 : 0.0) : s[i].value;
+  double c0 = vg.total_sill();
+  bool ordinary = kind != SIMPLE;
+  int dim = ordinary ? n + 1 : n;
+  std::vector<double> a(dim * dim, 0.0), b(dim, 0.0);
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) a[i * dim + j] = vg.cov_points(s[i].loc, s[j].loc);
+    a[i * dim + i] += s[i].err;
+    b[i] = vg.cov_points(s[i].loc, target);
+  }
+  std::vector<double> rhs = b;
+  if (ordinary) {
+    for (int i = 0; i < n; ++i) {
+      a[i * dim + n] = 1.0;
+      a[n * dim + i] = 1.0;
+    }
+    a[n * dim + n] = 0.0;
+    b[n] = 1.0;
+  }
+  if (!lu_solve(a, b, dim)) return BK_SINGULAR;
+  out.weights.assign(b.begin(), b.begin() + n);
+  double sum = -0.0;
+  for (int i = 0; i < n; ++i) sum += out.weights[i] * (kind == SIMPLE ? z[i] - param : z[i]);
+  if (kind == SIMPLE)
+    out.value = param + sum;
+  else if (kind == INDICATOR)
+    out.value = std::min(std::max(sum, 0.0), 1.0);
+  else
+    out.value = sum;
+  double swc = -0.0;
+  for (int i = 0; i < n; ++i) swc += out.weights[i] * rhs[i];
+  double mu = ordinary ? b[n] : 0.0;
+  out.variance = std::max(c0 - swc - mu, 0.0);
+  out.n_used = n;
+  out.lagrange = mu;
+  out.support = c0;
+  return BK_OK;
+}
+
+// krige.rs:83-95
+int krige(int kind, double param, const P3& target, const std::vector<Sample>& s, const Variogram& vg,
+          Estimate& out) {
+  Clusters c;
+  if (Clusters::find(s, c)) {
+    std::vector<Sample> merged = c.merge(s, vg);
+    int st = krige(kind, param, target, merged, vg, out);
+    if (st != BK_OK) return st;
+    spread(c, out, kind, param, s);
+    return BK_OK;
+  }
+  int st = krige_kernel(kind, param, target, s, vg, out);
+  if (st >= 0) return st;
+  return krige_lu(kind, param, target, s, vg, out);
+}
+
+// ---------------------------------------------------------------------------
+// Block support, crates/estimation/src/block.rs
+// block.rs:40-57 Discretization::points
+static std::vector<P3> disc_points(const P3& center, const P3& size, int nx, int ny, int nz) {
+  std::vector<P3> pts;
+  double dx = size[0] / nx, dy = size[1] / ny, dz = size[2] / nz;
+  double x0 = center[0] - size[0] / 2.0 + dx / 2.0;
+  double y0 = center[1] - size[1] / 2.0 + dy / 2.0;
+  double z0 = center[2] - size[2] / 2.0 + dz / 2.0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) pts.push_back({x0 + i * dx, y0 + j * dy, z0 + k * dz});
+  return pts;
+}
+
+// block.rs:60-70 offsets + py/estimation.rs:947-958 block_points
+std::vector<P3> block_points(const P3& t, const bk_block& b) {
+  M3 f = block_frame(b.rotation);
+  M3 ft{{f(0, 0), f(1, 0), f(2, 0), f(0, 1), f(1, 1), f(2, 1), f(0, 2), f(1, 2), f(2, 2)}};
+  std::vector<P3> off = disc_points({0.0, 0.0, 0.0}, {b.size[0], b.size[1], b.size[2]}, b.nx, b.ny, b.nz);
+  std::vector<P3> out;
+  for (auto& o : off) {
+    P3 p = mv(ft, o);
+    out.push_back({t[0] + p[0], t[1] + p[1], t[2] + p[2]});
+  }
+  return out;
+}
+
+// block.rs:122-198
+int block_krige_points(int kind, double param, const std::vector<P3>& pts, const std::vector<Sample>& s,
+                       const Variogram& vg, Estimate& out) {
+  int n = (int)s.size();
+  if (n == 0) return BK_INSUFFICIENT_DATA;
+  if (pts.empty()) return BK_INVALID_PARAMETERS;
+  for (auto& p : pts)
+    if (!(std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]))) return BK_INVALID_PARAMETERS;
+  if (!check_simple(kind, vg)) return BK_INVALID_PARAMETERS;
+  Clusters cl;
+  if (Clusters::find(s, cl)) {
+    int st = block_krige_points(kind, param, pts, cl.merge(s, vg), vg, out);
+    if (st != BK_OK) return st;
+    spread(cl, out, kind, param, s);
+    return BK_OK;
+  }
+  bool ordinary = kind != SIMPLE;
+  int dim = ordinary ? n + 1 : n;
+  // block.rs:100-118
+  double np = (double)pts.size();
+  std::vector<double> rhs(n);
+  for (int i = 0; i < n; ++i) {
+    double acc = -0.0;
+    for (auto& p : pts) acc += pts.size() == 1 ? vg.cov_points(s[i].loc, p) : vg.block_cov_points(s[i].loc, p);
+    rhs[i] = acc / np;
+  }
+  double cbb = vg.block_mean_cov(pts);
+  std::vector<double> b(dim, 0.0), a(dim * dim, 0.0);
+  for (int i = 0; i < n; ++i) b[i] = rhs[i];
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) a[i * dim + j] = vg.cov_points(s[i].loc, s[j].loc);
+    a[i * dim + i] += s[i].err;
+    if (ordinary) {
+      a[i * dim + n] = 1.0;
+      a[n * dim + i] = 1.0;
+    }
+  }
+  if (ordinary) b[n] = 1.0;
+  std::vector<double> x = b;
+  if (!lu_solve(a, x, dim)) return BK_SINGULAR;
+  out.weights.assign(x.begin(), x.begin() + n);
+  out.value = kind_value(kind, param, out.weights, s);
+  double mu = ordinary ? x[n] : 0.0;
+  double swc = -0.0;
+  for (int i = 0; i < n; ++i) swc += out.weights[i] * b[i];
+  out.variance = std::max(cbb - swc - mu, 0.0);
+  out.n_used = n;
+  out.lagrange = mu;
+  out.support = cbb;
+  return BK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Search, crates/estimation/src/search.rs
+struct Search {
+  int64_t min_samples = 4, max_samples = 16;
+  double radius = INFINITY;
+  bool has_aniso = false;
+  Aniso aniso;
+  bool octant = false;
+  int64_t max_per_hole = -1, max_per_octant = -1, sectors = 0, max_per_sector = 0;
+  int64_t min_sectors = -1, max_empty_sectors = -1, min_holes = -1;
+  bool balances() const { return octant || sectors > 0; }
+};
+
+static Search make_search(const bk_search* p) {
+  Search s;
+  s.min_samples = p->min_samples;
+  s.max_samples = p->max_samples;
+  s.radius = p->radius;
+  s.has_aniso = p->has_anisotropy != 0;
+  if (s.has_aniso) {
+    s.aniso.azimuth = p->azimuth;
+    s.aniso.dip = p->dip;
+    s.aniso.rake = p->rake;
+    s.aniso.major = 1.0;
+    s.aniso.semi = p->ratio_semi;
+    s.aniso.minor = p->ratio_minor;
+    s.aniso.init();
+  }
+  s.octant = p->octant != 0;
+  s.max_per_hole = p->max_per_hole;
+  s.max_per_octant = p->max_per_octant;
+  s.sectors = p->sectors;
+  s.max_per_sector = p->max_per_sector;
+  s.min_sectors = p->min_sectors;
+  s.max_empty_sectors = p->max_empty_sectors;
+  s.min_holes = p->min_holes;
+  return s;
+}
+
+// search.rs:214-264
+static bool check(const Search& s) {
+  if (std::isnan(s.radius) || s.radius <= 0.0 || s.max_samples <= 0 || s.min_samples > s.max_samples ||
+      s.min_samples < 0)
+    return false;
+  int64_t count = 0, ring = 0;
+  if (s.sectors > 0) {
+    if (s.octant) return false;
+    if (s.sectors < 2 || s.sectors > 360 || s.max_per_sector <= 0) return false;
+    count = ring = s.sectors;
+  } else if (s.octant) {
+    count = 8;
+    ring = 4;
+  }
+  if (s.max_per_octant >= 0) {
+    if (!s.octant) return false;
+    if (s.max_per_octant == 0) return false;
+  }
+  bool rules = s.min_sectors >= 0 || s.max_empty_sectors >= 0;
+  if (rules && !s.balances()) return false;
+  if (s.min_sectors >= 0 && (s.min_sectors == 0 || s.min_sectors > count || s.min_sectors > s.max_samples))
+    return false;
+  if (s.max_empty_sectors >= 0 && s.max_empty_sectors >= ring) return false;
+  if (s.min_holes >= 0 && (s.min_holes == 0 || s.min_holes > s.max_samples)) return false;
+  return true;
+}
+
+// search.rs:396-467
+struct Sectors {
+  M3 axes = identity();
+  bool planar = false;
+  int64_t plane = 0;  // 0: none
+  int count() const { return plane > 0 ? (int)plane : (planar ? 4 : 8); }
+  int of(const P3& t, const P3& s) const {
+    P3 d = mv(axes, {s[0] - t[0], s[1] - t[1], s[2] - t[2]});
+    if (plane > 0) {
+      const double tau = 2.0 * kPi;
+      double turn = std::fmod(std::atan2(-d[0], d[1]), tau);
+      if (turn < 0.0) turn = turn + tau;
+      double f = turn / tau * (double)plane;
+      int64_t k = f >= 0.0 ? (int64_t)f : 0;  // `as usize` saturates at 0
+      return (int)std::min<int64_t>(k, plane - 1);
+    }
+    int bx = d[0] >= 0.0, by = d[1] >= 0.0, bz = d[2] >= 0.0;
+    return planar ? (bx << 1) | by : (bx << 2) | (by << 1) | bz;
+  }
+  int longest_empty(const std::vector<int64_t>& counts) const {
+    std::vector<std::vector<int>> rings;
+    if (plane > 0) {
+      std::vector<int> r;
+      for (int i = 0; i < plane; ++i) r.push_back(i);
+      rings.push_back(r);
+    } else if (planar) {
+      rings.push_back({3, 1, 0, 2});
+    } else {
+      rings.push_back({7, 3, 1, 5});
+      rings.push_back({6, 2, 0, 4});
+    }
+    int best = 0;
+    for (auto& ring : rings) {
+      bool all = true;
+      for (int x : ring) all = all && counts[x] == 0;
+      if (all) {
+        best = std::max(best, (int)ring.size());
+        continue;
+      }
+      int run = 0, longest = 0;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int x : ring) {
+          run = counts[x] == 0 ? run + 1 : 0;
+          longest = std::max(longest, run);
+        }
+      best = std::max(best, longest);
+    }
+    return best;
+  }
+};
+
+static Sectors make_sectors(const Aniso* a, bool planar, const Search& p) {
+  Sectors s;
+  if (a) {
+    double rot[3] = {a->azimuth, a->dip, a->rake};
+    s.axes = block_frame(rot);
+  }
+  s.planar = planar;
+  s.plane = p.sectors;
+  return s;
+}
+
+// search.rs:476-577
+struct Selector {
+  const P3* target;
+  const Search* p;
+  const Sectors* sec;
+  std::vector<int64_t> chosen;
+  std::vector<std::pair<int32_t, int64_t>> per_hole;
+  int64_t untagged = 0;
+  std::vector<int64_t> per_sector;
+  int64_t cap;
+  Selector(const P3* t, const Search* params, const Sectors* s) : target(t), p(params), sec(s) {
+    per_sector.assign(p->balances() ? sec->count() : 0, 0);
+    if (p->sectors > 0)
+      cap = p->max_per_sector;
+    else if (p->octant)
+      cap = p->max_per_octant >= 0 ? p->max_per_octant : (p->max_samples + sec->count() - 1) / sec->count();
+    else
+      cap = std::numeric_limits<int64_t>::max();
+  }
+  bool full() const { return (int64_t)chosen.size() >= p->max_samples; }
+  bool offer(int64_t idx, const P3& loc, int32_t hole) {
+    if (full()) return true;
+    int seen = -1;
+    if (hole >= 0)
+      for (size_t k = 0; k < per_hole.size(); ++k)
+        if (per_hole[k].first == hole) {
+          seen = (int)k;
+          break;
+        }
+    if (p->max_per_hole >= 0 && seen >= 0 && per_hole[seen].second >= p->max_per_hole) return false;
+    if (p->balances()) {
+      int o = sec->of(*target, loc);
+      if (per_sector[o] >= cap) return false;
+      per_sector[o] += 1;
+    }
+    if (hole >= 0) {
+      if (seen >= 0)
+        per_hole[seen].second += 1;
+      else
+        per_hole.push_back({hole, 1});
+    } else {
+      untagged += 1;
+    }
+    chosen.push_back(idx);
+    return full();
+  }
+  int finish() const {
+    if ((int64_t)chosen.size() < p->min_samples) return BK_SEARCH_FAILED;
+    if (p->min_sectors >= 0) {
+      int64_t held = 0;
+      for (auto c : per_sector) held += c > 0;
+      if (held < p->min_sectors) return BK_SEARCH_FAILED;
+    }
+    if (p->max_empty_sectors >= 0) {
+      if (sec->longest_empty(per_sector) > p->max_empty_sectors) return BK_SEARCH_FAILED;
+    }
+    if (p->min_holes >= 0) {
+      if ((int64_t)per_hole.size() + untagged < p->min_holes) return BK_SEARCH_FAILED;
+    }
+    return BK_OK;
+  }
+};
+
+// search.rs:704-745
+struct Space {
+  M3 frame = identity();
+  bool has_origin = false;
+  P3 origin{0, 0, 0};
+  P3 project(const P3& p) const {
+    P3 o = has_origin ? origin : P3{0, 0, 0};
+    return mv(frame, {p[0] - o[0], p[1] - o[1], p[2] - o[2]});
+  }
+};
+
+static Space make_space(const Aniso* a, const std::vector<Sample>& s) {
+  Space sp;
+  sp.frame = a ? a->matrix() : identity();
+  for (auto& x : s)
+    if (std::isfinite(x.loc[0]) && std::isfinite(x.loc[1]) && std::isfinite(x.loc[2])) {
+      sp.origin = x.loc;
+      sp.has_origin = true;
+      break;
+    }
+  return sp;
+}
+
+// grid.rs:109-111
+static inline double d2(const P3& p, const P3& q) {
+  double a = p[0] - q[0], b = p[1] - q[1], c = p[2] - q[2];
+  return (a * a + b * b) + c * c;
+}
+
+static bool planar_locs(const std::vector<Sample>& s) {
+  if (s.empty()) return true;
+  double z = s[0].loc[2];
+  for (auto& x : s)
+    if (!(x.loc[2] == z)) return false;
+  return true;
+}
+
+static const Aniso* metric(const Search& p, const Variogram* vg) {
+  if (p.has_aniso) return &p.aniso;
+  if (vg && vg->has_aniso) return &vg->aniso;
+  return nullptr;
+}
+
+// search.rs:647-687 (the scan); no domains, soft boundaries or high-grade rules.
+// left_out >= 0 skips that sample as SearchTree::neighbors_around does
+// (search.rs:997): the frame's origin and the planar test still see it.
+int neighbors_scan(const P3& target, const std::vector<Sample>& s, const Search& p, const Variogram* vg,
+                   int64_t left_out, std::vector<int64_t>& out) {
+  const Aniso* a = metric(p, vg);
+  Space sp = make_space(a, s);
+  P3 q = sp.project(target);
+  double r2 = p.radius * p.radius;
+  std::vector<std::pair<double, int64_t>> cand;
+  for (size_t i = 0; i < s.size(); ++i) {
+    if ((int64_t)i == left_out) continue;
+    double dd = d2(sp.project(s[i].loc), q);
+    if (dd <= r2) cand.push_back({dd, (int64_t)i});
+  }
+  std::stable_sort(cand.begin(), cand.end(), [](auto& x, auto& y) { return x.first < y.first; });
+  Sectors sec = make_sectors(a, p.octant && planar_locs(s), p);
+  Selector sel(&target, &p, &sec);
+  for (auto& c : cand)
+    if (sel.offer(c.second, s[c.second].loc, s[c.second].hole)) break;
+  out = sel.chosen;
+  return sel.finish();
+}
+
+// crates/estimation/src/search/grid.rs
+static const int64_t REACH = 12;
+
+struct Table {
+  std::vector<std::array<int64_t, 3>> offsets;
+  std::vector<std::pair<uint64_t, size_t>> shells;
+  Table() {
+    std::vector<std::pair<uint64_t, std::array<int64_t, 3>>> keyed;
+    for (int64_t x = -REACH; x <= REACH; ++x)
+      for (int64_t y = -REACH; y <= REACH; ++y)
+        for (int64_t z = -REACH; z <= REACH; ++z) {
+          std::array<int64_t, 3> o{x, y, z};
+          uint64_t m = 0;
+          for (auto v : o) {
+            int64_t a = std::max<int64_t>(std::llabs(v) - 1, 0);
+            m += (uint64_t)(a * a);
+          }
+          keyed.push_back({m, o});
+        }
+    std::sort(keyed.begin(), keyed.end());
+    for (size_t k = 0; k < keyed.size(); ++k) {
+      if (!shells.empty() && shells.back().first == keyed[k].first)
+        shells.back().second = k + 1;
+      else
+        shells.push_back({keyed[k].first, k + 1});
+      offsets.push_back(keyed[k].second);
+    }
+  }
+};
+
+static const Table& table() {
+  static Table t;
+  return t;
+}
+
+// Rust `f64 as i64`: saturating, NaN -> 0.
+static inline int64_t sat_i64(double v) {
+  if (std::isnan(v)) return 0;
+  if (v >= 9.2233720368547758e18) return std::numeric_limits<int64_t>::max();
+  if (v < -9.2233720368547758e18) return std::numeric_limits<int64_t>::min();
+  return (int64_t)v;
+}
+
+struct Grid {
+  P3 lo{0, 0, 0}, hi{0, 0, 0};
+  double h = 1;
+  int64_t dims[3] = {1, 1, 1};
+  std::vector<uint32_t> start;
+  std::vector<P3> ep;
+  std::vector<uint32_t> ei;
+
+  // grid.rs:83-106
+  static double side(const P3& lo, const P3& hi, size_t n) {
+    double ext[3];
+    for (int d = 0; d < 3; ++d) ext[d] = hi[d] - lo[d];
+    double longest = 0.0;
+    for (double e : ext) longest = std::fmax(longest, e);
+    if (longest <= 0.0) return 1.0;
+    double prod = 1.0;
+    int cnt = 0;
+    for (double e : ext)
+      if (e > longest * 1e-9) {
+        prod *= e;
+        cnt++;
+      }
+    double h = std::pow(prod / (double)std::max<size_t>(n, 1), 1.0 / (double)cnt);
+    auto cells = [&](double hh) {
+      double c = 1.0;
+      for (double e : ext) c *= std::floor(e / hh) + 1.0;
+      return c;
+    };
+    while (cells(h) > 4.0 * (double)n + 64.0) h *= 2.0;
+    return h;
+  }
+
+  void build(const std::vector<P3>& pts) {
+    // grid.rs:66-80 bounds over finite points
+    P3 l{INFINITY, INFINITY, INFINITY}, u{-INFINITY, -INFINITY, -INFINITY};
+    for (auto& p : pts) {
+      if (!(std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]))) continue;
+      for (int d = 0; d < 3; ++d) {
+        l[d] = std::fmin(l[d], p[d]);
+        u[d] = std::fmax(u[d], p[d]);
+      }
+    }
+    if (!(l[0] <= u[0])) l = u = P3{0, 0, 0};
+    lo = l;
+    hi = u;
+    h = side(lo, hi, pts.size());
+    for (int d = 0; d < 3; ++d) dims[d] = sat_i64(std::floor((hi[d] - lo[d]) / h)) + 1;
+    size_t cells = (size_t)(dims[0] * dims[1] * dims[2]);
+    std::vector<size_t> ids(pts.size());
+    start.assign(cells + 1, 0);
+    for (size_t i = 0; i < pts.size(); ++i) {
+      ids[i] = id(cell(pts[i]));
+      start[ids[i] + 1]++;
+    }
+    for (size_t c = 0; c < cells; ++c) start[c + 1] += start[c];
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    ep.resize(pts.size());
+    ei.resize(pts.size());
+    for (size_t i = 0; i < pts.size(); ++i) {
+      ep[fill[ids[i]]] = pts[i];
+      ei[fill[ids[i]]] = (uint32_t)i;
+      fill[ids[i]]++;
+    }
+  }
+  std::array<int64_t, 3> cell(const P3& p) const {
+    std::array<int64_t, 3> c;
+    for (int d = 0; d < 3; ++d) c[d] = std::min(std::max(sat_i64(std::floor((p[d] - lo[d]) / h)), (int64_t)0), dims[d] - 1);
+    return c;
+  }
+  size_t id(const std::array<int64_t, 3>& c) const { return (size_t)(c[0] + dims[0] * (c[1] + dims[1] * c[2])); }
+
+  using Key = std::pair<uint64_t, uint32_t>;
+  using Heap = std::priority_queue<Key, std::vector<Key>, std::greater<Key>>;
+
+  void scan(std::array<int64_t, 3> c, const P3& q, double r2, Heap& heap) const {
+    for (int d = 0; d < 3; ++d)
+      if (c[d] < 0 || c[d] >= dims[d]) return;
+    size_t k = id(c);
+    for (uint32_t e = start[k]; e < start[k + 1]; ++e) {
+      double dd = d2(ep[e], q);
+      if (dd <= r2) {
+        uint64_t bits;
+        std::memcpy(&bits, &dd, 8);
+        heap.push({bits, ei[e]});
+      }
+    }
+  }
+  template <class F>
+  static bool release(Heap& heap, double bound, F& visit) {
+    while (!heap.empty()) {
+      Key top = heap.top();
+      double dd;
+      std::memcpy(&dd, &top.first, 8);
+      if (dd >= bound) return false;
+      heap.pop();
+      if (visit(dd, (int64_t)top.second)) return true;
+    }
+    return false;
+  }
+  void ring(std::array<int64_t, 3> c, int64_t r, const P3& q, double r2, Heap& heap) const {
+    auto lo_ = [&](int d) { return std::max(-r, -c[d]); };
+    auto hi_ = [&](int d) { return std::min(r, dims[d] - 1 - c[d]); };
+    for (int64_t x = lo_(0); x <= hi_(0); ++x)
+      for (int64_t y = lo_(1); y <= hi_(1); ++y) {
+        if (std::llabs(x) == r || std::llabs(y) == r) {
+          for (int64_t z = lo_(2); z <= hi_(2); ++z) scan({c[0] + x, c[1] + y, c[2] + z}, q, r2, heap);
+        } else {
+          for (int64_t z : {-r, r})
+            if (z >= lo_(2) && z <= hi_(2)) scan({c[0] + x, c[1] + y, c[2] + z}, q, r2, heap);
+        }
+      }
+  }
+  // grid.rs:209-255
+  template <class F>
+  void nearest(const P3& q, double r2, F visit) const {
+    const Table& t = table();
+    auto c = cell(q);
+    double h2 = h * h;
+    double beyond = h2 * (double)((REACH - 1) * (REACH - 1));
+    Heap heap;
+    size_t begin = 0;
+    for (auto& sh : t.shells) {
+      double bound = std::fmin(h2 * (double)sh.first, beyond) * (1.0 - 1e-9);
+      if (bound > r2) break;
+      if (release(heap, bound, visit)) return;
+      for (size_t k = begin; k < sh.second; ++k) {
+        auto& o = t.offsets[k];
+        scan({c[0] + o[0], c[1] + o[1], c[2] + o[2]}, q, r2, heap);
+      }
+      begin = sh.second;
+    }
+    int64_t far = 0;
+    for (int d = 0; d < 3; ++d) far = std::max(far, std::max(c[d], dims[d] - 1 - c[d]));
+    for (int64_t r = REACH + 1; r <= far; ++r) {
+      double x = (double)(r - 1) * h;
+      double bound = x * x * (1.0 - 1e-9);
+      if (bound > r2) break;
+      if (release(heap, bound, visit)) return;
+      ring(c, r, q, r2, heap);
+    }
+    release(heap, INFINITY, visit);
+  }
+};
+
+// search.rs:692-1006, without domains, soft boundaries, high grade or calibration.
+struct SearchTree {
+  const std::vector<Sample>* s;
+  Search p;
+  Space space;
+  Sectors sectors;
+  Grid grid;
+  std::vector<P3> points;
+  SearchTree(const std::vector<Sample>& samples, const Search& params, const Variogram* vg) : s(&samples), p(params) {
+    const Aniso* a = metric(p, vg);
+    space = make_space(a, samples);
+    sectors = make_sectors(a, planar_locs(samples), p);
+    points.resize(samples.size());
+    for (size_t i = 0; i < samples.size(); ++i) points[i] = space.project(samples[i].loc);
+    grid.build(points);
+  }
+  // search.rs:951-982
+  int neighbors_around(const P3& target, int64_t left_out, std::vector<int64_t>& out) const {
+    Selector sel(&target, &p, &sectors);
+    if (s->empty() || p.max_samples == 0) {
+      out = sel.chosen;
+      return sel.finish();
+    }
+    P3 q = space.project(target);
+    std::vector<int64_t> group;
+    double at = NAN;
+    bool full = false;
+    auto offer = [&]() {
+      bool f = false;
+      for (int64_t i : group) {
+        if (i == left_out) continue;
+        if (sel.offer(i, (*s)[i].loc, (*s)[i].hole)) {
+          f = true;
+          break;
+        }
+      }
+      group.clear();
+      return f;
+    };
+    grid.nearest(q, p.radius * p.radius, [&](double dd, int64_t i) {
+      if (dd != at && !group.empty()) {
+        full = offer();
+        if (full) return true;
+      }
+      at = dd;
+      group.push_back(i);
+      return false;
+    });
+    if (!full) offer();
+    out = sel.chosen;
+    return sel.finish();
+  }
+};
+
+static std::vector<Sample> make_samples(int64_t n, const double* x, const double* y, const double* z,
+                                        const double* v, const double* err, const int32_t* hole) {
+  std::vector<Sample> s(n);
+  for (int64_t i = 0; i < n; ++i) {
+    s[i].loc = {x[i], y[i], z[i]};
+    s[i].value = v ? v[i] : 0.0;
+    s[i].err = err ? err[i] : 0.0;
+    s[i].hole = hole ? hole[i] : -1;
+  }
+  return s;
+}
+
+static void write_estimate(int st, const Estimate& e, int64_t t, double* value, double* variance, double* lagrange,
+                           double* support, int32_t* n_used, int32_t* status) {
+  bool ok = st == BK_OK;
+  if (value) value[t] = ok ? e.value : NAN;
+  if (variance) variance[t] = ok ? e.variance : NAN;
+  if (lagrange) lagrange[t] = ok ? e.lagrange : NAN;
+  if (support) support[t] = ok ? e.support : NAN;
+  if (n_used) n_used[t] = ok ? e.n_used : 0;
+  if (status) status[t] = st;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+typedef struct {
+  int64_t n;
+  const double *x, *y, *z, *value, *err;
+  const int32_t* hole;
+} or_samples;
+
+static std::vector<Sample> samples_of(const or_samples* s) {
+  return make_samples(s->n, s->x, s->y, s->z, s->value, s->err, s->hole);
+}
+
+void or_rotation_matrix(double az, double dip, double rake, double* out) {
+  M3 r = rotation_matrix(az, dip, rake);
+  std::memcpy(out, r.m, sizeof(r.m));
+}
+
+void or_block_frame(const double* rot, double* out) {
+  M3 r = block_frame(rot);
+  std::memcpy(out, r.m, sizeof(r.m));
+}
+
+void or_aniso_matrix(double az, double dip, double rake, double major, double semi, double minor, double* out) {
+  Aniso a;
+  a.azimuth = az;
+  a.dip = dip;
+  a.rake = rake;
+  a.major = major;
+  a.semi = semi;
+  a.minor = minor;
+  a.init();
+  M3 m = a.matrix();
+  std::memcpy(out, m.m, sizeof(m.m));
+}
+
+double or_shape(int model, double param, double h, double range) { return shape(model, param, h, range); }
+
+void or_d_shape(int model, double param, double t, double* out) {
+  auto d = d_shape(model, param, t);
+  out[0] = d.first;
+  out[1] = d.second;
+}
+
+double or_cov(const bk_variogram* v, double h) { return make_variogram(v).cov(h); }
+double or_gamma(const bk_variogram* v, double h) { return make_variogram(v).gamma(h); }
+double or_total_sill(const bk_variogram* v) { return make_variogram(v).total_sill(); }
+
+double or_cov_points(const bk_variogram* v, const double* p, const double* q) {
+  return make_variogram(v).cov_points({p[0], p[1], p[2]}, {q[0], q[1], q[2]});
+}
+
+double or_block_cov_points(const bk_variogram* v, const double* p, const double* q) {
+  return make_variogram(v).block_cov_points({p[0], p[1], p[2]}, {q[0], q[1], q[2]});
+}
+
+int or_check_search(const bk_search* p) { return check(make_search(p)) ? 0 : -1; }
+
+void or_block_points(const double* t, const bk_block* b, double* out) {
+  auto pts = block_points({t[0], t[1], t[2]}, *b);
+  for (size_t i = 0; i < pts.size(); ++i)
+    for (int d = 0; d < 3; ++d) out[3 * i + d] = pts[i][d];
+}
+
+// One system: solver 0 = krige (guard, kernel, LU), 1 = kernel only (-1 when
+// beyond it), 2 = LU only. weights may be NULL (else n entries).
+int or_krige_system(const or_samples* sm, const bk_variogram* v, int kind, double param, const double* t,
+                    int solver, double* value, double* variance, double* lagrange, double* weights,
+                    int32_t* n_used) {
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  Estimate e;
+  P3 tt{t[0], t[1], t[2]};
+  int st = solver == 1 ? krige_kernel(kind, param, tt, s, vg, e)
+                       : (solver == 2 ? krige_lu(kind, param, tt, s, vg, e) : krige(kind, param, tt, s, vg, e));
+  if (st == BK_OK) {
+    *value = e.value;
+    *variance = e.variance;
+    *lagrange = e.lagrange;
+    *n_used = e.n_used;
+    if (weights)
+      for (size_t i = 0; i < e.weights.size(); ++i) weights[i] = e.weights[i];
+  }
+  return st;
+}
+
+int or_block_krige_system(const or_samples* sm, const bk_variogram* v, int kind, double param, const double* pts,
+                          int64_t npts, double* value, double* variance, double* lagrange, double* support,
+                          double* weights, int32_t* n_used) {
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  std::vector<P3> p(npts);
+  for (int64_t i = 0; i < npts; ++i) p[i] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  Estimate e;
+  int st = block_krige_points(kind, param, p, s, vg, e);
+  if (st == BK_OK) {
+    *value = e.value;
+    *variance = e.variance;
+    *lagrange = e.lagrange;
+    *support = e.support;
+    *n_used = e.n_used;
+    if (weights)
+      for (size_t i = 0; i < e.weights.size(); ++i) weights[i] = e.weights[i];
+  }
+  return st;
+}
+
+// Neighbour lists: use_tree 0 = the scan (search.rs:647-687), 1 = the grid tree.
+int or_neighbors(const or_samples* sm, const bk_search* sp, const bk_variogram* v, int64_t m, const double* tx,
+                 const double* ty, const double* tz, const int64_t* left_out, int use_tree, int32_t* idx,
+                 int32_t* count, int32_t* status, int nthreads) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Variogram vg;
+  if (v) vg = make_variogram(v);
+  const Variogram* pv = v ? &vg : nullptr;
+  SearchTree* tree = use_tree ? new SearchTree(s, p, pv) : nullptr;
+  int64_t stride = p.max_samples;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    std::vector<int64_t> out;
+    P3 q{tx[t], ty[t], tz[t]};
+    int64_t lo = left_out ? left_out[t] : -1;
+    int st = tree ? tree->neighbors_around(q, lo, out) : neighbors_scan(q, s, p, pv, lo, out);
+    count[t] = (int32_t)out.size();
+    status[t] = st;
+    for (size_t k = 0; k < out.size(); ++k) idx[t * stride + k] = (int32_t)out[k];
+  }
+  delete tree;
+  return 0;
+}
+
+// estimate_many with krige (batch.rs:15-38).
+int or_krige(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int kind, double param, int64_t m,
+             const double* tx, const double* ty, const double* tz, double* value, double* variance,
+             double* lagrange, int32_t* n_used, int32_t* status, int nthreads) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  SearchTree tree(s, p, &vg);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    P3 q{tx[t], ty[t], tz[t]};
+    std::vector<int64_t> chosen;
+    Estimate e;
+    int st = tree.neighbors_around(q, -1, chosen);
+    if (st == BK_OK) {
+      std::vector<Sample> sel;
+      for (auto i : chosen) sel.push_back(s[i]);
+      st = krige(kind, param, q, sel, vg, e);
+    }
+    write_estimate(st, e, t, value, variance, lagrange, nullptr, n_used, status);
+  }
+  return 0;
+}
+
+// estimate_many with block_krige_points (block.rs:122-198, py/estimation.rs:767-784).
+int or_block_krige(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int kind, double param,
+                   const bk_block* blk, int64_t m, const double* tx, const double* ty, const double* tz,
+                   double* value, double* variance, double* lagrange, double* support, int32_t* n_used,
+                   int32_t* status, int nthreads) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  SearchTree tree(s, p, &vg);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    P3 q{tx[t], ty[t], tz[t]};
+    std::vector<int64_t> chosen;
+    Estimate e;
+    int st = tree.neighbors_around(q, -1, chosen);
+    if (st == BK_OK) {
+      std::vector<Sample> sel;
+      for (auto i : chosen) sel.push_back(s[i]);
+      st = block_krige_points(kind, param, block_points(q, *blk), sel, vg, e);
+    }
+    write_estimate(st, e, t, value, variance, lagrange, support, n_used, status);
+  }
+  return 0;
+}
+
+// leave_one_out_at (batch.rs:165-190); which == NULL means every sample.
+int or_loo(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int kind, double param, int64_t m,
+           const int64_t* which, double* value, double* variance, int32_t* status, int nthreads) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  SearchTree tree(s, p, &vg);
+  if (!which) m = (int64_t)s.size();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t k = 0; k < m; ++k) {
+    int64_t i = which ? which[k] : k;
+    const P3& q = s[i].loc;
+    std::vector<int64_t> chosen;
+    Estimate e;
+    int st = tree.neighbors_around(q, i, chosen);
+    if (st == BK_OK && chosen.empty()) st = BK_SEARCH_FAILED;
+    if (st == BK_OK) {
+      std::vector<Sample> sel;
+      for (auto j : chosen) sel.push_back(s[j]);
+      st = krige(kind, param, q, sel, vg, e);
+    }
+    write_estimate(st, e, k, value, variance, nullptr, nullptr, nullptr, status);
+  }
+  return 0;
+}
+
+// Kriging with given neighbour lists (idx[t * stride + k], k < count[t]) at
+// given targets: the fixed-neighbourhood systems of the leave-one-out loss.
+int or_krige_lists(const or_samples* sm, const bk_variogram* v, int kind, double param, int64_t m,
+                   const double* tx, const double* ty, const double* tz, const int32_t* idx, const int32_t* count,
+                   int64_t stride, double* value, double* variance, double* lagrange, int32_t* status,
+                   int nthreads) {
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    P3 q{tx[t], ty[t], tz[t]};
+    std::vector<Sample> sel;
+    for (int32_t k = 0; k < count[t]; ++k) sel.push_back(s[idx[t * stride + k]]);
+    Estimate e;
+    int st = krige(kind, param, q, sel, vg, e);
+    write_estimate(st, e, t, value, variance, lagrange, nullptr, nullptr, status);
+  }
+  return 0;
+}
+
+}  // extern "C"
