@@ -185,4 +185,8 @@ SIGNATURES = [
 # Declared in include/boitata_gpu.h beside the reference-facing entry points.
 EXTRA_SIGNATURES = [
     ("bk_set_tuning", C.c_int, [P, P, I32, D, D]),
+    ("bk_profile", C.c_int, [P, I32]),
+    ("bk_profile_read", C.c_int, [P, P, P]),
 ]
+
+STAGES = ("grid", "search", "bucket", "solve", "deferred", "total")

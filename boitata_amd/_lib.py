@@ -18,6 +18,25 @@ class EngineUnavailable(RuntimeError):
     """The HIP library is not built, or cannot be loaded."""
 
 
+def _share_hip_runtime():
+    """One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64
+    (same SONAME, requested by an unversioned name), so if this library bound
+    /opt/rocm's first, importing torch afterwards would bring up a second HIP
+    and HSA runtime that sees no GPU. Loading torch's copy globally first makes
+    both bind to it; without torch, /opt/rocm's is used."""
+    import importlib.util
+
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    hip = Path(spec.origin).parent / "lib" / "libamdhip64.so"
+    if hip.exists():
+        try:
+            C.CDLL(str(hip), mode=C.RTLD_GLOBAL)
+        except OSError:
+            pass
+
+
 def load() -> C.CDLL:
     """The HIP library with every signature declared; raises if it is missing."""
     global _lib
@@ -25,6 +44,7 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise EngineUnavailable(f"{LIB_PATH} is not built; run `python -m boitata_amd.build`")
+    _share_hip_runtime()
     try:
         lib = C.CDLL(str(LIB_PATH))
     except OSError as e:  # pragma: no cover - depends on the machine

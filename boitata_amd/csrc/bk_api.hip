@@ -106,7 +106,30 @@ struct bk_ctx {
   int64_t loo_m = 0;
   int loo_stride = 0;
   Buf loo_idx, loo_count, loo_status;
+  // Stage timing (bk_profile): events on the context's stream.
+  bool prof = false;
+  hipEvent_t ev[7] = {};
+  double prof_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t prof_calls = 0;
 };
+
+namespace {
+void mark(bk_ctx* c, int k) {
+  if (c->prof) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+// After the stream has synchronised: add this call's stage times.
+void accumulate(bk_ctx* c) {
+  if (!c->prof) return;
+  float t[6];
+  const int from[6] = {0, 1, 2, 3, 4, 0}, to[6] = {1, 2, 3, 4, 5, 6};
+  for (int k = 0; k < 6; ++k) {
+    if (hipEventElapsedTime(&t[k], c->ev[from[k]], c->ev[to[k]]) != hipSuccess) return;
+  }
+  for (int k = 0; k < 6; ++k) c->prof_ms[k] += t[k];
+  c->prof_calls += 1;
+}
+}  // namespace
 
 namespace {
 
@@ -616,6 +639,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
   if (bk::launch_bucket(c->stream, m, (const int32_t*)c->ncount.p, (const int32_t*)c->sstatus.p, status, value,
                         variance, lists, counts, m, c->edges, nb, a.v.stationary))
     return fail(c, BK_E_HIP, "bucket launch failed");
+  mark(c, 3);
   int32_t hc[8];
   HIPOK(c, hipMemcpyAsync(hc, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPOK(c, hipStreamSynchronize(c->stream));
@@ -633,6 +657,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
                                     : bk::launch_krige64(c->stream, a, c->edges[b]);
     if (r) return fail(c, BK_E_HIP, "solve launch failed");
   }
+  mark(c, 4);
   // Systems the FP32 form handed back: FP64 over the redo list.
   if (precision == BK_FP32_IR) {
     int32_t h2[8];
@@ -661,6 +686,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
     b.n_list = h3[4];
     if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
   }
+  mark(c, 5);
   return 0;
 }
 
@@ -712,6 +738,8 @@ void bk_destroy(bk_ctx* c) {
     release(*b);
   bk::free_grid(c->gb);
   if (c->tab_storage) (void)hipFree(c->tab_storage);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -845,6 +873,7 @@ static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, i
         return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
   }
   HIPOK(c, hipSetDevice(c->device));
+  mark(c, 0);
   SearchArgs a;
   int64_t stride;
   if (int r = prepare_search(c, s, vg, a, stride)) return r;
@@ -886,9 +915,11 @@ static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, i
     if (int r = ensure(c, c->status, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
     ost = (int32_t*)c->status.p;
   }
+  mark(c, 1);
   if (int r = run_search(c, a, m, dx, dy, dz, dl)) return r;
   if (loo && m > 0) k_loo_nonempty<<<blocks_of(m), 256, 0, c->stream>>>(m, (const int32_t*)c->ncount.p,
                                                                        (int32_t*)c->sstatus.p);
+  mark(c, 2);
   if (int r = run_solve(c, vg, kind, param, precision, m, dx, dy, dz, (int)stride, blk, ov, ovar, olag, osup, onu,
                         ost))
     return r;
@@ -900,7 +931,9 @@ static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, i
   if (int r = stage_out(c, n_used, onu, m)) return r;
   if (status)
     if (int r = stage_out(c, status, ost, m)) return r;
+  mark(c, 6);
   HIPOK(c, hipStreamSynchronize(c->stream));
+  accumulate(c);
   return 0;
 }
 
@@ -953,6 +986,28 @@ int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol
     c->cell_factor = cell_factor;
     c->grid_valid = false;
   }
+  return 0;
+}
+
+int bk_profile(bk_ctx* c, int32_t enable) {
+  if (!c) return BK_E_INVALID;
+  HIPOK(c, hipSetDevice(c->device));
+  if (enable && !c->ev[0])
+    for (auto& e : c->ev) HIPOK(c, hipEventCreate(&e));
+  c->prof = enable != 0;
+  for (double& v : c->prof_ms) v = 0.0;
+  c->prof_calls = 0;
+  return 0;
+}
+
+int bk_profile_read(bk_ctx* c, double* ms6, int64_t* calls) {
+  if (!c) return BK_E_INVALID;
+  for (int k = 0; k < 6; ++k) {
+    if (ms6) ms6[k] = c->prof_ms[k];
+    c->prof_ms[k] = 0.0;
+  }
+  if (calls) *calls = c->prof_calls;
+  c->prof_calls = 0;
   return 0;
 }
 

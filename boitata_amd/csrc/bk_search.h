@@ -14,6 +14,7 @@
 
 #define BK_POOL 512       // candidates held per target between releases
 #define BK_MAX_SEARCH 256 // most samples one search selects on the device
+#define BK_MAX_KNN 64     // selections without caps up to this size run one target per lane
 #define BK_REACH 12
 
 // Internal per-target status: the candidate pool overflowed; the target is
@@ -87,4 +88,5 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
 void free_grid(GridBuffers& gb);
 int upload_shell_table(ShellTable& tab, void** storage);
 int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets);
+bool knn_path(const SearchArgs& a);
 }  // namespace bk

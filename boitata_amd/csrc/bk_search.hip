@@ -131,41 +131,40 @@ __device__ __forceinline__ int batch_lower(const SearchLds& L, int nb, uint64_t 
   return lo;
 }
 
-// Keeps the pool entries whose flag is set, in order (two phases: read, write).
+// Keeps the pool entries whose flag is set, in order. Chunks of one entry per
+// lane from the bottom: an entry only moves down, so a chunk's destinations
+// never reach entries not yet read. keep(in, j, sector) is called once per
+// chunk by the whole wave (it may use wave-wide operations).
 template <class Keep>
 __device__ __forceinline__ int pool_compact(SearchLds& L, int P, int lane, Keep keep) {
-  constexpr int R = BK_POOL / BK_WAVE;
-  uint64_t k[R];
-  uint32_t ii[R];
-  int s[R], h[R], pos[R];
-  int base = 0;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    int j = r * BK_WAVE + lane;
-    bool kp = false;
-    if (j < P) {
-      k[r] = L.pk[j];
-      ii[r] = L.pi[j];
-      s[r] = L.ps[j];
-      h[r] = L.ph[j];
-      kp = keep(r, j);
+  int out = 0;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int base = 0; base < P; base += BK_WAVE) {
+    int j = base + lane;
+    bool in = j < P;
+    uint64_t k = 0;
+    uint32_t ii = 0;
+    int s = -1, h = -1;
+    if (in) {
+      k = L.pk[j];
+      ii = L.pi[j];
+      s = L.ps[j];
+      h = L.ph[j];
     }
+    bool kp = keep(in, j, s);
     uint64_t mask = __ballot(kp);
-    pos[r] = kp ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
-    base += __popcll(mask);
-  }
-  wave_sync();
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (pos[r] >= 0) {
-      L.pk[pos[r]] = k[r];
-      L.pi[pos[r]] = ii[r];
-      L.ps[pos[r]] = s[r];
-      L.ph[pos[r]] = h[r];
+    int pos = out + __popcll(mask & lt);
+    wave_sync();
+    if (kp) {
+      L.pk[pos] = k;
+      L.pi[pos] = ii;
+      L.ps[pos] = s;
+      L.ph[pos] = h;
     }
+    wave_sync();
+    out += __popcll(mask);
   }
-  wave_sync();
-  return base;
+  return out;
 }
 
 struct Ctx {
@@ -220,27 +219,23 @@ struct Ctx {
       int running[8];
       for (int o = 0; o < 8; ++o) running[o] = 0;
       int sure = 0;
-      constexpr int R = BK_POOL / BK_WAVE;
-      int flags[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        int j = r * BK_WAVE + lane;
-        int s = j < st.P ? L->ps[j] : -1;
+      const uint64_t lt = (1ull << lane) - 1ull;
+      const SearchLds* Lc = L;
+      const int cap = a->sector_cap;
+      st.P = pool_compact(*L, st.P, lane, [&](bool in, int, int s) {
         int rank = 0;
         for (int o = 0; o < nsec; ++o) {
-          uint64_t mask = __ballot(s == o);
-          if (s == o) rank = running[o] + __popcll(mask & ((1ull << lane) - 1ull));
+          uint64_t mask = __ballot(in && s == o);
+          if (in && s == o) rank = running[o] + __popcll(mask & lt);
           running[o] += __popcll(mask);
         }
-        bool k = s >= 0 && rank < a->sector_cap - L->per_sector[s];
+        bool k = in && rank < cap - Lc->per_sector[s];
         // Sure picks in order; beyond what is left of max_samples none is taken.
         uint64_t km = __ballot(k);
-        int before = sure + __popcll(km & ((1ull << lane) - 1ull));
+        int before = sure + __popcll(km & lt);
         sure += __popcll(km);
-        flags[r] = k && before < rem;
-      }
-      int P = st.P;
-      st.P = pool_compact(*L, P, lane, [&](int r, int) { return flags[r] != 0; });
+        return k && before < rem;
+      });
       // Thresholds: an octant holding all it may still take, or a pool of
       // `rem` sure picks, makes larger keys useless.
       if (lane == 0) {
@@ -282,32 +277,31 @@ struct Ctx {
     L->bk[lane] = key;
     L->bi[lane] = idx;
     wave_sync();
-    constexpr int R = BK_POOL / BK_WAVE;
-    uint64_t k[R];
-    uint32_t ii[R];
-    int s[R], h[R], pos[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int j = r * BK_WAVE + lane;
-      pos[r] = -1;
-      if (j < st.P) {
-        k[r] = L->pk[j];
-        ii[r] = L->pi[j];
-        s[r] = L->ps[j];
-        h[r] = L->ph[j];
-        pos[r] = j + batch_lower(*L, nb, k[r], ii[r]);
-      }
-    }
+    // Final slot of each batch entry, against the pool before it moves.
     int bpos = lane < nb ? lane + pool_lower(*L, st.P, key, idx) : -1;
-    wave_sync();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (pos[r] >= 0) {
-        L->pk[pos[r]] = k[r];
-        L->pi[pos[r]] = ii[r];
-        L->ps[pos[r]] = s[r];
-        L->ph[pos[r]] = h[r];
+    // Pool entries move up by the batch entries below them; chunks from the
+    // top, so a destination never holds an entry not yet read.
+    for (int base = ((st.P - 1) / BK_WAVE) * BK_WAVE; base >= 0 && st.P > 0; base -= BK_WAVE) {
+      int j = base + lane;
+      int pos = -1;
+      uint64_t k = 0;
+      uint32_t ii = 0;
+      int s = 0, h = -1;
+      if (j < st.P) {
+        k = L->pk[j];
+        ii = L->pi[j];
+        s = L->ps[j];
+        h = L->ph[j];
+        pos = j + batch_lower(*L, nb, k, ii);
       }
+      wave_sync();
+      if (pos >= 0) {
+        L->pk[pos] = k;
+        L->pi[pos] = ii;
+        L->ps[pos] = s;
+        L->ph[pos] = h;
+      }
+      wave_sync();
     }
     if (bpos >= 0) {
       L->pk[bpos] = key;
@@ -423,7 +417,7 @@ struct Ctx {
       st.n_holes = L->scalar[1];
       st.untagged = L->scalar[2];
     }
-    st.P = pool_compact(*L, st.P, lane, [&](int, int j) { return j >= r; });
+    st.P = pool_compact(*L, st.P, lane, [&](bool in, int j, int) { return in && j >= r; });
     st.thr = ~0ull;
     st.thr_i = 0xffffffffu;
     if (!full()) prune();
@@ -590,6 +584,183 @@ __global__ void __launch_bounds__(BK_WAVE) k_search(const SearchArgs a) {
     a.count[t] = n;
     a.status[t] = status;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Searches without caps (no octants, sectors or hole cap): the selection is
+// the K = max_samples smallest (d2 bits, index) keys within the radius, in
+// that order (search.rs:519-543 reduces to "take until full"). One target per
+// lane: the lane walks the same shells with a max-heap of K keys in LDS and
+// stops once a shell's lower bound exceeds the heap's largest distance, since
+// no point of that shell or a later one can enter (grid.rs:209-255 bounds).
+
+struct KnnHeap {
+  uint64_t* k;  // [K][64], interleaved by lane
+  uint32_t* i;
+  int lane;
+  __device__ uint64_t key(int e) const { return k[e * BK_WAVE + lane]; }
+  __device__ uint32_t idx(int e) const { return i[e * BK_WAVE + lane]; }
+  __device__ void put(int e, uint64_t kk, uint32_t ii) {
+    k[e * BK_WAVE + lane] = kk;
+    i[e * BK_WAVE + lane] = ii;
+  }
+  // Max-heap on (key, idx).
+  __device__ void push(int& size, uint64_t kk, uint32_t ii) {
+    int e = size++;
+    while (e > 0) {
+      int p = (e - 1) >> 1;
+      uint64_t pk = key(p);
+      uint32_t pi = idx(p);
+      if (!key_less(pk, pi, kk, ii)) break;
+      put(e, pk, pi);
+      e = p;
+    }
+    put(e, kk, ii);
+  }
+  __device__ void replace_top(int size, uint64_t kk, uint32_t ii) {
+    int e = 0;
+    for (;;) {
+      int l = 2 * e + 1;
+      if (l >= size) break;
+      int r = l + 1;
+      uint64_t ck = key(l);
+      uint32_t ci = idx(l);
+      if (r < size) {
+        uint64_t rk = key(r);
+        uint32_t ri = idx(r);
+        if (key_less(ck, ci, rk, ri)) {
+          ck = rk;
+          ci = ri;
+          l = r;
+        }
+      }
+      if (!key_less(kk, ii, ck, ci)) break;
+      put(e, ck, ci);
+      e = l;
+    }
+    put(e, kk, ii);
+  }
+};
+
+__global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
+  extern __shared__ uint64_t knn_dyn[];
+  const int lane = threadIdx.x;
+  const int K = a.max_samples;
+  KnnHeap H{knn_dyn, (uint32_t*)(knn_dyn + (size_t)K * BK_WAVE), lane};
+  const int64_t b = (int64_t)blockIdx.x * BK_WAVE + lane;
+  const bool live = b < a.m;
+  const int64_t t = live ? (a.list ? a.list[b] : b) : 0;
+  const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
+  double q0 = 0, q1 = 0, q2 = 0;
+  if (live) mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
+  int size = 0;
+  uint64_t top = 0;  // largest key while the heap is full
+  uint32_t top_i = 0;
+  const GridDev& g = a.g;
+  bool go = live && g.start != nullptr && K > 0;
+  int c0 = 0, c1 = 0, c2 = 0;
+  if (go) {
+    c0 = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
+    c1 = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
+    c2 = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+  }
+  const double h2 = g.h * g.h;
+  const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
+  const double slack = 1.0 - 1e-9;
+  auto visit = [&](int x, int y, int z) {
+    if (x < 0 || x >= g.dims[0] || y < 0 || y >= g.dims[1] || z < 0 || z >= g.dims[2]) return;
+    int id = x + g.dims[0] * (y + g.dims[1] * z);
+    uint32_t e0 = g.start[id], e1 = g.start[id + 1];
+    for (uint32_t e = e0; e < e1; ++e) {
+      double4 E = g.entries[e];
+      double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
+      double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
+      uint32_t ii = (uint32_t)__double_as_longlong(E.w);
+      if (!(d2 <= a.radius2) || (int64_t)ii == left_out) continue;
+      uint64_t kk = dbits(d2);
+      if (size < K) {
+        H.push(size, kk, ii);
+        if (size == K) {
+          top = H.key(0);
+          top_i = H.idx(0);
+        }
+      } else if (key_less(kk, ii, top, top_i)) {
+        H.replace_top(size, kk, ii);
+        top = H.key(0);
+        top_i = H.idx(0);
+      }
+    }
+  };
+  uint32_t begin = 0;
+  bool open = go;
+  for (int s = 0; s < a.tab.n; ++s) {
+    uint32_t end = a.tab.end[s];
+    double bound = fmin(h2 * (double)a.tab.m[s], beyond) * slack;
+    if (open && (bound > a.radius2 || (size == K && bound > __longlong_as_double((long long)top)))) open = false;
+    if (!__any(open)) break;
+    if (open)
+      for (uint32_t o = begin; o < end; ++o) {
+        char4 off = a.tab.offsets[o];
+        visit(c0 + off.x, c1 + off.y, c2 + off.z);
+      }
+    begin = end;
+  }
+  if (__any(open)) {
+    int far = 0;
+    if (open)
+      far = max(max(max(c0, g.dims[0] - 1 - c0), max(c1, g.dims[1] - 1 - c1)), max(c2, g.dims[2] - 1 - c2));
+    for (int r = BK_REACH + 1;; ++r) {
+      double x = (double)(r - 1) * g.h;
+      double bound = x * x * slack;
+      if (open && (r > far || bound > a.radius2 || (size == K && bound > __longlong_as_double((long long)top))))
+        open = false;
+      if (!__any(open)) break;
+      if (!open) continue;
+      int xlo = max(-r, -c0), xhi = min(r, g.dims[0] - 1 - c0);
+      int ylo = max(-r, -c1), yhi = min(r, g.dims[1] - 1 - c1);
+      int zlo = max(-r, -c2), zhi = min(r, g.dims[2] - 1 - c2);
+      for (int dx = xlo; dx <= xhi; ++dx)
+        for (int dy = ylo; dy <= yhi; ++dy) {
+          if (abs(dx) == r || abs(dy) == r) {
+            for (int dz = zlo; dz <= zhi; ++dz) visit(c0 + dx, c1 + dy, c2 + dz);
+          } else {
+            if (zlo <= -r) visit(c0 + dx, c1 + dy, c2 - r);
+            if (r <= zhi) visit(c0 + dx, c1 + dy, c2 + r);
+          }
+        }
+    }
+  }
+  if (!live) return;
+  // Heap sort: the selection in increasing (d2 bits, index).
+  const int n = size;
+  for (int end = n - 1; end > 0; --end) {
+    uint64_t mk = H.key(0);
+    uint32_t mi = H.idx(0);
+    uint64_t lk = H.key(end);
+    uint32_t li = H.idx(end);
+    H.put(end, mk, mi);
+    H.replace_top(end, lk, li);
+  }
+  int* out = a.idx + t * a.stride;
+  for (int k = 0; k < n; ++k) out[k] = (int)H.idx(k);
+  // Selector::finish (search.rs:547-576) for the rules that apply here.
+  int status = n < a.min_samples ? BK_SEARCH_FAILED : BK_OK;
+  if (status == BK_OK && a.min_holes >= 0) {
+    int distinct = 0;
+    for (int k = 0; k < n; ++k) {
+      int h = a.hole ? a.hole[out[k]] : -1;
+      if (h < 0) {
+        distinct++;
+        continue;
+      }
+      bool seen = false;
+      for (int j = 0; j < k && !seen; ++j) seen = a.hole[out[j]] == h;
+      distinct += !seen;
+    }
+    if (distinct < a.min_holes) status = BK_SEARCH_FAILED;
+  }
+  a.count[t] = n;
+  a.status[t] = status;
 }
 
 // ---------------------------------------------------------------------------
@@ -824,7 +995,19 @@ int upload_shell_table(ShellTable& tab, void** storage) {
   return 0;
 }
 
+bool knn_path(const SearchArgs& a) { return !a.balances && a.max_per_hole < 0 && a.max_samples <= BK_MAX_KNN; }
+
 int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets) {
+  if (n_targets <= 0) return 0;
+  if (knn_path(a)) {
+    // Selections without caps: one target per lane.
+    SearchArgs c = a;
+    c.m = n_targets;
+    size_t sh = (size_t)a.max_samples * BK_WAVE * (sizeof(uint64_t) + sizeof(uint32_t));
+    k_knn<<<dim3((unsigned)((n_targets + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), sh, s>>>(c);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   for (int64_t b = 0; b < n_targets; b += 2147483647LL) {
     int64_t nb = std::min<int64_t>(n_targets - b, 2147483647LL);
     SearchArgs c = a;

@@ -707,11 +707,12 @@ __global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus
                          double* variance, int32_t* lists, int32_t* counts, int64_t list_stride, int e0, int e1,
                          int e2, int e3, int n_buckets, int stationary) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  int st = sstatus[t];
-  int n = count[t];
+  const int lane = threadIdx.x % BK_WAVE;
+  const bool live = t < m;
+  int st = live ? sstatus[t] : BK_SEARCH_FAILED;
+  int n = live ? count[t] : 0;
   int b = -1;
-  if (st == BK_OK) {
+  if (live && st == BK_OK) {
     if (n == 0) {
       st = BK_INSUFFICIENT_DATA;
     } else if (!stationary || n > BK_MAX_CHOL) {
@@ -725,8 +726,19 @@ __global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus
         }
     }
   }
+  // One atomic per wave and bucket: the lanes of a bucket take consecutive slots.
+  int pos = -1;
+  for (int k = 0; k <= n_buckets; ++k) {
+    uint64_t mask = __ballot(b == k);
+    if (mask == 0) continue;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counts[k], __popcll(mask));
+    base = __shfl(base, leader, BK_WAVE);
+    if (b == k) pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+  }
+  if (!live) return;
   if (b >= 0) {
-    int pos = atomicAdd(&counts[b], 1);
     lists[(int64_t)b * list_stride + pos] = (int32_t)t;
     status[t] = BK_OK;
   } else {

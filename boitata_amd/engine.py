@@ -97,6 +97,17 @@ class Engine:
         self._check(self._lib.bk_set_samples(self._h, self.n, _ptr(x), _ptr(y), _ptr(z), _ptr(value), _ptr(err),
                                              _ptr(hole)))
 
+    def profile(self, enable: bool = True):
+        """Start (or stop) timing the stages of each call with HIP events."""
+        self._check(self._lib.bk_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        """Milliseconds per stage summed since the last read, and the calls timed."""
+        ms = np.zeros(6)
+        calls = C.c_int64()
+        self._check(self._lib.bk_profile_read(self._h, _ptr(ms), C.byref(calls)))
+        return dict(zip(_abi.STAGES, ms.tolist())), calls.value
+
     def tuning(self, edges=None, max_iter=0, tol=0.0, cell_factor=0.0):
         e = None if edges is None else (C.c_int32 * 3)(*edges)
         self._check(self._lib.bk_set_tuning(self._h, e, int(max_iter), float(tol), float(cell_factor)))

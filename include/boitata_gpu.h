@@ -202,6 +202,15 @@ int bk_loo_loss_grad(bk_ctx* ctx, const bk_variogram* vg, int32_t precision, dou
  * keep the current setting. Results do not depend on the cell factor. */
 int bk_set_tuning(bk_ctx* ctx, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor);
 
+/* Stage timing with HIP events recorded on the context's stream (no
+ * counterpart in the reference). bk_profile(ctx, 1) starts accumulating;
+ * bk_profile_read returns the summed milliseconds of each stage since the last
+ * read and clears them: ms[0] grid build, ms[1] search kernel, ms[2] bucketing,
+ * ms[3] solve kernels of the size buckets, ms[4] deferred systems (FP64 redo,
+ * merged and LU systems), ms[5] the whole call; calls = calls timed. */
+int bk_profile(bk_ctx* ctx, int32_t enable);
+int bk_profile_read(bk_ctx* ctx, double* ms6, int64_t* calls);
+
 #ifdef __cplusplus
 }
 #endif
