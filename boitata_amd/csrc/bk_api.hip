@@ -960,12 +960,143 @@ int bk_loo(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, 
 }
 
 int bk_loo_prepare(bk_ctx* c, const bk_search* s, const bk_variogram* vg) {
-  return fail(c, BK_E_STATE, "not built yet");
+  if (!c || !s) return BK_E_INVALID;
+  HIPOK(c, hipSetDevice(c->device));
+  SearchArgs a;
+  int64_t stride;
+  if (int r = prepare_search(c, s, vg, a, stride)) return r;
+  const int64_t m = c->n;
+  for (Buf* b : {&c->tx, &c->ty, &c->tz})
+    if (int r = ensure(c, *b, std::max<int64_t>(m, 1) * sizeof(double))) return r;
+  if (int r = ensure(c, c->left, std::max<int64_t>(m, 1) * sizeof(int64_t))) return r;
+  if (m > 0)
+    k_loo_targets<<<blocks_of(m), 256, 0, c->stream>>>(m, nullptr, (const double*)c->x.p, (const double*)c->y.p,
+                                                       (const double*)c->z.p, (double*)c->tx.p, (double*)c->ty.p,
+                                                       (double*)c->tz.p, (int64_t*)c->left.p);
+  if (int r = run_search(c, a, m, (const double*)c->tx.p, (const double*)c->ty.p, (const double*)c->tz.p,
+                         (const int64_t*)c->left.p))
+    return r;
+  if (m > 0) k_loo_nonempty<<<blocks_of(m), 256, 0, c->stream>>>(m, (const int32_t*)c->ncount.p, (int32_t*)c->sstatus.p);
+  if (int r = ensure(c, c->loo_idx, std::max<int64_t>(m * stride, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->loo_count, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->loo_status, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (m > 0) {
+    HIPOK(c, hipMemcpyAsync(c->loo_idx.p, c->nidx.p, m * stride * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIPOK(c, hipMemcpyAsync(c->loo_count.p, c->ncount.p, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIPOK(c, hipMemcpyAsync(c->loo_status.p, c->sstatus.p, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  c->loo_ready = true;
+  c->loo_m = m;
+  c->loo_stride = (int)stride;
+  return 0;
 }
 
 int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, double* loss, double* grad,
                      int64_t* n_used) {
-  return fail(c, BK_E_STATE, "not built yet");
+  if (!c || !vg) return BK_E_INVALID;
+  if (!c->loo_ready) return fail(c, BK_E_STATE, "call bk_loo_prepare first");
+  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
+  for (int k = 0; k < vg->n_structures; ++k) {
+    int md = vg->structures[k].model;
+    if (md != BK_SPHERICAL && md != BK_EXPONENTIAL && md != BK_GAUSSIAN && md != BK_CUBIC && md != BK_PENTASPHERICAL)
+      return fail(c, BK_E_INVALID, "the gradient covers spherical, exponential, gaussian, cubic and pentaspherical");
+  }
+  (void)precision;
+  HIPOK(c, hipSetDevice(c->device));
+  const int64_t m = c->loo_m;
+  SolveArgs a;
+  std::memset(&a, 0, sizeof(a));
+  dev_variogram(vg, a.v, a.v32);
+  GradArgs gp;
+  std::memset(&gp, 0, sizeof(gp));
+  gp.S = vg->n_structures;
+  gp.P = 4 + 4 * gp.S;
+  double az = vg->has_anisotropy ? vg->azimuth : 0.0, dip = vg->has_anisotropy ? vg->dip : 0.0,
+         rk = vg->has_anisotropy ? vg->rake : 0.0;
+  for (int k = 0; k < gp.S; ++k) {
+    const bk_structure& st = vg->structures[k];
+    bool own = st.ratio_semi > 0.0 && st.ratio_minor > 0.0;
+    double rs = own ? st.ratio_semi : (vg->has_anisotropy ? vg->ratio_semi : 1.0);
+    double rm = own ? st.ratio_minor : (vg->has_anisotropy ? vg->ratio_minor : 1.0);
+    gp.model[k] = st.model;
+    gp.sill[k] = st.sill;
+    gp.r[k][0] = st.range;
+    gp.r[k][1] = st.range * rs;
+    gp.r[k][2] = st.range * rm;
+  }
+  {
+    // R = r3 r2 r1 (rotation.rs:8-16) and its derivative per degree of each angle.
+    const double d2r = kPi / 180.0;
+    double a1 = az * d2r, a2 = dip * d2r, a3 = rk * d2r;
+    double sa = std::sin(a1), ca = std::cos(a1), sd = std::sin(a2), cd = std::cos(a2), sr = std::sin(a3),
+           cr = std::cos(a3);
+    M3 r1{{sa, ca, 0.0, -ca, sa, 0.0, 0.0, 0.0, 1.0}}, dr1{{ca, -sa, 0.0, sa, ca, 0.0, 0.0, 0.0, 0.0}};
+    M3 r2{{cd, 0.0, -sd, 0.0, 1.0, 0.0, sd, 0.0, cd}}, dr2{{-sd, 0.0, -cd, 0.0, 0.0, 0.0, cd, 0.0, -sd}};
+    M3 r3{{1.0, 0.0, 0.0, 0.0, cr, sr, 0.0, -sr, cr}}, dr3{{0.0, 0.0, 0.0, 0.0, -sr, cr, 0.0, -cr, -sr}};
+    M3 R = mm(mm(r3, r2), r1);
+    M3 D[3] = {mm(mm(r3, r2), dr1), mm(mm(r3, dr2), r1), mm(mm(dr3, r2), r1)};
+    std::memcpy(gp.R, R.m, sizeof(R.m));
+    for (int b = 0; b < 3; ++b)
+      for (int q = 0; q < 9; ++q) gp.dR[b][q] = D[b].m[q] * d2r;
+  }
+  a.kind = BK_ORDINARY;
+  a.sx = (const double*)c->x.p;
+  a.sy = (const double*)c->y.p;
+  a.sz = (const double*)c->z.p;
+  a.sv = (const double*)c->v.p;
+  a.serr = c->has_err ? (const double*)c->e.p : nullptr;
+  a.tx = a.sx;
+  a.ty = a.sy;
+  a.tz = a.sz;
+  a.nidx = (const int32_t*)c->loo_idx.p;
+  a.ncount = (const int32_t*)c->loo_count.p;
+  a.stride = c->loo_stride;
+  const int cols = gp.P + 2;
+  for (Buf* b : {&c->value, &c->variance})
+    if (int r = ensure(c, *b, std::max<int64_t>(m, 1) * sizeof(double))) return r;
+  if (int r = ensure(c, c->status, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->support, std::max<int64_t>(m * cols, 32) * sizeof(double))) return r;
+  a.value = (double*)c->value.p;
+  a.variance = (double*)c->variance.p;
+  a.status = (int32_t*)c->status.p;
+  a.grad_part = (double*)c->support.p;
+  const int nb = 3;
+  if (int r = ensure(c, c->lists, std::max<int64_t>(m, 1) * (nb + 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  HIPOK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(int32_t), c->stream));
+  if (m > 0) HIPOK(c, hipMemsetAsync(c->support.p, 0, m * cols * sizeof(double), c->stream));
+  int32_t* lists = (int32_t*)c->lists.p;
+  int32_t* counts = (int32_t*)c->counts.p;
+  a.guard = (int32_t*)c->guard.p;
+  a.guard_count = counts + 4;
+  if (bk::launch_bucket(c->stream, m, (const int32_t*)c->loo_count.p, (const int32_t*)c->loo_status.p, a.status,
+                        a.value, a.variance, lists, counts, m, c->edges, nb, a.v.stationary))
+    return fail(c, BK_E_HIP, "bucket launch failed");
+  int32_t hc[8];
+  HIPOK(c, hipMemcpyAsync(hc, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b) {
+    if (hc[b] == 0) continue;
+    a.list = lists + (int64_t)b * m;
+    a.n_list = hc[b];
+    if (bk::launch_loo_grad(c->stream, a, gp, c->edges[b])) return fail(c, BK_E_HIP, "gradient launch failed");
+  }
+  // Systems beyond the Cholesky kernels (n > 64) stay out of the loss.
+  if (int r = ensure(c, c->offs, 64 * sizeof(double))) return r;
+  if (bk::launch_reduce_rows(c->stream, m, cols, a.grad_part, (double*)c->offs.p))
+    return fail(c, BK_E_HIP, "reduction launch failed");
+  double sums[BK_MAX_GRAD + 2];
+  HIPOK(c, hipMemcpyAsync(sums, c->offs.p, cols * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  double used = sums[1];
+  if (n_used) *n_used = (int64_t)used;
+  if (used <= 0) return fail(c, BK_E_STATE, "no held-out sample was estimated");
+  if (loss) *loss = sums[0] / used;
+  if (grad)
+    for (int p = 0; p < gp.P; ++p) grad[p] = sums[2 + p] / used;
+  return 0;
 }
 
 // Tuning knobs (not part of the reference interface): bucket edges, FP32+IR

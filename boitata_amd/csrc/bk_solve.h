@@ -56,12 +56,31 @@ struct SolveArgs {
   double tol;           // FP32+IR: stop when the correction is below tol * |x|
   // Per-system diagnostics of the refined form (may be null): sweeps taken.
   int32_t* iters;
+  // Leave-one-out gradient: the sample held out by each system, and per
+  // system (loss, used, gradient[P]).
+  const int64_t* left_out_ids;
+  double* grad_part;
+};
+
+#define BK_MAX_GRAD (4 + 4 * BK_MAX_STRUCTURES)
+
+// The parameters of the leave-one-out gradient (layout in boitata_gpu.h).
+struct GradArgs {
+  int S;
+  int P;  // 4 + 4 S
+  int model[BK_MAX_STRUCTURES];
+  double sill[BK_MAX_STRUCTURES];
+  double r[BK_MAX_STRUCTURES][3];  // major, semi, minor ranges (m)
+  double R[9];                     // rotation_matrix(azimuth, dip, rake)
+  double dR[3][9];                 // its derivatives per degree of each angle
 };
 
 namespace bk {
 int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_guarded(hipStream_t s, const SolveArgs& a);
+int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax);
+int launch_reduce_rows(hipStream_t s, int64_t m, int cols, const double* part, double* out);
 int launch_lu(hipStream_t s, const SolveArgs& a);
 // Buckets targets by neighbour count after the search (status OK): lists[b]
 // holds targets with n in (edge[b-1], edge[b]]; others get their status.

@@ -472,6 +472,278 @@ __global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Leave-one-out cross-validation loss and its adjoint gradient (no
+// counterpart in the reference). Per held-out sample i with its fixed
+// neighbours, the bordered ordinary system K w = k (w = [lambda; mu],
+// k = [c; 1]) gives zhat = lambda'z and sigma2 = c0 - w'k. With the adjoint
+// v = K^-1 [z; 0] (one more solve on the same factor):
+//   d zhat   = v_l'(dc - dC lambda)
+//   d sigma2 = dc0 - 2 lambda'dc + lambda'dC lambda
+// and the loss l = (z_i - zhat)^2 + log sigma2 has
+//   dl = alpha d zhat + beta d sigma2,  alpha = -2 (z_i - zhat), beta = 1 / sigma2.
+// dC and dc are recomputed pair by pair from the parameters (never stored).
+
+template <int NMAX>
+__device__ __forceinline__ void group_sum_vec(double* v, int P) {
+  for (int p = 0; p < P; ++p) v[p] = group_sum<NMAX>(v[p]);
+}
+
+// dg/dt of the normalised shape (crates/variogram/src/model.rs:207-294).
+__device__ __forceinline__ double dshape64(int model, double t) {
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0 ? 0.0 : 1.5 - 1.5 * t * t;
+    case BK_EXPONENTIAL:
+      return 3.0 * exp(-3.0 * t);
+    case BK_GAUSSIAN:
+      return 6.0 * t * exp(-3.0 * t * t);
+    case BK_CUBIC:
+      return t >= 1.0 ? 0.0 : 14.0 * t - 26.25 * t * t + 17.5 * t * t * t * t - 5.25 * t * t * t * t * t * t;
+    case BK_PENTASPHERICAL:
+      return t >= 1.0 ? 0.0 : 1.875 - 3.75 * t * t + 1.875 * t * t * t * t;
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ double shape_t64(int model, double t) {
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0 ? 1.0 : 1.5 * t - 0.5 * t * t * t;
+    case BK_EXPONENTIAL:
+      return 1.0 - exp(-3.0 * t);
+    case BK_GAUSSIAN:
+      return 1.0 - exp(-3.0 * t * t);
+    case BK_CUBIC: {
+      if (t >= 1.0) return 1.0;
+      double t2 = t * t;
+      return 7.0 * t2 - 8.75 * t2 * t + 3.5 * t2 * t2 * t - 0.75 * t2 * t2 * t2 * t;
+    }
+    case BK_PENTASPHERICAL: {
+      if (t >= 1.0) return 1.0;
+      double t2 = t * t;
+      return 1.875 * t - 1.25 * t2 * t + 0.375 * t2 * t2 * t;
+    }
+  }
+  return 0.0;
+}
+
+// Gradient of C(p, q) with respect to the packed parameters, for world
+// separation d = q - p. Layout: [0] nugget, [1 + s] sill_s, [1 + S + 3 s + a]
+// range a (major, semi, minor) of structure s, [1 + 4 S + b] angle b (azimuth,
+// dip, rake; per degree). Coincident points: C = c0.
+__device__ __forceinline__ void cov_grad64(const GradArgs& gp, double d0, double d1, double d2, double coef,
+                                           double* acc) {
+  const int S = gp.S;
+  if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0) {
+    acc[0] += coef;
+    for (int s = 0; s < S; ++s) acc[1 + s] += coef;
+    return;
+  }
+  double r0, r1, r2;
+  mv3(gp.R, d0, d1, d2, r0, r1, r2);
+  double q[3][3];
+  for (int b = 0; b < 3; ++b) mv3(gp.dR[b], d0, d1, d2, q[b][0], q[b][1], q[b][2]);
+  for (int s = 0; s < S; ++s) {
+    const double* rr = gp.r[s];
+    double u0 = r0 / rr[0], u1 = r1 / rr[1], u2 = r2 / rr[2];
+    double t = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    double g = shape_t64(gp.model[s], t);
+    double gd = dshape64(gp.model[s], t);
+    acc[1 + s] += coef * (1.0 - g);
+    // dC/dt = -sill g'(t); dt/dr_a = -u_a^2 / (t r_a); dt/dphi = u . (dR_phi d / r) / t
+    double k = coef * gp.sill[s] * gd / t;
+    acc[1 + S + 3 * s + 0] += k * u0 * u0 / rr[0];
+    acc[1 + S + 3 * s + 1] += k * u1 * u1 / rr[1];
+    acc[1 + S + 3 * s + 2] += k * u2 * u2 / rr[2];
+    for (int b = 0; b < 3; ++b) {
+      double dt = u0 * q[b][0] / rr[0] + u1 * q[b][1] / rr[1] + u2 * q[b][2] / rr[2];
+      acc[1 + 4 * S + b] -= k * dt;
+    }
+  }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(128) k_loo_grad(const SolveArgs a, const GradArgs gp) {
+  constexpr int G = BK_WAVE / NMAX;
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int per = sys_doubles(NMAX, nf);
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int g = lane / NMAX, r = lane % NMAX;
+  double* S = dyn + (wave * G + g) * per;
+  double* L = S;
+  double* W = L + NMAX * (NMAX + 1) / 2;
+  double* V = W + 3 * NMAX;
+  double* E = V + NMAX;
+  double* PF = E + NMAX;
+  double* misc = PF + nf * 3 * NMAX;
+  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
+  const bool active = sys < a.n_list;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z = 0.0, e = 0.0;
+  bool guard = load_system<NMAX>(a, t, r, n, g, W, V, E, PF, z, e);
+  const bool go = active && !guard;
+  const double c0 = a.v.c0;
+  const int npairs = n * (n - 1) / 2;
+  if (go)
+    for (int q = r; q < npairs; q += NMAX) {
+      int i, j;
+      pair_of(q, i, j);
+      L[tri(i) + j] = pair_cov64(a, W, PF, NMAX, i, j);
+    }
+  double c = 0.0;
+  if (go && r < n) {
+    L[tri(r) + r] = c0 + e;
+    c = rhs64(a, t, W, PF, NMAX, r);
+  }
+  wave_sync();
+  double reg[NMAX];
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) reg[k] = (go && k <= r && r < n) ? L[tri(r) + k] : 0.0;
+  bool sing = false;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    const bool act = go && j < n;
+    double acc = -0.0;
+#pragma unroll
+    for (int k = 0; k < j; ++k) acc += reg[k] * L[tri(j) + k];
+    double s = reg[j] - acc;
+    if (act && r == j) misc[0] = s;
+    wave_sync();
+    double d = misc[0];
+    if (act && (isnan(d) || d <= PIVOT * c0)) sing = true;
+    if (act && !sing) {
+      d = sqrt(d);
+      if (r == j) {
+        reg[j] = d;
+        L[tri(j) + j] = d;
+      } else if (r > j && r < n) {
+        double x = s / d;
+        reg[j] = x;
+        L[tri(r) + j] = x;
+      }
+    }
+    wave_sync();
+  }
+  // Three right-hand sides: c, 1 and the data z.
+  double ya = c, yb = 1.0, yz = z, aa = -0.0, ab = -0.0, az = -0.0;
+#pragma unroll
+  for (int k = 0; k < NMAX; ++k) {
+    const bool act = go && !sing && k < n;
+    if (act && r == k) {
+      ya = (ya - aa) / reg[k];
+      yb = (yb - ab) / reg[k];
+      yz = (yz - az) / reg[k];
+      misc[1] = ya;
+      misc[2] = yb;
+      misc[3] = yz;
+    }
+    wave_sync();
+    if (act && r > k && r < n) {
+      aa += reg[k] * misc[1];
+      ab += reg[k] * misc[2];
+      az += reg[k] * misc[3];
+    }
+    wave_sync();
+  }
+  double xa = ya, xb = yb, xz = yz;
+#pragma unroll
+  for (int k = NMAX - 1; k >= 0; --k) {
+    const bool act = go && !sing && k < n;
+    if (act && r == k) {
+      xa = xa / reg[k];
+      xb = xb / reg[k];
+      xz = xz / reg[k];
+      misc[1] = xa;
+      misc[2] = xb;
+      misc[3] = xz;
+    }
+    wave_sync();
+    if (act && r < k) {
+      double lk = L[tri(k) + r];
+      xa -= lk * misc[1];
+      xb -= lk * misc[2];
+      xz -= lk * misc[3];
+    }
+    wave_sync();
+  }
+  const bool in = go && !sing && r < n;
+  double sa = group_sum<NMAX>(in ? xa : 0.0);
+  double sb = group_sum<NMAX>(in ? xb : 0.0);
+  double sz = group_sum<NMAX>(in ? xz : 0.0);
+  double mu = (sa - 1.0) / sb;
+  double vmu = sz / sb;
+  double lam = xa - mu * xb;
+  double vl = xz - vmu * xb;
+  double zhat = group_sum<NMAX>(in ? lam * z : 0.0);
+  double swc = group_sum<NMAX>(in ? lam * c : 0.0);
+  double sigma2 = c0 - swc - mu;
+  // The held-out value.
+  const int64_t held = a.left_out_ids ? a.left_out_ids[t] : t;
+  double zi = active ? a.sv[held] : 0.0;
+  const bool ok = go && !sing && sigma2 > 0.0 && n > 0;
+  double alpha = -2.0 * (zi - zhat), beta = 1.0 / sigma2;
+  if (ok && r < n) {
+    V[r] = lam;
+    E[r] = vl;
+  }
+  wave_sync();
+  double acc[BK_MAX_GRAD];
+  for (int p = 0; p < BK_MAX_GRAD; ++p) acc[p] = 0.0;
+  if (ok) {
+    for (int q = r; q < npairs; q += NMAX) {
+      int i, j;
+      pair_of(q, i, j);
+      double coef = -alpha * (E[i] * V[j] + E[j] * V[i]) + 2.0 * beta * V[i] * V[j];
+      cov_grad64(gp, W[i] - W[j], W[NMAX + i] - W[NMAX + j], W[2 * NMAX + i] - W[2 * NMAX + j], coef, acc);
+    }
+    if (r < n) {
+      double coefd = -alpha * vl * lam + beta * lam * lam;
+      acc[0] += coefd;
+      for (int s = 0; s < gp.S; ++s) acc[1 + s] += coefd;
+      double gr = alpha * vl - 2.0 * beta * lam;
+      cov_grad64(gp, a.tx[t] - W[r], a.ty[t] - W[NMAX + r], a.tz[t] - W[2 * NMAX + r], gr, acc);
+    }
+    if (r == 0) {
+      acc[0] += beta;
+      for (int s = 0; s < gp.S; ++s) acc[1 + s] += beta;
+    }
+  }
+  group_sum_vec<NMAX>(acc, gp.P);
+  if (active && r == 0) {
+    double* out = a.grad_part + (int64_t)t * (gp.P + 2);
+    if (ok) {
+      out[0] = (zi - zhat) * (zi - zhat) + log(sigma2);
+      out[1] = 1.0;
+      for (int p = 0; p < gp.P; ++p) out[2 + p] = acc[p];
+    } else {
+      for (int p = 0; p < gp.P + 2; ++p) out[p] = 0.0;
+    }
+    a.value[t] = ok ? zhat : __longlong_as_double(0x7ff8000000000000LL);
+    a.variance[t] = ok ? sigma2 : __longlong_as_double(0x7ff8000000000000LL);
+    a.status[t] = ok ? BK_OK : (guard ? BK_UNSUPPORTED : BK_SINGULAR);
+  }
+}
+
+// Sums grad_part[0..m) column by column in a fixed order (one block).
+__global__ void __launch_bounds__(256) k_reduce_rows(int64_t m, int cols, const double* part, double* out) {
+  __shared__ double red[256];
+  for (int col = 0; col < cols; ++col) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < m; i += 256) s += part[i * cols + col];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[col] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Systems with coincident samples (coincident.rs:27-112): merged on lane 0,
 // solved in FP64 by LU on the bordered system of the merged samples (the
 // merged system is small and rare), weights spread back to the members.
@@ -752,6 +1024,13 @@ __global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus
 
 namespace bk {
 
+// Dynamic LDS above the default needs the attribute; a refused request must
+// not linger as the runtime's last error.
+static void allow_lds(const void* fn, size_t bytes) {
+  if (bytes > 48 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipGetLastError();
+}
+
 static size_t shmem64(int nmax, int nf) { return (size_t)2 * (BK_WAVE / nmax) * sys_doubles(nmax, nf) * sizeof(double); }
 static size_t shmem32(int nmax, int nf) {
   return (size_t)2 * (BK_WAVE / nmax) * (sys_doubles(nmax, nf) + (nmax * (nmax + 1) / 2 + 1) / 2 + 2) * sizeof(double);
@@ -763,11 +1042,7 @@ static int run64(hipStream_t s, const SolveArgs& a) {
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
   size_t sh = shmem64(NMAX, a.v.nf);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_krige64<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  allow_lds((const void*)k_krige64<NMAX>, sh);
   k_krige64<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -778,12 +1053,36 @@ static int run32(hipStream_t s, const SolveArgs& a) {
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
   size_t sh = shmem32(NMAX, a.v.nf);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_krige32<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  allow_lds((const void*)k_krige32<NMAX>, sh);
   k_krige32<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <int NMAX>
+static int run_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp) {
+  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  int64_t blocks = (a.n_list + per_block - 1) / per_block;
+  if (blocks == 0) return 0;
+  size_t sh = shmem64(NMAX, a.v.nf);
+  allow_lds((const void*)k_loo_grad<NMAX>, sh);
+  k_loo_grad<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a, gp);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax) {
+  switch (nmax) {
+    case 16:
+      return run_grad<16>(s, a, gp);
+    case 32:
+      return run_grad<32>(s, a, gp);
+    case 64:
+      return run_grad<64>(s, a, gp);
+  }
+  return -1;
+}
+
+int launch_reduce_rows(hipStream_t s, int64_t m, int cols, const double* part, double* out) {
+  k_reduce_rows<<<1, 256, 0, s>>>(m, cols, part, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -816,11 +1115,7 @@ int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax) {
 int launch_guarded(hipStream_t s, const SolveArgs& a) {
   if (a.n_list <= 0) return 0;
   size_t sh = (size_t)(LU_MAX * LU_MAX + LU_MAX + 7 * LU_MAX) * sizeof(double) + (2 * LU_MAX + 4) * sizeof(int);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_guarded, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  allow_lds((const void*)k_guarded, sh);
   k_guarded<<<dim3((unsigned)a.n_list), dim3(64), sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

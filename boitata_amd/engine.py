@@ -159,6 +159,21 @@ class Engine:
         return dict(value=value, variance=variance, lagrange=lagrange, support=support, n_used=n_used,
                     status=status)
 
+    def loo_prepare(self, search, variogram=None):
+        """Fixes the leave-one-out neighbour sets of every sample for loo_loss_grad."""
+        vg = C.byref(variogram) if variogram is not None else None
+        self._check(self._lib.bk_loo_prepare(self._h, C.byref(search), vg))
+
+    def loo_loss_grad(self, variogram, precision=_abi.FP64):
+        """Leave-one-out loss and its gradient (4 + 4 S values) on the fixed sets."""
+        S = int(variogram.n_structures)
+        grad = np.zeros(4 + 4 * S)
+        loss = C.c_double()
+        used = C.c_int64()
+        self._check(self._lib.bk_loo_loss_grad(self._h, C.byref(variogram), int(precision), C.byref(loss),
+                                               _ptr(grad), C.byref(used)))
+        return loss.value, grad, used.value
+
     def loo(self, variogram, search, which=None, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64):
         which = _i64(which)
         m = self.n if which is None else int(len(which))

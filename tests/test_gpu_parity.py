@@ -1,0 +1,197 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Neighbour sets are compared bit for bit (indices in selection order);
+estimates and variances at 1e-10 relative for the FP64 form and 1e-6 for
+FP32 + refinement, relative with a floor of 1e-3 * sill (the floor of the
+reference's own close(), crates/estimation/src/kernel.rs:178-180)."""
+import numpy as np
+import pytest
+
+import oracle
+from boitata_amd import _abi, datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def sill_of(vg):
+    return vg.nugget + sum(vg.structures[k].sill for k in range(vg.n_structures))
+
+
+def assert_neighbours(engine, d, search, vg, T, left=None):
+    gi, gc, gs = engine.neighbors(search, vg, T[:, 0], T[:, 1], T[:, 2], left_out=left)
+    oi, oc, os_ = oracle.neighbors(d, search, vg, T, left_out=left)
+    assert (gs == os_).all(), np.nonzero(gs != os_)[0][:10]
+    assert (gc == oc).all()
+    for i in range(len(T)):
+        assert (gi[i, :oc[i]] == oi[i, :oc[i]]).all(), i
+
+
+def assert_estimates(g, o, vg, tol, keys=("value", "variance")):
+    assert (g["status"] == o["status"]).all(), (np.bincount(g["status"]), np.bincount(o["status"]))
+    ok = o["status"] == 0
+    floor = 1e-3 * sill_of(vg)
+    for k in keys:
+        a, b = np.asarray(g[k])[ok], np.asarray(o[k])[ok]
+        err = np.abs(a - b) / np.maximum(np.abs(b), floor)
+        assert err.max(initial=0.0) <= tol, (k, err.max())
+    assert np.isnan(np.asarray(g["value"])[~ok]).all()
+
+
+@pytest.fixture(scope="module")
+def c1_data():
+    w = datasets.config1()
+    return w, oracle.Data(w.x, w.y, w.z, w.value, None, w.hole), w.targets[::23].copy()
+
+
+@pytest.fixture
+def c1(engine, c1_data):
+    """Config 1, its samples loaded on the engine (other tests load others)."""
+    w = c1_data[0]
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    return c1_data
+
+
+def test_c1_neighbour_sets_bit_identical(engine, c1):
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    assert_neighbours(engine, d, w.search, w.variogram, T)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(octant=True),
+    dict(octant=True, max_per_octant=2, min_sectors=4, max_empty_sectors=2),
+    dict(max_per_hole=3),
+    dict(max_per_hole=2, octant=True, min_holes=3),
+    dict(sectors=6, max_per_sector=3, min_sectors=3),
+    dict(min_holes=3),
+])
+def test_capped_searches_bit_identical(engine, c1, kw):
+    w, d, T = c1
+    s = _abi.make_search(1, 24, 120.0, rotation=(30, 0, 0), ratios=(80 / 120, 30 / 120), **kw)
+    assert_neighbours(engine, d, s, w.variogram, T[:1200])
+
+
+def test_leave_one_out_neighbours(engine, c1):
+    w, d, _ = c1
+    idx = np.arange(0, w.n, 7)
+    P = np.stack([w.x, w.y, w.z], 1)[idx]
+    assert_neighbours(engine, d, w.search, w.variogram, P, left=idx)
+
+
+@pytest.mark.parametrize("kind,param", [(_abi.ORDINARY, 0.0), (_abi.SIMPLE, 1.0), (_abi.INDICATOR, 1.2)])
+@pytest.mark.parametrize("precision,tol", [(_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)])
+def test_c1_kriging(engine, c1, kind, param, precision, tol):
+    w, d, T = c1
+    g = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], kind, param, precision)
+    o = oracle.krige(d, w.variogram, w.search, T, kind, param)
+    assert_estimates(g, o, w.variogram, tol, ("value", "variance", "lagrange"))
+    assert (g["n_used"][o["status"] == 0] == o["n_used"][o["status"] == 0]).all()
+
+
+def test_c2_per_structure_ranges(engine):
+    w = datasets.config2(holes=(8, 8), target_counts=(40, 40, 10))
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    d = oracle.Data(w.x, w.y, w.z, w.value, None, w.hole)
+    T = w.targets[::3].copy()
+    assert_neighbours(engine, d, w.search, w.variogram, T[:2000])
+    for vg in (w.variogram, w.variogram_shared):
+        o = oracle.krige(d, vg, w.search, T)
+        for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+            assert_estimates(engine.krige(vg, w.search, T[:, 0], T[:, 1], T[:, 2], precision=prec), o, vg, tol)
+
+
+def test_c3_clustered_octants(engine):
+    w = datasets.config3(target_counts=(40, 40, 20))
+    engine.set_samples(w.x, w.y, w.z, w.value)
+    d = oracle.Data(w.x, w.y, w.z, w.value)
+    T = w.targets[::5].copy()
+    assert_neighbours(engine, d, w.search, w.variogram, T)
+    o = oracle.krige(d, w.variogram, w.search, T)
+    for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+        g = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=prec)
+        assert_estimates(g, o, w.variogram, tol)
+        counts = np.bincount(g["n_used"], minlength=65)
+        assert counts[:16].sum() > 0 and counts[33:].sum() > 0  # uneven neighbourhoods
+
+
+def test_edge_cases(engine):
+    rng = np.random.default_rng(8)
+    P = rng.uniform(0, 100, (300, 3))
+    v = rng.uniform(0, 5, 300)
+    err = np.where(rng.uniform(size=300) < 0.2, 0.3, 0.0)
+    # Coincident samples (merged, coincident.rs) and near ones.
+    P[10] = P[11]
+    P[20] = P[21] + 1e-8
+    P[30] = P[31] + 1e-3
+    vg = _abi.make_variogram([("spherical", 0.8, 60.0)], nugget=0.2, rotation=(20, 10, 5), ratios=(0.7, 0.5))
+    engine.set_samples(P[:, 0], P[:, 1], P[:, 2], v, err, None)
+    d = oracle.Data(P[:, 0], P[:, 1], P[:, 2], v, err)
+    T = np.concatenate([P[[10, 20, 30]] + 0.5, rng.uniform(-20, 120, (300, 3))])
+    for s in (_abi.make_search(1, 12, 40.0), _abi.make_search(6, 64, 80.0), _abi.make_search(1, 70, 200.0),
+              _abi.make_search(10, 16, 15.0)):
+        o = oracle.krige(d, vg, s, T)
+        for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+            assert_estimates(engine.krige(vg, s, T[:, 0], T[:, 1], T[:, 2], precision=prec), o, vg, tol)
+    # A model without a sill goes to the bordered LU.
+    pw = _abi.make_variogram([("power", 1.0, 50.0, 1.5)], nugget=0.1)
+    s = _abi.make_search(1, 16, 60.0)
+    o = oracle.krige(d, pw, s, T)
+    assert_estimates(engine.krige(pw, s, T[:, 0], T[:, 1], T[:, 2]), o, pw, 1e-9)
+
+
+def test_no_samples_and_invalid_parameters(engine):
+    engine.set_samples(np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
+    vg = _abi.make_variogram([("spherical", 1.0, 10.0)])
+    g = engine.krige(vg, _abi.make_search(1, 4, 10.0), np.zeros(3), np.zeros(3), np.zeros(3))
+    assert (g["status"] == _abi.SEARCH_FAILED).all() and np.isnan(g["value"]).all()
+    from boitata_amd.engine import EngineError
+
+    with pytest.raises(EngineError):
+        engine.krige(vg, _abi.make_search(5, 4, 10.0), np.zeros(1), np.zeros(1), np.zeros(1))
+    with pytest.raises(EngineError):
+        engine.krige(_abi.make_variogram([("power", 1.0, 10.0, 1.0)]), _abi.make_search(1, 4, 10.0), np.zeros(1),
+                     np.zeros(1), np.zeros(1), kind=_abi.SIMPLE)
+
+
+def test_block_kriging(engine):
+    w = datasets.config5(target_counts=(40, 40, 10), holes=(16, 16))
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    d = oracle.Data(w.x, w.y, w.z, w.value, None, w.hole)
+    T = w.targets[::7].copy()
+    o = oracle.block_krige(d, w.variogram, w.search, w.block, T)
+    # Gaussian covariance with a 1e-4 nugget and 1 cm twins: the LU of the
+    # reference and the Cholesky of the device agree to kappa * eps only.
+    for prec in (_abi.FP64, _abi.FP32_IR):
+        g = engine.block_krige(w.variogram, w.search, w.block, T[:, 0], T[:, 1], T[:, 2], precision=prec)
+        assert_estimates(g, o, w.variogram, 1e-6, ("value", "variance", "support"))
+
+
+def test_block_kriging_well_conditioned(engine, c1):
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    blk = _abi.make_block((10, 10, 10), (4, 4, 2), rotation=(15, 0, 0))
+    o = oracle.block_krige(d, w.variogram, w.search, blk, T)
+    for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+        g = engine.block_krige(w.variogram, w.search, blk, T[:, 0], T[:, 1], T[:, 2], precision=prec)
+        assert_estimates(g, o, w.variogram, tol, ("value", "variance", "support"))
+
+
+def test_leave_one_out(engine, c1):
+    w, d, _ = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    which = np.arange(0, w.n, 3)
+    o = oracle.loo(d, w.variogram, w.search, which)
+    for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+        g = engine.loo(w.variogram, w.search, which, precision=prec)
+        assert_estimates(g, o, w.variogram, tol)
+
+
+def test_results_are_deterministic(engine, c1):
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    a = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
+    engine.tuning(cell_factor=1.7)
+    b = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
+    engine.tuning(cell_factor=1.0)
+    assert (a["value"].view(np.uint64) == b["value"].view(np.uint64)).all()
+    assert (a["variance"].view(np.uint64) == b["variance"].view(np.uint64)).all()
