@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--targets", type=int, default=0, help="use the first N targets only (0: all)")
+    ap.add_argument("--loo-steps", type=int, default=50, help="Adam steps of the LOO-CV gradient leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine shards targets across GPUs "
@@ -151,6 +152,36 @@ def main():
                  solve32_tflops=flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else None,
                  solve64_tflops=flops / (st64["solve"] * 1e-3) / 1e12 if st64["solve"] > 0 else None)
 
+    # Second half of the metric: LOO-CV gradient steps/s on 200,000 samples of
+    # the same generator (800 holes), 12 parameters, fixed neighbour sets of
+    # the explicit 400/250/80 ellipsoid (<= 32 neighbours), Adam from
+    # parameters 30% off the generating values.
+    from boitata_amd import loocv
+
+    loo_line = None
+    if args.loo_steps > 0:
+        w4 = datasets.config4()
+        models = ["exponential", "spherical"]
+        e.set_samples(tt(w4.x), tt(w4.y), tt(w4.z), tt(w4.value), None, tt(w4.hole.astype(np.int32)))
+        e.loo_prepare(w4.search, None)
+        th_true = loocv.theta_of(w4.variogram)
+        sign = np.random.default_rng(4).choice([-1.0, 1.0], len(th_true))
+        th0 = th_true * (1.0 + 0.3 * sign)
+        th0[-3:] = th_true[-3:] + 0.3 * sign[-3:] * np.array([30.0, 20.0, 10.0])  # angles: degrees off
+        loocv.loss_and_grad(e, th0, models)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        th, hist = loocv.fit_adam(e, th0, models, steps=args.loo_steps, lr=0.02)
+        torch.cuda.synchronize()
+        dtl = time.perf_counter() - t0
+        loss, grad, used = loocv.loss_and_grad(e, th, models)
+        loo_line = {"value": args.loo_steps / dtl, "unit": "Adam steps/s", "steps": args.loo_steps,
+                    "ms_per_step": dtl / args.loo_steps * 1e3, "samples": w4.n, "systems_per_step": used,
+                    "loss_first": hist[0], "loss_last": hist[-1], "precision": "fp64 (adjoint on the FP64 factor)"}
+        print(f"[bench] LOO-CV: {loo_line['value']:.1f} Adam steps/s on {w4.n} samples, loss {hist[0]:.4f} -> "
+              f"{hist[-1]:.4f}", file=sys.stderr, flush=True)
+        e.set_samples(sx, sy, sz, sv, None, hole)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -196,6 +227,7 @@ def main():
         "roofline": roof,
         "roofline_other": other,
         "cpu_baseline": cpu,
+        "loo_cv_grad": loo_line,
     }
     print(json.dumps(line), flush=True)
     e.close()

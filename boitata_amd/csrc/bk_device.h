@@ -194,6 +194,110 @@ __device__ __forceinline__ double block_cov_points64(const DevVariogram& v, doub
 }
 
 // ---------------------------------------------------------------------------
+// Fast FP64 covariance for the refined path: relative error about 1e-15
+// (the refinement needs ~1e-10), with no IEEE division or correctly rounded
+// square root. exp by Cody-Waite reduction x = k ln2 + r, |r| <= ln2 / 2, and
+// a degree-12 Taylor polynomial of exp(r) (truncation below 3e-17).
+__device__ __forceinline__ double exp_fast(double x) {
+#pragma clang fp contract(fast)
+  if (x < -708.0) return 0.0;
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  double k = rint(x * 1.44269504088896338700);
+  double r = (x - k * ln2_hi) - k * ln2_lo;
+  double p = 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+
+// sqrt(x) from the hardware reciprocal square root and one Newton step.
+__device__ __forceinline__ double sqrt_fast(double x) {
+#pragma clang fp contract(fast)
+  if (!(x > 0.0)) return 0.0;
+  double y = __builtin_amdgcn_rsq(x);
+  double s = x * y;
+  double e = fma(-s, s, x);  // x - s^2
+  return s + 0.5 * e * y;
+}
+
+__device__ __forceinline__ double shape_fast64(int model, double param, double h, double inv_r) {
+#pragma clang fp contract(fast)
+  if (!(h > 0.0)) return 0.0;
+  double t = h * inv_r;
+  switch (model) {
+    case BK_SPHERICAL:
+      return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t);
+    case BK_EXPONENTIAL:
+      return 1.0 - exp_fast(-3.0 * t);
+    case BK_GAUSSIAN:
+      return 1.0 - exp_fast(-3.0 * t * t);
+    case BK_CUBIC: {
+      if (t >= 1.0) return 1.0;
+      double t2 = t * t;
+      return t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
+    }
+    case BK_PENTASPHERICAL: {
+      if (t >= 1.0) return 1.0;
+      double t2 = t * t;
+      return t * (1.875 + t2 * (-1.25 + 0.375 * t2));
+    }
+    case BK_MATERN: {
+      double nu = param < 1.0 ? 0.5 : (param < 2.0 ? 1.5 : 2.5);
+      double delta = (nu == 0.5 ? 3.0 : (nu == 1.5 ? 5.196152422706632 : 6.708203932499369)) * t;
+      double e = exp_fast(-delta);
+      if (nu == 0.5) return 1.0 - e;
+      if (nu == 1.5) return 1.0 - (1.0 + delta) * e;
+      return 1.0 - (1.0 + delta + delta * delta * (1.0 / 3.0)) * e;
+    }
+  }
+  // Circular, sine hole, power: the library forms.
+  return shape64(model, param, h, 1.0 / inv_r);
+}
+
+__device__ __forceinline__ double cov_fast64(const DevVariogram& v, const double* h) {
+#pragma clang fp contract(fast)
+  if (!(h[v.nug_frame] > 0.0)) return v.c0;
+  double s = 0.0;
+  for (int k = 0; k < v.S; ++k) s += v.sill[k] * shape_fast64(v.model[k], v.param[k], h[v.frame[k]], v.inv_range[k]);
+  return v.c0 - (v.nugget + s);
+}
+
+__device__ __forceinline__ double norm3_fast(double a, double b, double c) {
+#pragma clang fp contract(fast)
+  return sqrt_fast(a * a + b * b + c * c);
+}
+
+// Anisotropy::lag per frame (aniso.rs:101-111) in the fast arithmetic.
+__device__ __forceinline__ void lags_world_fast(const DevVariogram& v, double vx, double vy, double vz, double* h) {
+#pragma clang fp contract(fast)
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  bool rotated = false;
+  for (int f = 0; f < v.nf; ++f) {
+    if (v.ident[f]) {
+      h[f] = norm3_fast(vx, vy, vz);
+      continue;
+    }
+    if (!rotated) {
+      r0 = v.rot[0] * vx + v.rot[1] * vy + v.rot[2] * vz;
+      r1 = v.rot[3] * vx + v.rot[4] * vy + v.rot[5] * vz;
+      r2 = v.rot[6] * vx + v.rot[7] * vy + v.rot[8] * vz;
+      rotated = true;
+    }
+    h[f] = norm3_fast(r0 / v.lag_major[f], r1 / v.lag_semi[f], r2 / v.lag_minor[f]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP32 forms for the refined path (contraction allowed, native transcendentals).
 struct DevVariogram32 {
   int S, nf, nug_frame;

@@ -926,7 +926,7 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
   k_cell_count<<<dim3((unsigned)blocks), dim3(T), 0, s>>>(n, px, py, pz, lo[0], lo[1], lo[2], h, dims[0], dims[1],
                                                          dims[2], gb.cell_of, gb.fill);
   size_t need = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, need, gb.fill, gb.start, (int)(cells + 1), s);
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, gb.fill, gb.start, (int)(cells + 1), s));
   if (need > gb.scan_tmp_bytes) {
     (void)hipFree(gb.scan_tmp);
     HIPCHK(hipMalloc(&gb.scan_tmp, need));

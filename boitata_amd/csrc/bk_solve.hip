@@ -46,6 +46,32 @@ __device__ __forceinline__ double group_max(double v) {
   return v;
 }
 
+// The value of lane k of this lane's group (k a compile-time constant after
+// unrolling): one v_readlane per group, no LDS round trip.
+template <int NMAX>
+__device__ __forceinline__ float group_bcastf(float v, int k, int g) {
+  constexpr int G = BK_WAVE / NMAX;
+  float out = 0.0f;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    float t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), gg * NMAX + k));
+    out = (g == gg) ? t : out;
+  }
+  return out;
+}
+
+template <int NMAX>
+__device__ __forceinline__ double group_bcast(double v, int k, int g) {
+  constexpr int G = BK_WAVE / NMAX;
+  double out = 0.0;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    double t = bcast(v, gg * NMAX + k);
+    out = (g == gg) ? t : out;
+  }
+  return out;
+}
+
 template <int NMAX>
 __device__ __forceinline__ bool group_any(bool p, int g) {
   uint64_t m = __ballot(p);
@@ -170,6 +196,41 @@ __device__ __forceinline__ double rhs64(const SolveArgs& a, int t, const double*
   return cov64(a.v, h);
 }
 
+// The same two covariances in the fast FP64 arithmetic of the refined form.
+__device__ __forceinline__ double pair_cov_fast(const SolveArgs& a, const double* W, const double* PF, int nmax,
+                                                int i, int j) {
+  double h[BK_MAX_FRAMES];
+  if (a.block) {
+    lags_world_fast(a.v, W[j] - W[i], W[nmax + j] - W[nmax + i], W[2 * nmax + j] - W[2 * nmax + i], h);
+    return cov_fast64(a.v, h);
+  }
+  for (int f = 0; f < a.v.nf; ++f) {
+    const double* P = PF + f * 3 * nmax;
+    h[f] = norm3_fast(P[i] - P[j], P[nmax + i] - P[nmax + j], P[2 * nmax + i] - P[2 * nmax + j]);
+  }
+  return cov_fast64(a.v, h);
+}
+
+__device__ __forceinline__ double rhs_fast(const SolveArgs& a, int t, const double* W, const double* PF, int nmax,
+                                           int r) {
+  double h[BK_MAX_FRAMES];
+  if (a.block) {
+    double x = W[r], y = W[nmax + r], z = W[2 * nmax + r];
+    double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+    double acc = 0.0;
+    for (int p = 0; p < a.npts; ++p) {
+      lags_world_fast(a.v, (t0 + a.offs[3 * p]) - x, (t1 + a.offs[3 * p + 1]) - y, (t2 + a.offs[3 * p + 2]) - z, h);
+      acc += (a.npts > 1 && !(h[a.v.nug_frame] > 0.0)) ? a.v.c0 - a.v.nugget : cov_fast64(a.v, h);
+    }
+    return acc / (double)a.npts;
+  }
+  for (int f = 0; f < a.v.nf; ++f) {
+    const double* P = PF + f * 3 * nmax;
+    h[f] = norm3_fast(P[r], P[nmax + r], P[2 * nmax + r]);
+  }
+  return cov_fast64(a.v, h);
+}
+
 // ---------------------------------------------------------------------------
 // FP64, the reference's operation order.
 template <int NMAX>
@@ -226,9 +287,7 @@ __global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
 #pragma unroll
     for (int k = 0; k < j; ++k) acc += reg[k] * L[tri(j) + k];
     double s = reg[j] - acc;
-    if (act && r == j) misc[0] = s;
-    wave_sync();
-    double d = misc[0];
+    double d = group_bcast<NMAX>(s, j, g);
     if (act && (isnan(d) || d <= PIVOT * c0)) sing = true;
     if (act && !sing) {
       d = sqrt(d);
@@ -243,7 +302,9 @@ __global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
     }
     wave_sync();
   }
-  // Forward solves L y = c and L y = 1 (kernel.rs:73-78), column by column.
+  (void)misc;
+  // Forward solves L y = c and L y = 1 (kernel.rs:73-78), column by column;
+  // each new y_k goes lane to lane by readlane.
   const bool ordinary = a.kind != BK_SIMPLE;
   double ya = c, yb = 1.0, acca = -0.0, accb = -0.0;
 #pragma unroll
@@ -252,15 +313,12 @@ __global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
     if (act && r == k) {
       ya = (ya - acca) / reg[k];
       yb = (yb - accb) / reg[k];
-      misc[1] = ya;
-      misc[2] = yb;
     }
-    wave_sync();
+    double ba = group_bcast<NMAX>(ya, k, g), bb = group_bcast<NMAX>(yb, k, g);
     if (act && r > k && r < n) {
-      acca += reg[k] * misc[1];
-      accb += reg[k] * misc[2];
+      acca += reg[k] * ba;
+      accb += reg[k] * bb;
     }
-    wave_sync();
   }
   // Back solves L' x = y (kernel.rs:79-84), row k of the factor from LDS.
   double xa = ya, xb = yb;
@@ -270,16 +328,13 @@ __global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
     if (act && r == k) {
       xa = xa / reg[k];
       xb = xb / reg[k];
-      misc[1] = xa;
-      misc[2] = xb;
     }
-    wave_sync();
+    double ba = group_bcast<NMAX>(xa, k, g), bb = group_bcast<NMAX>(xb, k, g);
     if (act && r < k) {
       double lk = L[tri(k) + r];
-      xa -= lk * misc[1];
-      xb -= lk * misc[2];
+      xa -= lk * ba;
+      xb -= lk * bb;
     }
-    wave_sync();
   }
   // Schur scalar and weights (kernel.rs:86-97).
   const bool in = go && !sing && r < n;
@@ -328,7 +383,7 @@ __global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
     for (int q = r; q < npairs; q += NMAX) {
       int i, j;
       pair_of(q, i, j);
-      double cij = pair_cov64(a, W, PF, NMAX, i, j);
+      double cij = pair_cov_fast(a, W, PF, NMAX, i, j);
       C[tri(i) + j] = cij;
       L[tri(i) + j] = (float)cij;
     }
@@ -336,32 +391,38 @@ __global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
   if (go && r < n) {
     C[tri(r) + r] = c0 + e;
     L[tri(r) + r] = (float)(c0 + e);
-    c = rhs64(a, t, W, PF, NMAX, r);
+    c = rhs_fast(a, t, W, PF, NMAX, r);
   }
   wave_sync();
   float reg[NMAX];
 #pragma unroll
   for (int k = 0; k < NMAX; ++k) reg[k] = (go && k <= r && r < n) ? L[tri(r) + k] : 0.0f;
-  // FP32 Cholesky. A pivot that loses most of its bits sends the system to FP64.
+  (void)miscf;
+  // FP32 Cholesky, left-looking; the pivot goes lane to lane by readlane. A
+  // pivot that loses most of its bits sends the system to FP64.
   bool bad = false;
+  float invd = 0.0f;  // 1 / L[r][r]
   const float c0f = (float)c0;
   {
 #pragma clang fp contract(fast)
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) {
       const bool act = go && j < n;
-      float acc = 0.0f;
+      float acc0 = 0.0f, acc1 = 0.0f;
 #pragma unroll
-      for (int k = 0; k < j; ++k) acc += reg[k] * L[tri(j) + k];
-      float s = reg[j] - acc;
-      if (act && r == j) miscf[0] = s;
-      wave_sync();
-      float d = miscf[0];
+      for (int k = 0; k + 1 < j; k += 2) {
+        acc0 += reg[k] * L[tri(j) + k];
+        acc1 += reg[k + 1] * L[tri(j) + k + 1];
+      }
+      if (j & 1) acc0 += reg[j - 1] * L[tri(j) + j - 1];
+      float s = reg[j] - (acc0 + acc1);
+      float d = group_bcastf<NMAX>(s, j, g);
       if (act && !(d > 1e-5f * c0f)) bad = true;
       if (act && !bad) {
         float inv = rsqrtf(d);
         if (r == j) {
           reg[j] = d * inv;
+          invd = inv;
           L[tri(j) + j] = reg[j];
         } else if (r > j && r < n) {
           float x = s * inv;
@@ -372,44 +433,37 @@ __global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
       wave_sync();
     }
   }
-  // Solve with the FP32 factor: (L L') dx = res, on both right-hand sides.
+  // Solve with the FP32 factor, (L L') v = rhs on both right-hand sides:
+  // column sweeps, values passed by readlane; the back sweep reads row k of L.
   auto solve32 = [&](float& va, float& vb) {
 #pragma clang fp contract(fast)
     float acca = 0.0f, accb = 0.0f;
 #pragma unroll
     for (int k = 0; k < NMAX; ++k) {
       const bool act = go && !bad && k < n;
-      if (act && r == k) {
-        float inv = 1.0f / reg[k];
-        va = (va - acca) * inv;
-        vb = (vb - accb) * inv;
-        miscf[1] = va;
-        miscf[2] = vb;
+      if (r == k) {
+        va = (va - acca) * invd;
+        vb = (vb - accb) * invd;
       }
-      wave_sync();
+      float ba = group_bcastf<NMAX>(va, k, g), bb = group_bcastf<NMAX>(vb, k, g);
       if (act && r > k && r < n) {
-        acca += reg[k] * miscf[1];
-        accb += reg[k] * miscf[2];
+        acca += reg[k] * ba;
+        accb += reg[k] * bb;
       }
-      wave_sync();
     }
 #pragma unroll
     for (int k = NMAX - 1; k >= 0; --k) {
       const bool act = go && !bad && k < n;
-      if (act && r == k) {
-        float inv = 1.0f / reg[k];
-        va = va * inv;
-        vb = vb * inv;
-        miscf[1] = va;
-        miscf[2] = vb;
+      if (r == k) {
+        va = va * invd;
+        vb = vb * invd;
       }
-      wave_sync();
+      float ba = group_bcastf<NMAX>(va, k, g), bb = group_bcastf<NMAX>(vb, k, g);
       if (act && r < k) {
         float lk = L[tri(k) + r];
-        va -= lk * miscf[1];
-        vb -= lk * miscf[2];
+        va -= lk * ba;
+        vb -= lk * bb;
       }
-      wave_sync();
     }
   };
   const bool ordinary = a.kind != BK_SIMPLE;
