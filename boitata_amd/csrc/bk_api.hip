@@ -133,8 +133,12 @@ void accumulate(bk_ctx* c) {
 
 namespace {
 
+// Message of the last failure of a call made without a context (validation
+// before bk_create, or bk_create itself), per thread.
+thread_local std::string g_err;
+
 int fail(bk_ctx* c, int code, const std::string& m) {
-  if (c) c->err = m;
+  (c ? c->err : g_err) = m;
   return code;
 }
 
@@ -705,23 +709,22 @@ extern "C" {
 const char* bk_version(void) { return "boitata-mi355x 0.1.0 (gfx950)"; }
 
 int bk_create(bk_ctx** out, int device) {
-  if (!out) return BK_E_INVALID;
+  if (!out) return fail(nullptr, BK_E_INVALID, "no output pointer");
   *out = nullptr;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess)
+    return fail(nullptr, BK_E_HIP, std::string("device ") + std::to_string(device) + ": " + hipGetErrorString(e));
   bk_ctx* c = new bk_ctx();
   c->device = device;
-  hipError_t e = hipSetDevice(device);
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
-    return BK_E_HIP;
-  }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return BK_E_HIP;
+    return fail(nullptr, BK_E_HIP, std::string("stream: ") + hipGetErrorString(e));
   }
   if (bk::upload_shell_table(c->tab, &c->tab_storage)) {
     (void)hipStreamDestroy(c->stream);
     delete c;
-    return BK_E_HIP;
+    return fail(nullptr, BK_E_HIP, "shell table upload failed");
   }
   *out = c;
   return 0;
@@ -744,7 +747,7 @@ void bk_destroy(bk_ctx* c) {
   delete c;
 }
 
-const char* bk_last_error(const bk_ctx* c) { return c ? c->err.c_str() : "no context"; }
+const char* bk_last_error(const bk_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
 int bk_check_search(bk_ctx* c, const bk_search* s) {
   if (!s) return fail(c, BK_E_INVALID, "no search");
@@ -1012,6 +1015,7 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
   std::memset(&gp, 0, sizeof(gp));
   gp.S = vg->n_structures;
   gp.P = 4 + 4 * gp.S;
+  gp.m = m;
   double az = vg->has_anisotropy ? vg->azimuth : 0.0, dip = vg->has_anisotropy ? vg->dip : 0.0,
          rk = vg->has_anisotropy ? vg->rake : 0.0;
   for (int k = 0; k < gp.S; ++k) {
@@ -1084,8 +1088,9 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     if (bk::launch_loo_grad(c->stream, a, gp, c->edges[b])) return fail(c, BK_E_HIP, "gradient launch failed");
   }
   // Systems beyond the Cholesky kernels (n > 64) stay out of the loss.
-  if (int r = ensure(c, c->offs, 64 * sizeof(double))) return r;
-  if (bk::launch_reduce_rows(c->stream, m, cols, a.grad_part, (double*)c->offs.p))
+  const int64_t rb = bk::reduce_blocks(m);
+  if (int r = ensure(c, c->offs, (64 + cols * rb) * sizeof(double))) return r;
+  if (bk::launch_reduce_cols(c->stream, m, cols, a.grad_part, (double*)c->offs.p + 64, (double*)c->offs.p))
     return fail(c, BK_E_HIP, "reduction launch failed");
   double sums[BK_MAX_GRAD + 2];
   HIPOK(c, hipMemcpyAsync(sums, c->offs.p, cols * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -67,7 +67,8 @@ struct SolveArgs {
 // The parameters of the leave-one-out gradient (layout in boitata_gpu.h).
 struct GradArgs {
   int S;
-  int P;  // 4 + 4 S
+  int P;     // 4 + 4 S
+  int64_t m; // held-out samples: the column length of grad_part
   int model[BK_MAX_STRUCTURES];
   double sill[BK_MAX_STRUCTURES];
   double r[BK_MAX_STRUCTURES][3];  // major, semi, minor ranges (m)
@@ -80,7 +81,11 @@ int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_guarded(hipStream_t s, const SolveArgs& a);
 int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax);
-int launch_reduce_rows(hipStream_t s, int64_t m, int cols, const double* part, double* out);
+// Column sums of part (cols columns of m values each, column-major) into
+// out[0..cols), in an order fixed by m alone; work holds
+// cols * reduce_blocks(m) doubles.
+int64_t reduce_blocks(int64_t m);
+int launch_reduce_cols(hipStream_t s, int64_t m, int cols, const double* part, double* work, double* out);
 int launch_lu(hipStream_t s, const SolveArgs& a);
 // Buckets targets by neighbour count after the search (status OK): lists[b]
 // holds targets with n in (edge[b-1], edge[b]]; others get their status.

@@ -766,13 +766,15 @@ __global__ void __launch_bounds__(128) k_loo_grad(const SolveArgs a, const GradA
   }
   group_sum_vec<NMAX>(acc, gp.P);
   if (active && r == 0) {
-    double* out = a.grad_part + (int64_t)t * (gp.P + 2);
+    // Column-major partials: column p holds entry p of every held-out sample.
+    double* out = a.grad_part + t;
+    const int64_t m = gp.m;
     if (ok) {
       out[0] = (zi - zhat) * (zi - zhat) + log(sigma2);
-      out[1] = 1.0;
-      for (int p = 0; p < gp.P; ++p) out[2 + p] = acc[p];
+      out[m] = 1.0;
+      for (int p = 0; p < gp.P; ++p) out[(2 + p) * m] = acc[p];
     } else {
-      for (int p = 0; p < gp.P + 2; ++p) out[p] = 0.0;
+      for (int p = 0; p < gp.P + 2; ++p) out[p * m] = 0.0;
     }
     a.value[t] = ok ? zhat : __longlong_as_double(0x7ff8000000000000LL);
     a.variance[t] = ok ? sigma2 : __longlong_as_double(0x7ff8000000000000LL);
@@ -780,21 +782,41 @@ __global__ void __launch_bounds__(128) k_loo_grad(const SolveArgs a, const GradA
   }
 }
 
-// Sums grad_part[0..m) column by column in a fixed order (one block).
-__global__ void __launch_bounds__(256) k_reduce_rows(int64_t m, int cols, const double* part, double* out) {
-  __shared__ double red[256];
-  for (int col = 0; col < cols; ++col) {
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < m; i += 256) s += part[i * cols + col];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[col] = red[0];
+// Deterministic column sums of a column-major m x cols array: block
+// (b, col) sums rows [b*chunk, (b+1)*chunk) of its column (coalesced, fixed
+// thread order, LDS tree), then one block per column adds the partials in
+// block order. The order depends on m only.
+constexpr int RED_T = 256;
+constexpr int64_t RED_CHUNK = 4096;
+
+__device__ double block_tree(double s, double* red) {
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = RED_T / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
+  return red[0];
+}
+
+__global__ void __launch_bounds__(RED_T) k_reduce_part(int64_t m, const double* part, double* work) {
+  __shared__ double red[RED_T];
+  const int64_t nb = gridDim.x, b = blockIdx.x, col = blockIdx.y;
+  const double* p = part + col * m;
+  const int64_t lo = b * RED_CHUNK, hi = lo + RED_CHUNK < m ? lo + RED_CHUNK : m;
+  double s = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += RED_T) s += p[i];
+  s = block_tree(s, red);
+  if (threadIdx.x == 0) work[col * nb + b] = s;
+}
+
+__global__ void __launch_bounds__(RED_T) k_reduce_final(int64_t nb, const double* work, double* out) {
+  __shared__ double red[RED_T];
+  const int col = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += RED_T) s += work[col * nb + i];
+  s = block_tree(s, red);
+  if (threadIdx.x == 0) out[col] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -1135,8 +1157,12 @@ int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int n
   return -1;
 }
 
-int launch_reduce_rows(hipStream_t s, int64_t m, int cols, const double* part, double* out) {
-  k_reduce_rows<<<1, 256, 0, s>>>(m, cols, part, out);
+int64_t reduce_blocks(int64_t m) { return m <= 0 ? 1 : (m + RED_CHUNK - 1) / RED_CHUNK; }
+
+int launch_reduce_cols(hipStream_t s, int64_t m, int cols, const double* part, double* work, double* out) {
+  const int64_t nb = reduce_blocks(m);
+  k_reduce_part<<<dim3((unsigned)nb, (unsigned)cols), dim3(RED_T), 0, s>>>(m, part, work);
+  k_reduce_final<<<dim3((unsigned)cols), dim3(RED_T), 0, s>>>(nb, work, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

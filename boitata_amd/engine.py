@@ -14,7 +14,11 @@ from ._lib import load
 
 
 class EngineError(RuntimeError):
-    """A call of the C ABI returned an error."""
+    """A call of the C ABI returned an error; `code` is the BK_E_* value."""
+
+    def __init__(self, message, code=0):
+        super().__init__(message)
+        self.code = code
 
 
 def _ptr(a):
@@ -86,7 +90,7 @@ class Engine:
     def _check(self, rc):
         if rc != 0:
             msg = self._lib.bk_last_error(self._h).decode()
-            raise EngineError(f"{msg} ({rc})")
+            raise EngineError(f"{msg} ({rc})", rc)
 
     def set_samples(self, x, y, z, value, error_variance=None, hole=None):
         x, y, z, value = _f64(x), _f64(y), _f64(z), _f64(value)

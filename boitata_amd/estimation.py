@@ -1,0 +1,502 @@
+"""Drop-in estimators on the MI355X engine.
+
+The classes keep the constructor, ``fit``, ``predict`` and ``cross_validate``
+signatures of the reference's Python API (python/boitata/estimation.py:251-485,
+through ``_Estimator``, crates/py/src/estimation.rs:1213-1866) for the methods
+the engine covers: ``OrdinaryKriging``, ``SimpleKriging``,
+``IndicatorKriging`` and ``BlockKriging``; and of ``Variogram``
+(crates/py/src/variogram.rs:329-362), ``Structure`` (:211) and ``Search``
+(crates/py/src/estimation.rs:415-531).
+
+Behaviour kept from the reference:
+
+* ``fit`` keeps the first of samples sharing a location bit for bit (after
+  ``+ 0.0``), with a ``UserWarning`` naming their holes or rows
+  (crates/py/src/args.rs:305-363, crates/eda/src/lib.rs:952 at tolerance 0);
+  hole labels become integer codes in order of first appearance, an unlabelled
+  sample gets a code of its own (args.rs:278-292).
+* A sequence of searches is a list of passes: targets a pass leaves
+  unestimated (any per-target failure) go to the next (batch.rs:132-147).
+* Per-target failures are NaN; parameters out of range raise InvalidInput.
+* ``(n, 2)`` coordinates lie on z = 0; coordinates must be finite
+  (crates/py/src/containers.rs:15-21).
+
+Not covered (raise NotImplementedError): domains, local anisotropy,
+diagnostics tables, k-fold and block cross-validation, units, soft / high-grade data and
+calibrated searches, and the other estimation methods.
+
+Extension: ``precision="fp32_ir"`` solves in FP32 with FP64 refinement; the
+default ``"fp64"`` is the reference's arithmetic.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+from collections.abc import Sequence
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .errors import InvalidInput
+
+__all__ = [
+    "BlockKriging",
+    "CrossValidation",
+    "IndicatorKriging",
+    "InvalidInput",
+    "OrdinaryKriging",
+    "Search",
+    "SimpleKriging",
+    "Structure",
+    "Variogram",
+]
+
+_PRECISIONS = {"fp64": _abi.FP64, "fp32_ir": _abi.FP32_IR}
+
+
+def _lib():
+    from ._lib import load
+
+    return load()
+
+
+def _checked(fn, obj, what):
+    lib = _lib()
+    import ctypes as C
+
+    if fn(lib)(None, C.byref(obj)) != 0:
+        raise InvalidInput(f"{what}: {lib.bk_last_error(None).decode()}")
+
+
+def _number(v, what):
+    if isinstance(v, str):
+        raise NotImplementedError(f"{what} with a unit is not supported by this engine")
+    return float(v)
+
+
+class Structure:
+    """One nested structure: `sill` is the partial sill, `range` the range
+    along the major axis; `order` for matern, `exponent` for power
+    (crates/py/src/variogram.rs:211). `ratios` (extension) gives the structure
+    its own (semi/major, minor/major) ratios under the variogram's rotation."""
+
+    def __init__(self, model: str, sill: float, range, *, order=None, exponent=None, ratios=None):
+        if model not in _abi.MODELS:
+            raise InvalidInput(f"unknown model {model!r}; one of {', '.join(_abi.MODELS)}")
+        if order is not None and model != "matern":
+            raise InvalidInput("order is for the matern model")
+        if exponent is not None and model != "power":
+            raise InvalidInput("exponent is for the power model")
+        if model == "matern" and order is None:
+            order = 0.5
+        if model == "power" and exponent is None:
+            exponent = 1.0
+        self.model = model
+        self.sill = float(sill)
+        self.range = _number(range, "range")
+        self.param = order if model == "matern" else exponent
+        self.ratios = None if ratios is None else (float(ratios[0]), float(ratios[1]))
+
+    def _tuple(self):
+        return (self.model, self.sill, self.range, self.param, self.ratios)
+
+    def __repr__(self):
+        return f"Structure({self.model!r}, {self.sill}, {self.range})"
+
+
+class Variogram:
+    """Nugget plus nested structures sharing one anisotropy: `rotation` is
+    (azimuth, dip, rake) in degrees, `ratios` (semi/major, minor/major)
+    (crates/py/src/variogram.rs:329-362). Structures are Structure objects or
+    (model, sill, range) tuples."""
+
+    def __init__(self, structures, *, nugget=0.0, rotation=(0.0, 0.0, 0.0), ratios=(1.0, 1.0), length_unit=None):
+        if length_unit is not None:
+            raise NotImplementedError("length units are not supported by this engine")
+        nugget = float(nugget)
+        if not (nugget >= 0.0 and math.isfinite(nugget)):
+            raise InvalidInput("nugget must be finite and >= 0")
+        parsed = []
+        for s in structures:
+            if isinstance(s, Structure):
+                parsed.append(s)
+                continue
+            try:
+                name, sill, rng = s
+            except (TypeError, ValueError):
+                raise InvalidInput("structures are Structure or (model, sill, range) tuples") from None
+            parsed.append(Structure(name, sill, rng))
+        if len(parsed) > _abi.MAX_STRUCTURES:
+            raise InvalidInput(f"at most {_abi.MAX_STRUCTURES} structures")
+        self.structures = parsed
+        self.nugget = nugget
+        self.rotation = tuple(float(a) for a in rotation)
+        self.ratios = tuple(float(r) for r in ratios)
+        self._c = _abi.make_variogram([s._tuple() for s in parsed], nugget=nugget, rotation=self.rotation,
+                                      ratios=self.ratios)
+        _checked(lambda lib: lib.bk_check_variogram, self._c, "variogram")
+
+    @property
+    def sill(self) -> float:
+        return self.nugget + sum(s.sill for s in self.structures)
+
+    @property
+    def has_sill(self) -> bool:
+        return all(s.model != "power" for s in self.structures)
+
+    def __repr__(self):
+        return f"Variogram({self.structures!r}, nugget={self.nugget})"
+
+
+class Search:
+    """Search neighbourhood (crates/py/src/estimation.rs:415-531): `radius`, or
+    `ranges` (major, semi, minor) giving radius = major and ratios semi/major,
+    minor/major; `max_per_sector` caps octants with `octant=True` or plane
+    sectors with `sectors`."""
+
+    def __init__(self, radius=None, *, max_samples=16, min_samples=1, octant=False, sectors=None, max_per_sector=None,
+                 max_per_hole=None, rotation=None, ratios=None, ranges=None, min_sectors=None, max_empty_sectors=None,
+                 min_holes=None, high_grade=None, soft=None, target_slope=None, target_efficiency=None,
+                 length_unit=None):
+        for name, v in (("high_grade", high_grade), ("soft", soft), ("target_slope", target_slope),
+                        ("target_efficiency", target_efficiency), ("length_unit", length_unit)):
+            if v is not None:
+                raise NotImplementedError(f"{name} is not supported by this engine")
+        if radius is not None and ranges is not None:
+            raise InvalidInput("give radius or ranges, not both")
+        if radius is None and ranges is None:
+            raise InvalidInput("give radius or ranges")
+        if radius is not None:
+            r = _number(radius, "radius")
+        else:
+            if ratios is not None:
+                raise InvalidInput("give ratios with radius, not with ranges")
+            major, semi, minor = (float(v) for v in ranges)
+            if not (major > 0.0 and semi > 0.0 and minor > 0.0):
+                raise InvalidInput("ranges must be > 0")
+            r, ratios = major, (semi / major, minor / major)
+        for name, v in (("max_samples", max_samples), ("min_samples", min_samples), ("sectors", sectors),
+                        ("max_per_sector", max_per_sector), ("max_per_hole", max_per_hole),
+                        ("min_sectors", min_sectors), ("max_empty_sectors", max_empty_sectors),
+                        ("min_holes", min_holes)):
+            if v is not None and (not isinstance(v, (int, np.integer)) or isinstance(v, bool) or v < 0):
+                raise InvalidInput(f"{name} must be a non-negative integer")
+        if max_per_sector is not None and sectors is None and not octant:
+            raise InvalidInput("max_per_sector needs octant or sectors")
+        max_per_octant = max_per_sector if (octant and sectors is None) else None
+        per_sector = None
+        if sectors is not None:
+            per_sector = max_per_sector if max_per_sector is not None else -(-int(max_samples) // max(int(sectors), 1))
+        self.radius, self.max_samples, self.min_samples = r, int(max_samples), int(min_samples)
+        self.octant = bool(octant)
+        self.rotation, self.ratios = rotation, ratios
+        self._c = _abi.make_search(min_samples=min_samples, max_samples=max_samples, radius=r, rotation=rotation,
+                                   ratios=ratios, octant=octant, max_per_hole=max_per_hole,
+                                   max_per_octant=max_per_octant, sectors=sectors, max_per_sector=per_sector,
+                                   min_sectors=min_sectors, max_empty_sectors=max_empty_sectors,
+                                   min_holes=min_holes)
+        if rotation is not None or ratios is not None:
+            rot = tuple(rotation) if rotation is not None else (0.0, 0.0, 0.0)
+            rat = tuple(ratios) if ratios is not None else (1.0, 1.0)
+            if any(not math.isfinite(float(a)) for a in rot) or any(not (float(q) > 0.0) for q in rat):
+                raise InvalidInput("anisotropy needs finite angles and ratios > 0")
+        _checked(lambda lib: lib.bk_check_search, self._c, "search")
+
+    def __repr__(self):
+        return f"Search(radius={self.radius}, max_samples={self.max_samples}, min_samples={self.min_samples})"
+
+
+@dataclass(frozen=True)
+class CrossValidation:
+    """Cross-validation results; NaN where a sample had too few neighbours
+    (python/boitata/estimation.py:113-149)."""
+
+    actual: np.ndarray
+    estimate: np.ndarray
+    variance: np.ndarray
+
+    @property
+    def error(self) -> np.ndarray:
+        return self.estimate - self.actual
+
+    @property
+    def mean_error(self) -> float:
+        return float(np.nanmean(self.error))
+
+    @property
+    def rmse(self) -> float:
+        return float(np.sqrt(np.nanmean(self.error ** 2)))
+
+    @property
+    def correlation(self) -> float:
+        ok = ~np.isnan(self.estimate)
+        return float(np.corrcoef(self.actual[ok], self.estimate[ok])[0, 1])
+
+    @property
+    def slope(self) -> float:
+        ok = ~np.isnan(self.estimate)
+        return float(np.polyfit(self.estimate[ok], self.actual[ok], 1)[0])
+
+    @property
+    def standardized_squared_error(self) -> float:
+        return float(np.nanmean(self.error ** 2 / self.variance))
+
+
+# --------------------------------------------------------------------------
+# Host-side input handling (no device needed).
+
+def coords_of(coords) -> np.ndarray:
+    """(n, 3) float64 rows of an (n, 2) or (n, 3) array; (n, 2) lies on z = 0."""
+    if hasattr(coords, "to_length_unit") or hasattr(coords, "centroids"):
+        raise NotImplementedError("PointSet / BlockModel containers are not supported by this engine; pass arrays")
+    try:
+        a = np.asarray(coords, dtype=np.float64)
+    except (TypeError, ValueError):
+        raise InvalidInput("coordinates must be a 2-D array of shape (n, 2) or (n, 3)") from None
+    if a.ndim != 2 or a.shape[1] not in (2, 3):
+        raise InvalidInput("coordinates must be a 2-D array of shape (n, 2) or (n, 3)")
+    if np.isnan(a).any():
+        raise InvalidInput("coordinates must be finite")
+    if a.shape[1] == 2:
+        a = np.concatenate([a, np.zeros((len(a), 1))], axis=1)
+    return np.ascontiguousarray(a)
+
+
+def hole_codes(holes, n):
+    """Labels (text) and integer codes of drill holes: codes in order of first
+    appearance, a sample without a label gets a fresh code of its own."""
+    if holes is None:
+        return None, None
+    if isinstance(holes, str):
+        raise NotImplementedError("hole columns by name need a container; pass an array")
+    labels = list(holes)
+    if len(labels) != n:
+        raise InvalidInput(f"holes: expected {n} values, got {len(labels)}")
+    text = [None if (h is None or (isinstance(h, float) and math.isnan(h))) else str(h) for h in labels]
+    codes, ids, nxt = {}, np.empty(n, dtype=np.int32), 0
+    for i, t in enumerate(text):
+        if t is None:
+            ids[i] = nxt
+            nxt += 1
+        else:
+            if t not in codes:
+                codes[t] = nxt
+                nxt += 1
+            ids[i] = codes[t]
+    return text, ids
+
+
+def distinct_rows(xyz: np.ndarray, hole_text=None):
+    """Rows to keep so that no two samples share a location (the first of each
+    group, compared bit for bit after + 0.0), and a warning text naming the
+    dropped groups (None when there is none)."""
+    keys = (xyz + 0.0).view(np.uint64).reshape(len(xyz), 3)
+    groups = {}
+    for i, k in enumerate(map(tuple, keys)):
+        groups.setdefault(k, []).append(i)
+    shared = sorted((g for g in groups.values() if len(g) > 1), key=lambda g: g[0])
+    if not shared:
+        return np.arange(len(xyz)), None
+    dropped = np.zeros(len(xyz), dtype=bool)
+    lines = []
+    for g in shared:
+        dropped[g[1:]] = True
+        tagged = None if hole_text is None else [hole_text[i] for i in g]
+        if tagged is not None and all(t is not None for t in tagged):
+            who = "holes " + ", ".join(tagged)
+        else:
+            who = "rows " + ", ".join(str(i) for i in g)
+        x, y, z = xyz[g[0]]
+        lines.append(f"{who} at ({x}, {y}, {z})")
+    more = max(len(lines) - 20, 0)
+    text = "samples sharing a location keep the first one: " + "; ".join(lines[:20])
+    if more:
+        text += f"; and {more} more"
+    return np.nonzero(~dropped)[0], text
+
+
+def run_passes(n, searches, one_pass):
+    """by_pass (crates/estimation/src/batch.rs:132-147): each search in turn
+    on the rows earlier passes left unestimated. one_pass(search, rows) returns
+    (value, variance, ok) for those rows. Returns value, variance, the pass of
+    each row (-1 where none) as arrays."""
+    value = np.full(n, np.nan)
+    variance = np.full(n, np.nan)
+    which = np.full(n, -1, dtype=np.int64)
+    remaining = np.arange(n)
+    for p, s in enumerate(searches):
+        if len(remaining) == 0:
+            break
+        v, q, ok = one_pass(s, remaining)
+        ok = np.asarray(ok, dtype=bool)
+        rows = remaining[ok]
+        value[rows] = np.asarray(v)[ok]
+        variance[rows] = np.asarray(q)[ok]
+        which[rows] = p
+        remaining = remaining[~ok]
+    return value, variance, which
+
+
+def _searches(search):
+    if isinstance(search, Search):
+        return [search]
+    if isinstance(search, Sequence) and len(search) > 0 and all(isinstance(s, Search) for s in search):
+        return list(search)
+    raise InvalidInput("search must be a Search or a non-empty sequence of Search")
+
+
+class _Base:
+    _kind = _abi.ORDINARY
+
+    def __init__(self, variogram: Variogram, search, *, precision: str = "fp64", device: int = 0):
+        if not isinstance(variogram, Variogram):
+            raise InvalidInput("kriging needs a Variogram")
+        if precision not in _PRECISIONS:
+            raise InvalidInput(f"precision is one of {', '.join(_PRECISIONS)}")
+        self._variogram = variogram
+        self._passes = _searches(search)
+        self._precision = _PRECISIONS[precision]
+        self._device = device
+        self._engine = None
+        self._values = None
+        self._param = 0.0
+
+    @property
+    def variogram(self) -> Variogram:
+        return self._variogram
+
+    @property
+    def values(self) -> np.ndarray:
+        """Values of the fitted samples, after dropping shared locations."""
+        if self._values is None:
+            raise InvalidInput("call fit first")
+        return self._values
+
+    def _device_engine(self):
+        if self._engine is None:
+            from .engine import Engine
+
+            self._engine = Engine(self._device)
+        return self._engine
+
+    def fit(self, coords, values, *, holes=None, error_variance=None, domains=None, domain_column=None):
+        """Stores the samples; samples sharing a location keep the first one."""
+        if domains is not None or domain_column is not None:
+            raise NotImplementedError("domains are not supported by this engine")
+        if isinstance(values, str):
+            raise NotImplementedError("value columns by name need a container; pass an array")
+        xyz = coords_of(coords)
+        n = len(xyz)
+        v = np.asarray(values, dtype=np.float64).reshape(-1)
+        if len(v) != n:
+            raise InvalidInput(f"values: expected {n} values, got {len(v)}")
+        err = None
+        if error_variance is not None:
+            err = np.asarray(error_variance, dtype=np.float64).reshape(-1)
+            if len(err) != n:
+                raise InvalidInput(f"error_variance: expected {n} values, got {len(err)}")
+            if not (np.isfinite(err).all() and (err >= 0).all()):
+                raise InvalidInput("error_variance must be finite and >= 0")
+        text, ids = hole_codes(holes, n)
+        keep, note = distinct_rows(xyz, text)
+        if note:
+            warnings.warn(note, UserWarning, stacklevel=2)
+        xyz, v = xyz[keep], v[keep]
+        err = None if err is None else err[keep]
+        ids = None if ids is None else ids[keep]
+        eng = self._device_engine()
+        eng.set_samples(xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy(), v, err, ids)
+        self._values = v
+        return self
+
+    def _one_pass(self, s, T):
+        out = self._engine.krige(self._variogram._c, s._c, T[:, 0].copy(), T[:, 1].copy(), T[:, 2].copy(),
+                                 self._kind, self._param, self._precision)
+        return out["value"], out["variance"], out["status"] == _abi.OK
+
+    def predict(self, targets, *, return_variance=False, anisotropy=None, diagnostics=False, domains=None,
+                domain_column=None):
+        """Estimates at targets; NaN where the search found too few samples."""
+        if anisotropy is not None:
+            raise NotImplementedError("local anisotropy is not supported by this engine")
+        if diagnostics:
+            raise NotImplementedError("diagnostics tables are not supported by this engine")
+        if domains is not None or domain_column is not None:
+            raise NotImplementedError("domains are not supported by this engine")
+        if self._values is None:
+            raise InvalidInput("call fit first")
+        T = coords_of(targets)
+        value, variance, _ = run_passes(len(T), self._passes, lambda s, rows: self._one_pass(s, T[rows]))
+        return (value, variance) if return_variance else value
+
+    def cross_validate(self, *, folds=None) -> CrossValidation:
+        """Re-estimates every sample from the others, through the same passes."""
+        if folds is not None:
+            raise NotImplementedError("k-fold cross-validation is not supported by this engine")
+        if self._values is None:
+            raise InvalidInput("call fit first")
+
+        def one(s, rows):
+            out = self._engine.loo(self._variogram._c, s._c, rows.astype(np.int64), self._kind, self._param,
+                                   self._precision)
+            return out["value"], out["variance"], out["status"] == _abi.OK
+
+        value, variance, _ = run_passes(len(self._values), self._passes, one)
+        return CrossValidation(self._values.copy(), value, variance)
+
+
+class OrdinaryKriging(_Base):
+    """Kriging with an unknown, locally constant mean (weights sum to 1)."""
+
+    def __init__(self, variogram: Variogram, search, **engine):
+        super().__init__(variogram, search, **engine)
+
+
+class SimpleKriging(_Base):
+    """Kriging with a known global `mean`; a variogram with a structure without
+    a sill (power) raises InvalidInput."""
+
+    _kind = _abi.SIMPLE
+
+    def __init__(self, variogram: Variogram, search, *, mean: float = 0.0, **engine):
+        super().__init__(variogram, search, **engine)
+        if not variogram.has_sill:
+            raise InvalidInput("simple kriging needs a variogram with a sill")
+        self._param = float(mean)
+
+
+class IndicatorKriging(_Base):
+    """Ordinary kriging of the indicator `value <= threshold`; estimates are probabilities."""
+
+    _kind = _abi.INDICATOR
+
+    def __init__(self, variogram: Variogram, search, threshold: float, **engine):
+        if isinstance(threshold, str):
+            raise NotImplementedError("thresholds with units are not supported by this engine")
+        super().__init__(variogram, search, **engine)
+        self._param = float(threshold)
+
+
+class BlockKriging(_Base):
+    """Ordinary kriging of block averages; targets are block centres of `size`
+    along the world axes, discretised `discretization` per axis; (1, 1, 1) is
+    point support, nugget included."""
+
+    def __init__(self, variogram: Variogram, search, size, *, discretization=(4, 4, 1), **engine):
+        super().__init__(variogram, search, **engine)
+        size = [float(s) for s in size]
+        disc = tuple(int(k) for k in discretization)
+        if len(size) != 3 or not all(math.isfinite(s) and s > 0 for s in size):
+            raise InvalidInput("block size must be three finite values > 0")
+        if len(disc) != 3 or min(disc) < 1:
+            raise InvalidInput("discretization must be at least 1 per axis")
+        self._block = _abi.make_block(size, disc)
+
+    def cross_validate(self, *, folds=None) -> CrossValidation:
+        raise NotImplementedError("block cross-validation is not supported by this engine")
+
+    def _one_pass(self, s, T):
+        out = self._engine.block_krige(self._variogram._c, s._c, self._block, T[:, 0].copy(), T[:, 1].copy(),
+                                       T[:, 2].copy(), self._kind, self._param, self._precision)
+        return out["value"], out["variance"], out["status"] == _abi.OK

@@ -135,7 +135,8 @@ typedef struct {
 
 typedef struct bk_ctx bk_ctx;
 
-/* Context on HIP device `device`. */
+/* Context on HIP device `device`. On failure *out is NULL and the message is
+ * in bk_last_error(NULL). */
 int bk_create(bk_ctx** out, int device);
 void bk_destroy(bk_ctx* ctx);
 const char* bk_last_error(const bk_ctx* ctx);
@@ -143,7 +144,9 @@ const char* bk_last_error(const bk_ctx* ctx);
 const char* bk_version(void);
 
 /* Validation, as crates/estimation/src/search.rs:214-264 (Search::check) and
- * crates/variogram/src/composite.rs:37-48 (Variogram::validate). */
+ * crates/variogram/src/composite.rs:37-48 (Variogram::validate). ctx may be
+ * NULL (no device needed); the message is then read with bk_last_error(NULL),
+ * which holds the last context-free failure of the calling thread. */
 int bk_check_search(bk_ctx* ctx, const bk_search* search);
 int bk_check_variogram(bk_ctx* ctx, const bk_variogram* vg);
 
