@@ -187,6 +187,10 @@ EXTRA_SIGNATURES = [
     ("bk_set_tuning", C.c_int, [P, P, I32, D, D]),
     ("bk_profile", C.c_int, [P, I32]),
     ("bk_profile_read", C.c_int, [P, P, P]),
+    ("bk_set_search_tuning", C.c_int, [P, I32]),
+    ("bk_get_stat", C.c_int, [P, I32, P]),
 ]
+
+STAT_SEARCH_SPILL = 0
 
 STAGES = ("grid", "search", "bucket", "solve", "deferred", "total")

@@ -88,6 +88,8 @@ struct bk_ctx {
   bool planar = true;
   // Grid cache, keyed by the search frame.
   bk::GridBuffers gb;
+  bk::TileBuffers tiles;
+  int tile_reach = 0;  // uncapped searches on sorted tiles (0: one walk per lane)
   GridDev grid{};
   bool grid_valid = false;
   double grid_F[9] = {0};
@@ -488,7 +490,7 @@ int run_search(bk_ctx* c, SearchArgs& a, int64_t m, const double* tx, const doub
   a.count = (int32_t*)c->ncount.p;
   a.status = (int32_t*)c->sstatus.p;
   if (m == 0) return 0;
-  if (bk::launch_search(c->stream, a, m)) return fail(c, BK_E_HIP, "search launch failed");
+  if (bk::launch_search(c->stream, a, m, c->tile_reach, c->tiles)) return fail(c, BK_E_HIP, "search launch failed");
   k_map_overflow<<<blocks_of(m), 256, 0, c->stream>>>(m, a.status);
   return 0;
 }
@@ -653,17 +655,25 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
   a.redo = (int32_t*)c->redo.p;
   a.guard_count = counts + 4;
   a.redo_count = counts + 5;
+  // The refined form covers spherical, exponential, Gaussian, cubic and
+  // pentaspherical structures (fast FP64 covariances); a variogram with any
+  // other model takes the FP64 kernel in either precision.
+  bool refined = precision == BK_FP32_IR;
+  for (int k = 0; k < vg->n_structures; ++k) {
+    const int md = vg->structures[k].model;
+    if (md != BK_SPHERICAL && md != BK_EXPONENTIAL && md != BK_GAUSSIAN && md != BK_CUBIC && md != BK_PENTASPHERICAL)
+      refined = false;
+  }
   for (int b = 0; b < nb; ++b) {
     if (hc[b] == 0) continue;
     a.list = lists + (int64_t)b * m;
     a.n_list = hc[b];
-    int r = precision == BK_FP32_IR ? bk::launch_krige32(c->stream, a, c->edges[b])
-                                    : bk::launch_krige64(c->stream, a, c->edges[b]);
+    int r = refined ? bk::launch_krige32(c->stream, a, c->edges[b]) : bk::launch_krige64(c->stream, a, c->edges[b]);
     if (r) return fail(c, BK_E_HIP, "solve launch failed");
   }
   mark(c, 4);
   // Systems the FP32 form handed back: FP64 over the redo list.
-  if (precision == BK_FP32_IR) {
+  if (refined) {
     int32_t h2[8];
     HIPOK(c, hipMemcpyAsync(h2, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIPOK(c, hipStreamSynchronize(c->stream));
@@ -740,6 +750,7 @@ void bk_destroy(bk_ctx* c) {
                  &c->loo_status})
     release(*b);
   bk::free_grid(c->gb);
+  bk::free_tiles(c->tiles);
   if (c->tab_storage) (void)hipFree(c->tab_storage);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1123,6 +1134,23 @@ int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol
     c->grid_valid = false;
   }
   return 0;
+}
+
+int bk_set_search_tuning(bk_ctx* c, int32_t tile_reach) {
+  if (!c) return BK_E_INVALID;
+  if (tile_reach < 0 || tile_reach > 8) return fail(c, BK_E_INVALID, "tile_reach is 0 (off) to 8 cells");
+  c->tile_reach = tile_reach;
+  return 0;
+}
+
+int bk_get_stat(bk_ctx* c, int32_t which, int64_t* value) {
+  if (!c || !value) return BK_E_INVALID;
+  switch (which) {
+    case BK_STAT_SEARCH_SPILL:
+      *value = c->tiles.last_spill;
+      return 0;
+  }
+  return fail(c, BK_E_INVALID, "unknown statistic");
 }
 
 int bk_profile(bk_ctx* c, int32_t enable) {

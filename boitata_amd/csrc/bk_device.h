@@ -260,8 +260,9 @@ __device__ __forceinline__ double shape_fast64(int model, double param, double h
       return 1.0 - (1.0 + delta + delta * delta * (1.0 / 3.0)) * e;
     }
   }
-  // Circular, sine hole, power: the library forms.
-  return shape64(model, param, h, 1.0 / inv_r);
+  // Circular, sine hole and power have no fast form: NaN sends the system
+  // to the FP64 kernel (the host routes such variograms there directly).
+  return __builtin_nan("");
 }
 
 __device__ __forceinline__ double cov_fast64(const DevVariogram& v, const double* h) {

@@ -87,6 +87,19 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
                const double origin[3], double cell_factor, GridBuffers& gb, GridDev& out);
 void free_grid(GridBuffers& gb);
 int upload_shell_table(ShellTable& tab, void** storage);
-int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets);
+// Scratch of the tiled search: Morton keys and target ids (sorted copies),
+// the spill list and its count, the radix sort's temporary storage.
+struct TileBuffers {
+  uint64_t* keys = nullptr;
+  int32_t* ids = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int64_t cap_keys = 0, cap_ids = 0;
+  int32_t last_spill = 0;     // targets the last tiled search left to k_knn
+};
+void free_tiles(TileBuffers& tb);
+// tile_reach > 0: uncapped searches over all targets run on sorted tiles
+// staging the cells within tile_reach cells of each tile (see k_knn_tile).
+int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int tile_reach, TileBuffers& tb);
 bool knn_path(const SearchArgs& a);
 }  // namespace bk

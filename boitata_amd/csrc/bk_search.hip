@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -764,6 +765,188 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The same selections on tiles of 64 targets that lie close in the grid
+// (targets sorted by the Morton code of their cell). The wave copies the
+// entries of every cell within `reach` cells of the tile's cell box into LDS
+// in chunks (cells along x are contiguous in the CSR, so each (y, z) row of
+// the box is one coalesced range), and each lane offers the staged
+// candidates to its own heap, as k_knn does cell by cell. Exactness: a sample
+// outside the box lies beyond the nearest face of the box that is not a face
+// of the grid, at distance at least `safe` from the target; a lane whose K-th
+// key lies below safe^2 (or, with fewer than K found, whose radius does) has
+// its exact selection. Any other lane goes to the spill list, which k_knn
+// searches afterwards.
+
+__device__ __forceinline__ uint64_t morton_spread(uint32_t v) {
+  uint64_t x = v & 0x1fffff;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+
+__global__ void k_cell_keys(int64_t m, const SearchArgs a, uint64_t* keys, int32_t* ids) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t t = a.list ? a.list[i] : i;
+  const GridDev& g = a.g;
+  double q0, q1, q2;
+  mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
+  const uint32_t c0 = (uint32_t)clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
+  const uint32_t c1 = (uint32_t)clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
+  const uint32_t c2 = (uint32_t)clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+  keys[i] = morton_spread(c0) | (morton_spread(c1) << 1) | (morton_spread(c2) << 2);
+  ids[i] = (int32_t)t;
+}
+
+constexpr int TILE_CHUNK = 256;
+constexpr int TILE_SPAN = 3;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, BK_WAVE));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, BK_WAVE));
+  return v;
+}
+
+__global__ void __launch_bounds__(BK_WAVE) k_knn_tile(const SearchArgs a, int reach, int32_t* spill,
+                                                      int32_t* spill_count) {
+  extern __shared__ uint64_t knn_dyn[];
+  const int lane = threadIdx.x;
+  const int K = a.max_samples;
+  KnnHeap H{knn_dyn, (uint32_t*)(knn_dyn + (size_t)K * BK_WAVE), lane};
+  double4* stage = (double4*)(knn_dyn + (size_t)K * BK_WAVE + ((size_t)K * BK_WAVE + 1) / 2);
+  const int64_t b = (int64_t)blockIdx.x * BK_WAVE + lane;
+  const bool live = b < a.m;
+  const int64_t t = live ? a.list[b] : 0;
+  const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
+  const GridDev& g = a.g;
+  double q0 = 0, q1 = 0, q2 = 0;
+  if (live) mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
+  int c[3] = {0, 0, 0};
+  if (live) {
+    c[0] = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
+    c[1] = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
+    c[2] = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+  }
+  // A Morton run can jump across the grid: lanes farther than TILE_SPAN cells
+  // from the cell of the tile's first target take the walk instead, so the
+  // box stays small. The staged box: the remaining cells widened by `reach`,
+  // inside the grid.
+  const int first = __builtin_amdgcn_readfirstlane(lane);
+  int ref[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ref[d] = __shfl(c[d], first, BK_WAVE);
+  const bool near = live && abs(c[0] - ref[0]) <= TILE_SPAN && abs(c[1] - ref[1]) <= TILE_SPAN &&
+                    abs(c[2] - ref[2]) <= TILE_SPAN;
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = max(wave_min_i(near ? c[d] : INT_MAX) - reach, 0);
+    hi[d] = min(wave_max_i(near ? c[d] : INT_MIN) + reach, g.dims[d] - 1);
+  }
+  // Distance from the target to the nearest face of the box inside the grid,
+  // less a margin for the rounding of the cell assignment.
+  const double qq[3] = {q0, q1, q2};
+  double safe = INFINITY;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (lo[d] > 0) safe = fmin(safe, qq[d] - (g.lo[d] + (double)lo[d] * g.h));
+    if (hi[d] < g.dims[d] - 1) safe = fmin(safe, (g.lo[d] + (double)(hi[d] + 1) * g.h) - qq[d]);
+  }
+  safe -= 1e-6 * g.h;
+  const double safe2 = safe > 0.0 ? safe * safe : 0.0;
+  int size = 0;
+  uint64_t top = 0;
+  uint32_t top_i = 0;
+  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  const int rows = ny * nz;
+  int row = 0;
+  uint32_t e = 0, e_end = 0;
+  auto row_range = [&](int rr) {
+    const int y = lo[1] + rr % ny, z = lo[2] + rr / ny;
+    const int id = lo[0] + g.dims[0] * (y + g.dims[1] * z);
+    e = g.start[id];
+    e_end = g.start[id + nx];
+  };
+  if (rows > 0) row_range(0);
+  while (row < rows) {
+    // Fill one chunk.
+    int fill = 0;
+    while (fill < TILE_CHUNK && row < rows) {
+      const int take = (int)min((uint32_t)(TILE_CHUNK - fill), e_end - e);
+      for (int k = lane; k < take; k += BK_WAVE) stage[fill + k] = g.entries[e + k];
+      fill += take;
+      e += take;
+      if (e == e_end && ++row < rows) row_range(row);
+    }
+    __syncthreads();
+    if (near) {
+      for (int k = 0; k < fill; ++k) {
+        const double4 E = stage[k];
+        const double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
+        const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
+        const uint32_t ii = (uint32_t)__double_as_longlong(E.w);
+        if (!(d2 <= a.radius2) || (int64_t)ii == left_out) continue;
+        const uint64_t kk = dbits(d2);
+        if (size < K) {
+          H.push(size, kk, ii);
+          if (size == K) {
+            top = H.key(0);
+            top_i = H.idx(0);
+          }
+        } else if (key_less(kk, ii, top, top_i)) {
+          H.replace_top(size, kk, ii);
+          top = H.key(0);
+          top_i = H.idx(0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!live) return;
+  const bool exact = near && (size == K ? __longlong_as_double((long long)top) < safe2 : a.radius2 < safe2);
+  if (!exact) {
+    spill[atomicAdd(spill_count, 1)] = (int32_t)t;
+    return;
+  }
+  const int n = size;
+  for (int end = n - 1; end > 0; --end) {
+    uint64_t mk = H.key(0);
+    uint32_t mi = H.idx(0);
+    uint64_t lk = H.key(end);
+    uint32_t li = H.idx(end);
+    H.put(end, mk, mi);
+    H.replace_top(end, lk, li);
+  }
+  int* out = a.idx + t * a.stride;
+  for (int k = 0; k < n; ++k) out[k] = (int)H.idx(k);
+  int status = n < a.min_samples ? BK_SEARCH_FAILED : BK_OK;
+  if (status == BK_OK && a.min_holes >= 0) {
+    int distinct = 0;
+    for (int k = 0; k < n; ++k) {
+      int h = a.hole ? a.hole[out[k]] : -1;
+      if (h < 0) {
+        distinct++;
+        continue;
+      }
+      bool seen = false;
+      for (int j = 0; j < k && !seen; ++j) seen = a.hole[out[j]] == h;
+      distinct += !seen;
+    }
+    if (distinct < a.min_holes) status = BK_SEARCH_FAILED;
+  }
+  a.count[t] = n;
+  a.status[t] = status;
+}
+
+// ---------------------------------------------------------------------------
 // Grid build.
 __global__ void k_project(int64_t n, const double* x, const double* y, const double* z, const double* F,
                           double o0, double o1, double o2, double* px, double* py, double* pz) {
@@ -997,9 +1180,73 @@ int upload_shell_table(ShellTable& tab, void** storage) {
 
 bool knn_path(const SearchArgs& a) { return !a.balances && a.max_per_hole < 0 && a.max_samples <= BK_MAX_KNN; }
 
-int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets) {
+template <class T>
+static int grow(T*& p, int64_t& cap, int64_t n) {
+  if (n <= cap) return 0;
+  (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  HIPCHK(hipMalloc((void**)&p, (size_t)n * sizeof(T)));
+  cap = n;
+  return 0;
+}
+
+// Uncapped selections on Morton-sorted tiles with LDS-staged candidates, the
+// targets the tile cannot settle by k_knn.
+static int launch_tiles(hipStream_t s, const SearchArgs& a, int64_t m, int reach, TileBuffers& tb) {
+  if (m > 2147483647LL) return -1;
+  if (grow(tb.keys, tb.cap_keys, 2 * m)) return -1;
+  if (grow(tb.ids, tb.cap_ids, 3 * m + 1)) return -1;
+  uint64_t* kin = tb.keys;
+  uint64_t* kout = tb.keys + m;
+  int32_t* iin = tb.ids;
+  int32_t* iout = tb.ids + m;
+  int32_t* spill = tb.ids + 2 * m;
+  int32_t* count = tb.ids + 3 * m;
+  k_cell_keys<<<dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s>>>(m, a, kin, iin);
+  size_t need = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, iin, iout, (int)m, 0, 64, s));
+  if (need > tb.tmp_bytes) {
+    (void)hipFree(tb.tmp);
+    tb.tmp = nullptr;
+    tb.tmp_bytes = 0;
+    HIPCHK(hipMalloc(&tb.tmp, need));
+    tb.tmp_bytes = need;
+  }
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tb.tmp, need, kin, kout, iin, iout, (int)m, 0, 64, s));
+  HIPCHK(hipMemsetAsync(count, 0, sizeof(int32_t), s));
+  SearchArgs c = a;
+  c.list = iout;
+  c.m = m;
+  const size_t heap = (size_t)a.max_samples * BK_WAVE * (sizeof(uint64_t) + sizeof(uint32_t));
+  const size_t sh = heap + (size_t)TILE_CHUNK * sizeof(double4);
+  k_knn_tile<<<dim3((unsigned)((m + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), sh, s>>>(c, reach, spill, count);
+  HIPCHK(hipGetLastError());
+  int32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  tb.last_spill = h;
+  if (h > 0) {
+    SearchArgs d = a;
+    d.list = spill;
+    d.m = h;
+    k_knn<<<dim3((unsigned)((h + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), heap, s>>>(d);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+void free_tiles(TileBuffers& tb) {
+  (void)hipFree(tb.keys);
+  (void)hipFree(tb.ids);
+  (void)hipFree(tb.tmp);
+  tb = TileBuffers();
+}
+
+int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int tile_reach, TileBuffers& tb) {
   if (n_targets <= 0) return 0;
   if (knn_path(a)) {
+    if (tile_reach > 0 && a.list == nullptr) return launch_tiles(s, a, n_targets, tile_reach, tb);
     // Selections without caps: one target per lane.
     SearchArgs c = a;
     c.m = n_targets;

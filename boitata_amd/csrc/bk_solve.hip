@@ -231,6 +231,20 @@ __device__ __forceinline__ double rhs_fast(const SolveArgs& a, int t, const doub
   return cov_fast64(a.v, h);
 }
 
+// Mean covariance between world point (x, y, z) and the discretisation of the
+// block centred on (t0, t1, t2), nugget left out between distinct points
+// (block.rs:100-118), in the fast arithmetic.
+__device__ __forceinline__ double rhs_fast_world(const SolveArgs& a, double t0, double t1, double t2, double x,
+                                                 double y, double z) {
+  double h[BK_MAX_FRAMES];
+  double acc = 0.0;
+  for (int p = 0; p < a.npts; ++p) {
+    lags_world_fast(a.v, (t0 + a.offs[3 * p]) - x, (t1 + a.offs[3 * p + 1]) - y, (t2 + a.offs[3 * p + 2]) - z, h);
+    acc += (a.npts > 1 && !(h[a.v.nug_frame] > 0.0)) ? a.v.c0 - a.v.nugget : cov_fast64(a.v, h);
+  }
+  return acc / (double)a.npts;
+}
+
 // ---------------------------------------------------------------------------
 // FP64, the reference's operation order.
 template <int NMAX>
@@ -347,160 +361,279 @@ __global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// FP32 factor, FP64 residual refinement and FP64 Schur scalar.
+// FP32 + FP64 refinement. The system is scaled by 1/c0 and inverted in FP32 by
+// Gauss-Jordan elimination without pivoting (the matrix is SPD, so pivots are
+// the Cholesky pivots d_j of kernel.rs:58-72 and stay positive): lane r holds
+// row r of the working matrix in registers, the pivot row goes to the group
+// through LDS as float4. Every solve is then one matrix-vector product X v, so
+// the initial solve and each refinement sweep are fully parallel. Residuals
+// b - C x are taken in FP64 against the FP64 matrix (row r in registers for
+// NMAX <= 32), the iterates are FP64, and so are the Schur scalar and the
+// weights. A pivot below 1e-5 of the scaled diagonal (FP32 cannot resolve the
+// reference's 1e-12 test) or a system that does not converge goes to the FP64
+// kernel, which applies the reference's pivot test exactly.
 template <int NMAX>
-__global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
+__device__ __forceinline__ bool group_all(bool p, int g) {
+  uint64_t m = __ballot(p);
+  uint64_t gm = NMAX == 64 ? ~0ull : (((1ull << NMAX) - 1ull) << (g * NMAX));
+  return (m & gm) == gm;
+}
+
+template <int NMAX>
+__device__ __forceinline__ float group_maxf(float v) {
+#pragma unroll
+  for (int o = NMAX / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, BK_WAVE));
+  return v;
+}
+
+// LDS layout of one system of the refined form: per sample j a record of SJ
+// doubles (world x, y, z, then the projected coordinates of each frame,
+// padded to an even count), iterates (NMAX double2), pivot rows (2 x NMAX
+// floats), product vectors (NMAX float2), and for NMAX = 64 the FP64 rows of
+// the wave's lanes ([k][lane], lane-private).
+__host__ __device__ constexpr int rec_doubles(int nf) { return (3 + 3 * nf + 1) / 2 * 2; }
+__host__ __device__ constexpr int sys32_doubles(int nmax, int nf) {
+  return nmax * rec_doubles(nf) + 2 * nmax + nmax + nmax + (nmax > 32 ? nmax * nmax : 0) + 8;
+}
+
+// acc[j] += sill * shape(|P_r - P_j| / range) over the group's samples j for
+// one structure whose frame starts at `off` in each record; sets bit j of
+// `apart` where the lag is positive (the nugget's h > 0 test, done on the
+// frame of structure 0 as composite.rs:69-85 does on the shared one).
+template <int NMAX, class Shape>
+__device__ __forceinline__ void row_pass(const double* PJ, int SJ, int off, const double* me, double sill, double ir,
+                                         bool first, uint64_t& apart, double* acc, Shape shape) {
+#pragma clang fp contract(fast)
+  const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    const double* pj = PJ + j * SJ + off;
+    const double d0 = q0 - pj[0], d1 = q1 - pj[1], d2 = q2 - pj[2];
+    const double h = sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2);
+    if (first && h > 0.0) apart |= 1ull << j;
+    acc[j] += sill * shape(h * ir);
+    // Keeps the compiler from hoisting every record's load to the top (which
+    // would hold 3 NMAX doubles in registers).
+    if (j % 4 == 3) asm volatile("" ::: "memory");
+  }
+}
+
+// Row r of the covariance matrix in the fast FP64 arithmetic, without the
+// diagonal: out[j] = C(s_r, s_j) for j < NMAX (values beyond n unused).
+template <int NMAX>
+__device__ __forceinline__ void row_cov_fast(const DevVariogram& v, const double* PJ, int SJ, int r, double* out) {
+#pragma clang fp contract(fast)
+  const double* me = PJ + r * SJ;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) out[j] = 0.0;
+  uint64_t apart = 0;
+  for (int s = 0; s < v.S; ++s) {
+    const int off = 3 + 3 * v.frame[s];
+    const double sill = v.sill[s], ir = v.inv_range[s];
+    const bool first = s == 0;
+    switch (v.model[s]) {
+      case BK_SPHERICAL:
+        row_pass<NMAX>(PJ, SJ, off, me, sill, ir, first, apart, out,
+                       [](double t) { return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t); });
+        break;
+      case BK_EXPONENTIAL:
+        row_pass<NMAX>(PJ, SJ, off, me, sill, ir, first, apart, out,
+                       [](double t) { return 1.0 - exp_fast(-3.0 * t); });
+        break;
+      case BK_GAUSSIAN:
+        row_pass<NMAX>(PJ, SJ, off, me, sill, ir, first, apart, out,
+                       [](double t) { return 1.0 - exp_fast(-3.0 * t * t); });
+        break;
+      case BK_CUBIC:
+        row_pass<NMAX>(PJ, SJ, off, me, sill, ir, first, apart, out, [](double t) {
+          const double t2 = t * t;
+          return t >= 1.0 ? 1.0 : t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
+        });
+        break;
+      case BK_PENTASPHERICAL:
+        row_pass<NMAX>(PJ, SJ, off, me, sill, ir, first, apart, out, [](double t) {
+          const double t2 = t * t;
+          return t >= 1.0 ? 1.0 : t * (1.875 + t2 * (-1.25 + 0.375 * t2));
+        });
+        break;
+      default:
+        // Other models never reach the refined kernel (the host sends them to
+        // the FP64 one); NaN would send the system there as well.
+        row_pass<NMAX>(PJ, SJ, off, me, sill, ir, first, apart, out, [](double) { return __builtin_nan(""); });
+    }
+  }
+  const double c0 = v.c0, nug = v.nugget;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) out[j] = ((apart >> j) & 1) ? c0 - (nug + out[j]) : c0;
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
+k_krige32(const SolveArgs a) {
   constexpr int G = BK_WAVE / NMAX;
+  constexpr bool ROWREG = NMAX <= 32;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
-  // Per system: C64 packed, L32 packed (as floats in half the doubles), W, V, E, PF, misc.
-  const int per = sys_doubles(NMAX, nf) + (NMAX * (NMAX + 1) / 2 + 1) / 2 + 2;
+  const int SJ = rec_doubles(nf);
+  const int per = sys32_doubles(NMAX, nf);
   const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
   const int g = lane / NMAX, r = lane % NMAX;
   double* S = dyn + (wave * G + g) * per;
-  double* C = S;
-  double* W = C + NMAX * (NMAX + 1) / 2;
-  double* V = W + 3 * NMAX;
-  double* E = V + NMAX;
-  double* PF = E + NMAX;
-  double* misc = PF + nf * 3 * NMAX;
-  float* L = (float*)(misc + 8);
-  float* miscf = (float*)(misc + 4);
+  double* PJ = S;
+  double* X2 = PJ + NMAX * SJ;
+  float* P2 = (float*)(X2 + 2 * NMAX);
+  float* B2 = P2 + 2 * NMAX;
+  double* RowL = (double*)(B2 + 2 * NMAX);  // NMAX = 64 only: [k][lane]
   const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
   const bool active = sys < a.n_list;
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
-  double z = 0.0, e = 0.0;
-  bool guard = load_system<NMAX>(a, t, r, n, g, W, V, E, PF, z, e);
+  const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+  // Sample r of the system: world coordinates and its projection on every
+  // frame relative to the target (kernel.rs:36-42), as a record in LDS.
+  double z = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  {
+#pragma clang fp contract(fast)
+    double* me = PJ + r * SJ;
+    if (r < n) {
+      const int id = a.nidx[(int64_t)t * a.stride + r];
+      wx = a.sx[id];
+      wy = a.sy[id];
+      wz = a.sz[id];
+      z = a.sv[id];
+      e = a.serr ? a.serr[id] : 0.0;
+    }
+    const double d0 = wx - t0, d1 = wy - t1, d2 = wz - t2;
+    me[0] = wx;
+    me[1] = wy;
+    me[2] = wz;
+    for (int f = 0; f < nf; ++f) {
+      const double* F = a.v.F[f];
+      const bool id = a.v.ident[f];
+      me[3 + 3 * f] = id ? d0 : F[0] * d0 + F[1] * d1 + F[2] * d2;
+      me[4 + 3 * f] = id ? d1 : F[3] * d0 + F[4] * d1 + F[5] * d2;
+      me[5 + 3 * f] = id ? d2 : F[6] * d0 + F[7] * d1 + F[8] * d2;
+    }
+  }
+  wave_sync();
+  // Coincident samples (coincident.rs:13-15, 27-35) send the system to the
+  // guarded kernel.
+  bool close = false;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    if (j < r && r < n) {
+      const double* pj = PJ + j * SJ;
+      const double u = wx - pj[0], vv = wy - pj[1], w = wz - pj[2];
+      close |= (u * u + vv * vv) + w * w <= EPS2;
+    }
+    if (j % 4 == 3) asm volatile("" ::: "memory");
+  }
+  const bool guard = group_any<NMAX>(close, g);
   if (guard && r == 0 && active) {
     a.status[t] = BK_ST_GUARD;
     a.guard[atomicAdd(a.guard_count, 1)] = t;
   }
   const bool go = active && !guard;
+  const bool live = go && r < n;
   const double c0 = a.v.c0;
-  const int npairs = n * (n - 1) / 2;
-  if (go)
-    for (int q = r; q < npairs; q += NMAX) {
-      int i, j;
-      pair_of(q, i, j);
-      double cij = pair_cov_fast(a, W, PF, NMAX, i, j);
-      C[tri(i) + j] = cij;
-      L[tri(i) + j] = (float)cij;
-    }
   double c = 0.0;
-  if (go && r < n) {
-    C[tri(r) + r] = c0 + e;
-    L[tri(r) + r] = (float)(c0 + e);
-    c = rhs_fast(a, t, W, PF, NMAX, r);
+  if (live) {
+    if (a.block) {
+      c = rhs_fast_world(a, t0, t1, t2, wx, wy, wz);
+    } else {
+      double h[BK_MAX_FRAMES];
+      const double* me = PJ + r * SJ;
+      for (int f = 0; f < nf; ++f) h[f] = norm3_fast(me[3 + 3 * f], me[4 + 3 * f], me[5 + 3 * f]);
+      c = cov_fast64(a.v, h);
+    }
   }
-  wave_sync();
-  float reg[NMAX];
+  // Row r of C (c0 + e on the diagonal; c0 on the diagonal and 0 elsewhere
+  // beyond n) and its FP32 copy scaled by 1/c0.
+  const double ic0 = 1.0 / c0;
+  double crow[NMAX];
+  row_cov_fast<NMAX>(a.v, PJ, SJ, r, crow);
+  float x[NMAX];
 #pragma unroll
-  for (int k = 0; k < NMAX; ++k) reg[k] = (go && k <= r && r < n) ? L[tri(r) + k] : 0.0f;
-  (void)miscf;
-  // FP32 Cholesky, left-looking; the pivot goes lane to lane by readlane. A
-  // pivot that loses most of its bits sends the system to FP64.
+  for (int k = 0; k < NMAX; ++k) {
+    double v = (k == r) ? (live ? c0 + e : c0) : ((live && k < n) ? crow[k] : 0.0);
+    crow[k] = v;
+    x[k] = (float)(v * ic0);
+    if constexpr (!ROWREG) RowL[k * BK_WAVE + lane] = v;
+  }
+  // Gauss-Jordan: after step k, column k holds the inverse's column.
   bool bad = false;
-  float invd = 0.0f;  // 1 / L[r][r]
-  const float c0f = (float)c0;
-  {
-#pragma clang fp contract(fast)
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      const bool act = go && j < n;
-      float acc0 = 0.0f, acc1 = 0.0f;
+  for (int k = 0; k < NMAX; ++k) {
+    float* prow = P2 + (k & 1) * NMAX;
+    if (r == k) {
 #pragma unroll
-      for (int k = 0; k + 1 < j; k += 2) {
-        acc0 += reg[k] * L[tri(j) + k];
-        acc1 += reg[k + 1] * L[tri(j) + k + 1];
-      }
-      if (j & 1) acc0 += reg[j - 1] * L[tri(j) + j - 1];
-      float s = reg[j] - (acc0 + acc1);
-      float d = group_bcastf<NMAX>(s, j, g);
-      if (act && !(d > 1e-5f * c0f)) bad = true;
-      if (act && !bad) {
-        float inv = rsqrtf(d);
-        if (r == j) {
-          reg[j] = d * inv;
-          invd = inv;
-          L[tri(j) + j] = reg[j];
-        } else if (r > j && r < n) {
-          float x = s * inv;
-          reg[j] = x;
-          L[tri(r) + j] = x;
-        }
-      }
-      wave_sync();
+      for (int c4 = 0; c4 < NMAX / 4; ++c4)
+        reinterpret_cast<float4*>(prow)[c4] = make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
     }
+    wave_sync();
+    const float p = prow[k];
+    bad |= !(p > 1e-5f);
+    const float ip = __builtin_amdgcn_rcpf(p);
+    const float sk = (r == k) ? 1.0f - ip : x[k] * ip;
+#pragma unroll
+    for (int c4 = 0; c4 < NMAX / 4; ++c4) {
+      const float4 q = reinterpret_cast<const float4*>(prow)[c4];
+      const float pv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (4 * c4 + u != k) x[4 * c4 + u] = fmaf(-sk, pv[u], x[4 * c4 + u]);
+    }
+    x[k] = (r == k) ? ip : -sk;
   }
-  // Solve with the FP32 factor, (L L') v = rhs on both right-hand sides:
-  // column sweeps, values passed by readlane; the back sweep reads row k of L.
-  auto solve32 = [&](float& va, float& vb) {
-#pragma clang fp contract(fast)
-    float acca = 0.0f, accb = 0.0f;
+  // y = X v for two right-hand sides (v in FP32, one float2 per lane).
+  auto apply = [&](float va, float vb, float& ya, float& yb) {
+    reinterpret_cast<float2*>(B2)[r] = make_float2(va, vb);
+    wave_sync();
+    float sa0 = 0.0f, sb0 = 0.0f, sa1 = 0.0f, sb1 = 0.0f;
 #pragma unroll
-    for (int k = 0; k < NMAX; ++k) {
-      const bool act = go && !bad && k < n;
-      if (r == k) {
-        va = (va - acca) * invd;
-        vb = (vb - accb) * invd;
-      }
-      float ba = group_bcastf<NMAX>(va, k, g), bb = group_bcastf<NMAX>(vb, k, g);
-      if (act && r > k && r < n) {
-        acca += reg[k] * ba;
-        accb += reg[k] * bb;
-      }
+    for (int c2 = 0; c2 < NMAX / 2; ++c2) {
+      float4 q = reinterpret_cast<const float4*>(B2)[c2];
+      sa0 = fmaf(x[2 * c2], q.x, sa0);
+      sb0 = fmaf(x[2 * c2], q.y, sb0);
+      sa1 = fmaf(x[2 * c2 + 1], q.z, sa1);
+      sb1 = fmaf(x[2 * c2 + 1], q.w, sb1);
+      if (c2 % 4 == 3) asm volatile("" ::: "memory");
     }
-#pragma unroll
-    for (int k = NMAX - 1; k >= 0; --k) {
-      const bool act = go && !bad && k < n;
-      if (r == k) {
-        va = va * invd;
-        vb = vb * invd;
-      }
-      float ba = group_bcastf<NMAX>(va, k, g), bb = group_bcastf<NMAX>(vb, k, g);
-      if (act && r < k) {
-        float lk = L[tri(k) + r];
-        va -= lk * ba;
-        vb -= lk * bb;
-      }
-    }
+    ya = sa0 + sa1;
+    yb = sb0 + sb1;
+    wave_sync();
   };
   const bool ordinary = a.kind != BK_SIMPLE;
-  float fa = (float)c, fb = 1.0f;
-  solve32(fa, fb);
-  double xa = fa, xb = ordinary ? (double)fb : 0.0;
-  // Refinement: residuals against the FP64 matrix, corrections through the
-  // FP32 factor, iterates kept in FP64.
+  const double ba = live ? c : 0.0, bb = (live && ordinary) ? 1.0 : 0.0;
+  float fa, fb;
+  apply((float)(ba * ic0), (float)(bb * ic0), fa, fb);
+  double xa = fa, xb = fb;
+  const float tolf = (float)a.tol;
+  const float ma = group_maxf<NMAX>(fabsf(fa)), mb = group_maxf<NMAX>(fabsf(fb));
   int it = 0;
   bool conv = false;
   for (; it < a.max_iter; ++it) {
-    // Share the iterates through LDS (V and E are free now: z, e live in registers).
-    if (go && r < n) {
-      V[r] = xa;
-      E[r] = xb;
+    reinterpret_cast<double2*>(X2)[r] = make_double2(xa, xb);
+    wave_sync();
+    double ra = ba, rb = bb;
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+      double2 q = reinterpret_cast<const double2*>(X2)[k];
+      double ck;
+      if constexpr (ROWREG)
+        ck = crow[k];
+      else
+        ck = RowL[k * BK_WAVE + lane];
+      ra = fma(-ck, q.x, ra);
+      rb = fma(-ck, q.y, rb);
+      if (k % 4 == 3) asm volatile("" ::: "memory");
     }
     wave_sync();
-    double ra = c, rb = 1.0;
-    if (go && !bad && r < n) {
-      double sa = -0.0, sb = -0.0;
-      for (int j = 0; j < n; ++j) {
-        double cij = j <= r ? C[tri(r) + j] : C[tri(j) + r];
-        sa = fma(cij, V[j], sa);
-        sb = fma(cij, E[j], sb);
-      }
-      ra = c - sa;
-      rb = 1.0 - sb;
-    }
-    wave_sync();
-    float da = (float)ra, db = ordinary ? (float)rb : 0.0f;
-    solve32(da, db);
-    const bool live = go && !bad && r < n;
-    double na = group_max<NMAX>(live ? fabs((double)da) : 0.0), nx = group_max<NMAX>(live ? fabs(xa) : 0.0);
-    double nb = group_max<NMAX>(live ? fabs((double)db) : 0.0), nxb = group_max<NMAX>(live ? fabs(xb) : 0.0);
+    float da, db;
+    apply((float)(ra * ic0), (float)(rb * ic0), da, db);
     xa += (double)da;
-    if (ordinary) xb += (double)db;
-    if (na <= a.tol * nx && nb <= a.tol * nxb) {
+    xb += (double)db;
+    if (group_all<NMAX>(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb, g)) {
       conv = true;
       ++it;
       break;
@@ -513,7 +646,7 @@ __global__ void __launch_bounds__(128) k_krige32(const SolveArgs a) {
     }
   }
   const bool ok = go && !bad && conv;
-  // The Schur scalar 1'C^-1 1 and the weights in FP64.
+  // The Schur scalar 1'C^-1 1 and the weights in FP64 (kernel.rs:86-97).
   const bool in = ok && r < n;
   double sa = group_sum<NMAX>(in ? xa : 0.0);
   double sb = group_sum<NMAX>(in ? xb : 0.0);
@@ -1108,9 +1241,7 @@ static void allow_lds(const void* fn, size_t bytes) {
 }
 
 static size_t shmem64(int nmax, int nf) { return (size_t)2 * (BK_WAVE / nmax) * sys_doubles(nmax, nf) * sizeof(double); }
-static size_t shmem32(int nmax, int nf) {
-  return (size_t)2 * (BK_WAVE / nmax) * (sys_doubles(nmax, nf) + (nmax * (nmax + 1) / 2 + 1) / 2 + 2) * sizeof(double);
-}
+static size_t shmem32(int nmax, int nf) { return (size_t)2 * (BK_WAVE / nmax) * sys32_doubles(nmax, nf) * sizeof(double); }
 
 template <int NMAX>
 static int run64(hipStream_t s, const SolveArgs& a) {

@@ -116,6 +116,15 @@ class Engine:
         e = None if edges is None else (C.c_int32 * 3)(*edges)
         self._check(self._lib.bk_set_tuning(self._h, e, int(max_iter), float(tol), float(cell_factor)))
 
+    def search_tuning(self, tile_reach: int):
+        """Tiled search reach in cells (0: one cell-by-cell walk per target)."""
+        self._check(self._lib.bk_set_search_tuning(self._h, int(tile_reach)))
+
+    def stat(self, which: int = _abi.STAT_SEARCH_SPILL) -> int:
+        v = C.c_int64()
+        self._check(self._lib.bk_get_stat(self._h, int(which), C.byref(v)))
+        return v.value
+
     def neighbors(self, search: _abi.Search, variogram: _abi.Variogram | None, tx, ty, tz, left_out=None):
         tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
         left_out = _i64(left_out)

@@ -205,6 +205,20 @@ int bk_loo_loss_grad(bk_ctx* ctx, const bk_variogram* vg, int32_t precision, dou
  * keep the current setting. Results do not depend on the cell factor. */
 int bk_set_tuning(bk_ctx* ctx, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor);
 
+/* Search tuning (no counterpart in the reference): searches without caps
+ * (no octants, sectors or hole cap, max_samples <= 64) run on tiles of 64
+ * targets sorted along a Morton curve of their grid cells, each tile staging
+ * the samples of the cells within tile_reach cells of its cells; a target the
+ * staged cells cannot settle is searched again cell by cell. 0 searches every
+ * target cell by cell. Results do not depend on it. Default 0. */
+int bk_set_search_tuning(bk_ctx* ctx, int32_t tile_reach);
+
+/* Counters of the last call (no counterpart in the reference):
+ * BK_STAT_SEARCH_SPILL, targets the tiled search handed back to the
+ * cell-by-cell search. */
+#define BK_STAT_SEARCH_SPILL 0
+int bk_get_stat(bk_ctx* ctx, int32_t which, int64_t* value);
+
 /* Stage timing with HIP events recorded on the context's stream (no
  * counterpart in the reference). bk_profile(ctx, 1) starts accumulating;
  * bk_profile_read returns the summed milliseconds of each stage since the last

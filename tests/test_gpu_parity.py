@@ -71,6 +71,22 @@ def test_capped_searches_bit_identical(engine, c1, kw):
     assert_neighbours(engine, d, s, w.variogram, T[:1200])
 
 
+@pytest.mark.parametrize("reach", [1, 2, 3])
+def test_tiled_search_equals_the_walk(engine, c1, reach):
+    # Tiles that cannot settle every lane hand targets back (reach 1 spills
+    # many); every reach gives the oracle's sets.
+    w, d, T = c1
+    try:
+        engine.search_tuning(reach)
+        assert_neighbours(engine, d, w.search, w.variogram, T)
+        s = _abi.make_search(1, 64, 200.0, rotation=(30, 0, 0), ratios=(80 / 120, 30 / 120))
+        assert_neighbours(engine, d, s, w.variogram, T[:3000])
+        if reach == 1:
+            assert engine.stat() > 0
+    finally:
+        engine.search_tuning(0)
+
+
 def test_leave_one_out_neighbours(engine, c1):
     w, d, _ = c1
     idx = np.arange(0, w.n, 7)
