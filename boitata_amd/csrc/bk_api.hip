@@ -89,11 +89,11 @@ struct bk_ctx {
   // Grid cache, keyed by the search frame.
   bk::GridBuffers gb;
   bk::TileBuffers tiles;
-  int tile_reach = 0;  // uncapped searches on sorted tiles (0: one walk per lane)
+  int search_mode = 0;  // uncapped searches: 0 wave per target, 1..8 tiles of that reach, -1 lane per target
   GridDev grid{};
   bool grid_valid = false;
   double grid_F[9] = {0};
-  double cell_factor = 1.0;
+  double cell_factor = 0.65;  // grid side as a factor of the reference's rule (measured best for the wave search)
   ShellTable tab{};
   void* tab_storage = nullptr;
   // Scratch (grow-only).
@@ -490,7 +490,7 @@ int run_search(bk_ctx* c, SearchArgs& a, int64_t m, const double* tx, const doub
   a.count = (int32_t*)c->ncount.p;
   a.status = (int32_t*)c->sstatus.p;
   if (m == 0) return 0;
-  if (bk::launch_search(c->stream, a, m, c->tile_reach, c->tiles)) return fail(c, BK_E_HIP, "search launch failed");
+  if (bk::launch_search(c->stream, a, m, c->search_mode, c->tiles)) return fail(c, BK_E_HIP, "search launch failed");
   k_map_overflow<<<blocks_of(m), 256, 0, c->stream>>>(m, a.status);
   return 0;
 }
@@ -1136,10 +1136,10 @@ int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol
   return 0;
 }
 
-int bk_set_search_tuning(bk_ctx* c, int32_t tile_reach) {
+int bk_set_search_tuning(bk_ctx* c, int32_t mode) {
   if (!c) return BK_E_INVALID;
-  if (tile_reach < 0 || tile_reach > 8) return fail(c, BK_E_INVALID, "tile_reach is 0 (off) to 8 cells");
-  c->tile_reach = tile_reach;
+  if (mode < -1 || mode > 8) return fail(c, BK_E_INVALID, "search mode is -1, 0 or a tile reach of 1 to 8 cells");
+  c->search_mode = mode;
   return 0;
 }
 

@@ -95,11 +95,12 @@ struct TileBuffers {
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int64_t cap_keys = 0, cap_ids = 0;
-  int32_t last_spill = 0;     // targets the last tiled search left to k_knn
+  int32_t last_spill = 0;     // targets the last wave or tile search left to k_knn
 };
 void free_tiles(TileBuffers& tb);
-// tile_reach > 0: uncapped searches over all targets run on sorted tiles
-// staging the cells within tile_reach cells of each tile (see k_knn_tile).
-int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int tile_reach, TileBuffers& tb);
+// Searches without caps by `mode`: 0 one wavefront per target (k_knn_wave),
+// 1..8 sorted tiles staging the cells within `mode` cells (k_knn_tile), -1
+// one lane per target (k_knn). Capped searches always run k_search.
+int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mode, TileBuffers& tb);
 bool knn_path(const SearchArgs& a);
 }  // namespace bk

@@ -765,6 +765,194 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The same selections with one wavefront per target. The K best keys so far
+// are held sorted across the lanes (lane l has rank l, +inf padding), so the
+// K-th key is a wave-uniform threshold. The wave takes the shell table's cell
+// offsets 64 at a time (lane l one cell), offers every entry of those cells,
+// keeps the survivors (within the radius, below the threshold) in an LDS
+// buffer, and merges them into the list with bitonic networks across lanes.
+// It stops at the first offset whose shell's lower bound exceeds the
+// threshold (or the radius), exactly where k_knn stops, since the offsets are
+// sorted by that bound; visiting cells of a later shell first would only offer
+// keys the threshold rejects. A target still open after the table (its walk
+// would continue on rings) is handed to k_knn through the spill list.
+
+__device__ __forceinline__ void cx_keys(uint64_t& k, uint32_t& i, int lane, int stride, bool up) {
+  const uint64_t ok = shfl_xor_u64(k, stride);
+  const uint32_t oi = __shfl_xor(i, stride, BK_WAVE);
+  const bool lower = (lane & stride) == 0;
+  // The lower lane keeps the smaller key when sorting up.
+  const bool other_less = key_less(ok, oi, k, i);
+  const bool take = (lower == up) ? other_less : !other_less && !(ok == k && oi == i);
+  if (take) {
+    k = ok;
+    i = oi;
+  }
+}
+
+// Ascending bitonic sort of one key per lane.
+__device__ __forceinline__ void wave_sort_keys(uint64_t& k, uint32_t& i, int lane) {
+#pragma unroll
+  for (int size = 2; size <= BK_WAVE; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) cx_keys(k, i, lane, stride, (lane & size) == 0);
+  }
+}
+
+// A (sorted ascending across lanes) becomes the 64 smallest of A and B (B
+// sorted ascending), sorted.
+__device__ __forceinline__ void wave_merge_keys(uint64_t& ak, uint32_t& ai, uint64_t bk, uint32_t bi, int lane) {
+  // Reverse B, take the pairwise minimum: a bitonic sequence of the 64 smallest.
+  const int src = BK_WAVE - 1 - lane;
+  const uint32_t lo = __shfl((unsigned)(bk & 0xffffffffu), src, BK_WAVE);
+  const uint32_t hi = __shfl((unsigned)(bk >> 32), src, BK_WAVE);
+  const uint64_t rk = ((uint64_t)hi << 32) | lo;
+  const uint32_t ri = __shfl(bi, src, BK_WAVE);
+  if (key_less(rk, ri, ak, ai)) {
+    ak = rk;
+    ai = ri;
+  }
+#pragma unroll
+  for (int stride = BK_WAVE / 2; stride > 0; stride >>= 1) cx_keys(ak, ai, lane, stride, true);
+}
+
+constexpr uint64_t KEY_INF = ~0ull;
+
+__global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
+  __shared__ uint64_t buf_k[4][2 * BK_WAVE];
+  __shared__ uint32_t buf_i[4][2 * BK_WAVE];
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
+  if (b >= a.m) return;  // whole wave
+  const int64_t t = a.list ? a.list[b] : b;
+  const int64_t left_out = a.left_out ? a.left_out[t] : -1;
+  const int K = a.max_samples;
+  const GridDev& g = a.g;
+  double q0, q1, q2;
+  mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
+  uint64_t ak = KEY_INF;
+  uint32_t ai = 0xffffffffu;
+  uint64_t* bk = buf_k[wave];
+  uint32_t* bi = buf_i[wave];
+  bool open = g.start != nullptr && K > 0;
+  if (open) {
+    const int c0 = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
+    const int c1 = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
+    const int c2 = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+    const double h2 = g.h * g.h;
+    const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
+    const double slack = 1.0 - 1e-9;
+    const uint32_t n_off = a.tab.end[a.tab.n - 1];
+    int nbuf = 0;
+    auto merge = [&]() {
+      uint64_t k = lane < nbuf ? bk[lane] : KEY_INF;
+      uint32_t i = lane < nbuf ? bi[lane] : 0xffffffffu;
+      wave_sort_keys(k, i, lane);
+      wave_merge_keys(ak, ai, k, i, lane);
+      // Keep what did not fit in this merge.
+      const int rest = nbuf > BK_WAVE ? nbuf - BK_WAVE : 0;
+      uint64_t kk = 0;
+      uint32_t ii = 0;
+      if (lane < rest) {
+        kk = bk[BK_WAVE + lane];
+        ii = bi[BK_WAVE + lane];
+      }
+      wave_sync();
+      if (lane < rest) {
+        bk[lane] = kk;
+        bi[lane] = ii;
+      }
+      wave_sync();
+      nbuf = rest;
+    };
+    for (uint32_t o0 = 0; o0 < n_off; o0 += BK_WAVE) {
+      // The threshold: the K-th key (rank K - 1), +inf until K are held.
+      const uint32_t tlo = __shfl((unsigned)(ak & 0xffffffffu), K - 1, BK_WAVE);
+      const uint32_t thi = __shfl((unsigned)(ak >> 32), K - 1, BK_WAVE);
+      const uint64_t tk = ((uint64_t)thi << 32) | tlo;
+      const uint32_t ti = __shfl(ai, K - 1, BK_WAVE);
+      // Lower bound of every offset from o0 on (sorted by m).
+      const char4 f0 = a.tab.offsets[o0];
+      const int m0 = max(abs(f0.x) - 1, 0) * max(abs(f0.x) - 1, 0) + max(abs(f0.y) - 1, 0) * max(abs(f0.y) - 1, 0) +
+                     max(abs(f0.z) - 1, 0) * max(abs(f0.z) - 1, 0);
+      const double bound = fmin(h2 * (double)m0, beyond) * slack;
+      if (bound > a.radius2 || (tk != KEY_INF && bound > __longlong_as_double((long long)tk))) {
+        open = false;
+        break;
+      }
+      // Lane l: cell o0 + l.
+      uint32_t e = 0, e1 = 0;
+      const uint32_t o = o0 + lane;
+      if (o < n_off) {
+        const char4 f = a.tab.offsets[o];
+        const int x = c0 + f.x, y = c1 + f.y, z = c2 + f.z;
+        if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2]) {
+          const int id = x + g.dims[0] * (y + g.dims[1] * z);
+          e = g.start[id];
+          e1 = g.start[id + 1];
+        }
+      }
+      while (__any(e < e1)) {
+        bool keep = false;
+        uint64_t kk = 0;
+        uint32_t ii = 0;
+        if (e < e1) {
+          const double4 E = g.entries[e];
+          ++e;
+          const double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
+          const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
+          ii = (uint32_t)__double_as_longlong(E.w);
+          kk = dbits(d2);
+          keep = d2 <= a.radius2 && (int64_t)ii != left_out && key_less(kk, ii, tk, ti);
+        }
+        const uint64_t mask = __ballot(keep);
+        if (mask) {
+          const int pos = nbuf + __popcll(mask & ((1ull << lane) - 1ull));
+          if (keep) {
+            bk[pos] = kk;
+            bi[pos] = ii;
+          }
+          nbuf += __popcll(mask);
+          wave_sync();
+          if (nbuf >= BK_WAVE) merge();
+        }
+      }
+      if (nbuf > 0) merge();
+    }
+  }
+  if (open) {
+    if (lane == 0) spill[atomicAdd(spill_count, 1)] = (int32_t)t;
+    return;
+  }
+  // The selection: ranks below K with a finite key, in order.
+  const bool have = lane < K && ak != KEY_INF;
+  const int n = __popcll(__ballot(have));
+  int* out = a.idx + t * a.stride;
+  if (have) out[lane] = (int)ai;
+  bi[lane] = ai;
+  wave_sync();
+  if (lane == 0) {
+    int status = n < a.min_samples ? BK_SEARCH_FAILED : BK_OK;
+    if (status == BK_OK && a.min_holes >= 0) {
+      int distinct = 0;
+      for (int k = 0; k < n; ++k) {
+        const int hk = a.hole ? a.hole[bi[k]] : -1;
+        if (hk < 0) {
+          distinct++;
+          continue;
+        }
+        bool seen = false;
+        for (int j = 0; j < k && !seen; ++j) seen = a.hole[bi[j]] == hk;
+        distinct += !seen;
+      }
+      if (distinct < a.min_holes) status = BK_SEARCH_FAILED;
+    }
+    a.count[t] = n;
+    a.status[t] = status;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The same selections on tiles of 64 targets that lie close in the grid
 // (targets sorted by the Morton code of their cell). The wave copies the
 // entries of every cell within `reach` cells of the tile's cell box into LDS
@@ -1243,10 +1431,39 @@ void free_tiles(TileBuffers& tb) {
   tb = TileBuffers();
 }
 
-int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int tile_reach, TileBuffers& tb) {
+// One wavefront per target, the targets left open after the shell table by
+// k_knn.
+static int launch_waves(hipStream_t s, const SearchArgs& a, int64_t m, TileBuffers& tb) {
+  if (m > 2147483647LL) return -1;
+  if (grow(tb.ids, tb.cap_ids, 3 * m + 1)) return -1;
+  int32_t* spill = tb.ids + 2 * m;
+  int32_t* count = tb.ids + 3 * m;
+  HIPCHK(hipMemsetAsync(count, 0, sizeof(int32_t), s));
+  SearchArgs c = a;
+  c.m = m;
+  k_knn_wave<<<dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s>>>(c, spill, count);
+  HIPCHK(hipGetLastError());
+  int32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  tb.last_spill = h;
+  if (h > 0) {
+    SearchArgs d = a;
+    d.list = spill;
+    d.m = h;
+    const size_t heap = (size_t)a.max_samples * BK_WAVE * (sizeof(uint64_t) + sizeof(uint32_t));
+    k_knn<<<dim3((unsigned)((h + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), heap, s>>>(d);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mode, TileBuffers& tb) {
   if (n_targets <= 0) return 0;
+  tb.last_spill = 0;
   if (knn_path(a)) {
-    if (tile_reach > 0 && a.list == nullptr) return launch_tiles(s, a, n_targets, tile_reach, tb);
+    if (mode > 0 && a.list == nullptr) return launch_tiles(s, a, n_targets, mode, tb);
+    if (mode == 0) return launch_waves(s, a, n_targets, tb);
     // Selections without caps: one target per lane.
     SearchArgs c = a;
     c.m = n_targets;

@@ -116,9 +116,9 @@ class Engine:
         e = None if edges is None else (C.c_int32 * 3)(*edges)
         self._check(self._lib.bk_set_tuning(self._h, e, int(max_iter), float(tol), float(cell_factor)))
 
-    def search_tuning(self, tile_reach: int):
-        """Tiled search reach in cells (0: one cell-by-cell walk per target)."""
-        self._check(self._lib.bk_set_search_tuning(self._h, int(tile_reach)))
+    def search_tuning(self, mode: int):
+        """Uncapped searches: 0 one wave per target, 1..8 tiles of that reach, -1 one lane per target."""
+        self._check(self._lib.bk_set_search_tuning(self._h, int(mode)))
 
     def stat(self, which: int = _abi.STAT_SEARCH_SPILL) -> int:
         v = C.c_int64()

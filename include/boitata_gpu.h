@@ -205,17 +205,19 @@ int bk_loo_loss_grad(bk_ctx* ctx, const bk_variogram* vg, int32_t precision, dou
  * keep the current setting. Results do not depend on the cell factor. */
 int bk_set_tuning(bk_ctx* ctx, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor);
 
-/* Search tuning (no counterpart in the reference): searches without caps
- * (no octants, sectors or hole cap, max_samples <= 64) run on tiles of 64
- * targets sorted along a Morton curve of their grid cells, each tile staging
- * the samples of the cells within tile_reach cells of its cells; a target the
- * staged cells cannot settle is searched again cell by cell. 0 searches every
- * target cell by cell. Results do not depend on it. Default 0. */
-int bk_set_search_tuning(bk_ctx* ctx, int32_t tile_reach);
+/* Search tuning (no counterpart in the reference) for searches without caps
+ * (no octants, sectors or hole cap, max_samples <= 64). mode 0 (default):
+ * one wavefront per target walking the grid's shells 64 cells at a time with
+ * the selection held across the lanes; 1..8: tiles of 64 targets sorted along
+ * a Morton curve of their grid cells, each staging the samples of the cells
+ * within `mode` cells of its cells in LDS; -1: one lane per target walking
+ * cell by cell. Targets the first two cannot settle are searched by the
+ * last. Results do not depend on the mode. */
+int bk_set_search_tuning(bk_ctx* ctx, int32_t mode);
 
 /* Counters of the last call (no counterpart in the reference):
- * BK_STAT_SEARCH_SPILL, targets the tiled search handed back to the
- * cell-by-cell search. */
+ * BK_STAT_SEARCH_SPILL, targets the wave or tiled search handed back to the
+ * lane-per-target search. */
 #define BK_STAT_SEARCH_SPILL 0
 int bk_get_stat(bk_ctx* ctx, int32_t which, int64_t* value);
 

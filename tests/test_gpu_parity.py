@@ -71,20 +71,34 @@ def test_capped_searches_bit_identical(engine, c1, kw):
     assert_neighbours(engine, d, s, w.variogram, T[:1200])
 
 
-@pytest.mark.parametrize("reach", [1, 2, 3])
-def test_tiled_search_equals_the_walk(engine, c1, reach):
-    # Tiles that cannot settle every lane hand targets back (reach 1 spills
-    # many); every reach gives the oracle's sets.
+@pytest.mark.parametrize("mode", [-1, 1, 2, 3])
+def test_search_modes_agree(engine, c1, mode):
+    # Lane per target, and tiles of several reaches (reach 1 hands many
+    # targets back to the lane walk): every mode gives the oracle's sets.
     w, d, T = c1
     try:
-        engine.search_tuning(reach)
+        engine.search_tuning(mode)
         assert_neighbours(engine, d, w.search, w.variogram, T)
         s = _abi.make_search(1, 64, 200.0, rotation=(30, 0, 0), ratios=(80 / 120, 30 / 120))
         assert_neighbours(engine, d, s, w.variogram, T[:3000])
-        if reach == 1:
+        if mode == 1:
             assert engine.stat() > 0
     finally:
         engine.search_tuning(0)
+
+
+def test_wave_search_hands_sparse_targets_back(engine):
+    # Targets far outside the data: the shell table ends before K are found,
+    # the lane walk continues on rings.
+    rng = np.random.default_rng(5)
+    P = rng.uniform(0, 100, (2000, 3))
+    v = rng.uniform(size=2000)
+    engine.set_samples(P[:, 0], P[:, 1], P[:, 2], v)
+    d = oracle.Data(P[:, 0], P[:, 1], P[:, 2], v)
+    T = np.concatenate([rng.uniform(0, 100, (500, 3)), rng.uniform(2000, 3000, (64, 3))])
+    s = _abi.make_search(1, 16, 1e5)
+    assert_neighbours(engine, d, s, None, T)
+    assert engine.stat() > 0
 
 
 def test_leave_one_out_neighbours(engine, c1):
@@ -208,6 +222,6 @@ def test_results_are_deterministic(engine, c1):
     a = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
     engine.tuning(cell_factor=1.7)
     b = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
-    engine.tuning(cell_factor=1.0)
+    engine.tuning(cell_factor=0.65)
     assert (a["value"].view(np.uint64) == b["value"].view(np.uint64)).all()
     assert (a["variance"].view(np.uint64) == b["variance"].view(np.uint64)).all()
