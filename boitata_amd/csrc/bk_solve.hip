@@ -389,11 +389,13 @@ __device__ __forceinline__ float group_maxf(float v) {
 // LDS layout of one system of the refined form: per sample j a record of SJ
 // doubles (world x, y, z, then the projected coordinates of each frame,
 // padded to an even count), iterates (NMAX double2), pivot rows (2 x NMAX
-// floats), product vectors (NMAX float2), and for NMAX = 64 the FP64 rows of
-// the wave's lanes ([k][lane], lane-private).
+// floats), product vectors (NMAX float2), the FP64 matrix as a packed strict
+// lower triangle, and for NMAX = 64 the FP64 rows of the wave's lanes
+// ([k][lane], lane-private).
 __host__ __device__ constexpr int rec_doubles(int nf) { return (3 + 3 * nf + 1) / 2 * 2; }
 __host__ __device__ constexpr int sys32_doubles(int nmax, int nf) {
-  return nmax * rec_doubles(nf) + 2 * nmax + nmax + nmax + (nmax > 32 ? nmax * nmax : 0) + 8;
+  return nmax * rec_doubles(nf) + 2 * nmax + nmax + nmax + (nmax * (nmax + 1) / 2 + 1) / 2 * 2 +
+         (nmax > 32 ? nmax * nmax : 0) + 8;
 }
 
 // acc[j] += sill * shape(|P_r - P_j| / range) over the group's samples j for
@@ -467,6 +469,93 @@ __device__ __forceinline__ void row_cov_fast(const DevVariogram& v, const double
   for (int j = 0; j < NMAX; ++j) out[j] = ((apart >> j) & 1) ? c0 - (nug + out[j]) : c0;
 }
 
+// Each unordered pair of the group's n samples once: lane r takes the pairs
+// (r, (r + d) mod n) for d = 1 .. NMAX/2 (d = n/2 only from r < n/2 when n is
+// even, d <= (n - 1)/2 otherwise), so every lane evaluates about n/2 pairs.
+// acc[d - 1] receives C(s_r, s_j) for valid d.
+template <int NMAX, class Shape>
+__device__ __forceinline__ void half_pass(const double* PJ, int SJ, int off, const double* me, int r, int n,
+                                          double sill, double ir, bool first, uint64_t& apart, double* acc,
+                                          Shape shape) {
+#pragma clang fp contract(fast)
+  const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
+#pragma unroll
+  for (int d = 1; d <= NMAX / 2; ++d) {
+    // For small n, r + d can pass n twice; such d are not valid pairs and
+    // read sample 0 instead of beyond the records.
+    int j = r + d;
+    j = j >= n ? j - n : j;
+    j = j < n ? j : 0;
+    const double* pj = PJ + j * SJ + off;
+    const double d0 = q0 - pj[0], d1 = q1 - pj[1], d2 = q2 - pj[2];
+    const double h = sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2);
+    if (first && h > 0.0) apart |= 1ull << d;
+    acc[d - 1] += sill * shape(h * ir);
+    if (d % 4 == 0) asm volatile("" ::: "memory");
+  }
+}
+
+// The strict lower triangle of the group's covariance matrix into T (packed,
+// T[i (i + 1) / 2 + j], j < i), each pair evaluated once (see half_pass).
+template <int NMAX>
+__device__ __forceinline__ void pairs_cov_fast(const DevVariogram& v, const double* PJ, int SJ, int r, int n,
+                                               double* T) {
+#pragma clang fp contract(fast)
+  if (r >= n) return;
+  const double* me = PJ + r * SJ;
+  double acc[NMAX / 2];
+#pragma unroll
+  for (int d = 0; d < NMAX / 2; ++d) acc[d] = 0.0;
+  uint64_t apart = 0;
+  for (int s = 0; s < v.S; ++s) {
+    const int off = 3 + 3 * v.frame[s];
+    const double sill = v.sill[s], ir = v.inv_range[s];
+    const bool first = s == 0;
+    switch (v.model[s]) {
+      case BK_SPHERICAL:
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+                        [](double t) { return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t); });
+        break;
+      case BK_EXPONENTIAL:
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+                        [](double t) { return 1.0 - exp_fast(-3.0 * t); });
+        break;
+      case BK_GAUSSIAN:
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+                        [](double t) { return 1.0 - exp_fast(-3.0 * t * t); });
+        break;
+      case BK_CUBIC:
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc, [](double t) {
+          const double t2 = t * t;
+          return t >= 1.0 ? 1.0 : t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
+        });
+        break;
+      case BK_PENTASPHERICAL:
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc, [](double t) {
+          const double t2 = t * t;
+          return t >= 1.0 ? 1.0 : t * (1.875 + t2 * (-1.25 + 0.375 * t2));
+        });
+        break;
+      default:
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+                        [](double) { return __builtin_nan(""); });
+    }
+  }
+  const double c0 = v.c0, nug = v.nugget;
+  const int half = (n - 1) / 2;
+  const bool even = (n & 1) == 0;
+#pragma unroll
+  for (int d = 1; d <= NMAX / 2; ++d) {
+    const bool valid = d <= half || (even && d == n / 2 && r < n / 2);
+    if (valid) {
+      int j = r + d;
+      j = j >= n ? j - n : j;
+      const double c = ((apart >> d) & 1) ? c0 - (nug + acc[d - 1]) : c0;
+      T[j < r ? r * (r + 1) / 2 + j : j * (j + 1) / 2 + r] = c;
+    }
+  }
+}
+
 template <int NMAX>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
 k_krige32(const SolveArgs a) {
@@ -483,7 +572,8 @@ k_krige32(const SolveArgs a) {
   double* X2 = PJ + NMAX * SJ;
   float* P2 = (float*)(X2 + 2 * NMAX);
   float* B2 = P2 + 2 * NMAX;
-  double* RowL = (double*)(B2 + 2 * NMAX);  // NMAX = 64 only: [k][lane]
+  double* T = (double*)(B2 + 2 * NMAX);  // packed strict lower triangle
+  double* RowL = T + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;  // NMAX = 64 only: [k][lane]
   const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
   const bool active = sys < a.n_list;
   const int t = active ? a.list[sys] : 0;
@@ -550,15 +640,22 @@ k_krige32(const SolveArgs a) {
   // Row r of C (c0 + e on the diagonal; c0 on the diagonal and 0 elsewhere
   // beyond n) and its FP32 copy scaled by 1/c0.
   const double ic0 = 1.0 / c0;
-  double crow[NMAX];
-  row_cov_fast<NMAX>(a.v, PJ, SJ, r, crow);
+  if (go) pairs_cov_fast<NMAX>(a.v, PJ, SJ, r, n, T);
+  wave_sync();
+  // Row r of the FP64 matrix: in registers for NMAX <= 32, else in LDS
+  // ([k][lane], lane-private); the FP32 copy scaled by 1/c0 starts the
+  // elimination.
+  double crow[ROWREG ? NMAX : 1];
   float x[NMAX];
 #pragma unroll
   for (int k = 0; k < NMAX; ++k) {
-    double v = (k == r) ? (live ? c0 + e : c0) : ((live && k < n) ? crow[k] : 0.0);
-    crow[k] = v;
+    double v = (k == r) ? (live ? c0 + e : c0) : 0.0;
+    if (live && k < n && k != r) v = k < r ? T[r * (r + 1) / 2 + k] : T[k * (k + 1) / 2 + r];
+    if constexpr (ROWREG)
+      crow[k] = v;
+    else
+      RowL[k * BK_WAVE + lane] = v;
     x[k] = (float)(v * ic0);
-    if constexpr (!ROWREG) RowL[k * BK_WAVE + lane] = v;
   }
   // Gauss-Jordan: after step k, column k holds the inverse's column.
   bool bad = false;
