@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--targets", type=int, default=0, help="use the first N targets only (0: all)")
-    ap.add_argument("--loo-steps", type=int, default=50, help="Adam steps of the LOO-CV gradient leg (0: skip)")
+    ap.add_argument("--loo-steps", type=int, default=200, help="Adam steps of the LOO-CV gradient leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine shards targets across GPUs "
@@ -168,7 +168,7 @@ def main():
         sign = np.random.default_rng(4).choice([-1.0, 1.0], len(th_true))
         th0 = th_true * (1.0 + 0.3 * sign)
         th0[-3:] = th_true[-3:] + 0.3 * sign[-3:] * np.array([30.0, 20.0, 10.0])  # angles: degrees off
-        loocv.loss_and_grad(e, th0, models)  # warm-up
+        loocv.fit_adam(e, th0, models, steps=2, lr=0.02)  # warm-up (torch's first optimizer step)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         th, hist = loocv.fit_adam(e, th0, models, steps=args.loo_steps, lr=0.02)

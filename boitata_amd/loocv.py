@@ -118,7 +118,7 @@ def fit_adam(engine, theta0, models, steps=200, lr=0.01, precision=_abi.FP64, sc
             # Keep sills, nugget and ranges positive.
             n_pos = 1 + len(models) * 4
             x[:n_pos].clamp_(min=1e-6)
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         if callback:
-            callback(it, float(loss))
+            callback(it, hist[-1])
     return (x.detach() * sc_t).numpy(), hist
