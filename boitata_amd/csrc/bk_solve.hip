@@ -657,30 +657,61 @@ k_krige32(const SolveArgs a) {
       RowL[k * BK_WAVE + lane] = v;
     x[k] = (float)(v * ic0);
   }
-  // Gauss-Jordan: after step k, column k holds the inverse's column.
+  // Gauss-Jordan: after step k, column k holds the inverse's column. Steps go
+  // in pairs (k, k + 1) to halve the LDS round trips: lanes k and k + 1
+  // publish their rows, and every lane applies step k to row k + 1 itself
+  // (the same operations lane k + 1 performs), which gives the pivot row of
+  // step k + 1 without a second round trip.
   bool bad = false;
+  float* rowA = P2;
+  float* rowB = P2 + NMAX;
 #pragma unroll
-  for (int k = 0; k < NMAX; ++k) {
-    float* prow = P2 + (k & 1) * NMAX;
-    if (r == k) {
+  for (int k = 0; k < NMAX; k += 2) {
+    if (r == k || r == k + 1) {
+      float* dst = (r == k) ? rowA : rowB;
 #pragma unroll
       for (int c4 = 0; c4 < NMAX / 4; ++c4)
-        reinterpret_cast<float4*>(prow)[c4] = make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
+        reinterpret_cast<float4*>(dst)[c4] = make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
     }
     wave_sync();
-    const float p = prow[k];
-    bad |= !(p > 1e-5f);
-    const float ip = __builtin_amdgcn_rcpf(p);
-    const float sk = (r == k) ? 1.0f - ip : x[k] * ip;
+    float pa[NMAX], pb[NMAX];
 #pragma unroll
     for (int c4 = 0; c4 < NMAX / 4; ++c4) {
-      const float4 q = reinterpret_cast<const float4*>(prow)[c4];
-      const float pv[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (4 * c4 + u != k) x[4 * c4 + u] = fmaf(-sk, pv[u], x[4 * c4 + u]);
+      const float4 qa = reinterpret_cast<const float4*>(rowA)[c4];
+      const float4 qb = reinterpret_cast<const float4*>(rowB)[c4];
+      pa[4 * c4] = qa.x;
+      pa[4 * c4 + 1] = qa.y;
+      pa[4 * c4 + 2] = qa.z;
+      pa[4 * c4 + 3] = qa.w;
+      pb[4 * c4] = qb.x;
+      pb[4 * c4 + 1] = qb.y;
+      pb[4 * c4 + 2] = qb.z;
+      pb[4 * c4 + 3] = qb.w;
     }
+    // Step k.
+    const float p = pa[k];
+    bad |= !(p > 1e-5f);
+    const float ip = __builtin_amdgcn_rcpf(p);
+    const float f1 = pb[k] * ip;
+#pragma unroll
+    for (int cc = 0; cc < NMAX; ++cc)
+      if (cc != k) pb[cc] = fmaf(-f1, pa[cc], pb[cc]);
+    pb[k] = -f1;
+    const float sk = (r == k) ? 1.0f - ip : x[k] * ip;
+#pragma unroll
+    for (int cc = 0; cc < NMAX; ++cc)
+      if (cc != k) x[cc] = fmaf(-sk, pa[cc], x[cc]);
     x[k] = (r == k) ? ip : -sk;
+    // Step k + 1, pivot row pb.
+    const float p2 = pb[k + 1];
+    bad |= !(p2 > 1e-5f);
+    const float ip2 = __builtin_amdgcn_rcpf(p2);
+    const float sk2 = (r == k + 1) ? 1.0f - ip2 : x[k + 1] * ip2;
+#pragma unroll
+    for (int cc = 0; cc < NMAX; ++cc)
+      if (cc != k + 1) x[cc] = fmaf(-sk2, pb[cc], x[cc]);
+    x[k + 1] = (r == k + 1) ? ip2 : -sk2;
+    wave_sync();
   }
   // y = X v for two right-hand sides (v in FP32, one float2 per lane).
   auto apply = [&](float va, float vb, float& ya, float& yb) {
