@@ -168,16 +168,30 @@ def main():
         sign = np.random.default_rng(4).choice([-1.0, 1.0], len(th_true))
         th0 = th_true * (1.0 + 0.3 * sign)
         th0[-3:] = th_true[-3:] + 0.3 * sign[-3:] * np.array([30.0, 20.0, 10.0])  # angles: degrees off
-        loocv.fit_adam(e, th0, models, steps=2, lr=0.02)  # warm-up (torch's first optimizer step)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        th, hist = loocv.fit_adam(e, th0, models, steps=args.loo_steps, lr=0.02)
-        torch.cuda.synchronize()
-        dtl = time.perf_counter() - t0
-        loss, grad, used = loocv.loss_and_grad(e, th, models)
+
+        def adam_rate(prec, steps):
+            loocv.fit_adam(e, th0, models, steps=2, lr=0.02, precision=prec)  # warm-up (first optimizer step)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            th, hist = loocv.fit_adam(e, th0, models, steps=steps, lr=0.02, precision=prec)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, th, hist
+
+        dtl, th, hist = adam_rate(_abi.FP32_IR, args.loo_steps)
+        # The same steps with the FP64 forward and adjoint, and the agreement of
+        # the two at the final parameters.
+        dtl64, _, hist64 = adam_rate(_abi.FP64, max(args.loo_steps // 4, 1))
+        loss, grad, used = loocv.loss_and_grad(e, th, models, _abi.FP32_IR)
+        loss64, grad64, _ = loocv.loss_and_grad(e, th, models, _abi.FP64)
+        gfloor = 1e-3 * float(np.linalg.norm(grad64))
         loo_line = {"value": args.loo_steps / dtl, "unit": "Adam steps/s", "steps": args.loo_steps,
                     "ms_per_step": dtl / args.loo_steps * 1e3, "samples": w4.n, "systems_per_step": used,
-                    "loss_first": hist[0], "loss_last": hist[-1], "precision": "fp64 (adjoint on the FP64 factor)"}
+                    "parameters": len(th0), "lr": 0.02, "loss_first": hist[0], "loss_last": hist[-1],
+                    "precision": "FP32 Gauss-Jordan forward with FP64 refinement, FP32 adjoint, FP64 reduction",
+                    "fp64": {"value": max(args.loo_steps // 4, 1) / dtl64, "unit": "Adam steps/s"},
+                    "rel_err_vs_fp64": {"loss": abs(loss - loss64) / abs(loss64),
+                                        "grad": float(np.max(np.abs(grad - grad64) /
+                                                             np.maximum(np.abs(grad64), gfloor)))}}
         print(f"[bench] LOO-CV: {loo_line['value']:.1f} Adam steps/s on {w4.n} samples, loss {hist[0]:.4f} -> "
               f"{hist[-1]:.4f}", file=sys.stderr, flush=True)
         e.set_samples(sx, sy, sz, sv, None, hole)

@@ -1016,7 +1016,7 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     if (md != BK_SPHERICAL && md != BK_EXPONENTIAL && md != BK_GAUSSIAN && md != BK_CUBIC && md != BK_PENTASPHERICAL)
       return fail(c, BK_E_INVALID, "the gradient covers spherical, exponential, gaussian, cubic and pentaspherical");
   }
-  (void)precision;
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
   HIPOK(c, hipSetDevice(c->device));
   const int64_t m = c->loo_m;
   SolveArgs a;
@@ -1055,6 +1055,21 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     for (int b = 0; b < 3; ++b)
       for (int q = 0; q < 9; ++q) gp.dR[b][q] = D[b].m[q] * d2r;
   }
+  // The FP32 copy for the refined kernel.
+  GradArgs32 g32;
+  std::memset(&g32, 0, sizeof(g32));
+  g32.S = gp.S;
+  g32.P = gp.P;
+  for (int k = 0; k < gp.S; ++k) {
+    g32.model[k] = gp.model[k];
+    g32.sill[k] = (float)gp.sill[k];
+    for (int q = 0; q < 3; ++q) g32.inv_r[k][q] = (float)(1.0 / gp.r[k][q]);
+  }
+  for (int q = 0; q < 9; ++q) g32.R[q] = (float)gp.R[q];
+  for (int b = 0; b < 3; ++b)
+    for (int q = 0; q < 9; ++q) g32.dR[b][q] = (float)gp.dR[b][q];
+  a.max_iter = c->max_iter;
+  a.tol = c->tol;
   a.kind = BK_ORDINARY;
   a.sx = (const double*)c->x.p;
   a.sy = (const double*)c->y.p;
@@ -1096,7 +1111,9 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     if (hc[b] == 0) continue;
     a.list = lists + (int64_t)b * m;
     a.n_list = hc[b];
-    if (bk::launch_loo_grad(c->stream, a, gp, c->edges[b])) return fail(c, BK_E_HIP, "gradient launch failed");
+    const int r = precision == BK_FP32_IR ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->edges[b])
+                                          : bk::launch_loo_grad(c->stream, a, gp, c->edges[b]);
+    if (r) return fail(c, BK_E_HIP, "gradient launch failed");
   }
   // Systems beyond the Cholesky kernels (n > 64) stay out of the loss.
   const int64_t rb = bk::reduce_blocks(m);

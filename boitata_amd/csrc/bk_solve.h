@@ -76,11 +76,26 @@ struct GradArgs {
   double dR[3][9];                 // its derivatives per degree of each angle
 };
 
+// The same parameters in FP32 for the refined gradient kernel: inverse
+// ranges instead of ranges.
+struct GradArgs32 {
+  int S;
+  int P;
+  int model[BK_MAX_STRUCTURES];
+  float sill[BK_MAX_STRUCTURES];
+  float inv_r[BK_MAX_STRUCTURES][3];
+  float R[9];
+  float dR[3][9];
+};
+
 namespace bk {
 int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_guarded(hipStream_t s, const SolveArgs& a);
 int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax);
+// The refined form: FP32 Gauss-Jordan forward with FP64 refinement, FP32
+// adjoint accumulation.
+int launch_loo_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32, int nmax);
 // Column sums of part (cols columns of m values each, column-major) into
 // out[0..cols), in an order fixed by m alone; work holds
 // cols * reduce_blocks(m) doubles.

@@ -556,32 +556,28 @@ __device__ __forceinline__ void pairs_cov_fast(const DevVariogram& v, const doub
   }
 }
 
+// The front of the refined kernels for the system of target t (n samples;
+// lane r of group g): loads the samples as records in LDS (world coordinates
+// and the projection on every frame relative to the target, kernel.rs:36-42),
+// tests coincident samples (coincident.rs:13-15, 27-35; with report_guard the
+// system is handed to the guarded kernel), computes the right-hand side c of
+// sample r (the covariance to the target, or to the block), the FP64 matrix
+// (pairs once, packed triangle T), keeps row r of it in crow (NMAX <= 32) or
+// RowL, and leaves row r of (C / c0)^-1 in x (FP32 Gauss-Jordan; bad is set
+// where a pivot falls below 1e-5 of the scaled diagonal). Returns the guard.
 template <int NMAX>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
-k_krige32(const SolveArgs a) {
-  constexpr int G = BK_WAVE / NMAX;
+__device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, int r, int g, int lane, bool active,
+                                              bool report_guard, double* PJ, int SJ, float* P2, double* T,
+                                              double* RowL, double& z, double& e, double& c, float (&x)[NMAX],
+                                              double (&crow)[NMAX <= 32 ? NMAX : 1], bool& bad) {
   constexpr bool ROWREG = NMAX <= 32;
-  extern __shared__ double dyn[];
   const int nf = a.v.nf;
-  const int SJ = rec_doubles(nf);
-  const int per = sys32_doubles(NMAX, nf);
-  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
-  const int g = lane / NMAX, r = lane % NMAX;
-  double* S = dyn + (wave * G + g) * per;
-  double* PJ = S;
-  double* X2 = PJ + NMAX * SJ;
-  float* P2 = (float*)(X2 + 2 * NMAX);
-  float* B2 = P2 + 2 * NMAX;
-  double* T = (double*)(B2 + 2 * NMAX);  // packed strict lower triangle
-  double* RowL = T + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;  // NMAX = 64 only: [k][lane]
-  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
-  const bool active = sys < a.n_list;
-  const int t = active ? a.list[sys] : 0;
-  const int n = active ? a.ncount[t] : 0;
   const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
   // Sample r of the system: world coordinates and its projection on every
   // frame relative to the target (kernel.rs:36-42), as a record in LDS.
-  double z = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  z = 0.0;
+  e = 0.0;
   {
 #pragma clang fp contract(fast)
     double* me = PJ + r * SJ;
@@ -619,14 +615,14 @@ k_krige32(const SolveArgs a) {
     if (j % 4 == 3) asm volatile("" ::: "memory");
   }
   const bool guard = group_any<NMAX>(close, g);
-  if (guard && r == 0 && active) {
+  if (report_guard && guard && r == 0 && active) {
     a.status[t] = BK_ST_GUARD;
     a.guard[atomicAdd(a.guard_count, 1)] = t;
   }
   const bool go = active && !guard;
   const bool live = go && r < n;
   const double c0 = a.v.c0;
-  double c = 0.0;
+  c = 0.0;
   if (live) {
     if (a.block) {
       c = rhs_fast_world(a, t0, t1, t2, wx, wy, wz);
@@ -645,8 +641,6 @@ k_krige32(const SolveArgs a) {
   // Row r of the FP64 matrix: in registers for NMAX <= 32, else in LDS
   // ([k][lane], lane-private); the FP32 copy scaled by 1/c0 starts the
   // elimination.
-  double crow[ROWREG ? NMAX : 1];
-  float x[NMAX];
 #pragma unroll
   for (int k = 0; k < NMAX; ++k) {
     double v = (k == r) ? (live ? c0 + e : c0) : 0.0;
@@ -662,7 +656,7 @@ k_krige32(const SolveArgs a) {
   // publish their rows, and every lane applies step k to row k + 1 itself
   // (the same operations lane k + 1 performs), which gives the pivot row of
   // step k + 1 without a second round trip.
-  bool bad = false;
+  bad = false;
   float* rowA = P2;
   float* rowB = P2 + NMAX;
 #pragma unroll
@@ -713,6 +707,40 @@ k_krige32(const SolveArgs a) {
     x[k + 1] = (r == k + 1) ? ip2 : -sk2;
     wave_sync();
   }
+  return guard;
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
+k_krige32(const SolveArgs a) {
+  constexpr int G = BK_WAVE / NMAX;
+  constexpr bool ROWREG = NMAX <= 32;
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int SJ = rec_doubles(nf);
+  const int per = sys32_doubles(NMAX, nf);
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int g = lane / NMAX, r = lane % NMAX;
+  double* S = dyn + (wave * G + g) * per;
+  double* PJ = S;
+  double* X2 = PJ + NMAX * SJ;
+  float* P2 = (float*)(X2 + 2 * NMAX);
+  float* B2 = P2 + 2 * NMAX;
+  double* T = (double*)(B2 + 2 * NMAX);  // packed strict lower triangle
+  double* RowL = T + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;  // NMAX = 64 only: [k][lane]
+  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
+  const bool active = sys < a.n_list;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z, e, c;
+  double crow[ROWREG ? NMAX : 1];
+  float x[NMAX];
+  bool bad;
+  const bool guard = refined_front<NMAX>(a, t, n, r, g, lane, active, true, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
+  const bool go = active && !guard;
+  const bool live = go && r < n;
+  const double c0 = a.v.c0;
+  const double ic0 = 1.0 / c0;
   // y = X v for two right-hand sides (v in FP32, one float2 per lane).
   auto apply = [&](float va, float vb, float& ya, float& yb) {
     reinterpret_cast<float2*>(B2)[r] = make_float2(va, vb);
@@ -1036,6 +1064,295 @@ __global__ void __launch_bounds__(128) k_loo_grad(const SolveArgs a, const GradA
       for (int p = 0; p < gp.P; ++p) out[(2 + p) * m] = acc[p];
     } else {
       for (int p = 0; p < gp.P + 2; ++p) out[p * m] = 0.0;
+    }
+    a.value[t] = ok ? zhat : __longlong_as_double(0x7ff8000000000000LL);
+    a.variance[t] = ok ? sigma2 : __longlong_as_double(0x7ff8000000000000LL);
+    a.status[t] = ok ? BK_OK : (guard ? BK_UNSUPPORTED : BK_SINGULAR);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Leave-one-out loss and gradient in the refined form: the forward solve of
+// k_krige32 (FP32 Gauss-Jordan inverse, FP64 residuals) on three right-hand
+// sides (c, 1, z), the Schur scalar, weights and the adjoint vector in FP64,
+// and the adjoint accumulation over pairs in FP32 (the gradient is compared
+// with finite differences at 1e-5; FP32 products carry 1e-7).
+
+// shape(t) and its derivative in FP32 (model.rs:76-126, :181-311).
+__device__ __forceinline__ void shape_d32(int model, float t, float& g, float& gd) {
+#pragma clang fp contract(fast)
+  switch (model) {
+    case BK_SPHERICAL:
+      if (t >= 1.0f) {
+        g = 1.0f;
+        gd = 0.0f;
+      } else {
+        g = t * (1.5f - 0.5f * t * t);
+        gd = 1.5f - 1.5f * t * t;
+      }
+      return;
+    case BK_EXPONENTIAL: {
+      const float e = __expf(-3.0f * t);
+      g = 1.0f - e;
+      gd = 3.0f * e;
+      return;
+    }
+    case BK_GAUSSIAN: {
+      const float e = __expf(-3.0f * t * t);
+      g = 1.0f - e;
+      gd = 6.0f * t * e;
+      return;
+    }
+    case BK_CUBIC: {
+      const float t2 = t * t;
+      if (t >= 1.0f) {
+        g = 1.0f;
+        gd = 0.0f;
+      } else {
+        g = t2 * (7.0f + t * (-8.75f + t2 * (3.5f - 0.75f * t2)));
+        gd = t * (14.0f + t * (-26.25f + t2 * (17.5f * t - 5.25f * t2 * t)));
+      }
+      return;
+    }
+    case BK_PENTASPHERICAL: {
+      const float t2 = t * t;
+      if (t >= 1.0f) {
+        g = 1.0f;
+        gd = 0.0f;
+      } else {
+        g = t * (1.875f + t2 * (-1.25f + 0.375f * t2));
+        gd = 1.875f + t2 * (-3.75f + 1.875f * t2);
+      }
+      return;
+    }
+  }
+  g = gd = 0.0f;
+}
+
+// cov_grad64 in FP32: the gradient of C(p, q), q - p = d, weighted by coef,
+// into a fixed layout with compile-time indices (registers, no scratch):
+// [0] nugget, [1 + s] sill_s, [5 + 3 s + a] range a of structure s, [17 + b]
+// angle b. The packed layout of the C ABI is formed at the output.
+constexpr int G32_SILL = 1, G32_RANGE = 1 + BK_MAX_STRUCTURES, G32_ANGLE = 1 + 4 * BK_MAX_STRUCTURES;
+
+__device__ __forceinline__ void cov_grad32(const GradArgs32& gp, float d0, float d1, float d2, float coef,
+                                           float* acc) {
+#pragma clang fp contract(fast)
+  const int S = gp.S;
+  if (d0 == 0.0f && d1 == 0.0f && d2 == 0.0f) {
+    acc[0] += coef;
+#pragma unroll
+    for (int s = 0; s < BK_MAX_STRUCTURES; ++s)
+      if (s < S) acc[G32_SILL + s] += coef;
+    return;
+  }
+  const float r0 = gp.R[0] * d0 + gp.R[1] * d1 + gp.R[2] * d2;
+  const float r1 = gp.R[3] * d0 + gp.R[4] * d1 + gp.R[5] * d2;
+  const float r2 = gp.R[6] * d0 + gp.R[7] * d1 + gp.R[8] * d2;
+  float q[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    q[b][0] = gp.dR[b][0] * d0 + gp.dR[b][1] * d1 + gp.dR[b][2] * d2;
+    q[b][1] = gp.dR[b][3] * d0 + gp.dR[b][4] * d1 + gp.dR[b][5] * d2;
+    q[b][2] = gp.dR[b][6] * d0 + gp.dR[b][7] * d1 + gp.dR[b][8] * d2;
+  }
+#pragma unroll
+  for (int s = 0; s < BK_MAX_STRUCTURES; ++s) {
+    if (s >= S) break;
+    const float i0 = gp.inv_r[s][0], i1 = gp.inv_r[s][1], i2 = gp.inv_r[s][2];
+    const float u0 = r0 * i0, u1 = r1 * i1, u2 = r2 * i2;
+    const float tt = u0 * u0 + u1 * u1 + u2 * u2;
+    const float rs = __builtin_amdgcn_rsqf(tt);
+    const float t = tt * rs;
+    float g, gd;
+    shape_d32(gp.model[s], t, g, gd);
+    acc[G32_SILL + s] += coef * (1.0f - g);
+    const float k = coef * gp.sill[s] * gd * rs;
+    acc[G32_RANGE + 3 * s + 0] += k * u0 * u0 * i0;
+    acc[G32_RANGE + 3 * s + 1] += k * u1 * u1 * i1;
+    acc[G32_RANGE + 3 * s + 2] += k * u2 * u2 * i2;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      acc[G32_ANGLE + b] -= k * (u0 * q[b][0] * i0 + u1 * q[b][1] * i1 + u2 * q[b][2] * i2);
+  }
+}
+
+// Position in the C ABI's packed layout of fixed-layout index p (S structures).
+__device__ __forceinline__ int g32_packed(int p, int S) {
+  if (p < G32_RANGE) return p;  // nugget, sills
+  if (p < G32_ANGLE) return 1 + S + (p - G32_RANGE);
+  return 1 + 4 * S + (p - G32_ANGLE);
+}
+
+__device__ __forceinline__ bool g32_used(int p, int S) {
+  if (p == 0) return true;
+  if (p < G32_RANGE) return p - G32_SILL < S;
+  if (p < G32_ANGLE) return (p - G32_RANGE) / 3 < S;
+  return true;
+}
+
+// LDS doubles per system of the refined gradient kernel: records, iterates
+// (NMAX x 4 doubles), pivot rows (2 NMAX floats), product vectors (NMAX
+// float4), the packed triangle and, for NMAX = 64, the lanes' FP64 rows.
+__host__ __device__ constexpr int loo32_doubles(int nmax, int nf) {
+  return nmax * rec_doubles(nf) + 4 * nmax + nmax + 2 * nmax + (nmax * (nmax + 1) / 2 + 1) / 2 * 2 +
+         (nmax > 32 ? nmax * nmax : 0) + 8;
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
+k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
+  constexpr int G = BK_WAVE / NMAX;
+  constexpr bool ROWREG = NMAX <= 32;
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int SJ = rec_doubles(nf);
+  const int per = loo32_doubles(NMAX, nf);
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int g = lane / NMAX, r = lane % NMAX;
+  double* S = dyn + (wave * G + g) * per;
+  double* PJ = S;
+  double* X4 = PJ + NMAX * SJ;
+  float* P2 = (float*)(X4 + 4 * NMAX);
+  float* B4 = P2 + 2 * NMAX;
+  double* T = (double*)(B4 + 4 * NMAX);
+  double* RowL = T + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;
+  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
+  const bool active = sys < a.n_list;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z, e, c;
+  double crow[ROWREG ? NMAX : 1];
+  float x[NMAX];
+  bool bad;
+  const bool guard = refined_front<NMAX>(a, t, n, r, g, lane, active, false, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
+  const bool go = active && !guard;
+  const bool live = go && r < n;
+  const double c0 = a.v.c0;
+  const double ic0 = 1.0 / c0;
+  // y = X v for three right-hand sides.
+  auto apply3 = [&](float va, float vb, float vz, float& ya, float& yb, float& yz) {
+    reinterpret_cast<float4*>(B4)[r] = make_float4(va, vb, vz, 0.0f);
+    wave_sync();
+    float sa = 0.0f, sb = 0.0f, sz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+      const float4 q = reinterpret_cast<const float4*>(B4)[k];
+      sa = fmaf(x[k], q.x, sa);
+      sb = fmaf(x[k], q.y, sb);
+      sz = fmaf(x[k], q.z, sz);
+      if (k % 8 == 7) asm volatile("" ::: "memory");
+    }
+    ya = sa;
+    yb = sb;
+    yz = sz;
+    wave_sync();
+  };
+  const double ba = live ? c : 0.0, bb = live ? 1.0 : 0.0, bz = live ? z : 0.0;
+  float fa, fb, fz;
+  apply3((float)(ba * ic0), (float)(bb * ic0), (float)(bz * ic0), fa, fb, fz);
+  double xa = fa, xb = fb, xz = fz;
+  const float tolf = (float)a.tol;
+  const float ma = group_maxf<NMAX>(fabsf(fa)), mb = group_maxf<NMAX>(fabsf(fb)), mz = group_maxf<NMAX>(fabsf(fz));
+  bool conv = false;
+  for (int it = 0; it < a.max_iter; ++it) {
+    double2* X = reinterpret_cast<double2*>(X4);
+    X[2 * r] = make_double2(xa, xb);
+    X[2 * r + 1] = make_double2(xz, 0.0);
+    wave_sync();
+    double ra = ba, rb = bb, rz = bz;
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+      const double2 q0 = X[2 * k], q1 = X[2 * k + 1];
+      double ck;
+      if constexpr (ROWREG)
+        ck = crow[k];
+      else
+        ck = RowL[k * BK_WAVE + lane];
+      ra = fma(-ck, q0.x, ra);
+      rb = fma(-ck, q0.y, rb);
+      rz = fma(-ck, q1.x, rz);
+      if (k % 4 == 3) asm volatile("" ::: "memory");
+    }
+    wave_sync();
+    float da, db, dz;
+    apply3((float)(ra * ic0), (float)(rb * ic0), (float)(rz * ic0), da, db, dz);
+    xa += (double)da;
+    xb += (double)db;
+    xz += (double)dz;
+    if (group_all<NMAX>(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb && fabsf(dz) <= tolf * mz, g)) {
+      conv = true;
+      break;
+    }
+  }
+  const bool okf = go && !bad && conv;
+  // Forward quantities in FP64 (kernel.rs:86-97), adjoint vector v = K^-1 [z; 0].
+  const bool in = okf && r < n;
+  const double sa = group_sum<NMAX>(in ? xa : 0.0);
+  const double sb = group_sum<NMAX>(in ? xb : 0.0);
+  const double sz = group_sum<NMAX>(in ? xz : 0.0);
+  const double mu = (sa - 1.0) / sb;
+  const double vmu = sz / sb;
+  const double lam = xa - mu * xb;
+  const double vl = xz - vmu * xb;
+  const double zhat = group_sum<NMAX>(in ? lam * z : 0.0);
+  const double swc = group_sum<NMAX>(in ? lam * c : 0.0);
+  const double sigma2 = c0 - swc - mu;
+  const int64_t held = a.left_out_ids ? a.left_out_ids[t] : t;
+  const double zi = active ? a.sv[held] : 0.0;
+  const bool ok = okf && sigma2 > 0.0 && n > 0;
+  const double alpha = -2.0 * (zi - zhat), beta = 1.0 / sigma2;
+  double2* VE = reinterpret_cast<double2*>(X4);
+  wave_sync();
+  if (r < NMAX) VE[r] = make_double2(ok && r < n ? lam : 0.0, ok && r < n ? vl : 0.0);
+  wave_sync();
+  float acc[BK_MAX_GRAD];
+#pragma unroll
+  for (int p = 0; p < BK_MAX_GRAD; ++p) acc[p] = 0.0f;
+  if (ok && r < n) {
+    const float al = (float)alpha, be = (float)beta;
+    const float Vr = (float)lam, Er = (float)vl;
+    const double* me = PJ + r * SJ;
+    const int half = (n - 1) / 2;
+    const bool even = (n & 1) == 0;
+    const int dmax = (even && r < n / 2) ? n / 2 : half;
+    for (int d = 1; d <= dmax; ++d) {
+      int j = r + d;
+      j = j >= n ? j - n : j;
+      const double2 vj = VE[j];
+      const float Vj = (float)vj.x, Ej = (float)vj.y;
+      const float coef = -al * (Er * Vj + Ej * Vr) + 2.0f * be * Vr * Vj;
+      const double* pj = PJ + j * SJ;
+      cov_grad32(g32, (float)(me[0] - pj[0]), (float)(me[1] - pj[1]), (float)(me[2] - pj[2]), coef, acc);
+    }
+    const float coefd = (float)(-alpha * vl * lam + beta * lam * lam);
+    acc[0] += coefd;
+#pragma unroll
+    for (int s = 0; s < BK_MAX_STRUCTURES; ++s)
+      if (s < g32.S) acc[G32_SILL + s] += coefd;
+    const float gr = (float)(alpha * vl - 2.0 * beta * lam);
+    cov_grad32(g32, (float)(a.tx[t] - me[0]), (float)(a.ty[t] - me[1]), (float)(a.tz[t] - me[2]), gr, acc);
+  }
+  if (ok && r == 0) {
+    acc[0] += (float)beta;
+#pragma unroll
+    for (int s = 0; s < BK_MAX_STRUCTURES; ++s)
+      if (s < g32.S) acc[G32_SILL + s] += (float)beta;
+  }
+#pragma unroll
+  for (int p = 0; p < BK_MAX_GRAD; ++p)
+    if (g32_used(p, g32.S)) acc[p] = group_sumf<NMAX>(acc[p]);
+  if (active && r == 0) {
+    double* out = a.grad_part + t;
+    const int64_t m = gp.m;
+    if (ok) {
+      out[0] = (zi - zhat) * (zi - zhat) + log(sigma2);
+      out[m] = 1.0;
+#pragma unroll
+      for (int p = 0; p < BK_MAX_GRAD; ++p)
+        if (g32_used(p, g32.S)) out[(2 + g32_packed(p, g32.S)) * m] = (double)acc[p];
+    } else {
+      for (int p = 0; p < g32.P + 2; ++p) out[p * m] = 0.0;
     }
     a.value[t] = ok ? zhat : __longlong_as_double(0x7ff8000000000000LL);
     a.variance[t] = ok ? sigma2 : __longlong_as_double(0x7ff8000000000000LL);
@@ -1402,6 +1719,29 @@ static int run_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp) {
   allow_lds((const void*)k_loo_grad<NMAX>, sh);
   k_loo_grad<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a, gp);
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <int NMAX>
+static int run_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32) {
+  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  int64_t blocks = (a.n_list + per_block - 1) / per_block;
+  if (blocks == 0) return 0;
+  size_t sh = (size_t)per_block * loo32_doubles(NMAX, a.v.nf) * sizeof(double);
+  allow_lds((const void*)k_loo_grad32<NMAX>, sh);
+  k_loo_grad32<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a, gp, g32);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_loo_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32, int nmax) {
+  switch (nmax) {
+    case 16:
+      return run_grad32<16>(s, a, gp, g32);
+    case 32:
+      return run_grad32<32>(s, a, gp, g32);
+    case 64:
+      return run_grad32<64>(s, a, gp, g32);
+  }
+  return -1;
 }
 
 int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax) {

@@ -50,15 +50,20 @@ def oracle_loss(setup, theta):
     return float(np.mean(r * r) + np.mean(np.log(out["variance"][ok]))), int(ok.sum())
 
 
-def test_loss_matches_the_oracle(engine, setup):
-    loss, grad, used = engine.loo_loss_grad(loocv.variogram_of(THETA, MODELS))
+PRECISIONS = [(_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-9)]
+
+
+@pytest.mark.parametrize("precision,tol", PRECISIONS)
+def test_loss_matches_the_oracle(engine, setup, precision, tol):
+    loss, grad, used = engine.loo_loss_grad(loocv.variogram_of(THETA, MODELS), precision)
     ref, n = oracle_loss(setup, THETA)
     assert used == n
-    assert abs(loss - ref) <= 1e-10 * abs(ref)
+    assert abs(loss - ref) <= tol * abs(ref)
 
 
-def test_gradient_matches_central_differences(engine, setup):
-    _, grad, _ = engine.loo_loss_grad(loocv.variogram_of(THETA, MODELS))
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32_IR])
+def test_gradient_matches_central_differences(engine, setup, precision):
+    _, grad, _ = engine.loo_loss_grad(loocv.variogram_of(THETA, MODELS), precision)
     fd = np.zeros_like(THETA)
     for p in range(len(THETA)):
         h = 1e-5 * max(abs(THETA[p]), 1.0)
@@ -148,10 +153,11 @@ def dense_loss_torch(setup, theta_t):
     return total_sq / used + total_log / used
 
 
-def test_gradient_matches_dense_autograd(engine, setup):
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32_IR])
+def test_gradient_matches_dense_autograd(engine, setup, precision):
     import torch
 
-    _, grad, _ = engine.loo_loss_grad(loocv.variogram_of(THETA, MODELS))
+    _, grad, _ = engine.loo_loss_grad(loocv.variogram_of(THETA, MODELS), precision)
     th = torch.tensor(THETA, dtype=torch.float64, requires_grad=True)
     loss = dense_loss_torch(setup, th)
     loss.backward()
@@ -171,3 +177,22 @@ def test_autograd_function_and_adam_reduce_the_loss(engine, setup):
     assert np.allclose(th.grad.numpy(), grad)
     _, hist = loocv.fit_adam(engine, THETA, MODELS, steps=25, lr=0.02)
     assert hist[-1] < hist[0]
+
+
+@pytest.mark.parametrize("max_samples", [24, 40])
+def test_refined_gradient_on_larger_neighbourhoods(engine, setup, max_samples):
+    # The 32 and 64 size classes of the refined kernel against the FP64 one
+    # (itself pinned above by finite differences and autograd).
+    s = _abi.make_search(min_samples=1, max_samples=max_samples, radius=400.0, rotation=(30.0, 20.0, 0.0),
+                         ratios=(250.0 / 400.0, 80.0 / 400.0))
+    try:
+        engine.loo_prepare(s, None)
+        vg = loocv.variogram_of(THETA, MODELS)
+        l64, g64, u64 = engine.loo_loss_grad(vg, _abi.FP64)
+        l32, g32, u32 = engine.loo_loss_grad(vg, _abi.FP32_IR)
+        assert u32 == u64
+        assert abs(l32 - l64) <= 1e-9 * abs(l64)
+        floor = 1e-3 * np.linalg.norm(g64)
+        assert (np.abs(g32 - g64) / np.maximum(np.abs(g64), floor)).max() <= 1e-5
+    finally:
+        engine.loo_prepare(SEARCH, None)
