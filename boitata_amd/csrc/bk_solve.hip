@@ -79,6 +79,10 @@ __device__ __forceinline__ bool group_any(bool p, int g) {
   return (m & gm) != 0;
 }
 
+// Doubles of one sample record in LDS: world x, y, z, then the projected
+// coordinates on every frame, padded to an even count.
+__host__ __device__ constexpr int rec_doubles(int nf) { return (3 + 3 * nf + 1) / 2 * 2; }
+
 // LDS doubles per system.
 __host__ __device__ constexpr int sys_doubles(int nmax, int nf) { return nmax * (nmax + 1) / 2 + 5 * nmax + nf * 3 * nmax + 8; }
 
@@ -247,47 +251,129 @@ __device__ __forceinline__ double rhs_fast_world(const SolveArgs& a, double t0, 
 
 // ---------------------------------------------------------------------------
 // FP64, the reference's operation order.
+//
+// LDS per system: the packed lower triangle with the diagonal (the factor in
+// place), the sample records (world coordinates and frame projections, as
+// the refined kernels) and a few scalars.
+__host__ __device__ constexpr int sys64_doubles(int nmax, int nf) {
+  return (nmax * (nmax + 1) / 2 + 1) / 2 * 2 + nmax * ((3 + 3 * nf + 1) / 2 * 2) + 8;
+}
+
+// The reference-order front of the FP64 kernels: records, coincident test,
+// right-hand side c of sample r, and the matrix (each pair once, balanced over
+// the lanes as in pairs_cov_fast) into L with the diagonal c0 + e. Lags are
+// norm3 of frame-coordinate differences and covariances cov64, exactly the
+// reference's arithmetic (kernel.rs:36-56, composite.rs:69-85); block
+// support takes the world-coordinate forms (block.rs:100-158). Returns the
+// guard flag (coincident samples).
+template <int NMAX>
+__device__ __forceinline__ bool fp64_front(const SolveArgs& a, int t, int n, int r, int g, bool active, double* L,
+                                           double* PJ, int SJ, double& z, double& e, double& c) {
+  const int nf = a.v.nf;
+  const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  z = 0.0;
+  e = 0.0;
+  double* me = PJ + r * SJ;
+  if (r < n) {
+    const int id = a.nidx[(int64_t)t * a.stride + r];
+    wx = a.sx[id];
+    wy = a.sy[id];
+    wz = a.sz[id];
+    z = a.sv[id];
+    e = a.serr ? a.serr[id] : 0.0;
+  }
+  {
+    const double d0 = wx - t0, d1 = wy - t1, d2 = wz - t2;
+    me[0] = wx;
+    me[1] = wy;
+    me[2] = wz;
+    for (int f = 0; f < nf; ++f) {
+      double p0 = d0, p1 = d1, p2 = d2;
+      if (!a.v.ident[f]) mv3(a.v.F[f], d0, d1, d2, p0, p1, p2);
+      me[3 + 3 * f] = p0;
+      me[4 + 3 * f] = p1;
+      me[5 + 3 * f] = p2;
+    }
+  }
+  wave_sync();
+  bool close = false;
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    if (j < r && r < n) {
+      const double* pj = PJ + j * SJ;
+      const double u = wx - pj[0], vv = wy - pj[1], w = wz - pj[2];
+      close |= (u * u + vv * vv) + w * w <= EPS2;
+    }
+    if (j % 4 == 3) asm volatile("" ::: "memory");
+  }
+  const bool guard = group_any<NMAX>(close, g);
+  const bool go = active && !guard;
+  c = 0.0;
+  if (go && r < n) {
+    const double c0 = a.v.c0;
+    L[tri(r) + r] = c0 + e;
+    if (a.block) {
+      double acc = -0.0;
+      for (int p = 0; p < a.npts; ++p) {
+        const double qx = t0 + a.offs[3 * p], qy = t1 + a.offs[3 * p + 1], qz = t2 + a.offs[3 * p + 2];
+        acc += a.npts == 1 ? cov_points64(a.v, wx, wy, wz, qx, qy, qz)
+                           : block_cov_points64(a.v, wx, wy, wz, qx, qy, qz);
+      }
+      c = acc / (double)a.npts;
+    } else {
+      double h[BK_MAX_FRAMES];
+      for (int f = 0; f < nf; ++f) h[f] = norm3(me[3 + 3 * f] - 0.0, me[4 + 3 * f] - 0.0, me[5 + 3 * f] - 0.0);
+      c = cov64(a.v, h);
+    }
+    const int half = (n - 1) / 2;
+    const bool even = (n & 1) == 0;
+    const int dmax = (even && r < n / 2) ? n / 2 : half;
+    for (int d = 1; d <= dmax; ++d) {
+      int j = r + d;
+      j = j >= n ? j - n : j;
+      const double* pj = PJ + j * SJ;
+      double cij;
+      if (a.block) {
+        cij = cov_points64(a.v, wx, wy, wz, pj[0], pj[1], pj[2]);
+      } else {
+        double h[BK_MAX_FRAMES];
+        for (int f = 0; f < nf; ++f)
+          h[f] = norm3(me[3 + 3 * f] - pj[3 + 3 * f], me[4 + 3 * f] - pj[4 + 3 * f], me[5 + 3 * f] - pj[5 + 3 * f]);
+        cij = cov64(a.v, h);
+      }
+      L[j < r ? tri(r) + j : tri(j) + r] = cij;
+    }
+  }
+  wave_sync();
+  return guard;
+}
+
 template <int NMAX>
 __global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
   constexpr int G = BK_WAVE / NMAX;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
-  const int per = sys_doubles(NMAX, nf);
+  const int SJ = rec_doubles(nf);
+  const int per = sys64_doubles(NMAX, nf);
   const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
   const int g = lane / NMAX, r = lane % NMAX;
   double* S = dyn + (wave * G + g) * per;
   double* L = S;
-  double* W = L + NMAX * (NMAX + 1) / 2;
-  double* V = W + 3 * NMAX;
-  double* E = V + NMAX;
-  double* PF = E + NMAX;
-  double* misc = PF + nf * 3 * NMAX;
+  double* PJ = L + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;
+  double* misc = PJ + NMAX * SJ;
   const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
   const bool active = sys < a.n_list;
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
-  double z = 0.0, e = 0.0;
-  bool guard = load_system<NMAX>(a, t, r, n, g, W, V, E, PF, z, e);
+  double z, e, c;
+  const bool guard = fp64_front<NMAX>(a, t, n, r, g, active, L, PJ, SJ, z, e, c);
   if (guard && r == 0 && active) {
     a.status[t] = BK_ST_GUARD;
     a.guard[atomicAdd(a.guard_count, 1)] = t;
   }
   const bool go = active && !guard;
   const double c0 = a.v.c0;
-  // Packed lower triangle (kernel.rs:47-56), pairs spread over the group.
-  const int npairs = n * (n - 1) / 2;
-  if (go)
-    for (int q = r; q < npairs; q += NMAX) {
-      int i, j;
-      pair_of(q, i, j);
-      L[tri(i) + j] = pair_cov64(a, W, PF, NMAX, i, j);
-    }
-  double c = 0.0;
-  if (go && r < n) {
-    L[tri(r) + r] = c0 + e;
-    c = rhs64(a, t, W, PF, NMAX, r);
-  }
-  wave_sync();
   double reg[NMAX];
 #pragma unroll
   for (int k = 0; k < NMAX; ++k) reg[k] = (go && k <= r && r < n) ? L[tri(r) + k] : 0.0;
@@ -392,7 +478,6 @@ __device__ __forceinline__ float group_maxf(float v) {
 // floats), product vectors (NMAX float2), the FP64 matrix as a packed strict
 // lower triangle, and for NMAX = 64 the FP64 rows of the wave's lanes
 // ([k][lane], lane-private).
-__host__ __device__ constexpr int rec_doubles(int nf) { return (3 + 3 * nf + 1) / 2 * 2; }
 __host__ __device__ constexpr int sys32_doubles(int nmax, int nf) {
   return nmax * rec_doubles(nf) + 2 * nmax + nmax + nmax + (nmax * (nmax + 1) / 2 + 1) / 2 * 2 +
          (nmax > 32 ? nmax * nmax : 0) + 8;
@@ -1693,7 +1778,7 @@ static int run64(hipStream_t s, const SolveArgs& a) {
   constexpr int per_block = 2 * (BK_WAVE / NMAX);
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
-  size_t sh = shmem64(NMAX, a.v.nf);
+  size_t sh = (size_t)per_block * sys64_doubles(NMAX, a.v.nf) * sizeof(double);
   allow_lds((const void*)k_krige64<NMAX>, sh);
   k_krige64<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
