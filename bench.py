@@ -94,6 +94,12 @@ def main():
     vg, search = w.variogram, w.search
     run = lambda prec: e.krige(vg, search, tx, ty, tz, precision=prec)  # noqa: E731
 
+    # The autotuner picks grid, search mode, workgroup size and bucket edges on
+    # a strided sample of 64k targets (results do not depend on them).
+    t0 = time.perf_counter()
+    tuned = e.autotune(vg, search, tx, ty, tz, precision=_abi.FP32_IR, sample=65536)
+    tuned["seconds"] = time.perf_counter() - t0
+    print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         run(_abi.FP32_IR)
     ref = run(_abi.FP64)
@@ -248,6 +254,7 @@ def main():
         "roofline_other": other,
         "cpu_baseline": cpu,
         "loo_cv_grad": loo_line,
+        "autotune": tuned,
     }
     print(json.dumps(line), flush=True)
     e.close()

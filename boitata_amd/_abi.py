@@ -189,7 +189,12 @@ EXTRA_SIGNATURES = [
     ("bk_profile_read", C.c_int, [P, P, P]),
     ("bk_set_search_tuning", C.c_int, [P, I32]),
     ("bk_get_stat", C.c_int, [P, I32, P]),
+    ("bk_set_solve_tuning", C.c_int, [P, I32]),
+    ("bk_autotune", C.c_int, [P, P, P, I32, D, I32, I64, P, P, P, P]),
 ]
+
+AUTOTUNE_REPORT = ("cell_factor", "search_mode", "waves", "edge0", "edge1", "edge2", "search_ms_before",
+                   "solve_ms_before", "search_ms_after", "solve_ms_after")
 
 STAT_SEARCH_SPILL = 0
 

@@ -90,6 +90,7 @@ struct bk_ctx {
   bk::GridBuffers gb;
   bk::TileBuffers tiles;
   int search_mode = 0;  // uncapped searches: 0 wave per target, 1..8 tiles of that reach, -1 lane per target
+  int solve_waves = 2;  // wavefronts per workgroup of the solve kernels
   GridDev grid{};
   bool grid_valid = false;
   double grid_F[9] = {0};
@@ -523,6 +524,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
   a.status = status;
   a.max_iter = c->max_iter;
   a.tol = c->tol;
+  a.waves = c->solve_waves;
   if (blk) {
     // block.rs:40-70 and py/estimation.rs:947-958: offsets along the block frame.
     int npts = blk->nx * blk->ny * blk->nz;
@@ -1070,6 +1072,7 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     for (int q = 0; q < 9; ++q) g32.dR[b][q] = (float)gp.dR[b][q];
   a.max_iter = c->max_iter;
   a.tol = c->tol;
+  a.waves = c->solve_waves;
   a.kind = BK_ORDINARY;
   a.sx = (const double*)c->x.p;
   a.sy = (const double*)c->y.p;
@@ -1160,6 +1163,13 @@ int bk_set_search_tuning(bk_ctx* c, int32_t mode) {
   return 0;
 }
 
+int bk_set_solve_tuning(bk_ctx* c, int32_t waves) {
+  if (!c) return BK_E_INVALID;
+  if (waves != 1 && waves != 2 && waves != 4) return fail(c, BK_E_INVALID, "waves per workgroup is 1, 2 or 4");
+  c->solve_waves = waves;
+  return 0;
+}
+
 int bk_get_stat(bk_ctx* c, int32_t which, int64_t* value) {
   if (!c || !value) return BK_E_INVALID;
   switch (which) {
@@ -1189,6 +1199,95 @@ int bk_profile_read(bk_ctx* c, double* ms6, int64_t* calls) {
   }
   if (calls) *calls = c->prof_calls;
   c->prof_calls = 0;
+  return 0;
+}
+
+
+int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                int32_t precision, int64_t m, const double* tx, const double* ty, const double* tz, double* report) {
+  if (!c || !vg || !s) return BK_E_INVALID;
+  if (m <= 0 || !tx || !ty || !tz) return fail(c, BK_E_INVALID, "autotune needs targets");
+  HIPOK(c, hipSetDevice(c->device));
+  struct Scratch {
+    void* p[3] = {nullptr, nullptr, nullptr};
+    ~Scratch() {
+      for (void* q : p)
+        if (q) (void)hipFree(q);
+    }
+  } out;
+  HIPOK(c, hipMalloc(&out.p[0], m * sizeof(double)));
+  HIPOK(c, hipMalloc(&out.p[1], m * sizeof(double)));
+  HIPOK(c, hipMalloc(&out.p[2], m * sizeof(int32_t)));
+  const bool was_prof = c->prof;
+  if (int r = bk_profile(c, 1)) return r;
+  (void)bk_profile_read(c, nullptr, nullptr);
+  // One untimed pass (grid build, buffer growth), then two timed ones.
+  auto trial = [&](double& search_ms, double& solve_ms) -> int {
+    for (int k = 0; k < 3; ++k) {
+      if (k == 1) (void)bk_profile_read(c, nullptr, nullptr);
+      if (int r = krige_common(c, vg, s, kind, param, nullptr, precision, m, tx, ty, tz, nullptr, false,
+                               (double*)out.p[0], (double*)out.p[1], nullptr, nullptr, nullptr, (int32_t*)out.p[2]))
+        return r;
+    }
+    double ms[6];
+    int64_t calls = 0;
+    (void)bk_profile_read(c, ms, &calls);
+    if (calls <= 0) return fail(c, BK_E_STATE, "no timed calls");
+    search_ms = (ms[0] + ms[1]) / (double)calls;
+    solve_ms = (ms[2] + ms[3] + ms[4]) / (double)calls;
+    return 0;
+  };
+  double def_search = 0.0, def_solve = 0.0;
+  if (int r = trial(def_search, def_solve)) return r;
+  // The search: grid cell size and the uncapped search's mode.
+  const double cfs[] = {0.45, 0.55, 0.65, 0.8, 1.0};
+  const int modes[] = {0, -1, 2};
+  double best_cf = c->cell_factor, best_search = def_search;
+  int best_mode = c->search_mode;
+  for (double cf : cfs)
+    for (int mode : modes) {
+      if (cf != c->cell_factor) {
+        c->cell_factor = cf;
+        c->grid_valid = false;
+      }
+      c->search_mode = mode;
+      double sm = 0.0, so = 0.0;
+      if (int r = trial(sm, so)) return r;
+      if (sm < best_search) {
+        best_search = sm;
+        best_cf = cf;
+        best_mode = mode;
+      }
+    }
+  if (best_cf != c->cell_factor) c->grid_valid = false;
+  c->cell_factor = best_cf;
+  c->search_mode = best_mode;
+  // The solves: workgroup size and bucket edges.
+  const int ws[] = {1, 2, 4};
+  const int edge_sets[4][3] = {{16, 32, 64}, {32, 32, 64}, {16, 64, 64}, {64, 64, 64}};
+  int best_w = c->solve_waves, best_e[3] = {c->edges[0], c->edges[1], c->edges[2]};
+  double best_solve = def_solve;
+  for (int w : ws)
+    for (const auto& es : edge_sets) {
+      c->solve_waves = w;
+      for (int k = 0; k < 3; ++k) c->edges[k] = es[k];
+      double sm = 0.0, so = 0.0;
+      if (int r = trial(sm, so)) return r;
+      if (so < best_solve) {
+        best_solve = so;
+        best_w = w;
+        for (int k = 0; k < 3; ++k) best_e[k] = es[k];
+      }
+    }
+  c->solve_waves = best_w;
+  for (int k = 0; k < 3; ++k) c->edges[k] = best_e[k];
+  if (!was_prof) (void)bk_profile(c, 0);
+  if (report) {
+    const double rep[BK_AUTOTUNE_REPORT] = {best_cf,     (double)best_mode, (double)best_w, (double)best_e[0],
+                                            (double)best_e[1], (double)best_e[2], def_search,  def_solve,
+                                            best_search, best_solve};
+    std::memcpy(report, rep, sizeof(rep));
+  }
   return 0;
 }
 

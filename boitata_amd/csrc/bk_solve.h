@@ -52,6 +52,7 @@ struct SolveArgs {
   int32_t* guard_count;
   int32_t* redo;        // FP32 systems handed to the FP64 kernel
   int32_t* redo_count;
+  int waves;            // wavefronts per workgroup of the solve kernels (1, 2 or 4)
   int max_iter;         // FP32+IR refinement sweeps
   double tol;           // FP32+IR: stop when the correction is below tol * |x|
   // Per-system diagnostics of the refined form (may be null): sweeps taken.

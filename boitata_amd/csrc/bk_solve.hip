@@ -350,7 +350,7 @@ __device__ __forceinline__ bool fp64_front(const SolveArgs& a, int t, int n, int
 }
 
 template <int NMAX>
-__global__ void __launch_bounds__(128) k_krige64(const SolveArgs a) {
+__global__ void __launch_bounds__(256) k_krige64(const SolveArgs a) {
   constexpr int G = BK_WAVE / NMAX;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
@@ -796,7 +796,7 @@ __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, 
 }
 
 template <int NMAX>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
 k_krige32(const SolveArgs a) {
   constexpr int G = BK_WAVE / NMAX;
   constexpr bool ROWREG = NMAX <= 32;
@@ -991,7 +991,7 @@ __device__ __forceinline__ void cov_grad64(const GradArgs& gp, double d0, double
 }
 
 template <int NMAX>
-__global__ void __launch_bounds__(128) k_loo_grad(const SolveArgs a, const GradArgs gp) {
+__global__ void __launch_bounds__(256) k_loo_grad(const SolveArgs a, const GradArgs gp) {
   constexpr int G = BK_WAVE / NMAX;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
@@ -1285,7 +1285,7 @@ __host__ __device__ constexpr int loo32_doubles(int nmax, int nf) {
 }
 
 template <int NMAX>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
 k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
   constexpr int G = BK_WAVE / NMAX;
   constexpr bool ROWREG = NMAX <= 32;
@@ -1770,50 +1770,60 @@ static void allow_lds(const void* fn, size_t bytes) {
   (void)hipGetLastError();
 }
 
-static size_t shmem64(int nmax, int nf) { return (size_t)2 * (BK_WAVE / nmax) * sys_doubles(nmax, nf) * sizeof(double); }
-static size_t shmem32(int nmax, int nf) { return (size_t)2 * (BK_WAVE / nmax) * sys32_doubles(nmax, nf) * sizeof(double); }
+
+// Wavefronts per workgroup: the requested count (1, 2 or 4), halved until the
+// workgroup's LDS fits the 160 KiB of a CU.
+static int fit_waves(int requested, size_t bytes_per_wave) {
+  int w = requested >= 1 && requested <= 4 ? requested : 2;
+  while (w > 1 && (size_t)w * bytes_per_wave > 160 * 1024) w >>= 1;
+  return w;
+}
 
 template <int NMAX>
 static int run64(hipStream_t s, const SolveArgs& a) {
-  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * sys64_doubles(NMAX, a.v.nf) * sizeof(double));
+  const int per_block = W * (BK_WAVE / NMAX);
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
   size_t sh = (size_t)per_block * sys64_doubles(NMAX, a.v.nf) * sizeof(double);
   allow_lds((const void*)k_krige64<NMAX>, sh);
-  k_krige64<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
+  k_krige64<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 template <int NMAX>
 static int run32(hipStream_t s, const SolveArgs& a) {
-  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * sys32_doubles(NMAX, a.v.nf) * sizeof(double));
+  const int per_block = W * (BK_WAVE / NMAX);
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
-  size_t sh = shmem32(NMAX, a.v.nf);
+  size_t sh = (size_t)per_block * sys32_doubles(NMAX, a.v.nf) * sizeof(double);
   allow_lds((const void*)k_krige32<NMAX>, sh);
-  k_krige32<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a);
+  k_krige32<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 template <int NMAX>
 static int run_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp) {
-  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * sys_doubles(NMAX, a.v.nf) * sizeof(double));
+  const int per_block = W * (BK_WAVE / NMAX);
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
-  size_t sh = shmem64(NMAX, a.v.nf);
+  size_t sh = (size_t)per_block * sys_doubles(NMAX, a.v.nf) * sizeof(double);
   allow_lds((const void*)k_loo_grad<NMAX>, sh);
-  k_loo_grad<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a, gp);
+  k_loo_grad<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a, gp);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 template <int NMAX>
 static int run_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32) {
-  constexpr int per_block = 2 * (BK_WAVE / NMAX);
+  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * loo32_doubles(NMAX, a.v.nf) * sizeof(double));
+  const int per_block = W * (BK_WAVE / NMAX);
   int64_t blocks = (a.n_list + per_block - 1) / per_block;
   if (blocks == 0) return 0;
   size_t sh = (size_t)per_block * loo32_doubles(NMAX, a.v.nf) * sizeof(double);
   allow_lds((const void*)k_loo_grad32<NMAX>, sh);
-  k_loo_grad32<NMAX><<<dim3((unsigned)blocks), dim3(128), sh, s>>>(a, gp, g32);
+  k_loo_grad32<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a, gp, g32);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

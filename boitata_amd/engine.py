@@ -120,6 +120,25 @@ class Engine:
         """Uncapped searches: 0 one wave per target, 1..8 tiles of that reach, -1 one lane per target."""
         self._check(self._lib.bk_set_search_tuning(self._h, int(mode)))
 
+    def solve_tuning(self, waves: int):
+        """Wavefronts per workgroup of the solve kernels (1, 2 or 4)."""
+        self._check(self._lib.bk_set_solve_tuning(self._h, int(waves)))
+
+    def autotune(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP32_IR,
+                 sample: int = 65536):
+        """Times the tunable settings on an evenly strided sample of the targets
+        and keeps the fastest in this context; returns the report as a dict."""
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        step = max(m // max(int(sample), 1), 1)
+        if step > 1:
+            tx, ty, tz = (v[::step].contiguous() if hasattr(v, "data_ptr") else np.ascontiguousarray(v[::step])
+                          for v in (tx, ty, tz))
+        rep = np.zeros(len(_abi.AUTOTUNE_REPORT))
+        self._check(self._lib.bk_autotune(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                          int(precision), int(len(tx)), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(rep)))
+        return dict(zip(_abi.AUTOTUNE_REPORT, rep.tolist()))
+
     def stat(self, which: int = _abi.STAT_SEARCH_SPILL) -> int:
         v = C.c_int64()
         self._check(self._lib.bk_get_stat(self._h, int(which), C.byref(v)))

@@ -215,6 +215,23 @@ int bk_set_tuning(bk_ctx* ctx, const int32_t* edges3, int32_t max_iter, double t
  * last. Results do not depend on the mode. */
 int bk_set_search_tuning(bk_ctx* ctx, int32_t mode);
 
+/* Workgroup size of the solve kernels (no counterpart in the reference): 1, 2
+ * (default) or 4 wavefronts. Results do not depend on it. */
+int bk_set_solve_tuning(bk_ctx* ctx, int32_t waves);
+
+/* Autotuner (no counterpart in the reference): times krige passes over the m
+ * targets given (a sample of a workload) with HIP events and keeps, in the
+ * context, the fastest grid cell factor and uncapped search mode (by the
+ * search stage) and the fastest solve workgroup size and bucket edges (by
+ * the bucket, solve and deferred stages). None of them changes results.
+ * report (may be NULL) receives BK_AUTOTUNE_REPORT doubles: cell factor,
+ * search mode, waves, edges[3], then search and solve milliseconds per pass
+ * before and after tuning. */
+#define BK_AUTOTUNE_REPORT 10
+int bk_autotune(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
+                int32_t precision, int64_t m, const double* tx, const double* ty, const double* tz,
+                double* report);
+
 /* Counters of the last call (no counterpart in the reference):
  * BK_STAT_SEARCH_SPILL, targets the wave or tiled search handed back to the
  * lane-per-target search. */

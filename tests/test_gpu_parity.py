@@ -225,3 +225,18 @@ def test_results_are_deterministic(engine, c1):
     engine.tuning(cell_factor=0.65)
     assert (a["value"].view(np.uint64) == b["value"].view(np.uint64)).all()
     assert (a["variance"].view(np.uint64) == b["variance"].view(np.uint64)).all()
+
+
+def test_autotune_keeps_results(engine, c1):
+    w, d, T = c1
+    before = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
+    rep = engine.autotune(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], sample=1500)
+    assert rep["waves"] in (1, 2, 4) and rep["search_mode"] in (-1, 0, 2)
+    assert rep["search_ms_after"] <= rep["search_ms_before"] and rep["solve_ms_after"] <= rep["solve_ms_before"]
+    after = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
+    for k in ("value", "variance", "status", "n_used"):
+        assert (np.asarray(before[k]).view(np.uint8) == np.asarray(after[k]).view(np.uint8)).all(), k
+    # Back to the defaults for the tests that follow.
+    engine.tuning(edges=(16, 32, 64), cell_factor=0.65)
+    engine.search_tuning(0)
+    engine.solve_tuning(2)
