@@ -80,8 +80,9 @@ __device__ __forceinline__ bool group_any(bool p, int g) {
 }
 
 // Doubles of one sample record in LDS: world x, y, z, then the projected
-// coordinates on every frame, padded to an even count.
-__host__ __device__ constexpr int rec_doubles(int nf) { return (3 + 3 * nf + 1) / 2 * 2; }
+// coordinates on every frame, padded to an odd count so that lanes reading
+// consecutive records (stride 2 * SJ dwords) spread over the banks.
+__host__ __device__ constexpr int rec_doubles(int nf) { return (3 + 3 * nf) | 1; }
 
 // LDS doubles per system.
 __host__ __device__ constexpr int sys_doubles(int nmax, int nf) { return nmax * (nmax + 1) / 2 + 5 * nmax + nf * 3 * nmax + 8; }
@@ -256,7 +257,7 @@ __device__ __forceinline__ double rhs_fast_world(const SolveArgs& a, double t0, 
 // place), the sample records (world coordinates and frame projections, as
 // the refined kernels) and a few scalars.
 __host__ __device__ constexpr int sys64_doubles(int nmax, int nf) {
-  return (nmax * (nmax + 1) / 2 + 1) / 2 * 2 + nmax * ((3 + 3 * nf + 1) / 2 * 2) + 8;
+  return (nmax * (nmax + 1) / 2 + 1) / 2 * 2 + nmax * rec_doubles(nf) + 8;
 }
 
 // The reference-order front of the FP64 kernels: records, coincident test,
@@ -473,8 +474,7 @@ __device__ __forceinline__ float group_maxf(float v) {
 }
 
 // LDS layout of one system of the refined form: per sample j a record of SJ
-// doubles (world x, y, z, then the projected coordinates of each frame,
-// padded to an even count), iterates (NMAX double2), pivot rows (2 x NMAX
+// doubles (rec_doubles), iterates (NMAX double2), pivot rows (2 x NMAX
 // floats), product vectors (NMAX float2), the FP64 matrix as a packed strict
 // lower triangle, and for NMAX = 64 the FP64 rows of the wave's lanes
 // ([k][lane], lane-private).
