@@ -15,7 +15,18 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libboitata_gpu.so"
-SOURCES = ["bk_search.hip", "bk_solve.hip", "bk_api.hip"]
+# (source, object, extra flags). bk_solve.hip is built as two objects: the
+# kriging pass with its persistent solve loops (BK_PART 1), where machine LICM
+# would hoist FP64 constants (polynomial coefficients, zeroed accumulators)
+# out of the loop over work units and spill them, so it is off there; and the
+# refined gradient (BK_PART 2), which has no such loop and keeps it.
+UNITS = [
+    ("bk_search.hip", "bk_search.o", []),
+    ("bk_solve.hip", "bk_solve_pass.o", ["-DBK_PART=1", "-mllvm", "-disable-machine-licm"]),
+    ("bk_solve.hip", "bk_solve_grad.o", ["-DBK_PART=2"]),
+    ("bk_api.hip", "bk_api.o", []),
+]
+SOURCES = sorted({u[0] for u in UNITS})
 ARCH = os.environ.get("BK_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}",
@@ -34,9 +45,9 @@ def build(verbose: bool = False) -> Path:
     objs = []
     out_dir = CSRC / "_obj"
     out_dir.mkdir(exist_ok=True)
-    for src in SOURCES:
-        obj = out_dir / (Path(src).stem + ".o")
-        cmd = [hipcc, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    for src, name, extra in UNITS:
+        obj = out_dir / name
+        cmd = [hipcc, *FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

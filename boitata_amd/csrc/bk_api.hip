@@ -99,7 +99,7 @@ struct bk_ctx {
   void* tab_storage = nullptr;
   // Scratch (grow-only).
   Buf tx, ty, tz, left, nidx, ncount, sstatus, lists, counts, status, value, variance, lagrange, support, n_used,
-      guard, redo, offs, which, iters;
+      guard, redo, offs, which, iters, work;
   // Tuning.
   int edges[3] = {16, 32, 64};
   int max_iter = 8;
@@ -109,6 +109,25 @@ struct bk_ctx {
   int64_t loo_m = 0;
   int loo_stride = 0;
   Buf loo_idx, loo_count, loo_status;
+  // The held-out systems bucketed by neighbour count once, at prepare time:
+  // lists of loo_hc[b] samples with n in (loo_edges[b-1], loo_edges[b]].
+  Buf loo_lists;
+  int32_t loo_hc[4] = {0, 0, 0, 0};
+  int loo_edges[3] = {16, 32, 64};
+  // Graph replay of point-kriging passes over device arrays: a pass repeated
+  // with the same inputs and unchanged state (gen) is captured once and then
+  // launched as one graph.
+  bool graphs = true;
+  bool capturing = false;
+  uint64_t gen = 1;  // bumped whenever a buffer moves, the samples, grid or a tuning knob change
+  struct Graph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;  // null: seen once, not captured yet
+    uint64_t used = 0;
+  };
+  std::vector<Graph> graph_cache;  // a few passes (say FP32+IR and FP64 of one workload)
+  uint64_t graph_clock = 0;
+  int64_t graph_launches = 0;
   // Stage timing (bk_profile): events on the context's stream.
   bool prof = false;
   hipEvent_t ev[7] = {};
@@ -153,6 +172,8 @@ int fail(bk_ctx* c, int code, const std::string& m) {
 
 int ensure(bk_ctx* c, Buf& b, size_t bytes) {
   if (bytes <= b.cap) return 0;
+  if (c->capturing) return fail(c, BK_E_STATE, "a buffer would move while a pass is captured");
+  c->gen++;
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
@@ -467,6 +488,7 @@ int prepare_search(bk_ctx* c, const bk_search* s, const bk_variogram* vg, Search
       if (r) return fail(c, BK_E_HIP, "grid build failed");
       std::memcpy(c->grid_F, F.m, sizeof(F.m));
       c->grid_valid = true;
+      c->gen++;
     }
     a.g = c->grid;
   }
@@ -491,7 +513,12 @@ int run_search(bk_ctx* c, SearchArgs& a, int64_t m, const double* tx, const doub
   a.count = (int32_t*)c->ncount.p;
   a.status = (int32_t*)c->sstatus.p;
   if (m == 0) return 0;
+  const void* before[2] = {c->tiles.keys, c->tiles.ids};
   if (bk::launch_search(c->stream, a, m, c->search_mode, c->tiles)) return fail(c, BK_E_HIP, "search launch failed");
+  if (before[0] != c->tiles.keys || before[1] != c->tiles.ids) {
+    if (c->capturing) return fail(c, BK_E_STATE, "search buffers moved while a pass was captured");
+    c->gen++;
+  }
   k_map_overflow<<<blocks_of(m), 256, 0, c->stream>>>(m, a.status);
   return 0;
 }
@@ -637,22 +664,24 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
       }
     }
   }
-  // Bucket the searched targets by neighbour count.
+  // Bucket the searched targets by neighbour count. The list lengths stay on
+  // the device: every solve launch is persistent and reads its count there,
+  // so the pass has no host round trip and can be captured as one graph.
   const int nb = 3;
   if (int r = ensure(c, c->lists, std::max<int64_t>(m, 1) * (nb + 1) * sizeof(int32_t))) return r;
   if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
-  HIPOK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(int32_t), c->stream));
+  if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->redo, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
   int32_t* lists = (int32_t*)c->lists.p;
   int32_t* counts = (int32_t*)c->counts.p;
+  if (int r = ensure(c, c->work, 16 * sizeof(int32_t))) return r;
+  int32_t* work = (int32_t*)c->work.p;
+  if (bk::zero32(c->stream, counts, 8) || bk::zero32(c->stream, work, 16))
+    return fail(c, BK_E_HIP, "fill launch failed");
   if (bk::launch_bucket(c->stream, m, (const int32_t*)c->ncount.p, (const int32_t*)c->sstatus.p, status, value,
                         variance, lists, counts, m, c->edges, nb, a.v.stationary))
     return fail(c, BK_E_HIP, "bucket launch failed");
   mark(c, 3);
-  int32_t hc[8];
-  HIPOK(c, hipMemcpyAsync(hc, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPOK(c, hipStreamSynchronize(c->stream));
-  if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
-  if (int r = ensure(c, c->redo, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
   a.guard = (int32_t*)c->guard.p;
   a.redo = (int32_t*)c->redo.p;
   a.guard_count = counts + 4;
@@ -666,40 +695,40 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
     if (md != BK_SPHERICAL && md != BK_EXPONENTIAL && md != BK_GAUSSIAN && md != BK_CUBIC && md != BK_PENTASPHERICAL)
       refined = false;
   }
+  // Bucket b holds n in (edges[b-1], edges[b]]; one that no neighbour count
+  // can reach is not launched.
   for (int b = 0; b < nb; ++b) {
-    if (hc[b] == 0) continue;
+    const int lo = b ? c->edges[b - 1] : 0;
+    if (c->edges[b] <= lo || lo >= stride) continue;
     a.list = lists + (int64_t)b * m;
-    a.n_list = hc[b];
+    a.n_list = m;
+    a.n_list_dev = counts + b;
+    a.work = work + b;
     int r = refined ? bk::launch_krige32(c->stream, a, c->edges[b]) : bk::launch_krige64(c->stream, a, c->edges[b]);
     if (r) return fail(c, BK_E_HIP, "solve launch failed");
   }
   mark(c, 4);
   // Systems the FP32 form handed back: FP64 over the redo list.
   if (refined) {
-    int32_t h2[8];
-    HIPOK(c, hipMemcpyAsync(h2, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPOK(c, hipStreamSynchronize(c->stream));
-    if (h2[5] > 0) {
-      SolveArgs b = a;
-      b.list = a.redo;
-      b.n_list = h2[5];
-      if (bk::launch_krige64(c->stream, b, 64)) return fail(c, BK_E_HIP, "solve launch failed");
-    }
+    SolveArgs b = a;
+    b.list = a.redo;
+    b.n_list = m;
+    b.n_list_dev = a.redo_count;
+    b.work = work + nb;
+    if (bk::launch_krige64(c->stream, b, 64)) return fail(c, BK_E_HIP, "solve launch failed");
   }
-  // The LU list (n > 64, power models) joins the guarded systems.
-  int32_t h3[8];
-  HIPOK(c, hipMemcpyAsync(h3, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPOK(c, hipStreamSynchronize(c->stream));
-  if (hc[nb] > 0) {
+  // The LU list (n > 64, power models), then the guarded systems (merged
+  // coincident samples) of every solve kernel.
+  if (!a.v.stationary || stride > BK_MAX_CHOL) {
     SolveArgs b = a;
     b.guard = lists + (int64_t)nb * m;
     b.guard_count = counts + nb;
-    b.n_list = hc[nb];
+    b.n_list = m;
     if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
   }
-  if (h3[4] > 0) {
+  {
     SolveArgs b = a;
-    b.n_list = h3[4];
+    b.n_list = m;
     if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
   }
   mark(c, 5);
@@ -748,11 +777,13 @@ void bk_destroy(bk_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (Buf* b : {&c->x, &c->y, &c->z, &c->v, &c->e, &c->hole, &c->tx, &c->ty, &c->tz, &c->left, &c->nidx, &c->ncount,
                  &c->sstatus, &c->lists, &c->counts, &c->status, &c->value, &c->variance, &c->lagrange, &c->support,
-                 &c->n_used, &c->guard, &c->redo, &c->offs, &c->which, &c->iters, &c->loo_idx, &c->loo_count,
-                 &c->loo_status})
+                 &c->n_used, &c->guard, &c->redo, &c->offs, &c->which, &c->iters, &c->work, &c->loo_idx, &c->loo_count,
+                 &c->loo_status, &c->loo_lists})
     release(*b);
   bk::free_grid(c->gb);
   bk::free_tiles(c->tiles);
+  for (auto& g : c->graph_cache)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (c->tab_storage) (void)hipFree(c->tab_storage);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -801,6 +832,7 @@ int bk_set_samples(bk_ctx* c, int64_t n, const double* x, const double* y, const
   if (c->has_hole)
     if (int r = copy(c->hole, hole, 4)) return r;
   c->n = n;
+  c->gen++;
   c->grid_valid = false;
   c->loo_ready = false;
   // Origin: the first sample with finite coordinates; planar: one elevation.
@@ -866,29 +898,14 @@ int bk_neighbors(bk_ctx* c, const bk_search* s, const bk_variogram* vg, int64_t 
   return 0;
 }
 
-static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
-                        const bk_block* blk, int32_t precision, int64_t m, const double* tx, const double* ty,
-                        const double* tz, const int64_t* which_loo, bool loo, double* value, double* variance,
-                        double* lagrange, double* support, int32_t* n_used, int32_t* status) {
-  if (!c || !vg || !s) return BK_E_INVALID;
-  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
-  if (kind < BK_ORDINARY || kind > BK_INDICATOR) return fail(c, BK_E_INVALID, "unknown kind");
-  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
-  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
-  bool stationary = true;
-  for (int k = 0; k < vg->n_structures; ++k) stationary = stationary && vg->structures[k].model != BK_POWER;
-  if (kind == BK_SIMPLE && !stationary)
-    return fail(c, BK_E_INVALID,
-                "simple kriging needs a variogram with a sill: every structure must be stationary (not power); use "
-                "ordinary kriging");
-  if (blk) {
-    if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
-      return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
-    for (int d = 0; d < 3; ++d)
-      if (!(std::isfinite(blk->size[d]) && blk->size[d] >= 0.0))
-        return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
-  }
-  HIPOK(c, hipSetDevice(c->device));
+// The pass on the context's stream, without the closing synchronisation:
+// search, bucketing, solves and output copies. It neither allocates nor waits
+// on the host when the buffers already fit and the grid is current, which is
+// what lets it be captured into a graph.
+static int krige_enqueue(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                         const bk_block* blk, int32_t precision, int64_t m, const double* tx, const double* ty,
+                         const double* tz, const int64_t* which_loo, bool loo, double* value, double* variance,
+                         double* lagrange, double* support, int32_t* n_used, int32_t* status) {
   mark(c, 0);
   SearchArgs a;
   int64_t stride;
@@ -948,8 +965,125 @@ static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, i
   if (status)
     if (int r = stage_out(c, status, ost, m)) return r;
   mark(c, 6);
+  return 0;
+}
+
+// Inputs and state that fix a captured pass: the variogram, search and call
+// arguments, the array addresses and the context's generation.
+static std::vector<unsigned char> pass_key(const bk_ctx* c, const bk_variogram* vg, const bk_search* s,
+                                           int32_t kind, double param, int32_t precision, int64_t m,
+                                           const void* const* ptrs, int n_ptrs) {
+  std::vector<unsigned char> k;
+  auto put = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    k.insert(k.end(), b, b + n);
+  };
+  put(vg, sizeof(*vg));
+  put(s, sizeof(*s));
+  put(&kind, sizeof(kind));
+  put(&param, sizeof(param));
+  put(&precision, sizeof(precision));
+  put(&m, sizeof(m));
+  put(ptrs, n_ptrs * sizeof(void*));
+  put(&c->gen, sizeof(c->gen));
+  return k;
+}
+
+static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                        const bk_block* blk, int32_t precision, int64_t m, const double* tx, const double* ty,
+                        const double* tz, const int64_t* which_loo, bool loo, double* value, double* variance,
+                        double* lagrange, double* support, int32_t* n_used, int32_t* status) {
+  if (!c || !vg || !s) return BK_E_INVALID;
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  if (kind < BK_ORDINARY || kind > BK_INDICATOR) return fail(c, BK_E_INVALID, "unknown kind");
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
+  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
+  bool stationary = true;
+  for (int k = 0; k < vg->n_structures; ++k) stationary = stationary && vg->structures[k].model != BK_POWER;
+  if (kind == BK_SIMPLE && !stationary)
+    return fail(c, BK_E_INVALID,
+                "simple kriging needs a variogram with a sill: every structure must be stationary (not power); use "
+                "ordinary kriging");
+  if (blk) {
+    if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
+      return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
+    for (int d = 0; d < 3; ++d)
+      if (!(std::isfinite(blk->size[d]) && blk->size[d] >= 0.0))
+        return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
+  }
+  HIPOK(c, hipSetDevice(c->device));
+  // A point-kriging pass over device arrays, repeated with the same inputs,
+  // runs as a graph: the first repeat is captured, later ones launch it.
+  bool graph_ok = c->graphs && !c->prof && !loo && !blk && m > 0 && c->n > 0 && c->search_mode <= 0 &&
+                        is_device(tx) && is_device(ty) && is_device(tz) && is_device(value) &&
+                        is_device(variance) && (!lagrange || is_device(lagrange)) &&
+                        (!n_used || is_device(n_used)) && (!status || is_device(status));
+  const void* ptrs[8] = {tx, ty, tz, value, variance, lagrange, n_used, status};
+  bk_ctx::Graph* entry = nullptr;
+  if (graph_ok) {
+    std::vector<unsigned char> key = pass_key(c, vg, s, kind, param, precision, m, ptrs, 8);
+    for (auto& g : c->graph_cache)
+      if (g.key == key) entry = &g;
+    if (entry && entry->exec) {
+      entry->used = ++c->graph_clock;
+      HIPOK(c, hipGraphLaunch(entry->exec, c->stream));
+      HIPOK(c, hipStreamSynchronize(c->stream));
+      c->graph_launches++;
+      return 0;
+    }
+    if (entry) {
+      // Second pass with these inputs: capture it.
+      hipGraph_t g = nullptr;
+      c->capturing = true;
+      hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+      int r = e == hipSuccess ? krige_enqueue(c, vg, s, kind, param, blk, precision, m, tx, ty, tz, which_loo, loo,
+                                              value, variance, lagrange, support, n_used, status)
+                              : BK_E_HIP;
+      if (e == hipSuccess) e = hipStreamEndCapture(c->stream, &g);
+      c->capturing = false;
+      hipGraphExec_t x = nullptr;
+      if (r == 0 && e == hipSuccess && g) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (r == 0 && e == hipSuccess && x) {
+        entry->exec = x;
+        entry->used = ++c->graph_clock;
+        HIPOK(c, hipGraphLaunch(x, c->stream));
+        HIPOK(c, hipStreamSynchronize(c->stream));
+        c->graph_launches++;
+        return 0;
+      }
+      // Not capturable here (a runtime that refuses a call made by the pass):
+      // run passes as usual from now on.
+      if (x) (void)hipGraphExecDestroy(x);
+      (void)hipGetLastError();
+      c->err.clear();
+      (void)bk_set_graphs(c, 0);
+      graph_ok = false;
+    }
+  }
+  if (int r = krige_enqueue(c, vg, s, kind, param, blk, precision, m, tx, ty, tz, which_loo, loo, value, variance,
+                            lagrange, support, n_used, status))
+    return r;
   HIPOK(c, hipStreamSynchronize(c->stream));
   accumulate(c);
+  if (graph_ok) {
+    // Remember the pass (with the state it left) for a capture next time,
+    // in place of the least recently used of at most four.
+    const size_t kMax = 4;
+    bk_ctx::Graph* slot = nullptr;
+    if (c->graph_cache.size() < kMax) {
+      c->graph_cache.emplace_back();
+      slot = &c->graph_cache.back();
+    } else {
+      slot = &c->graph_cache[0];
+      for (auto& g : c->graph_cache)
+        if (g.used < slot->used) slot = &g;
+      if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    }
+    slot->key = pass_key(c, vg, s, kind, param, precision, m, ptrs, 8);
+    slot->exec = nullptr;
+    slot->used = ++c->graph_clock;
+  }
   return 0;
 }
 
@@ -1001,7 +1135,30 @@ int bk_loo_prepare(bk_ctx* c, const bk_search* s, const bk_variogram* vg) {
     HIPOK(c, hipMemcpyAsync(c->loo_count.p, c->ncount.p, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
     HIPOK(c, hipMemcpyAsync(c->loo_status.p, c->sstatus.p, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
   }
+  // The sets do not move with the parameters, so neither do the buckets.
+  const int nb = 3;
+  if (int r = ensure(c, c->loo_lists, std::max<int64_t>(m, 1) * (nb + 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
+  for (Buf* b : {&c->value, &c->variance})
+    if (int r = ensure(c, *b, std::max<int64_t>(m, 1) * sizeof(double))) return r;
+  if (int r = ensure(c, c->status, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (bk::zero32(c->stream, (int32_t*)c->counts.p, 8)) return fail(c, BK_E_HIP, "fill launch failed");
+  DevVariogram dv;
+  DevVariogram32 dv32;
+  bool stationary = true;
+  if (vg) {
+    dev_variogram(vg, dv, dv32);
+    stationary = dv.stationary != 0;
+  }
+  if (bk::launch_bucket(c->stream, m, (const int32_t*)c->loo_count.p, (const int32_t*)c->loo_status.p,
+                        (int32_t*)c->status.p, (double*)c->value.p, (double*)c->variance.p,
+                        (int32_t*)c->loo_lists.p, (int32_t*)c->counts.p, m, c->edges, nb, stationary))
+    return fail(c, BK_E_HIP, "bucket launch failed");
+  int32_t hc[8];
+  HIPOK(c, hipMemcpyAsync(hc, c->counts.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPOK(c, hipStreamSynchronize(c->stream));
+  for (int b = 0; b <= nb; ++b) c->loo_hc[b] = hc[b];
+  for (int b = 0; b < nb; ++b) c->loo_edges[b] = c->edges[b];
   c->loo_ready = true;
   c->loo_m = m;
   c->loo_stride = (int)stride;
@@ -1095,27 +1252,24 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
   a.status = (int32_t*)c->status.p;
   a.grad_part = (double*)c->support.p;
   const int nb = 3;
-  if (int r = ensure(c, c->lists, std::max<int64_t>(m, 1) * (nb + 1) * sizeof(int32_t))) return r;
   if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
   if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->work, 16 * sizeof(int32_t))) return r;
   HIPOK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(int32_t), c->stream));
+  HIPOK(c, hipMemsetAsync(c->work.p, 0, 16 * sizeof(int32_t), c->stream));
+  // Rows of samples outside every bucket stay zero (they are out of the loss).
   if (m > 0) HIPOK(c, hipMemsetAsync(c->support.p, 0, m * cols * sizeof(double), c->stream));
-  int32_t* lists = (int32_t*)c->lists.p;
   int32_t* counts = (int32_t*)c->counts.p;
   a.guard = (int32_t*)c->guard.p;
   a.guard_count = counts + 4;
-  if (bk::launch_bucket(c->stream, m, (const int32_t*)c->loo_count.p, (const int32_t*)c->loo_status.p, a.status,
-                        a.value, a.variance, lists, counts, m, c->edges, nb, a.v.stationary))
-    return fail(c, BK_E_HIP, "bucket launch failed");
-  int32_t hc[8];
-  HIPOK(c, hipMemcpyAsync(hc, counts, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPOK(c, hipStreamSynchronize(c->stream));
+  // The buckets of bk_loo_prepare, with the edges they were made with.
   for (int b = 0; b < nb; ++b) {
-    if (hc[b] == 0) continue;
-    a.list = lists + (int64_t)b * m;
-    a.n_list = hc[b];
-    const int r = precision == BK_FP32_IR ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->edges[b])
-                                          : bk::launch_loo_grad(c->stream, a, gp, c->edges[b]);
+    if (c->loo_hc[b] == 0) continue;
+    a.list = (const int32_t*)c->loo_lists.p + (int64_t)b * m;
+    a.n_list = c->loo_hc[b];
+    a.work = (int32_t*)c->work.p + b;
+    const int r = precision == BK_FP32_IR ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b])
+                                          : bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b]);
     if (r) return fail(c, BK_E_HIP, "gradient launch failed");
   }
   // Systems beyond the Cholesky kernels (n > 64) stay out of the loss.
@@ -1149,6 +1303,7 @@ int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol
   }
   if (max_iter > 0) c->max_iter = max_iter;
   if (tol > 0) c->tol = tol;
+  c->gen++;
   if (cell_factor > 0 && cell_factor != c->cell_factor) {
     c->cell_factor = cell_factor;
     c->grid_valid = false;
@@ -1160,6 +1315,7 @@ int bk_set_search_tuning(bk_ctx* c, int32_t mode) {
   if (!c) return BK_E_INVALID;
   if (mode < -1 || mode > 8) return fail(c, BK_E_INVALID, "search mode is -1, 0 or a tile reach of 1 to 8 cells");
   c->search_mode = mode;
+  c->gen++;
   return 0;
 }
 
@@ -1167,15 +1323,37 @@ int bk_set_solve_tuning(bk_ctx* c, int32_t waves) {
   if (!c) return BK_E_INVALID;
   if (waves != 1 && waves != 2 && waves != 4) return fail(c, BK_E_INVALID, "waves per workgroup is 1, 2 or 4");
   c->solve_waves = waves;
+  c->gen++;
+  return 0;
+}
+
+int bk_set_graphs(bk_ctx* c, int32_t enable) {
+  if (!c) return BK_E_INVALID;
+  c->graphs = enable != 0;
+  (void)hipSetDevice(c->device);
+  for (auto& g : c->graph_cache)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graph_cache.clear();
   return 0;
 }
 
 int bk_get_stat(bk_ctx* c, int32_t which, int64_t* value) {
   if (!c || !value) return BK_E_INVALID;
   switch (which) {
-    case BK_STAT_SEARCH_SPILL:
-      *value = c->tiles.last_spill;
+    case BK_STAT_GRAPH_LAUNCHES:
+      *value = c->graph_launches;
       return 0;
+    case BK_STAT_SEARCH_SPILL: {
+      // The count stays on the device during the pass; read it now.
+      *value = 0;
+      if (!c->tiles.spill_count) return 0;
+      HIPOK(c, hipSetDevice(c->device));
+      int32_t h = 0;
+      HIPOK(c, hipMemcpyAsync(&h, c->tiles.spill_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPOK(c, hipStreamSynchronize(c->stream));
+      *value = h;
+      return 0;
+    }
   }
   return fail(c, BK_E_INVALID, "unknown statistic");
 }
@@ -1219,6 +1397,7 @@ int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t k
   HIPOK(c, hipMalloc(&out.p[1], m * sizeof(double)));
   HIPOK(c, hipMalloc(&out.p[2], m * sizeof(int32_t)));
   const bool was_prof = c->prof;
+  c->gen++;  // the knobs below change without the setters
   if (int r = bk_profile(c, 1)) return r;
   (void)bk_profile_read(c, nullptr, nullptr);
   // One untimed pass (grid build, buffer growth), then two timed ones.
@@ -1281,6 +1460,7 @@ int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t k
     }
   c->solve_waves = best_w;
   for (int k = 0; k < 3; ++k) c->edges[k] = best_e[k];
+  c->gen++;
   if (!was_prof) (void)bk_profile(c, 0);
   if (report) {
     const double rep[BK_AUTOTUNE_REPORT] = {best_cf,     (double)best_mode, (double)best_w, (double)best_e[0],

@@ -26,6 +26,28 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Persistent launches hand out work units through a counter in device memory
+// (zeroed before the launch): one lane of each wavefront takes the next unit,
+// so the time does not depend on how many workgroups are resident at once.
+__device__ __forceinline__ int grab_unit(int32_t* counter) {
+  int v = 0;
+  if ((threadIdx.x & (BK_WAVE - 1)) == 0) v = atomicAdd(counter, 1);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// The kernel's first argument (a struct passed by value sits at offset 0 of
+// the kernarg segment), through a pointer the optimiser cannot see through.
+// A persistent loop that reads its arguments this way reloads them (scalar
+// loads, cache hits) per work block instead of hoisting dozens of them out
+// of the loop, where they outgrow the SGPR file and spill into VGPR lanes.
+template <class T>
+__device__ __forceinline__ const T& kernarg_first() {
+  typedef const __attribute__((address_space(4))) T* KPtr;
+  KPtr p = (KPtr)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *(const T*)p;
+}
+
 __device__ __forceinline__ double bcast(double v, int lane) {
   long long b = __double_as_longlong(v);
   int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);

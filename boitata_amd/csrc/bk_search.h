@@ -61,7 +61,8 @@ struct SearchArgs {
   const double* tz;
   const int64_t* left_out;    // may be null
   const int32_t* list;        // targets to search (null: 0..m)
-  int64_t m;
+  int64_t m;                  // their number, or a bound on it when m_dev is set
+  const int32_t* m_dev;       // the number on the device (a spill count), or null
   int32_t* idx;               // m * stride
   int32_t* count;
   int32_t* status;
@@ -95,9 +96,11 @@ struct TileBuffers {
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int64_t cap_keys = 0, cap_ids = 0;
-  int32_t last_spill = 0;     // targets the last wave or tile search left to k_knn
+  const int32_t* spill_count = nullptr;  // device count of the targets the last wave or tile search left to k_knn
 };
 void free_tiles(TileBuffers& tb);
+// p[0 .. n) = 0 by a kernel (graph-safe).
+int zero32(hipStream_t s, int32_t* p, int64_t n);
 // Searches without caps by `mode`: 0 one wavefront per target (k_knn_wave),
 // 1..8 sorted tiles staging the cells within `mode` cells (k_knn_tile), -1
 // one lane per target (k_knn). Capped searches always run k_search.

@@ -643,13 +643,13 @@ struct KnnHeap {
   }
 };
 
-__global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
+// Target b of the launch (lane-wise), the launch holding mm targets.
+__device__ __forceinline__ void knn_lane(const SearchArgs& a, int64_t b, int64_t mm) {
   extern __shared__ uint64_t knn_dyn[];
   const int lane = threadIdx.x;
   const int K = a.max_samples;
   KnnHeap H{knn_dyn, (uint32_t*)(knn_dyn + (size_t)K * BK_WAVE), lane};
-  const int64_t b = (int64_t)blockIdx.x * BK_WAVE + lane;
-  const bool live = b < a.m;
+  const bool live = b < mm;
   const int64_t t = live ? (a.list ? a.list[b] : b) : 0;
   const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
   double q0 = 0, q1 = 0, q2 = 0;
@@ -762,6 +762,17 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
   }
   a.count[t] = n;
   a.status[t] = status;
+}
+
+// One lane per target; persistent over the targets when their number is read
+// on the device (a spill count).
+__global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
+  int64_t mm = a.m;
+  if (a.m_dev) mm = min<int64_t>(*a.m_dev, a.m);
+  for (int64_t base = (int64_t)blockIdx.x * BK_WAVE; base < mm; base += (int64_t)gridDim.x * BK_WAVE) {
+    knn_lane(a, base + threadIdx.x, mm);
+    wave_sync();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1136,6 +1147,14 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn_tile(const SearchArgs a, int re
 
 // ---------------------------------------------------------------------------
 // Grid build.
+// Zero fill as a kernel rather than hipMemsetAsync: an enqueued pass may be
+// captured into a graph and replayed, and its nodes are kept to kernels and
+// copies.
+__global__ void k_zero32(int32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
 __global__ void k_project(int64_t n, const double* x, const double* y, const double* z, const double* F,
                           double o0, double o1, double o2, double* px, double* py, double* pz) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1379,6 +1398,33 @@ static int grow(T*& p, int64_t& cap, int64_t n) {
   return 0;
 }
 
+// The targets a wave or tile search handed back (spill[0 .. *count), at most
+// m): k_knn over them, persistent, reading the count on the device so that
+// no host round trip sits between the two kernels.
+static int launch_spill(hipStream_t s, const SearchArgs& a, int64_t m, const int32_t* spill, const int32_t* count) {
+  SearchArgs d = a;
+  d.list = spill;
+  d.m = m;
+  d.m_dev = count;
+  const size_t heap = (size_t)a.max_samples * BK_WAVE * (sizeof(uint64_t) + sizeof(uint32_t));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_knn, BK_WAVE, heap) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                              hipSuccess) {
+    (void)hipGetLastError();
+    cus = 256;
+  }
+  const int64_t grid = std::min<int64_t>((m + BK_WAVE - 1) / BK_WAVE, (int64_t)per_cu * cus);
+  k_knn<<<dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(BK_WAVE), heap, s>>>(d);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // Uncapped selections on Morton-sorted tiles with LDS-staged candidates, the
 // targets the tile cannot settle by k_knn.
 static int launch_tiles(hipStream_t s, const SearchArgs& a, int64_t m, int reach, TileBuffers& tb) {
@@ -1402,7 +1448,7 @@ static int launch_tiles(hipStream_t s, const SearchArgs& a, int64_t m, int reach
     tb.tmp_bytes = need;
   }
   HIPCHK(hipcub::DeviceRadixSort::SortPairs(tb.tmp, need, kin, kout, iin, iout, (int)m, 0, 64, s));
-  HIPCHK(hipMemsetAsync(count, 0, sizeof(int32_t), s));
+  k_zero32<<<1, 64, 0, s>>>(count, 1);
   SearchArgs c = a;
   c.list = iout;
   c.m = m;
@@ -1410,18 +1456,14 @@ static int launch_tiles(hipStream_t s, const SearchArgs& a, int64_t m, int reach
   const size_t sh = heap + (size_t)TILE_CHUNK * sizeof(double4);
   k_knn_tile<<<dim3((unsigned)((m + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), sh, s>>>(c, reach, spill, count);
   HIPCHK(hipGetLastError());
-  int32_t h = 0;
-  HIPCHK(hipMemcpyAsync(&h, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  tb.last_spill = h;
-  if (h > 0) {
-    SearchArgs d = a;
-    d.list = spill;
-    d.m = h;
-    k_knn<<<dim3((unsigned)((h + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), heap, s>>>(d);
-    HIPCHK(hipGetLastError());
-  }
-  return 0;
+  tb.spill_count = count;
+  return launch_spill(s, a, m, spill, count);
+}
+
+int zero32(hipStream_t s, int32_t* p, int64_t n) {
+  if (n <= 0) return 0;
+  k_zero32<<<dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s>>>(p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 void free_tiles(TileBuffers& tb) {
@@ -1438,29 +1480,18 @@ static int launch_waves(hipStream_t s, const SearchArgs& a, int64_t m, TileBuffe
   if (grow(tb.ids, tb.cap_ids, 3 * m + 1)) return -1;
   int32_t* spill = tb.ids + 2 * m;
   int32_t* count = tb.ids + 3 * m;
-  HIPCHK(hipMemsetAsync(count, 0, sizeof(int32_t), s));
+  k_zero32<<<1, 64, 0, s>>>(count, 1);
   SearchArgs c = a;
   c.m = m;
   k_knn_wave<<<dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s>>>(c, spill, count);
   HIPCHK(hipGetLastError());
-  int32_t h = 0;
-  HIPCHK(hipMemcpyAsync(&h, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  tb.last_spill = h;
-  if (h > 0) {
-    SearchArgs d = a;
-    d.list = spill;
-    d.m = h;
-    const size_t heap = (size_t)a.max_samples * BK_WAVE * (sizeof(uint64_t) + sizeof(uint32_t));
-    k_knn<<<dim3((unsigned)((h + BK_WAVE - 1) / BK_WAVE)), dim3(BK_WAVE), heap, s>>>(d);
-    HIPCHK(hipGetLastError());
-  }
-  return 0;
+  tb.spill_count = count;
+  return launch_spill(s, a, m, spill, count);
 }
 
 int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mode, TileBuffers& tb) {
   if (n_targets <= 0) return 0;
-  tb.last_spill = 0;
+  tb.spill_count = nullptr;
   if (knn_path(a)) {
     if (mode > 0 && a.list == nullptr) return launch_tiles(s, a, n_targets, mode, tb);
     if (mode == 0) return launch_waves(s, a, n_targets, tb);

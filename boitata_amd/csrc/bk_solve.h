@@ -41,7 +41,9 @@ struct SolveArgs {
   const int32_t* ncount;
   int stride;
   const int32_t* list;  // systems of this launch: target ids
-  int64_t n_list;
+  int64_t n_list;       // their number, or a bound on it when n_list_dev is set
+  const int32_t* n_list_dev;  // the number on the device (written by an earlier kernel), or null
+  int32_t* work;        // work units handed out so far (zeroed before the launch): 64 / NMAX systems each
   double* value;
   double* variance;
   double* lagrange;     // may be null

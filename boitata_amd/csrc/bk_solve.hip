@@ -5,8 +5,17 @@
 // unfused arithmetic (crates/estimation/src/kernel.rs:47-113). Refinement
 // code opts into contraction locally.
 #include <algorithm>
+#include <atomic>
 
 #include "bk_solve.h"
+
+// Built as two objects: BK_PART 1 holds the kriging pass (persistent solve
+// kernels, guarded systems, bucketing) and the FP64 gradient, BK_PART 2 the
+// refined gradient and the reductions; the build compiles them with different
+// backend options. Without BK_PART the file holds everything.
+#ifndef BK_PART
+#define BK_PART 0
+#endif
 
 namespace {
 
@@ -350,21 +359,38 @@ __device__ __forceinline__ bool fp64_front(const SolveArgs& a, int t, int n, int
   return guard;
 }
 
+// The thread index, opaque to the optimiser: solve bodies compute their lane
+// addresses from it, so a persistent loop does not hoist them (dozens per
+// system) out of the loop and hold them in VGPRs across it.
+__device__ __forceinline__ int opaque_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// Systems of a launch: the device count when the list was filled on the device.
+__device__ __forceinline__ int64_t list_len(const SolveArgs& a) {
+  if (!a.n_list_dev) return a.n_list;
+  const int64_t n = *a.n_list_dev;
+  return n < a.n_list ? n : a.n_list;
+}
+
 template <int NMAX>
-__global__ void __launch_bounds__(256) k_krige64(const SolveArgs a) {
+__device__ __forceinline__ void krige64_blk(const SolveArgs& a, int64_t nl, int unit) {
   constexpr int G = BK_WAVE / NMAX;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
   const int SJ = rec_doubles(nf);
   const int per = sys64_doubles(NMAX, nf);
-  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int tid = opaque_tid();
+  const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
   const int g = lane / NMAX, r = lane % NMAX;
   double* S = dyn + (wave * G + g) * per;
   double* L = S;
   double* PJ = L + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;
   double* misc = PJ + NMAX * SJ;
-  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
-  const bool active = sys < a.n_list;
+  const int64_t sys = (int64_t)unit * G + g;
+  const bool active = sys < nl;
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
   double z, e, c;
@@ -445,6 +471,20 @@ __global__ void __launch_bounds__(256) k_krige64(const SolveArgs a) {
   double w = xa - (ordinary ? mu * xb : 0.0);
   if (active && !guard)
     finish_outputs<NMAX>(a, t, r, n, !sing, BK_SINGULAR, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(256) k_krige64(const SolveArgs a) {
+  const int64_t nl = list_len(a);
+  const int64_t units = (nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX);
+  // The next unit is taken before this one is solved, hiding the atomic.
+  int unit = grab_unit(a.work);
+  while (unit < units) {
+    const int next = grab_unit(a.work);
+    krige64_blk<NMAX>(kernarg_first<SolveArgs>(), nl, unit);
+    wave_sync();
+    unit = next;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -796,15 +836,15 @@ __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, 
 }
 
 template <int NMAX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
-k_krige32(const SolveArgs a) {
+__device__ __forceinline__ void krige32_blk(const SolveArgs& a, int64_t nl, int unit) {
   constexpr int G = BK_WAVE / NMAX;
   constexpr bool ROWREG = NMAX <= 32;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
   const int SJ = rec_doubles(nf);
   const int per = sys32_doubles(NMAX, nf);
-  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int tid = opaque_tid();
+  const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
   const int g = lane / NMAX, r = lane % NMAX;
   double* S = dyn + (wave * G + g) * per;
   double* PJ = S;
@@ -813,8 +853,8 @@ k_krige32(const SolveArgs a) {
   float* B2 = P2 + 2 * NMAX;
   double* T = (double*)(B2 + 2 * NMAX);  // packed strict lower triangle
   double* RowL = T + (NMAX * (NMAX + 1) / 2 + 1) / 2 * 2;  // NMAX = 64 only: [k][lane]
-  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
-  const bool active = sys < a.n_list;
+  const int64_t sys = (int64_t)unit * G + g;
+  const bool active = sys < nl;
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
   double z, e, c;
@@ -896,6 +936,21 @@ k_krige32(const SolveArgs a) {
   if (ok) {
     finish_outputs<NMAX>(a, t, r, n, true, BK_OK, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
     if (a.iters && r == 0) a.iters[t] = it;
+  }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
+k_krige32(const SolveArgs a) {
+  const int64_t nl = list_len(a);
+  const int64_t units = (nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX);
+  // The next unit is taken before this one is solved, hiding the atomic.
+  int unit = grab_unit(a.work);
+  while (unit < units) {
+    const int next = grab_unit(a.work);
+    krige32_blk<NMAX>(kernarg_first<SolveArgs>(), nl, unit);
+    wave_sync();
+    unit = next;
   }
 }
 
@@ -991,12 +1046,13 @@ __device__ __forceinline__ void cov_grad64(const GradArgs& gp, double d0, double
 }
 
 template <int NMAX>
-__global__ void __launch_bounds__(256) k_loo_grad(const SolveArgs a, const GradArgs gp) {
+__device__ __forceinline__ void loo_grad_blk(const SolveArgs& a, const GradArgs& gp, int64_t nl, int unit) {
   constexpr int G = BK_WAVE / NMAX;
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
   const int per = sys_doubles(NMAX, nf);
-  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int tid = opaque_tid();
+  const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
   const int g = lane / NMAX, r = lane % NMAX;
   double* S = dyn + (wave * G + g) * per;
   double* L = S;
@@ -1005,8 +1061,8 @@ __global__ void __launch_bounds__(256) k_loo_grad(const SolveArgs a, const GradA
   double* E = V + NMAX;
   double* PF = E + NMAX;
   double* misc = PF + nf * 3 * NMAX;
-  const int64_t sys = ((int64_t)blockIdx.x * (blockDim.x / BK_WAVE) + wave) * G + g;
-  const bool active = sys < a.n_list;
+  const int64_t sys = (int64_t)unit * G + g;
+  const bool active = sys < nl;
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
   double z = 0.0, e = 0.0;
@@ -1156,6 +1212,20 @@ __global__ void __launch_bounds__(256) k_loo_grad(const SolveArgs a, const GradA
   }
 }
 
+template <int NMAX>
+__global__ void __launch_bounds__(256) k_loo_grad(const SolveArgs a, const GradArgs gp) {
+  const int64_t nl = list_len(a);
+  const int64_t units = (nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX);
+  // The next unit is taken before this one is solved, hiding the atomic.
+  int unit = grab_unit(a.work);
+  while (unit < units) {
+    const int next = grab_unit(a.work);
+    loo_grad_blk<NMAX>(a, gp, nl, unit);
+    wave_sync();
+    unit = next;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Leave-one-out loss and gradient in the refined form: the forward solve of
 // k_krige32 (FP32 Gauss-Jordan inverse, FP64 residuals) on three right-hand
@@ -1293,6 +1363,9 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
   const int nf = a.v.nf;
   const int SJ = rec_doubles(nf);
   const int per = loo32_doubles(NMAX, nf);
+  // One system group per wavefront of the grid: the list length is known on
+  // the host here, and this kernel's registers fit better without a
+  // persistent loop.
   const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
   const int g = lane / NMAX, r = lane % NMAX;
   double* S = dyn + (wave * G + g) * per;
@@ -1462,6 +1535,7 @@ __device__ double block_tree(double s, double* red) {
   return red[0];
 }
 
+#if BK_PART != 1
 __global__ void __launch_bounds__(RED_T) k_reduce_part(int64_t m, const double* part, double* work) {
   __shared__ double red[RED_T];
   const int64_t nb = gridDim.x, b = blockIdx.x, col = blockIdx.y;
@@ -1481,6 +1555,7 @@ __global__ void __launch_bounds__(RED_T) k_reduce_final(int64_t nb, const double
   s = block_tree(s, red);
   if (threadIdx.x == 0) out[col] = s;
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Systems with coincident samples (coincident.rs:27-112): merged on lane 0,
@@ -1693,25 +1768,30 @@ __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, doubl
   wave_sync();
 }
 
+// One wavefront per system of a.guard[0 .. *a.guard_count), persistent: the
+// grid strides over the list.
+#if BK_PART != 2
 __global__ void __launch_bounds__(64) k_guarded(const SolveArgs a) {
   extern __shared__ double dyn[];
   double* A = dyn;
   double* b = A + LU_MAX * LU_MAX;
   double* ws = b + LU_MAX;
   int* cl = (int*)(ws + 7 * LU_MAX);
-  const int64_t sys = blockIdx.x;
-  if (sys >= *a.guard_count) return;
-  const int t = a.guard[sys];
-  if (a.ncount[t] > BK_MAX_LU) {
-    if (threadIdx.x == 0) {
-      const double nan = __longlong_as_double(0x7ff8000000000000LL);
-      a.value[t] = nan;
-      a.variance[t] = nan;
-      a.status[t] = BK_UNSUPPORTED;
+  const int64_t count = *a.guard_count;
+  for (int64_t sys = blockIdx.x; sys < count; sys += gridDim.x) {
+    const int t = a.guard[sys];
+    if (a.ncount[t] > BK_MAX_LU) {
+      if (threadIdx.x == 0) {
+        const double nan = __longlong_as_double(0x7ff8000000000000LL);
+        a.value[t] = nan;
+        a.variance[t] = nan;
+        a.status[t] = BK_UNSUPPORTED;
+      }
+      continue;
     }
-    return;
+    lu_system(a, t, A, b, ws, cl, threadIdx.x);
+    __syncthreads();
   }
-  lu_system(a, t, A, b, ws, cl, threadIdx.x);
 }
 
 __global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus, int32_t* status, double* value,
@@ -1758,6 +1838,7 @@ __global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus
     variance[t] = __longlong_as_double(0x7ff8000000000000LL);
   }
 }
+#endif
 
 }  // namespace
 
@@ -1765,7 +1846,7 @@ namespace bk {
 
 // Dynamic LDS above the default needs the attribute; a refused request must
 // not linger as the runtime's last error.
-static void allow_lds(const void* fn, size_t bytes) {
+[[maybe_unused]] static void allow_lds(const void* fn, size_t bytes) {
   if (bytes > 48 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   (void)hipGetLastError();
 }
@@ -1773,57 +1854,100 @@ static void allow_lds(const void* fn, size_t bytes) {
 
 // Wavefronts per workgroup: the requested count (1, 2 or 4), halved until the
 // workgroup's LDS fits the 160 KiB of a CU.
-static int fit_waves(int requested, size_t bytes_per_wave) {
+[[maybe_unused]] static int fit_waves(int requested, size_t bytes_per_wave) {
   int w = requested >= 1 && requested <= 4 ? requested : 2;
   while (w > 1 && (size_t)w * bytes_per_wave > 160 * 1024) w >>= 1;
   return w;
 }
 
+// Compute units of the current device (cached per device).
+[[maybe_unused]] static int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  int cu = dev >= 0 && dev < 64 ? cache[dev].load(std::memory_order_relaxed) : 0;
+  if (cu > 0) return cu;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) {
+    (void)hipGetLastError();
+    cu = 256;
+  }
+  if (dev >= 0 && dev < 64) cache[dev].store(cu, std::memory_order_relaxed);
+  return cu;
+}
+
+// Grid of a persistent launch over `work` workgroups' worth of units: the
+// workgroups the device holds at once (by the kernel's registers and LDS), at
+// most `work`. The waves take their units from a counter, so a workgroup that
+// is not resident at first only finds the counter exhausted when it starts.
+[[maybe_unused]] static unsigned persistent_grid(const void* fn, int threads, size_t sh, int64_t work) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, sh) != hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  const int64_t resident = (int64_t)per_cu * device_cus();
+  return (unsigned)std::max<int64_t>(std::min(resident, work), 1);
+}
+
+// Launch shape of a solve kernel with `doubles` of LDS per system.
+struct SolveShape {
+  int waves;
+  size_t sh;
+  int64_t work;
+};
+template <int NMAX>
+[[maybe_unused]] static SolveShape solve_shape(const SolveArgs& a, int doubles) {
+  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * doubles * sizeof(double));
+  const int per_block = W * (BK_WAVE / NMAX);
+  return SolveShape{W, (size_t)per_block * doubles * sizeof(double), (a.n_list + per_block - 1) / per_block};
+}
+
+#if BK_PART != 2
 template <int NMAX>
 static int run64(hipStream_t s, const SolveArgs& a) {
-  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * sys64_doubles(NMAX, a.v.nf) * sizeof(double));
-  const int per_block = W * (BK_WAVE / NMAX);
-  int64_t blocks = (a.n_list + per_block - 1) / per_block;
-  if (blocks == 0) return 0;
-  size_t sh = (size_t)per_block * sys64_doubles(NMAX, a.v.nf) * sizeof(double);
-  allow_lds((const void*)k_krige64<NMAX>, sh);
-  k_krige64<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a);
+  const SolveShape q = solve_shape<NMAX>(a, sys64_doubles(NMAX, a.v.nf));
+  if (q.work == 0) return 0;
+  if (!a.work) return -1;  // persistent kernels take their units from this counter
+  const void* fn = (const void*)k_krige64<NMAX>;
+  allow_lds(fn, q.sh);
+  k_krige64<NMAX><<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 template <int NMAX>
 static int run32(hipStream_t s, const SolveArgs& a) {
-  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * sys32_doubles(NMAX, a.v.nf) * sizeof(double));
-  const int per_block = W * (BK_WAVE / NMAX);
-  int64_t blocks = (a.n_list + per_block - 1) / per_block;
-  if (blocks == 0) return 0;
-  size_t sh = (size_t)per_block * sys32_doubles(NMAX, a.v.nf) * sizeof(double);
-  allow_lds((const void*)k_krige32<NMAX>, sh);
-  k_krige32<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a);
+  const SolveShape q = solve_shape<NMAX>(a, sys32_doubles(NMAX, a.v.nf));
+  if (q.work == 0) return 0;
+  if (!a.work) return -1;  // persistent kernels take their units from this counter
+  const void* fn = (const void*)k_krige32<NMAX>;
+  allow_lds(fn, q.sh);
+  k_krige32<NMAX><<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 template <int NMAX>
 static int run_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp) {
-  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * sys_doubles(NMAX, a.v.nf) * sizeof(double));
-  const int per_block = W * (BK_WAVE / NMAX);
-  int64_t blocks = (a.n_list + per_block - 1) / per_block;
-  if (blocks == 0) return 0;
-  size_t sh = (size_t)per_block * sys_doubles(NMAX, a.v.nf) * sizeof(double);
-  allow_lds((const void*)k_loo_grad<NMAX>, sh);
-  k_loo_grad<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a, gp);
+  const SolveShape q = solve_shape<NMAX>(a, sys_doubles(NMAX, a.v.nf));
+  if (q.work == 0) return 0;
+  if (!a.work) return -1;  // persistent kernels take their units from this counter
+  const void* fn = (const void*)k_loo_grad<NMAX>;
+  allow_lds(fn, q.sh);
+  k_loo_grad<NMAX><<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a, gp);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
+#endif
 
+#if BK_PART != 1
 template <int NMAX>
 static int run_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32) {
-  const int W = fit_waves(a.waves, (size_t)(BK_WAVE / NMAX) * loo32_doubles(NMAX, a.v.nf) * sizeof(double));
-  const int per_block = W * (BK_WAVE / NMAX);
-  int64_t blocks = (a.n_list + per_block - 1) / per_block;
-  if (blocks == 0) return 0;
-  size_t sh = (size_t)per_block * loo32_doubles(NMAX, a.v.nf) * sizeof(double);
-  allow_lds((const void*)k_loo_grad32<NMAX>, sh);
-  k_loo_grad32<NMAX><<<dim3((unsigned)blocks), dim3(64 * W), sh, s>>>(a, gp, g32);
+  const SolveShape q = solve_shape<NMAX>(a, loo32_doubles(NMAX, a.v.nf));
+  if (q.work == 0) return 0;
+  const void* fn = (const void*)k_loo_grad32<NMAX>;
+  allow_lds(fn, q.sh);
+  k_loo_grad32<NMAX><<<dim3((unsigned)q.work), dim3(64 * q.waves), q.sh, s>>>(a, gp, g32);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -1838,7 +1962,9 @@ int launch_loo_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, con
   }
   return -1;
 }
+#endif
 
+#if BK_PART != 2
 int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax) {
   switch (nmax) {
     case 16:
@@ -1850,7 +1976,9 @@ int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int n
   }
   return -1;
 }
+#endif
 
+#if BK_PART != 1
 int64_t reduce_blocks(int64_t m) { return m <= 0 ? 1 : (m + RED_CHUNK - 1) / RED_CHUNK; }
 
 int launch_reduce_cols(hipStream_t s, int64_t m, int cols, const double* part, double* work, double* out) {
@@ -1859,7 +1987,9 @@ int launch_reduce_cols(hipStream_t s, int64_t m, int cols, const double* part, d
   k_reduce_final<<<dim3((unsigned)cols), dim3(RED_T), 0, s>>>(nb, work, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
+#endif
 
+#if BK_PART != 2
 int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax) {
   switch (nmax) {
     case 16:
@@ -1884,13 +2014,13 @@ int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax) {
   return -1;
 }
 
-// Handles a.guard[0 .. *a.guard_count) with a grid of a.n_list blocks (an
-// upper bound on that count).
+// Handles a.guard[0 .. *a.guard_count); a.n_list bounds that count.
 int launch_guarded(hipStream_t s, const SolveArgs& a) {
   if (a.n_list <= 0) return 0;
   size_t sh = (size_t)(LU_MAX * LU_MAX + LU_MAX + 7 * LU_MAX) * sizeof(double) + (2 * LU_MAX + 4) * sizeof(int);
-  allow_lds((const void*)k_guarded, sh);
-  k_guarded<<<dim3((unsigned)a.n_list), dim3(64), sh, s>>>(a);
+  const void* fn = (const void*)k_guarded;
+  allow_lds(fn, sh);
+  k_guarded<<<dim3(persistent_grid(fn, 64, sh, a.n_list)), dim3(64), sh, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -1905,5 +2035,6 @@ int launch_bucket(hipStream_t s, int64_t m, const int32_t* count, const int32_t*
                                                                      n_buckets, stationary);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
+#endif
 
 }  // namespace bk

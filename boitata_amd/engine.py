@@ -124,6 +124,10 @@ class Engine:
         """Wavefronts per workgroup of the solve kernels (1, 2 or 4)."""
         self._check(self._lib.bk_set_solve_tuning(self._h, int(waves)))
 
+    def graphs(self, enable: bool = True):
+        """Repeated krige passes over device arrays run as a captured HIP graph (default on)."""
+        self._check(self._lib.bk_set_graphs(self._h, 1 if enable else 0))
+
     def autotune(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP32_IR,
                  sample: int = 65536):
         """Times the tunable settings on an evenly strided sample of the targets
@@ -168,11 +172,22 @@ class Engine:
             i = lambda: np.empty(m, dtype=np.int32)  # noqa: E731
         return f, i
 
-    def krige(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64):
+    def krige(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64, out=None):
+        """Point kriging of the targets. `out` (a dict as returned, e.g. by an
+        earlier call) receives the results in place: with device arrays and
+        the same arguments, a repeated call then runs as a captured graph."""
         tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
         m = int(len(tx))
-        f, i = self._outputs(m, tx)
-        value, variance, lagrange, n_used, status = f(), f(), f(), i(), i()
+        if out is not None:
+            value, variance, lagrange, n_used, status = (out[k] for k in ("value", "variance", "lagrange", "n_used",
+                                                                           "status"))
+            for a, dt in ((value, "float64"), (variance, "float64"), (lagrange, "float64"), (n_used, "int32"),
+                          (status, "int32")):
+                if len(a) != m or str(a.dtype).rsplit(".", 1)[-1] != dt:
+                    raise ValueError(f"out arrays must hold {m} values of the returned dtypes")
+        else:
+            f, i = self._outputs(m, tx)
+            value, variance, lagrange, n_used, status = f(), f(), f(), i(), i()
         self._check(self._lib.bk_krige(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
                                        int(precision), m, _ptr(tx), _ptr(ty), _ptr(tz), _ptr(value), _ptr(variance),
                                        _ptr(lagrange), _ptr(n_used), _ptr(status)))

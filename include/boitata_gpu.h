@@ -232,10 +232,20 @@ int bk_autotune(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, in
                 int32_t precision, int64_t m, const double* tx, const double* ty, const double* tz,
                 double* report);
 
-/* Counters of the last call (no counterpart in the reference):
- * BK_STAT_SEARCH_SPILL, targets the wave or tiled search handed back to the
- * lane-per-target search. */
+/* Graph replay of bk_krige (no counterpart in the reference; on by default).
+ * A bk_krige call whose targets and outputs are device arrays, made again with
+ * the same arguments while the context's samples, buffers and tuning are
+ * unchanged, is captured into a HIP graph (search, bucketing, persistent
+ * solves, deferred systems: no host round trip inside the pass) and later
+ * calls launch that graph. Results do not depend on it; stage timing
+ * (bk_profile) and the tiled search keep calls on the ordinary path. */
+int bk_set_graphs(bk_ctx* ctx, int32_t enable);
+
+/* Counters (no counterpart in the reference): BK_STAT_SEARCH_SPILL, targets
+ * the last wave or tiled search handed back to the lane-per-target search;
+ * BK_STAT_GRAPH_LAUNCHES, bk_krige calls run as a graph so far. */
 #define BK_STAT_SEARCH_SPILL 0
+#define BK_STAT_GRAPH_LAUNCHES 1
 int bk_get_stat(bk_ctx* ctx, int32_t which, int64_t* value);
 
 /* Stage timing with HIP events recorded on the context's stream (no

@@ -240,3 +240,51 @@ def test_autotune_keeps_results(engine, c1):
     engine.tuning(edges=(16, 32, 64), cell_factor=0.65)
     engine.search_tuning(0)
     engine.solve_tuning(2)
+
+
+def test_graph_replay_matches_eager(engine, c1):
+    """Repeated passes over device arrays run as a captured graph: every replay
+    equals the ordinary pass bit for bit, with other work and synchronisations
+    in between, and new samples or tuning take effect (a new capture)."""
+    import torch
+
+    w, d, T = c1
+    dev = torch.device("cuda:0")
+    tx, ty, tz = (torch.from_numpy(np.ascontiguousarray(T[:, k])).to(dev) for k in range(3))
+    keys = ("value", "variance", "lagrange", "n_used", "status")
+
+    def bits(out):
+        return {k: out[k].cpu().numpy().view(np.uint8).copy() for k in keys}
+
+    engine.graphs(False)
+    ref = bits(engine.krige(w.variogram, w.search, tx, ty, tz, precision=_abi.FP32_IR))
+    engine.graphs(True)
+    out = engine.krige(w.variogram, w.search, tx, ty, tz, precision=_abi.FP32_IR)
+    start = engine.stat(_abi.STAT_GRAPH_LAUNCHES)
+    for rep in range(5):
+        for k in keys:
+            out[k].fill_(-1)
+        engine.krige(w.variogram, w.search, tx, ty, tz, precision=_abi.FP32_IR, out=out)
+        got = bits(out)
+        for k in keys:
+            assert (got[k] == ref[k]).all(), (rep, k)
+        # Other work on the context and on torch's stream between replays.
+        engine.krige(w.variogram, w.search, T[:50, 0], T[:50, 1], T[:50, 2], precision=_abi.FP64)
+        torch.cuda.synchronize()
+    assert engine.stat(_abi.STAT_GRAPH_LAUNCHES) - start >= 3
+    # New sample values: the captured pass must not be reused.
+    engine.set_samples(w.x, w.y, w.z, 2.0 * w.value, None, w.hole)
+    for _ in range(3):
+        engine.krige(w.variogram, w.search, tx, ty, tz, precision=_abi.FP32_IR, out=out)
+    scaled = out["value"].cpu().numpy()
+    ok = ref["status"].view(np.int32) == 0
+    assert np.allclose(scaled[ok], 2.0 * ref["value"].view(np.float64)[ok], rtol=1e-12, atol=0.0)
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    # A tuning change (workgroup size) is a new capture with the same results.
+    engine.solve_tuning(1)
+    for _ in range(3):
+        engine.krige(w.variogram, w.search, tx, ty, tz, precision=_abi.FP32_IR, out=out)
+        got = bits(out)
+        for k in keys:
+            assert (got[k] == ref[k]).all(), k
+    engine.solve_tuning(2)
