@@ -674,9 +674,10 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
   if (int r = ensure(c, c->redo, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
   int32_t* lists = (int32_t*)c->lists.p;
   int32_t* counts = (int32_t*)c->counts.p;
-  if (int r = ensure(c, c->work, 16 * sizeof(int32_t))) return r;
+  // Work-unit counters of the persistent launches: a line per XCD each.
+  if (int r = ensure(c, c->work, (nb + 1) * BK_CTR_INTS * sizeof(int32_t))) return r;
   int32_t* work = (int32_t*)c->work.p;
-  if (bk::zero32(c->stream, counts, 8) || bk::zero32(c->stream, work, 16))
+  if (bk::zero32(c->stream, counts, 8) || bk::zero32(c->stream, work, (nb + 1) * BK_CTR_INTS))
     return fail(c, BK_E_HIP, "fill launch failed");
   if (bk::launch_bucket(c->stream, m, (const int32_t*)c->ncount.p, (const int32_t*)c->sstatus.p, status, value,
                         variance, lists, counts, m, c->edges, nb, a.v.stationary))
@@ -703,7 +704,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
     a.list = lists + (int64_t)b * m;
     a.n_list = m;
     a.n_list_dev = counts + b;
-    a.work = work + b;
+    a.work = work + b * BK_CTR_INTS;
     int r = refined ? bk::launch_krige32(c->stream, a, c->edges[b]) : bk::launch_krige64(c->stream, a, c->edges[b]);
     if (r) return fail(c, BK_E_HIP, "solve launch failed");
   }
@@ -714,7 +715,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
     b.list = a.redo;
     b.n_list = m;
     b.n_list_dev = a.redo_count;
-    b.work = work + nb;
+    b.work = work + nb * BK_CTR_INTS;
     if (bk::launch_krige64(c->stream, b, 64)) return fail(c, BK_E_HIP, "solve launch failed");
   }
   // The LU list (n > 64, power models), then the guarded systems (merged
@@ -1254,9 +1255,9 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
   const int nb = 3;
   if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
   if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
-  if (int r = ensure(c, c->work, 16 * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->work, nb * BK_CTR_INTS * sizeof(int32_t))) return r;
   HIPOK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(int32_t), c->stream));
-  HIPOK(c, hipMemsetAsync(c->work.p, 0, 16 * sizeof(int32_t), c->stream));
+  HIPOK(c, hipMemsetAsync(c->work.p, 0, nb * BK_CTR_INTS * sizeof(int32_t), c->stream));
   // Rows of samples outside every bucket stay zero (they are out of the loss).
   if (m > 0) HIPOK(c, hipMemsetAsync(c->support.p, 0, m * cols * sizeof(double), c->stream));
   int32_t* counts = (int32_t*)c->counts.p;
@@ -1267,7 +1268,7 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     if (c->loo_hc[b] == 0) continue;
     a.list = (const int32_t*)c->loo_lists.p + (int64_t)b * m;
     a.n_list = c->loo_hc[b];
-    a.work = (int32_t*)c->work.p + b;
+    a.work = (int32_t*)c->work.p + b * BK_CTR_INTS;
     const int r = precision == BK_FP32_IR ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b])
                                           : bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b]);
     if (r) return fail(c, BK_E_HIP, "gradient launch failed");

@@ -26,14 +26,46 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Persistent launches hand out work units through a counter in device memory
-// (zeroed before the launch): one lane of each wavefront takes the next unit,
-// so the time does not depend on how many workgroups are resident at once.
+// Persistent launches. The work units [0, units) are split into BK_XCDS
+// contiguous ranges; range x is served by the workgroups with
+// blockIdx % BK_XCDS == x (gfx950 hands workgroups to its 8 XCDs round robin,
+// so a range stays in one XCD's L2, where neighbouring targets share
+// samples). Each wavefront takes its first unit of the range by its index,
+// then further ones from the range's counter (zeroed before the launch, one
+// 64-byte line per counter): the time does not depend on how many workgroups
+// are resident at once, a launch without units makes no atomic, and eight
+// counters spread the contention.
+#define BK_XCDS 8
+#define BK_CTR_STRIDE 16  // int32 per counter: one 64-byte line
+#define BK_CTR_INTS (BK_XCDS * BK_CTR_STRIDE)
+
 __device__ __forceinline__ int grab_unit(int32_t* counter) {
   int v = 0;
   if ((threadIdx.x & (BK_WAVE - 1)) == 0) v = atomicAdd(counter, 1);
   return __builtin_amdgcn_readfirstlane(v);
 }
+
+struct UnitLoop {
+  int unit;      // current unit (>= end: done)
+  int end;       // end of this wave's range
+  int base;      // first unit handed out by the counter
+  int32_t* ctr;
+};
+
+__device__ __forceinline__ UnitLoop unit_loop(int64_t units, int32_t* work) {
+  const int x = blockIdx.x % BK_XCDS;
+  const int lo = (int)(units * x / BK_XCDS), hi = (int)(units * (x + 1) / BK_XCDS);
+  const int waves = blockDim.x / BK_WAVE;
+  const int blocks = ((int)gridDim.x - x + BK_XCDS - 1) / BK_XCDS;  // workgroups serving range x
+  UnitLoop q;
+  q.unit = lo + (int)(blockIdx.x / BK_XCDS) * waves + (int)(threadIdx.x / BK_WAVE);
+  q.end = hi;
+  q.base = lo + blocks * waves;
+  q.ctr = work + x * BK_CTR_STRIDE;
+  return q;
+}
+
+__device__ __forceinline__ int unit_next(const UnitLoop& q) { return q.base + grab_unit(q.ctr); }
 
 // The kernel's first argument (a struct passed by value sits at offset 0 of
 // the kernarg segment), through a pointer the optimiser cannot see through.

@@ -43,7 +43,7 @@ struct SolveArgs {
   const int32_t* list;  // systems of this launch: target ids
   int64_t n_list;       // their number, or a bound on it when n_list_dev is set
   const int32_t* n_list_dev;  // the number on the device (written by an earlier kernel), or null
-  int32_t* work;        // work units handed out so far (zeroed before the launch): 64 / NMAX systems each
+  int32_t* work;        // BK_CTR_INTS of work-unit counters (zeroed before the launch); a unit is 64 / NMAX systems
   double* value;
   double* variance;
   double* lagrange;     // may be null

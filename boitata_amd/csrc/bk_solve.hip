@@ -476,14 +476,13 @@ __device__ __forceinline__ void krige64_blk(const SolveArgs& a, int64_t nl, int 
 template <int NMAX>
 __global__ void __launch_bounds__(256) k_krige64(const SolveArgs a) {
   const int64_t nl = list_len(a);
-  const int64_t units = (nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX);
-  // The next unit is taken before this one is solved, hiding the atomic.
-  int unit = grab_unit(a.work);
-  while (unit < units) {
-    const int next = grab_unit(a.work);
-    krige64_blk<NMAX>(kernarg_first<SolveArgs>(), nl, unit);
+  UnitLoop q = unit_loop((nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX), a.work);
+  while (q.unit < q.end) {
+    // The next unit is taken before this one is solved, hiding the atomic.
+    const int next = unit_next(q);
+    krige64_blk<NMAX>(kernarg_first<SolveArgs>(), nl, q.unit);
     wave_sync();
-    unit = next;
+    q.unit = next;
   }
 }
 
@@ -943,14 +942,13 @@ template <int NMAX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NMAX == 64 ? 1 : 2)))
 k_krige32(const SolveArgs a) {
   const int64_t nl = list_len(a);
-  const int64_t units = (nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX);
-  // The next unit is taken before this one is solved, hiding the atomic.
-  int unit = grab_unit(a.work);
-  while (unit < units) {
-    const int next = grab_unit(a.work);
-    krige32_blk<NMAX>(kernarg_first<SolveArgs>(), nl, unit);
+  UnitLoop q = unit_loop((nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX), a.work);
+  while (q.unit < q.end) {
+    // The next unit is taken before this one is solved, hiding the atomic.
+    const int next = unit_next(q);
+    krige32_blk<NMAX>(kernarg_first<SolveArgs>(), nl, q.unit);
     wave_sync();
-    unit = next;
+    q.unit = next;
   }
 }
 
@@ -1215,14 +1213,13 @@ __device__ __forceinline__ void loo_grad_blk(const SolveArgs& a, const GradArgs&
 template <int NMAX>
 __global__ void __launch_bounds__(256) k_loo_grad(const SolveArgs a, const GradArgs gp) {
   const int64_t nl = list_len(a);
-  const int64_t units = (nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX);
-  // The next unit is taken before this one is solved, hiding the atomic.
-  int unit = grab_unit(a.work);
-  while (unit < units) {
-    const int next = grab_unit(a.work);
-    loo_grad_blk<NMAX>(a, gp, nl, unit);
+  UnitLoop q = unit_loop((nl + BK_WAVE / NMAX - 1) / (BK_WAVE / NMAX), a.work);
+  while (q.unit < q.end) {
+    // The next unit is taken before this one is solved, hiding the atomic.
+    const int next = unit_next(q);
+    loo_grad_blk<NMAX>(a, gp, nl, q.unit);
     wave_sync();
-    unit = next;
+    q.unit = next;
   }
 }
 
@@ -1880,16 +1877,18 @@ namespace bk {
 
 // Grid of a persistent launch over `work` workgroups' worth of units: the
 // workgroups the device holds at once (by the kernel's registers and LDS), at
-// most `work`. The waves take their units from a counter, so a workgroup that
-// is not resident at first only finds the counter exhausted when it starts.
+// most `work`, as a multiple of the XCD count so that every unit range of
+// unit_loop has its workgroups. The waves take their units from counters, so
+// a workgroup that is not resident at first only finds its range done.
 [[maybe_unused]] static unsigned persistent_grid(const void* fn, int threads, size_t sh, int64_t work) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, sh) != hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 1;
   }
-  const int64_t resident = (int64_t)per_cu * device_cus();
-  return (unsigned)std::max<int64_t>(std::min(resident, work), 1);
+  const int64_t resident = (int64_t)per_cu * device_cus() / BK_XCDS * BK_XCDS;
+  const int64_t want = (work + BK_XCDS - 1) / BK_XCDS * BK_XCDS;
+  return (unsigned)std::max<int64_t>(std::min(resident, want), BK_XCDS);
 }
 
 // Launch shape of a solve kernel with `doubles` of LDS per system.
