@@ -83,6 +83,9 @@ class Search(C.Structure):
         ("min_sectors", C.c_int64),
         ("max_empty_sectors", C.c_int64),
         ("min_holes", C.c_int64),
+        ("calibration", C.c_int32),
+        ("reserved", C.c_int32),
+        ("calibration_target", C.c_double),
     ]
 
 
@@ -125,9 +128,13 @@ def make_variogram(structures, nugget=0.0, rotation=None, ratios=(1.0, 1.0)) -> 
     return v
 
 
+CALIBRATE_SLOPE = 1
+CALIBRATE_EFFICIENCY = 2
+
+
 def make_search(min_samples=4, max_samples=16, radius=float("inf"), rotation=None, ratios=None, octant=False,
                 max_per_hole=None, max_per_octant=None, sectors=None, max_per_sector=None, min_sectors=None,
-                max_empty_sectors=None, min_holes=None) -> Search:
+                max_empty_sectors=None, min_holes=None, target_slope=None, target_efficiency=None) -> Search:
     s = Search()
     s.min_samples = int(min_samples)
     s.max_samples = int(max_samples)
@@ -147,6 +154,10 @@ def make_search(min_samples=4, max_samples=16, radius=float("inf"), rotation=Non
     s.min_sectors = -1 if min_sectors is None else int(min_sectors)
     s.max_empty_sectors = -1 if max_empty_sectors is None else int(max_empty_sectors)
     s.min_holes = -1 if min_holes is None else int(min_holes)
+    if target_slope is not None:
+        s.calibration, s.calibration_target = CALIBRATE_SLOPE, float(target_slope)
+    elif target_efficiency is not None:
+        s.calibration, s.calibration_target = CALIBRATE_EFFICIENCY, float(target_efficiency)
     return s
 
 

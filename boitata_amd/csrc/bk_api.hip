@@ -273,6 +273,13 @@ const char* check_search(const bk_search* s) {
     return "all ranges must be finite and > 0";
   if (s->has_anisotropy && !(std::isfinite(s->azimuth) && std::isfinite(s->dip) && std::isfinite(s->rake)))
     return "azimuth, dip and rake must be finite";
+  // py/estimation.rs:458-472
+  if (s->calibration < 0 || s->calibration > BK_CALIBRATE_EFFICIENCY) return "unknown calibration";
+  if (s->calibration == BK_CALIBRATE_SLOPE && !(std::isfinite(s->calibration_target) && s->calibration_target > 0.0))
+    return "target_slope must be finite and > 0";
+  if (s->calibration == BK_CALIBRATE_EFFICIENCY &&
+      !(std::isfinite(s->calibration_target) && s->calibration_target <= 1.0))
+    return "target_efficiency must be finite and <= 1";
   return nullptr;
 }
 
@@ -524,7 +531,8 @@ int run_search(bk_ctx* c, SearchArgs& a, int64_t m, const double* tx, const doub
 }
 
 // Solves every target with status OK after the search; outputs per target.
-int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int32_t precision, int64_t m,
+int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t kind, double param, int32_t precision,
+              int64_t m,
               const double* tx, const double* ty, const double* tz, int stride, const bk_block* blk, double* value,
               double* variance, double* lagrange, double* support, int32_t* n_used, int32_t* status) {
   SolveArgs a;
@@ -663,6 +671,15 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, int32_t kind, double param, int
         a.cbb = total / (double)((int64_t)npts * npts);
       }
     }
+  }
+  // A calibrated search keeps, per target, the fewest leading samples that
+  // reach its target quality (the selection was put in balanced order after
+  // the search).
+  if (srch && srch->calibration && m > 0) {
+    CalibArgs k{(const int32_t*)c->sstatus.p, (int32_t*)c->ncount.p, m, srch->calibration, srch->calibration_target,
+                srch->min_samples, kind != BK_SIMPLE};
+    if (bk::launch_calibrate(c->stream, a, k, stride <= 16 ? 16 : (stride <= 32 ? 32 : 64)))
+      return fail(c, BK_E_HIP, "calibration launch failed");
   }
   // Bucket the searched targets by neighbour count. The list lengths stay on
   // the device: every solve launch is persistent and reads its count there,
@@ -953,8 +970,11 @@ static int krige_enqueue(bk_ctx* c, const bk_variogram* vg, const bk_search* s, 
   if (int r = run_search(c, a, m, dx, dy, dz, dl)) return r;
   if (loo && m > 0) k_loo_nonempty<<<blocks_of(m), 256, 0, c->stream>>>(m, (const int32_t*)c->ncount.p,
                                                                        (int32_t*)c->sstatus.p);
+  // search.rs:909-936: a calibrated search that balances sectors takes them in turn.
+  if (s->calibration && (s->octant || s->sectors > 0) && m > 0)
+    if (bk::launch_balance(c->stream, a, m)) return fail(c, BK_E_HIP, "balance launch failed");
   mark(c, 2);
-  if (int r = run_solve(c, vg, kind, param, precision, m, dx, dy, dz, (int)stride, blk, ov, ovar, olag, osup, onu,
+  if (int r = run_solve(c, vg, s, kind, param, precision, m, dx, dy, dz, (int)stride, blk, ov, ovar, olag, osup, onu,
                         ost))
     return r;
   if (int r = finalize(c, m, ost, ov, ovar, olag, osup, onu)) return r;
@@ -1005,6 +1025,10 @@ static int krige_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, i
     return fail(c, BK_E_INVALID,
                 "simple kriging needs a variogram with a sill: every structure must be stationary (not power); use "
                 "ordinary kriging");
+  if (s->calibration == BK_CALIBRATE_SLOPE && kind == BK_SIMPLE)
+    return fail(c, BK_E_INVALID, "simple kriging has a slope of regression of 1; calibrate it with target_efficiency");
+  if (s->calibration && s->max_samples > BK_MAX_CHOL)
+    return fail(c, BK_E_INVALID, "calibrated searches select at most 64 samples on the device");
   if (blk) {
     if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
       return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
@@ -1112,6 +1136,8 @@ int bk_loo(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, 
 
 int bk_loo_prepare(bk_ctx* c, const bk_search* s, const bk_variogram* vg) {
   if (!c || !s) return BK_E_INVALID;
+  if (s->calibration)
+    return fail(c, BK_E_INVALID, "the leave-one-out gradient needs fixed neighbour sets: a search without calibration");
   HIPOK(c, hipSetDevice(c->device));
   SearchArgs a;
   int64_t stride;

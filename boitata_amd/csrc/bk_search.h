@@ -99,6 +99,9 @@ struct TileBuffers {
   const int32_t* spill_count = nullptr;  // device count of the targets the last wave or tile search left to k_knn
 };
 void free_tiles(TileBuffers& tb);
+// SearchTree::balanced order of each target's selection (search.rs:909-936),
+// in place; selections of at most 64 samples.
+int launch_balance(hipStream_t s, const SearchArgs& a, int64_t m);
 // p[0 .. n) = 0 by a kernel (graph-safe).
 int zero32(hipStream_t s, int32_t* p, int64_t n);
 // Searches without caps by `mode`: 0 one wavefront per target (k_knn_wave),

@@ -168,6 +168,24 @@ __device__ __forceinline__ int pool_compact(SearchLds& L, int P, int lane, Keep 
   return out;
 }
 
+// Sectors::of (search.rs:428-441): the sector of sample i around the target at
+// world (t0, t1, t2).
+__device__ __forceinline__ int sector_at(const SearchArgs& a, double t0, double t1, double t2, uint32_t i) {
+  double d0, d1, d2v;
+  mv3(a.axes, a.sx[i] - t0, a.sy[i] - t1, a.sz[i] - t2, d0, d1, d2v);
+  if (a.plane) {
+    // search.rs:430-434
+    const double tau = 6.283185307179586;
+    double turn = fmod(atan2(-d0, d1), tau);
+    if (turn < 0.0) turn = turn + tau;
+    double f = turn / tau * (double)a.plane;
+    int k = f >= 0.0 ? (f < 2147483647.0 ? (int)f : 2147483647) : 0;
+    return k < a.plane - 1 ? k : a.plane - 1;
+  }
+  int bx = d0 >= 0.0, by = d1 >= 0.0, bz = d2v >= 0.0;
+  return a.planar ? (bx << 1) | by : (bx << 2) | (by << 1) | bz;
+}
+
 struct Ctx {
   const SearchArgs* a;
   SearchLds* L;
@@ -179,21 +197,7 @@ struct Ctx {
   int capped;
   __device__ bool full() const { return st.n >= a->max_samples; }
 
-  __device__ int sector_of(uint32_t i) const {
-    double d0, d1, d2v;
-    mv3(a->axes, a->sx[i] - t0, a->sy[i] - t1, a->sz[i] - t2, d0, d1, d2v);
-    if (a->plane) {
-      // search.rs:430-434
-      const double tau = 6.283185307179586;
-      double turn = fmod(atan2(-d0, d1), tau);
-      if (turn < 0.0) turn = turn + tau;
-      double f = turn / tau * (double)a->plane;
-      int k = f >= 0.0 ? (f < 2147483647.0 ? (int)f : 2147483647) : 0;
-      return k < a->plane - 1 ? k : a->plane - 1;
-    }
-    int bx = d0 >= 0.0, by = d1 >= 0.0, bz = d2v >= 0.0;
-    return a->planar ? (bx << 1) | by : (bx << 2) | (by << 1) | bz;
-  }
+  __device__ int sector_of(uint32_t i) const { return sector_at(*a, t0, t1, t2, i); }
 
   __device__ int hole_count(int h) const {
     for (int k = 0; k < st.n_holes; ++k)
@@ -1147,6 +1151,33 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn_tile(const SearchArgs a, int re
 
 // ---------------------------------------------------------------------------
 // Grid build.
+// SearchTree::balanced (search.rs:909-936) for calibrated searches that balance
+// octants or sectors: each target's selection taken round-robin over its
+// sectors, i.e. stably sorted by the rank of each sample within its sector,
+// so every leading part stays balanced. One wavefront per target, one lane
+// per selected sample (at most 64).
+__global__ void __launch_bounds__(256) k_balance(const SearchArgs a) {
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+  if (t >= a.m || a.status[t] != BK_OK) return;  // whole wave
+  const int n = a.count[t];
+  int32_t* idx = a.idx + t * a.stride;
+  const bool live = lane < n;
+  const int i = live ? idx[lane] : 0;
+  const int sec = live ? sector_at(a, a.tx[t], a.ty[t], a.tz[t], (uint32_t)i) : -1;
+  int rank = 1;  // 1 + earlier samples of the same sector
+  for (int j = 0; j < BK_WAVE; ++j) {
+    const int sj = __shfl(sec, j, BK_WAVE);
+    rank += (j < lane && sj == sec) ? 1 : 0;
+  }
+  int pos = 0;  // place in the stable order by rank
+  for (int j = 0; j < BK_WAVE; ++j) {
+    const int rj = __shfl(rank, j, BK_WAVE);
+    pos += (j < n && (rj < rank || (rj == rank && j < lane))) ? 1 : 0;
+  }
+  if (live) idx[pos] = i;
+}
+
 // Zero fill as a kernel rather than hipMemsetAsync: an enqueued pass may be
 // captured into a graph and replayed, and its nodes are kept to kernels and
 // copies.
@@ -1458,6 +1489,14 @@ static int launch_tiles(hipStream_t s, const SearchArgs& a, int64_t m, int reach
   HIPCHK(hipGetLastError());
   tb.spill_count = count;
   return launch_spill(s, a, m, spill, count);
+}
+
+int launch_balance(hipStream_t s, const SearchArgs& a, int64_t m) {
+  if (m <= 0) return 0;
+  SearchArgs c = a;
+  c.m = m;
+  k_balance<<<dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s>>>(c);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int zero32(hipStream_t s, int32_t* p, int64_t n) {

@@ -91,7 +91,22 @@ struct GradArgs32 {
   float dR[3][9];
 };
 
+// Calibrated searches (kriging_algebra.rs:338-415): per target, the quality of
+// kriging from each leading part of its selection, and the count trimmed to
+// the fewest leading samples, from min_samples, that reach the target.
+struct CalibArgs {
+  const int32_t* sstatus;  // search status per target (only BK_OK targets are calibrated)
+  int32_t* count;          // the selections' counts, trimmed in place
+  int64_t m;
+  int calibration;         // BK_CALIBRATE_SLOPE or BK_CALIBRATE_EFFICIENCY
+  double target;
+  int64_t min_samples;
+  int ordinary;            // the unbiasedness column of ones (not simple kriging)
+};
+
 namespace bk {
+// One FP64 system per NMAX lanes (NMAX >= the selections' size, <= 64).
+int launch_calibrate(hipStream_t s, const SolveArgs& a, const CalibArgs& k, int nmax);
 int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_guarded(hipStream_t s, const SolveArgs& a);

@@ -1768,6 +1768,144 @@ __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, doubl
 // One wavefront per system of a.guard[0 .. *a.guard_count), persistent: the
 // grid strides over the list.
 #if BK_PART != 2
+// ---------------------------------------------------------------------------
+// Calibrated search, kriging_algebra.rs:338-415 (prefix_quality and
+// calibrated_count), in the arithmetic of the reference, which delegates the
+// factorisation and the triangular solves to nalgebra 0.33: Cholesky::new
+// updates column j by the columns k < j in order (y <- (-L[j][k]) L[.][k] + y)
+// and divides it by the square root of its diagonal, failing (every prefix
+// None) unless each pivot is positive; solve_lower_triangular divides b[i] by
+// L[i][i] and updates the rows below by -b[i] L[.][i]. The matrix and the
+// right-hand side are the world-lag cov_points (plus the error variance on the
+// diagonal), or the block covariances for block support. Lane r holds row r;
+// the running sums over the leading parts are taken in order, as the
+// reference's fold does. The kernel only trims count[t]: the solve kernels then
+// krige the leading samples as krige() on samples[..n] does.
+__host__ __device__ constexpr int cal_doubles(int nmax) { return 3 * nmax + (nmax * (nmax + 1) / 2 + 1) / 2 * 2; }
+
+template <int NMAX>
+__global__ void __launch_bounds__(64) k_calibrate(const SolveArgs a, const CalibArgs k) {
+  constexpr int G = BK_WAVE / NMAX;
+  extern __shared__ double dyn[];
+  const int lane = threadIdx.x, g = lane / NMAX, r = lane % NMAX;
+  const int64_t t = (int64_t)blockIdx.x * G + g;
+  const bool active = t < k.m && k.sstatus[t] == BK_OK;
+  const int n = active ? k.count[t] : 0;
+  double* R = dyn + g * cal_doubles(NMAX);  // sample coordinates, 3 per sample
+  double* L = R + 3 * NMAX;                 // packed lower triangle, L[tri(i) + j]
+  double wx = 0.0, wy = 0.0, wz = 0.0, e = 0.0;
+  if (r < n) {
+    const int id = a.nidx[t * a.stride + r];
+    wx = a.sx[id];
+    wy = a.sy[id];
+    wz = a.sz[id];
+    e = a.serr ? a.serr[id] : 0.0;
+  }
+  R[3 * r] = wx;
+  R[3 * r + 1] = wy;
+  R[3 * r + 2] = wz;
+  wave_sync();
+  // Row r of the covariance matrix and its right-hand side entry.
+  double c = 0.0;
+  if (r < n) {
+    for (int j = 0; j < r; ++j) L[tri(r) + j] = cov_points64(a.v, wx, wy, wz, R[3 * j], R[3 * j + 1], R[3 * j + 2]) + 0.0;
+    L[tri(r) + r] = cov_points64(a.v, wx, wy, wz, wx, wy, wz) + e;
+    const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+    if (a.block) {
+      double acc = -0.0;
+      for (int p = 0; p < a.npts; ++p) {
+        const double qx = t0 + a.offs[3 * p], qy = t1 + a.offs[3 * p + 1], qz = t2 + a.offs[3 * p + 2];
+        acc += a.npts == 1 ? cov_points64(a.v, wx, wy, wz, qx, qy, qz)
+                           : block_cov_points64(a.v, wx, wy, wz, qx, qy, qz);
+      }
+      c = acc / (double)a.npts;
+    } else {
+      c = cov_points64(a.v, wx, wy, wz, t0, t1, t2);
+    }
+  }
+  wave_sync();
+  // Cholesky, column j after the columns before it.
+  bool fail = false;
+  for (int j = 0; j < NMAX; ++j) {
+    double v = 0.0;
+    if (j < n && j <= r && r < n) {
+      v = L[tri(r) + j];
+      for (int q = 0; q < j; ++q) v = -L[tri(j) + q] * L[tri(r) + q] + v;
+    }
+    const double d = group_bcast<NMAX>(v, j, g);
+    if (j < n && !(d > 0.0)) fail = true;
+    if (j < n && !fail) {
+      const double denom = sqrt(d);
+      if (r == j) L[tri(j) + j] = denom;
+      else if (r > j && r < n) L[tri(r) + j] = v / denom;
+    }
+    wave_sync();
+  }
+  // Forward solves of the right-hand side and of the column of ones.
+  double y = c, yx = 1.0;
+  for (int i = 0; i < NMAX; ++i) {
+    double ci = 0.0, xi = 0.0;
+    if (r == i && i < n && !fail) {
+      const double dii = L[tri(i) + i];
+      ci = y / dii;
+      xi = yx / dii;
+      y = ci;
+      yx = xi;
+    }
+    ci = group_bcast<NMAX>(ci, i, g);
+    xi = group_bcast<NMAX>(xi, i, g);
+    if (r > i && r < n && i < n && !fail) {
+      const double lri = L[tri(r) + i];
+      y = -ci * lri + y;
+      yx = -xi * lri + yx;
+    }
+  }
+  // Running sums over the leading parts; lane r keeps those of the first r + 1.
+  double ss = 0.0, aa = 0.0, bb = 0.0, my_s = 0.0, my_a = 0.0, my_b = 0.0;
+  for (int q = 0; q < NMAX; ++q) {
+    const double yq = group_bcast<NMAX>(y, q, g), xq = group_bcast<NMAX>(yx, q, g);
+    if (q < n) {
+      ss += yq * yq;
+      aa += xq * xq;
+      bb += xq * yq;
+    }
+    if (r == q) {
+      my_s = ss;
+      my_a = aa;
+      my_b = bb;
+    }
+  }
+  bool meets = false;
+  if (r < n && !fail) {
+    bool some = false;
+    double cov = 0.0, mu = 0.0;
+    if (!k.ordinary) {
+      cov = my_s;
+      some = true;
+    } else if (my_a > 0.0) {
+      // The 1 x 1 Cholesky of a and its solve of b - x0, x0 = 1.
+      const double dd = sqrt(my_a);
+      if (!(dd <= 1e-7 * dd)) {
+        const double mm = ((my_b - 1.0) / dd) / dd;
+        cov = my_s - mm * my_b;
+        mu = mm;
+        some = true;
+      }
+    }
+    const double support = a.block ? a.cbb : a.v.c0;
+    const double slope = cov / (cov - mu), eff = (cov + mu) / support;
+    const int64_t from = k.min_samples > 1 ? k.min_samples : 1;
+    meets = some && r + 1 >= from && (k.calibration == BK_CALIBRATE_SLOPE ? slope >= k.target : eff >= k.target);
+  }
+  const uint64_t m = __ballot(meets);
+  const uint64_t gm = NMAX == 64 ? ~0ull : (((1ull << NMAX) - 1ull) << (g * NMAX));
+  const uint64_t mine = m & gm;
+  if (active && r == 0 && mine) {
+    const int first = __ffsll((unsigned long long)mine) - 1 - g * NMAX;
+    if (first + 1 < n) k.count[t] = first + 1;
+  }
+}
+
 __global__ void __launch_bounds__(64) k_guarded(const SolveArgs a) {
   extern __shared__ double dyn[];
   double* A = dyn;
@@ -1989,6 +2127,26 @@ int launch_reduce_cols(hipStream_t s, int64_t m, int cols, const double* part, d
 #endif
 
 #if BK_PART != 2
+int launch_calibrate(hipStream_t s, const SolveArgs& a, const CalibArgs& k, int nmax) {
+  if (k.m <= 0) return 0;
+  auto run = [&](auto kern, int nm) {
+    const int G = BK_WAVE / nm;
+    const size_t sh = (size_t)G * cal_doubles(nm) * sizeof(double);
+    allow_lds((const void*)kern, sh);
+    kern<<<dim3((unsigned)((k.m + G - 1) / G)), dim3(BK_WAVE), sh, s>>>(a, k);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  };
+  switch (nmax) {
+    case 16:
+      return run(k_calibrate<16>, 16);
+    case 32:
+      return run(k_calibrate<32>, 32);
+    case 64:
+      return run(k_calibrate<64>, 64);
+  }
+  return -1;
+}
+
 int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax) {
   switch (nmax) {
     case 16:

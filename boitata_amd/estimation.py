@@ -153,16 +153,25 @@ class Search:
     """Search neighbourhood (crates/py/src/estimation.rs:415-531): `radius`, or
     `ranges` (major, semi, minor) giving radius = major and ratios semi/major,
     minor/major; `max_per_sector` caps octants with `octant=True` or plane
-    sectors with `sectors`."""
+    sectors with `sectors`. `target_slope` or `target_efficiency` calibrates
+    the search: each target is kriged from the fewest leading samples, from
+    min_samples, that reach it (kriging_algebra.rs:338-441)."""
 
     def __init__(self, radius=None, *, max_samples=16, min_samples=1, octant=False, sectors=None, max_per_sector=None,
                  max_per_hole=None, rotation=None, ratios=None, ranges=None, min_sectors=None, max_empty_sectors=None,
                  min_holes=None, high_grade=None, soft=None, target_slope=None, target_efficiency=None,
                  length_unit=None):
-        for name, v in (("high_grade", high_grade), ("soft", soft), ("target_slope", target_slope),
-                        ("target_efficiency", target_efficiency), ("length_unit", length_unit)):
+        for name, v in (("high_grade", high_grade), ("soft", soft), ("length_unit", length_unit)):
             if v is not None:
                 raise NotImplementedError(f"{name} is not supported by this engine")
+        # py/estimation.rs:458-472
+        if target_slope is not None and target_efficiency is not None:
+            raise InvalidInput("give at most one of target_slope and target_efficiency")
+        if target_slope is not None and not (math.isfinite(float(target_slope)) and float(target_slope) > 0.0):
+            raise InvalidInput("target_slope must be finite and > 0")
+        if target_efficiency is not None and not (math.isfinite(float(target_efficiency))
+                                                  and float(target_efficiency) <= 1.0):
+            raise InvalidInput("target_efficiency must be finite and <= 1")
         if radius is not None and ranges is not None:
             raise InvalidInput("give radius or ranges, not both")
         if radius is None and ranges is None:
@@ -195,7 +204,8 @@ class Search:
                                    ratios=ratios, octant=octant, max_per_hole=max_per_hole,
                                    max_per_octant=max_per_octant, sectors=sectors, max_per_sector=per_sector,
                                    min_sectors=min_sectors, max_empty_sectors=max_empty_sectors,
-                                   min_holes=min_holes)
+                                   min_holes=min_holes, target_slope=target_slope,
+                                   target_efficiency=target_efficiency)
         if rotation is not None or ratios is not None:
             rot = tuple(rotation) if rotation is not None else (0.0, 0.0, 0.0)
             rat = tuple(ratios) if ratios is not None else (1.0, 1.0)
@@ -463,6 +473,9 @@ class SimpleKriging(_Base):
         super().__init__(variogram, search, **engine)
         if not variogram.has_sill:
             raise InvalidInput("simple kriging needs a variogram with a sill")
+        # py/estimation.rs:820-838
+        if any(p._c.calibration == _abi.CALIBRATE_SLOPE for p in self._passes):
+            raise InvalidInput("simple kriging has a slope of regression of 1; calibrate it with target_efficiency")
         self._param = float(mean)
 
 

@@ -121,7 +121,18 @@ typedef struct {
   int64_t min_sectors;        /* -1: none */
   int64_t max_empty_sectors;  /* -1: none */
   int64_t min_holes;          /* -1: none */
+  /* Calibrated search (search.rs:59-61,94-99; py/estimation.rs:458-472): each
+   * target is kriged from the fewest leading samples, from min_samples, whose
+   * kriging reaches the target slope of regression or efficiency
+   * (kriging_algebra.rs:344-441), or from all of them. A search that balances
+   * octants or sectors then takes the sectors in turn (search.rs:909-936). */
+  int32_t calibration;        /* 0: none, BK_CALIBRATE_SLOPE or BK_CALIBRATE_EFFICIENCY */
+  int32_t reserved;
+  double calibration_target;  /* slope: finite and > 0; efficiency: finite and <= 1 */
 } bk_search;
+
+#define BK_CALIBRATE_SLOPE 1
+#define BK_CALIBRATE_EFFICIENCY 2
 
 /* Block support, crates/estimation/src/block.rs:18-71 and
  * crates/py/src/estimation.rs:947-958: a block of `size` along the axes of

@@ -69,6 +69,9 @@ def lib() -> C.CDLL:
             "or_block_krige": (I32, [SP, VP, SR, I32, D, BP, I64, P, P, P, P, P, P, P, P, P, I32]),
             "or_loo": (I32, [SP, VP, SR, I32, D, I64, P, P, P, P, I32]),
             "or_krige_lists": (I32, [SP, VP, I32, D, I64, P, P, P, P, P, I64, P, P, P, P, I32]),
+            "or_prefix_quality": (I32, [SP, VP, I32, P, BP, I32, P, P, P]),
+            "or_calibrated_count": (I32, [I32, D, I64, I32, P, P, P]),
+            "or_balanced": (I32, [SP, SR, VP, P, I32, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -262,3 +265,35 @@ def krige_lists(data: Data, vg, targets, idx, count, kind=_abi.ORDINARY, param=0
     lib().or_krige_lists(C.byref(data.s), C.byref(vg), kind, param, m, _p(tx), _p(ty), _p(tz), _p(idx), _p(count),
                          idx.shape[1], _p(value), _p(variance), _p(lagrange), _p(status), threads)
     return dict(value=value, variance=variance, lagrange=lagrange, status=status)
+
+
+def prefix_quality(data: Data, vg, target, idx, kind=_abi.ORDINARY, block=None):
+    """(slope, efficiency) of kriging `target` from each leading part of the
+    samples idx (in that order); NaN where the reference gives None."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    n = len(idx)
+    slope, eff = np.zeros(n), np.zeros(n)
+    lib().or_prefix_quality(C.byref(data.s), C.byref(vg), kind, _p(t), C.byref(block) if block is not None else None,
+                            n, _p(idx), _p(slope), _p(eff))
+    return slope, eff
+
+
+def calibrated_count(calibration, target, min_samples, slope, efficiency):
+    slope = np.ascontiguousarray(slope, dtype=np.float64)
+    efficiency = np.ascontiguousarray(efficiency, dtype=np.float64)
+    met = C.c_int32()
+    n = lib().or_calibrated_count(int(calibration), float(target), int(min_samples), len(slope), _p(slope),
+                                  _p(efficiency), C.byref(met))
+    return n, bool(met.value)
+
+
+def balanced(data: Data, search, vg, target, chosen):
+    chosen = np.ascontiguousarray(chosen, dtype=np.int64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    out = np.zeros(len(chosen), np.int64)
+    rc = lib().or_balanced(C.byref(data.s), C.byref(search), C.byref(vg) if vg is not None else None, _p(t),
+                           len(chosen), _p(chosen), _p(out))
+    if rc != 0:
+        raise ValueError("invalid search")
+    return out

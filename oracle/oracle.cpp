@@ -18,6 +18,9 @@
 //   crates/estimation/src/krige.rs:83-201      krige, value, check_simple, krige_lu
 //   crates/estimation/src/block.rs:18-198      Discretization, block_covariances, block_krige_points
 //   crates/estimation/src/batch.rs:15-38,165-190 estimate_many, leave_one_out_at
+//   crates/estimation/src/kriging_algebra.rs:320-441 Quality, prefix_quality,
+//                                              calibrated_count, krige_calibrated
+//   crates/estimation/src/search.rs:909-936    SearchTree::balanced
 //   crates/py/src/estimation.rs:947-958        block_points
 // Third-party arithmetic the reference delegates to nalgebra 0.33.3 (absent from
 // the reference tree): 3x3 products are restated in nalgebra's column-axpy order,
@@ -718,6 +721,101 @@ int block_krige_points(int kind, double param, const std::vector<P3>& pts, const
 }
 
 // ---------------------------------------------------------------------------
+// Calibrated kriging, crates/estimation/src/kriging_algebra.rs:320-441.
+
+// block.rs:100-118 block_covariances: the right-hand side (one point is point
+// support) and C(B, B).
+static std::vector<double> block_rhs(const std::vector<P3>& pts, const std::vector<Sample>& s, const Variogram& vg) {
+  double np = (double)pts.size();
+  std::vector<double> rhs(s.size());
+  for (size_t i = 0; i < s.size(); ++i) {
+    double acc = -0.0;
+    for (auto& p : pts) acc += pts.size() == 1 ? vg.cov_points(s[i].loc, p) : vg.block_cov_points(s[i].loc, p);
+    rhs[i] = acc / np;
+  }
+  return rhs;
+}
+
+// kriging_algebra.rs:320-336
+struct Quality {
+  bool some = false;
+  double slope = NAN, efficiency = NAN;
+  bool meets(int calibration, double target) const {
+    if (!some) return false;
+    return calibration == BK_CALIBRATE_SLOPE ? slope >= target : efficiency >= target;
+  }
+};
+
+// kriging_algebra.rs:338-397 prefix_quality for the ordinary (one drift
+// column of ones, x0 = 1) and simple (none) drifts. The factorisation and the
+// triangular solves are nalgebra 0.33's, restated: Cholesky::new updates
+// column j by the columns k < j in order (y <- (-L[j][k]) * L[.][k] + y) and
+// divides it by the square root of its diagonal, failing unless every pivot
+// is positive; solve_lower_triangular divides b[i] by L[i][i] and then
+// updates the rows below by -b[i] * L[.][i].
+std::vector<Quality> prefix_quality(const std::vector<Sample>& s, const Variogram& vg, const std::vector<double>& rhs,
+                                    bool ordinary, double support) {
+  const int n = (int)s.size();
+  std::vector<Quality> q(n);
+  std::vector<double> a((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) a[i * n + j] = vg.cov_points(s[i].loc, s[j].loc) + (i == j ? s[i].err : 0.0);
+  for (int j = 0; j < n; ++j) {
+    for (int k = 0; k < j; ++k) {
+      double factor = -a[j * n + k];
+      for (int r = j; r < n; ++r) a[r * n + j] = factor * a[r * n + k] + a[r * n + j];
+    }
+    double diag = a[j * n + j];
+    if (!(diag > 0.0)) return q;  // every prefix None
+    double denom = std::sqrt(diag);
+    for (int r = j + 1; r < n; ++r) a[r * n + j] = a[r * n + j] / denom;
+    a[j * n + j] = denom;
+  }
+  auto forward = [&](std::vector<double>& b) {
+    for (int i = 0; i < n; ++i) {
+      double coeff = b[i] / a[i * n + i];
+      b[i] = coeff;
+      for (int r = i + 1; r < n; ++r) b[r] = -coeff * a[r * n + i] + b[r];
+    }
+  };
+  std::vector<double> y = rhs, yx(n, 1.0);
+  forward(y);
+  if (ordinary) forward(yx);
+  double ss = 0.0, aa = 0.0, bb = 0.0;
+  for (int k = 0; k < n; ++k) {
+    ss += y[k] * y[k];
+    double cov, mu;
+    if (!ordinary) {
+      cov = ss;
+      mu = 0.0;
+    } else {
+      aa += yx[k] * yx[k];
+      bb += yx[k] * y[k];
+      // The 1 x 1 Cholesky of aa and its solve of (bb - x0), x0 = 1.
+      if (!(aa > 0.0)) continue;
+      double d = std::sqrt(aa);
+      if (d <= 1e-7 * d) continue;
+      double m = ((bb - 1.0) / d) / d;
+      cov = ss - m * bb;
+      mu = m;
+    }
+    q[k].some = true;
+    q[k].slope = cov / (cov - mu);
+    q[k].efficiency = (cov + mu) / support;
+  }
+  return q;
+}
+
+// kriging_algebra.rs:399-415
+std::pair<int, bool> calibrated_count(int calibration, double target, int64_t min_samples,
+                                      const std::vector<Quality>& q) {
+  int64_t from = std::max<int64_t>(min_samples, 1);
+  for (int64_t k = from - 1; k < (int64_t)q.size(); ++k)
+    if (q[k].meets(calibration, target)) return {(int)(k + 1), true};
+  return {(int)q.size(), false};
+}
+
+// ---------------------------------------------------------------------------
 // Search, crates/estimation/src/search.rs
 struct Search {
   int64_t min_samples = 4, max_samples = 16;
@@ -727,6 +825,8 @@ struct Search {
   bool octant = false;
   int64_t max_per_hole = -1, max_per_octant = -1, sectors = 0, max_per_sector = 0;
   int64_t min_sectors = -1, max_empty_sectors = -1, min_holes = -1;
+  int calibration = 0;  // 0 none, BK_CALIBRATE_SLOPE, BK_CALIBRATE_EFFICIENCY (search.rs:94-99)
+  double calibration_target = 0.0;
   bool balances() const { return octant || sectors > 0; }
 };
 
@@ -753,6 +853,8 @@ static Search make_search(const bk_search* p) {
   s.min_sectors = p->min_sectors;
   s.max_empty_sectors = p->max_empty_sectors;
   s.min_holes = p->min_holes;
+  s.calibration = p->calibration;
+  s.calibration_target = p->calibration_target;
   return s;
 }
 
@@ -780,6 +882,13 @@ static bool check(const Search& s) {
     return false;
   if (s.max_empty_sectors >= 0 && s.max_empty_sectors >= ring) return false;
   if (s.min_holes >= 0 && (s.min_holes == 0 || s.min_holes > s.max_samples)) return false;
+  // py/estimation.rs:458-472 (the Python Search constructor's checks)
+  if (s.calibration < 0 || s.calibration > BK_CALIBRATE_EFFICIENCY) return false;
+  if (s.calibration == BK_CALIBRATE_SLOPE && !(std::isfinite(s.calibration_target) && s.calibration_target > 0.0))
+    return false;
+  if (s.calibration == BK_CALIBRATE_EFFICIENCY &&
+      !(std::isfinite(s.calibration_target) && s.calibration_target <= 1.0))
+    return false;
   return true;
 }
 
@@ -1178,6 +1287,26 @@ struct SearchTree {
     for (size_t i = 0; i < samples.size(); ++i) points[i] = space.project(samples[i].loc);
     grid.build(points);
   }
+  // search.rs:909-936: `chosen` round-robin over the sectors around the
+  // target (each sector in selection order, a stable sort by the rank within
+  // its sector), for calibrated searches that balance sectors.
+  std::vector<int64_t> balanced(const P3& target, const std::vector<int64_t>& chosen) const {
+    if (!p.balances() || p.calibration == 0) return chosen;
+    std::vector<int64_t> seen(sectors.count(), 0);
+    std::vector<std::pair<int64_t, int64_t>> ranked;
+    for (int64_t i : chosen) {
+      int o = sectors.of(target, (*s)[i].loc);
+      seen[o] += 1;
+      ranked.push_back({seen[o], i});
+    }
+    std::stable_sort(ranked.begin(), ranked.end(),
+                     [](const std::pair<int64_t, int64_t>& a, const std::pair<int64_t, int64_t>& b) {
+                       return a.first < b.first;
+                     });
+    std::vector<int64_t> out;
+    for (auto& r : ranked) out.push_back(r.second);
+    return out;
+  }
   // search.rs:951-982
   int neighbors_around(const P3& target, int64_t left_out, std::vector<int64_t>& out) const {
     Selector sel(&target, &p, &sectors);
@@ -1215,6 +1344,31 @@ struct SearchTree {
     return sel.finish();
   }
 };
+
+// The leading samples a calibrated search keeps (py/estimation.rs:846-897 with
+// kriging_algebra.rs:421-441), for point support (pts null) or the block
+// discretised by pts; all of them without calibration.
+static int calibrated_n(const Search& p, int kind, const P3& t, const std::vector<P3>* pts,
+                        const std::vector<Sample>& sel, const Variogram& vg) {
+  if (p.calibration == 0 || sel.empty()) return (int)sel.size();
+  std::vector<double> rhs;
+  double support;
+  if (pts) {
+    rhs = block_rhs(*pts, sel, vg);
+    support = vg.block_mean_cov(*pts);
+  } else {
+    rhs.resize(sel.size());
+    for (size_t i = 0; i < sel.size(); ++i) rhs[i] = vg.cov_points(sel[i].loc, t);
+    support = vg.total_sill();
+  }
+  auto q = prefix_quality(sel, vg, rhs, kind != SIMPLE, support);
+  return calibrated_count(p.calibration, p.calibration_target, p.min_samples, q).first;
+}
+
+// Simple kriging has a slope of 1 by construction (kriging_algebra.rs:429-434).
+static bool calibration_fits(const Search& p, int kind) {
+  return !(kind == SIMPLE && p.calibration == BK_CALIBRATE_SLOPE);
+}
 
 static std::vector<Sample> make_samples(int64_t n, const double* x, const double* y, const double* z,
                                         const double* v, const double* err, const int32_t* hole) {
@@ -1375,6 +1529,64 @@ int or_neighbors(const or_samples* sm, const bk_search* sp, const bk_variogram* 
   return 0;
 }
 
+// prefix_quality (kriging_algebra.rs:338-397) of the samples idx[0 .. n), in
+// that order, for target t: point support, or the block `blk` centred on t.
+// slope[k], efficiency[k] for the first k + 1 samples, NaN where None.
+int or_prefix_quality(const or_samples* sm, const bk_variogram* v, int kind, const double* t, const bk_block* blk,
+                      int n, const int64_t* idx, double* slope, double* efficiency) {
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  std::vector<Sample> sel;
+  for (int k = 0; k < n; ++k) sel.push_back(s[idx[k]]);
+  P3 q{t[0], t[1], t[2]};
+  std::vector<double> rhs(n);
+  double support = vg.total_sill();
+  if (blk) {
+    auto pts = block_points(q, *blk);
+    rhs = block_rhs(pts, sel, vg);
+    support = vg.block_mean_cov(pts);
+  } else {
+    for (int k = 0; k < n; ++k) rhs[k] = vg.cov_points(sel[k].loc, q);
+  }
+  auto qs = prefix_quality(sel, vg, rhs, kind != SIMPLE, support);
+  for (int k = 0; k < n; ++k) {
+    slope[k] = qs[k].some ? qs[k].slope : NAN;
+    efficiency[k] = qs[k].some ? qs[k].efficiency : NAN;
+  }
+  return 0;
+}
+
+// calibrated_count (kriging_algebra.rs:399-415) over qualities given as
+// slope / efficiency arrays (NaN: None). Returns the count; *met whether it
+// meets the target.
+int or_calibrated_count(int calibration, double target, int64_t min_samples, int n, const double* slope,
+                        const double* efficiency, int32_t* met) {
+  std::vector<Quality> q(n);
+  for (int k = 0; k < n; ++k) {
+    q[k].some = !(std::isnan(slope[k]) && std::isnan(efficiency[k]));
+    q[k].slope = slope[k];
+    q[k].efficiency = efficiency[k];
+  }
+  auto r = calibrated_count(calibration, target, min_samples, q);
+  if (met) *met = r.second ? 1 : 0;
+  return r.first;
+}
+
+// SearchTree::balanced (search.rs:909-936) of the selection chosen[0 .. n)
+// around target t.
+int or_balanced(const or_samples* sm, const bk_search* sp, const bk_variogram* v, const double* t, int n,
+                const int64_t* chosen, int64_t* out) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Variogram vg;
+  if (v) vg = make_variogram(v);
+  SearchTree tree(s, p, v ? &vg : nullptr);
+  auto b = tree.balanced(P3{t[0], t[1], t[2]}, std::vector<int64_t>(chosen, chosen + n));
+  std::copy(b.begin(), b.end(), out);
+  return 0;
+}
+
 // estimate_many with krige (batch.rs:15-38).
 int or_krige(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int kind, double param, int64_t m,
              const double* tx, const double* ty, const double* tz, double* value, double* variance,
@@ -1383,6 +1595,7 @@ int or_krige(const or_samples* sm, const bk_variogram* v, const bk_search* sp, i
   if (!check(p)) return BK_E_INVALID;
   auto s = samples_of(sm);
   Variogram vg = make_variogram(v);
+  if (!calibration_fits(p, kind)) return BK_E_INVALID;
   SearchTree tree(s, p, &vg);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
   for (int64_t t = 0; t < m; ++t) {
@@ -1392,7 +1605,8 @@ int or_krige(const or_samples* sm, const bk_variogram* v, const bk_search* sp, i
     int st = tree.neighbors_around(q, -1, chosen);
     if (st == BK_OK) {
       std::vector<Sample> sel;
-      for (auto i : chosen) sel.push_back(s[i]);
+      for (auto i : tree.balanced(q, chosen)) sel.push_back(s[i]);
+      sel.resize(calibrated_n(p, kind, q, nullptr, sel, vg));
       st = krige(kind, param, q, sel, vg, e);
     }
     write_estimate(st, e, t, value, variance, lagrange, nullptr, n_used, status);
@@ -1410,6 +1624,7 @@ int or_block_krige(const or_samples* sm, const bk_variogram* v, const bk_search*
   if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0) return BK_E_INVALID;
   auto s = samples_of(sm);
   Variogram vg = make_variogram(v);
+  if (!calibration_fits(p, kind)) return BK_E_INVALID;
   SearchTree tree(s, p, &vg);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
   for (int64_t t = 0; t < m; ++t) {
@@ -1419,8 +1634,10 @@ int or_block_krige(const or_samples* sm, const bk_variogram* v, const bk_search*
     int st = tree.neighbors_around(q, -1, chosen);
     if (st == BK_OK) {
       std::vector<Sample> sel;
-      for (auto i : chosen) sel.push_back(s[i]);
-      st = block_krige_points(kind, param, block_points(q, *blk), sel, vg, e);
+      for (auto i : tree.balanced(q, chosen)) sel.push_back(s[i]);
+      const std::vector<P3> pts = block_points(q, *blk);
+      sel.resize(calibrated_n(p, kind, q, &pts, sel, vg));
+      st = block_krige_points(kind, param, pts, sel, vg, e);
     }
     write_estimate(st, e, t, value, variance, lagrange, support, n_used, status);
   }
@@ -1434,6 +1651,7 @@ int or_loo(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int
   if (!check(p)) return BK_E_INVALID;
   auto s = samples_of(sm);
   Variogram vg = make_variogram(v);
+  if (!calibration_fits(p, kind)) return BK_E_INVALID;
   SearchTree tree(s, p, &vg);
   if (!which) m = (int64_t)s.size();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
@@ -1446,7 +1664,8 @@ int or_loo(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int
     if (st == BK_OK && chosen.empty()) st = BK_SEARCH_FAILED;
     if (st == BK_OK) {
       std::vector<Sample> sel;
-      for (auto j : chosen) sel.push_back(s[j]);
+      for (auto j : tree.balanced(q, chosen)) sel.push_back(s[j]);
+      sel.resize(calibrated_n(p, kind, q, nullptr, sel, vg));
       st = krige(kind, param, q, sel, vg, e);
     }
     write_estimate(st, e, k, value, variance, nullptr, nullptr, nullptr, status);

@@ -33,7 +33,8 @@ def test_struct_layouts():
     assert C.sizeof(_abi.Structure) == 48
     assert C.sizeof(_abi.Variogram) == 8 + 8 + 48 * _abi.MAX_STRUCTURES + 40
     assert _abi.Search.radius.offset == 16 and _abi.Search.azimuth.offset == 32
-    assert C.sizeof(_abi.Search) == 16 + 8 + 8 + 40 + 7 * 8
+    assert C.sizeof(_abi.Search) == 16 + 8 + 8 + 40 + 7 * 8 + 16
+    assert _abi.Search.calibration_target.offset == 136
     assert C.sizeof(_abi.Block) == 24 + 16 + 24
 
 

@@ -58,10 +58,22 @@ def test_search_sector_caps():
     assert q._c.max_per_sector == 2 and q._c.max_per_octant == -1
 
 
-@pytest.mark.parametrize("kw", [dict(soft=5.0), dict(target_slope=0.9), dict(length_unit="m")])
+@pytest.mark.parametrize("kw", [dict(soft=5.0), dict(high_grade=3.0), dict(length_unit="m")])
 def test_search_features_out_of_scope(kw):
     with pytest.raises(NotImplementedError):
         Search(10.0, **kw)
+
+
+def test_calibrated_search_arguments():
+    # py/estimation.rs:458-472
+    s = Search(10.0, target_slope=0.9)
+    assert s._c.calibration == _abi.CALIBRATE_SLOPE and s._c.calibration_target == 0.9
+    e = Search(10.0, target_efficiency=0.5)
+    assert e._c.calibration == _abi.CALIBRATE_EFFICIENCY
+    for kw in (dict(target_slope=0.9, target_efficiency=0.5), dict(target_slope=0.0), dict(target_slope=float("nan")),
+               dict(target_efficiency=1.5)):
+        with pytest.raises(InvalidInput):
+            Search(10.0, **kw)
 
 
 def test_variogram_tuples_and_structures():
@@ -170,3 +182,10 @@ def test_fit_warns_on_shared_locations_before_any_device_call(monkeypatch):
     assert est.values.tolist() == [1.0, 3.0]
     with pytest.raises(NotImplementedError):
         est.predict([[0, 0, 0]], diagnostics=True)
+
+
+def test_simple_kriging_takes_no_slope_calibration():
+    vg = Variogram([("spherical", 1.0, 50.0)], nugget=0.1)
+    with pytest.raises(InvalidInput):
+        SimpleKriging(vg, Search(30.0, target_slope=0.9), mean=1.0)
+    SimpleKriging(vg, Search(30.0, target_efficiency=0.5), mean=1.0)
