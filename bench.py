@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--targets", type=int, default=0, help="use the first N targets only (0: all)")
     ap.add_argument("--loo-steps", type=int, default=200, help="Adam steps of the LOO-CV gradient leg (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tuning")
+    ap.add_argument("--mik-steps", type=int, default=3,
+                    help="timed passes of the multiple indicator kriging leg (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="finish with a device-to-device copy of this many MiB (byte calibration of PMC runs)")
     args = ap.parse_args()
@@ -231,6 +233,38 @@ def main():
               f"{hist[-1]:.4f}", file=sys.stderr, flush=True)
         e.set_samples(sx, sy, sz, sv, None, hole)
 
+    # Multiple indicator kriging on the same samples and targets: the nine
+    # deciles of the data as thresholds with the one variogram (median
+    # indicator kriging), three cutoffs and three quantiles; one pass = search,
+    # solves writing weights, indicator estimates, conditional distributions.
+    mik_line = None
+    if args.mik_steps > 0:
+        thr = np.quantile(w.value, np.linspace(0.1, 0.9, 9))
+        mk = _abi.make_mik(thr, 1)
+        kw = dict(cutoffs=[0.5, 1.0, 2.0], quantiles=[0.1, 0.5, 0.9])
+        mres = {}
+        for prec in (_abi.FP32_IR, _abi.FP64):
+            mres[prec] = e.mik_predict(mk, thr, [vg], search, tx, ty, tz, precision=prec, **kw)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.mik_steps):
+                mres[prec] = e.mik_predict(mk, thr, [vg], search, tx, ty, tz, precision=prec, **kw)
+            torch.cuda.synchronize()
+            mres[prec]["dt"] = (time.perf_counter() - t0) / args.mik_steps
+        a32, a64 = mres[_abi.FP32_IR], mres[_abi.FP64]
+        okm = ((a32["status"] == 0) & (a64["status"] == 0)).cpu().numpy()
+        def rel(k):
+            x, y = a32[k].cpu().numpy()[..., okm], a64[k].cpu().numpy()[..., okm]
+            return float(np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-3))) if okm.any() else float("nan")
+        mik_line = {"value": m / a32["dt"], "unit": "targets/s", "ms_per_step": a32["dt"] * 1e3,
+                    "steps": args.mik_steps, "thresholds": len(thr), "variograms": 1, "cutoffs": 3, "quantiles": 3,
+                    "precision": "FP32 Gauss-Jordan weights with FP64 refinement",
+                    "fp64": {"value": m / a64["dt"], "ms_per_step": a64["dt"] * 1e3},
+                    "rel_err_vs_fp64": {"cdf": rel("cdf"), "mean": rel("mean"), "variance": rel("variance")},
+                    "estimated": int(okm.sum())}
+        print(f"[bench] MIK: {mik_line['value']:.0f} targets/s FP32+IR, {mik_line['fp64']['value']:.0f} FP64",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -264,6 +298,14 @@ def main():
                                     sample=f"{len(one)} of the {m} targets (every {max(m // 60000, 1)}th), {dt1:.1f} s")
         print(f"[bench] CPU oracle {cpu['value']:.0f} targets/s on {threads} threads, "
               f"{cpu['single_thread']['value']:.0f} on one", file=sys.stderr, flush=True)
+        if mik_line is not None:
+            # The oracle's multiple indicator kriging on a strided sample.
+            few = T[:: max(m // 20000, 1)][:20000]
+            t0 = time.perf_counter()
+            oracle.mik_predict(d, mk, thr, [vg], search, few, threads=threads, **kw)
+            dtm = time.perf_counter() - t0
+            mik_line["cpu_baseline"] = dict(value=len(few) / dtm, unit="targets/s", cores=threads, kind="port",
+                                            sample=f"{len(few)} of the {m} targets, {dtm:.1f} s")
 
     line = {
         "metric": "kriged targets/s (FP32+IR vs FP64) at <=1e-6 rel err; LOO-CV grad steps/s",
@@ -291,6 +333,7 @@ def main():
         "roofline_other": other,
         "cpu_baseline": cpu,
         "loo_cv_grad": loo_line,
+        "mik": mik_line,
         "autotune": tuned,
         "graph_launches": e.stat(_abi.STAT_GRAPH_LAUNCHES),
     }

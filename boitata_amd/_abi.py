@@ -170,6 +170,47 @@ def make_block(size, discretization=(4, 4, 1), rotation=(0.0, 0.0, 0.0)) -> Bloc
     return b
 
 
+MIK_GLOBAL, MIK_LINEAR = 0, 1
+TAIL_NONE, TAIL_POWER, TAIL_HYPERBOLIC = 0, 1, 2
+
+
+class Mik(C.Structure):
+    """bk_mik: multiple indicator kriging parameters (indicator.rs:53-67)."""
+
+    _fields_ = [
+        ("n_thresholds", C.c_int32),
+        ("n_variograms", C.c_int32),
+        ("simple", C.c_int32),
+        ("interpolation", C.c_int32),
+        ("upper_tail", C.c_int32),
+        ("has_tails", C.c_int32),
+        ("omega", C.c_double),
+        ("tails", C.c_double * 2),
+    ]
+
+
+class MikOut(C.Structure):
+    """bk_mik_out: output pointers (entry-major [k * m + t]); NULL ones are skipped."""
+
+    _fields_ = [(name, C.c_void_p) for name in ("raw", "cdf", "correction", "mean", "variance", "probability_above",
+                                                 "mean_above", "quantile_values", "violations", "n_used", "status")]
+
+
+def make_mik(thresholds, n_variograms=1, simple=False, interpolation=MIK_GLOBAL, upper_tail=TAIL_NONE, omega=1.0,
+             tails=None) -> Mik:
+    p = Mik()
+    p.n_thresholds = len(thresholds)
+    p.n_variograms = int(n_variograms)
+    p.simple = 1 if simple else 0
+    p.interpolation = int(interpolation)
+    p.upper_tail = int(upper_tail)
+    p.omega = float(omega)
+    p.has_tails = 0 if tails is None else 1
+    if tails is not None:
+        p.tails[0], p.tails[1] = float(tails[0]), float(tails[1])
+    return p
+
+
 P = C.c_void_p
 I64 = C.c_int64
 I32 = C.c_int32
@@ -191,6 +232,10 @@ SIGNATURES = [
     ("bk_loo", C.c_int, [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, I32, I64, P, P, P, P]),
     ("bk_loo_prepare", C.c_int, [P, C.POINTER(Search), C.POINTER(Variogram)]),
     ("bk_loo_loss_grad", C.c_int, [P, C.POINTER(Variogram), I32, P, P, P]),
+    ("bk_mik_predict", C.c_int,
+     [P, C.POINTER(Mik), P, P, P, C.POINTER(Search), P, I32, I64, P, P, P, I32, P, I32, P, C.POINTER(MikOut)]),
+    ("bk_mik_cross_validate", C.c_int,
+     [P, C.POINTER(Mik), P, P, P, C.POINTER(Search), I32, I64, P, P, C.POINTER(MikOut)]),
 ]
 
 # Declared in include/boitata_gpu.h beside the reference-facing entry points.

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/boitata_gpu.h"
+#include "bk_mik.h"
 #include "bk_search.h"
 #include "bk_solve.h"
 
@@ -128,6 +129,15 @@ struct bk_ctx {
   std::vector<Graph> graph_cache;  // a few passes (say FP32+IR and FP64 of one workload)
   uint64_t graph_clock = 0;
   int64_t graph_launches = 0;
+  // Multiple indicator kriging: the global distribution of the last call on
+  // the host and the device, kept while its inputs (samples, thresholds,
+  // tails, weights) are the same; scratch for weights, estimates and status.
+  uint64_t samples_gen = 0;
+  std::vector<unsigned char> mik_key;
+  std::vector<double> mik_values, mik_sums, mik_prop;
+  std::vector<int64_t> mik_bounds;
+  double mik_lower = 0, mik_upper = 0;
+  Buf mik_gv, mik_gs, mik_par, mik_cls, mik_tab, mik_w, mik_raw, mik_st, mik_cdf, mik_out;
   // Stage timing (bk_profile): events on the context's stream.
   bool prof = false;
   hipEvent_t ev[7] = {};
@@ -534,9 +544,11 @@ int run_search(bk_ctx* c, SearchArgs& a, int64_t m, const double* tx, const doub
 int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t kind, double param, int32_t precision,
               int64_t m,
               const double* tx, const double* ty, const double* tz, int stride, const bk_block* blk, double* value,
-              double* variance, double* lagrange, double* support, int32_t* n_used, int32_t* status) {
+              double* variance, double* lagrange, double* support, int32_t* n_used, int32_t* status,
+              double* weights = nullptr) {
   SolveArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.weights = weights;
   dev_variogram(vg, a.v, a.v32);
   a.kind = kind;
   a.param = param;
@@ -796,7 +808,8 @@ void bk_destroy(bk_ctx* c) {
   for (Buf* b : {&c->x, &c->y, &c->z, &c->v, &c->e, &c->hole, &c->tx, &c->ty, &c->tz, &c->left, &c->nidx, &c->ncount,
                  &c->sstatus, &c->lists, &c->counts, &c->status, &c->value, &c->variance, &c->lagrange, &c->support,
                  &c->n_used, &c->guard, &c->redo, &c->offs, &c->which, &c->iters, &c->work, &c->loo_idx, &c->loo_count,
-                 &c->loo_status, &c->loo_lists})
+                 &c->loo_status, &c->loo_lists, &c->mik_gv, &c->mik_gs, &c->mik_par, &c->mik_cls, &c->mik_tab,
+                 &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out})
     release(*b);
   bk::free_grid(c->gb);
   bk::free_tiles(c->tiles);
@@ -851,6 +864,7 @@ int bk_set_samples(bk_ctx* c, int64_t n, const double* x, const double* y, const
     if (int r = copy(c->hole, hole, 4)) return r;
   c->n = n;
   c->gen++;
+  c->samples_gen++;
   c->grid_valid = false;
   c->loo_ready = false;
   // Origin: the first sample with finite coordinates; planar: one elevation.
@@ -1318,6 +1332,367 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
 
 // Tuning knobs (not part of the reference interface): bucket edges, FP32+IR
 // sweeps and tolerance, grid cell factor.
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Multiple indicator kriging (crates/estimation/src/indicator.rs): one search
+// with the median threshold's variogram, then per variogram the ordinary
+// solves writing their weights, the indicator estimates from those weights,
+// and the conditional distributions.
+
+// MultipleIndicator::validate (indicator.rs:291-317).
+const char* check_mik(const bk_mik* p, const double* thr) {
+  const int k = p->n_thresholds;
+  if (k < 1) return "need at least one finite threshold";
+  for (int i = 0; i < k; ++i)
+    if (!std::isfinite(thr[i])) return "need at least one finite threshold";
+  for (int i = 1; i < k; ++i)
+    if (thr[i - 1] >= thr[i]) return "thresholds must be strictly increasing";
+  if (p->n_variograms != 1 && p->n_variograms != k) return "need one variogram, or one per threshold";
+  if (p->has_tails && !(std::isfinite(p->tails[0]) && std::isfinite(p->tails[1]) && p->tails[0] <= p->tails[1]))
+    return "tails must be finite with lower <= upper";
+  if (p->interpolation != BK_MIK_GLOBAL && p->interpolation != BK_MIK_LINEAR) return "unknown interpolation";
+  if (p->upper_tail < BK_TAIL_NONE || p->upper_tail > BK_TAIL_HYPERBOLIC) return "unknown upper tail";
+  if (p->upper_tail != BK_TAIL_NONE && !(std::isfinite(p->omega) && p->omega > 0.0))
+    return "the upper tail exponent must be > 0";
+  if (p->upper_tail == BK_TAIL_HYPERBOLIC && thr[k - 1] <= 0.0)
+    return "a hyperbolic upper tail needs a last threshold > 0";
+  return nullptr;
+}
+
+// A host copy of a parameter array given in host or device memory.
+template <class T>
+int host_array(bk_ctx* c, const T* src, int64_t n, std::vector<T>& out) {
+  out.resize(std::max<int64_t>(n, 0));
+  if (n <= 0 || !src) return 0;
+  if (is_device(src)) {
+    HIPOK(c, hipMemcpy(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  } else {
+    std::memcpy(out.data(), src, n * sizeof(T));
+  }
+  return 0;
+}
+
+// f64::total_cmp as an integer key.
+inline int64_t total_order_key(double v) {
+  int64_t b;
+  std::memcpy(&b, &v, sizeof(b));
+  return b ^ (int64_t)((uint64_t)(b >> 63) >> 1);
+}
+
+// Global::new (indicator.rs:152-200) of the fitted samples with declustering
+// weights w (empty: all 1), kept in the context while its inputs repeat.
+int mik_global(bk_ctx* c, const bk_mik* p, const std::vector<double>& thr, const std::vector<double>& w) {
+  std::vector<unsigned char> key;
+  auto put = [&](const void* q, size_t n) {
+    const unsigned char* b = (const unsigned char*)q;
+    key.insert(key.end(), b, b + n);
+  };
+  put(&c->samples_gen, sizeof(c->samples_gen));
+  put(thr.data(), thr.size() * sizeof(double));
+  put(&p->has_tails, sizeof(p->has_tails));
+  put(p->tails, sizeof(p->tails));
+  const int64_t nw = (int64_t)w.size();
+  put(&nw, sizeof(nw));
+  put(w.data(), w.size() * sizeof(double));
+  if (key == c->mik_key) return 0;
+  c->mik_key.clear();
+  const int64_t n = c->n;
+  std::vector<double> values(n);
+  HIPOK(c, hipMemcpy(values.data(), c->v.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> wt = w.empty() ? std::vector<double>(n, 1.0) : w;
+  for (double x : wt)
+    if (!std::isfinite(x) || x < 0.0) return fail(c, BK_E_INVALID, "weights must be finite and >= 0");
+  double total = -0.0;
+  for (double x : wt) total += x;
+  if (total <= 0.0) return fail(c, BK_E_INVALID, "weights must not all be 0");
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int64_t i, int64_t j) {
+    const int64_t a = total_order_key(values[i]), b = total_order_key(values[j]);
+    return a != b ? a < b : i < j;
+  });
+  std::vector<double>& v = c->mik_values;
+  std::vector<double>& sums = c->mik_sums;
+  v.resize(n);
+  sums.assign(3 * (n + 1), 0.0);
+  for (int64_t k = 0; k < n; ++k) v[k] = values[order[k]];
+  for (int64_t k = 0; k < n; ++k) {
+    const double wk = wt[order[k]] / total;
+    for (int j = 0; j < 3; ++j) sums[j * (n + 1) + k + 1] = sums[j * (n + 1) + k] + wk * mik_powi(v[k], j);
+  }
+  const int K = (int)thr.size();
+  c->mik_bounds.assign(1, 0);
+  c->mik_prop.resize(K);
+  for (int k = 0; k < K; ++k) {
+    const int64_t cut = mik_partition(v.data(), 0, n, [&](double x) { return x <= thr[k]; });
+    c->mik_bounds.push_back(cut);
+    c->mik_prop[k] = sums[cut];
+  }
+  c->mik_bounds.push_back(n);
+  const double lo = p->has_tails ? p->tails[0] : v.front(), hi = p->has_tails ? p->tails[1] : v.back();
+  c->mik_lower = std::fmin(std::fmin(lo, v.front()), thr.front());
+  c->mik_upper = std::fmax(std::fmax(hi, v.back()), thr.back());
+  if (int r = ensure(c, c->mik_gv, std::max<int64_t>(n, 1) * sizeof(double))) return r;
+  if (int r = ensure(c, c->mik_gs, sums.size() * sizeof(double))) return r;
+  HIPOK(c, hipMemcpyAsync(c->mik_gv.p, v.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPOK(c, hipMemcpyAsync(c->mik_gs.p, sums.data(), sums.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  c->mik_key = key;
+  return 0;
+}
+
+int mik_common(bk_ctx* c, const bk_mik* p, const double* thresholds, const bk_variogram* vgs,
+                      const double* weights, const bk_search* s, const bk_block* blk, int32_t precision, int64_t m,
+                      const double* tx, const double* ty, const double* tz, bool loo, const int64_t* which,
+                      int32_t nc, const double* cutoffs, int32_t nq, const double* quantiles, double* pit,
+                      const bk_mik_out* out) {
+  if (!c) return BK_E_INVALID;
+  if (!p || !thresholds || !vgs || !s || !out) return fail(c, BK_E_INVALID, "null argument");
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  if (nc < 0 || nq < 0 || (nc > 0 && !cutoffs) || (nq > 0 && !quantiles))
+    return fail(c, BK_E_INVALID, "cutoffs or quantiles missing");
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
+  HIPOK(c, hipSetDevice(c->device));
+  std::vector<double> thr, cut, qs, w;
+  if (int r = host_array(c, thresholds, (int64_t)p->n_thresholds, thr)) return r;
+  if (const char* msg = check_mik(p, thr.data())) return fail(c, BK_E_INVALID, msg);
+  if (int r = host_array(c, cutoffs, (int64_t)nc, cut)) return r;
+  if (int r = host_array(c, quantiles, (int64_t)nq, qs)) return r;
+  for (double x : cut)
+    if (!std::isfinite(x)) return fail(c, BK_E_INVALID, "cutoffs must be finite");
+  for (double x : qs)
+    if (!(x >= 0.0 && x <= 1.0)) return fail(c, BK_E_INVALID, "quantiles must be in [0, 1]");
+  if (const char* msg = check_search(s)) return fail(c, BK_E_INVALID, msg);
+  if (s->calibration)
+    return fail(c, BK_E_INVALID, "multiple indicator kriging takes a search without target_slope or target_efficiency");
+  const int K = p->n_thresholds, V = p->n_variograms;
+  bool stationary = true;
+  for (int v = 0; v < V; ++v) {
+    if (const char* msg = check_variogram(&vgs[v])) return fail(c, BK_E_INVALID, msg);
+    for (int k = 0; k < vgs[v].n_structures; ++k) stationary = stationary && vgs[v].structures[k].model != BK_POWER;
+  }
+  if (blk) {
+    if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
+      return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
+    for (int d = 0; d < 3; ++d)
+      if (!(std::isfinite(blk->size[d]) && blk->size[d] >= 0.0))
+        return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
+  }
+  if (c->n == 0) return fail(c, BK_E_STATE, "no samples");
+  if (weights)
+    if (int r = host_array(c, weights, c->n, w)) return r;
+  if (int r = mik_global(c, p, thr, w)) return r;
+  // The classes (indicator.rs:347-378) and their moments above each shared
+  // cutoff (column 0: -inf), which no target changes.
+  const int CC = nc + 1;
+  std::vector<MikClass> cls(K + 1);
+  const MikGlobal hg{c->mik_values.data(), c->mik_sums.data(), c->n};
+  for (int k = 0; k <= K; ++k) {
+    MikClass& q = cls[k];
+    std::memset(&q, 0, sizeof(q));
+    q.a = k == 0 ? c->mik_lower : thr[k - 1];
+    q.b = k == K ? c->mik_upper : thr[k];
+    q.s = c->mik_bounds[k];
+    q.e = c->mik_bounds[k + 1];
+    q.w = 1.0;
+    const double weight = c->mik_sums[q.e] - c->mik_sums[q.s];
+    if (k == K && p->upper_tail == BK_TAIL_POWER) {
+      q.kind = BK_SHAPE_POWER;
+      q.w = p->omega;
+    } else if (k == K && p->upper_tail == BK_TAIL_HYPERBOLIC) {
+      q.kind = BK_SHAPE_HYPERBOLIC;
+      q.w = p->omega;
+    } else if (p->interpolation == BK_MIK_GLOBAL && weight > 0.0) {
+      q.kind = BK_SHAPE_DATA;
+    } else {
+      q.kind = BK_SHAPE_POWER;
+    }
+  }
+  std::vector<double> tab((size_t)(K + 1) * CC * 3);
+  for (int k = 0; k <= K; ++k)
+    for (int ci = 0; ci < CC; ++ci) {
+      const double at = ci == 0 ? -std::numeric_limits<double>::infinity() : cut[ci - 1];
+      for (int j = 0; j < 3; ++j) tab[((size_t)k * CC + ci) * 3 + j] = mik_above(hg, cls[k], at, j);
+    }
+  std::vector<double> par(2 * K + nq);
+  std::copy(thr.begin(), thr.end(), par.begin());
+  std::copy(c->mik_prop.begin(), c->mik_prop.end(), par.begin() + K);
+  std::copy(qs.begin(), qs.end(), par.begin() + 2 * K);
+  if (int r = ensure(c, c->mik_par, par.size() * sizeof(double))) return r;
+  if (int r = ensure(c, c->mik_cls, cls.size() * sizeof(MikClass))) return r;
+  if (int r = ensure(c, c->mik_tab, tab.size() * sizeof(double))) return r;
+  HIPOK(c, hipMemcpyAsync(c->mik_par.p, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPOK(c, hipMemcpyAsync(c->mik_cls.p, cls.data(), cls.size() * sizeof(MikClass), hipMemcpyHostToDevice, c->stream));
+  HIPOK(c, hipMemcpyAsync(c->mik_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const double* dthr = (const double*)c->mik_par.p;
+  // Targets and the search of the median threshold's variogram (indicator.rs:319-321).
+  SearchArgs a;
+  int64_t stride;
+  if (int r = prepare_search(c, s, &vgs[V / 2], a, stride)) return r;
+  const double *dx, *dy, *dz;
+  const int64_t* dl = nullptr;
+  if (loo) {
+    const int64_t* dw = nullptr;
+    if (which) {
+      if (int r = stage_in(c, which, m, c->which, &dw)) return r;
+    } else {
+      m = c->n;
+    }
+    for (Buf* b : {&c->tx, &c->ty, &c->tz})
+      if (int r = ensure(c, *b, std::max<int64_t>(m, 1) * sizeof(double))) return r;
+    if (int r = ensure(c, c->left, std::max<int64_t>(m, 1) * sizeof(int64_t))) return r;
+    if (m > 0)
+      k_loo_targets<<<blocks_of(m), 256, 0, c->stream>>>(m, dw, (const double*)c->x.p, (const double*)c->y.p,
+                                                         (const double*)c->z.p, (double*)c->tx.p, (double*)c->ty.p,
+                                                         (double*)c->tz.p, (int64_t*)c->left.p);
+    dx = (const double*)c->tx.p;
+    dy = (const double*)c->ty.p;
+    dz = (const double*)c->tz.p;
+    dl = (const int64_t*)c->left.p;
+  } else {
+    if (m > 0 && (!tx || !ty || !tz)) return fail(c, BK_E_INVALID, "targets missing");
+    if (int r = stage_in(c, tx, m, c->tx, &dx)) return r;
+    if (int r = stage_in(c, ty, m, c->ty, &dy)) return r;
+    if (int r = stage_in(c, tz, m, c->tz, &dz)) return r;
+  }
+  if (int r = run_search(c, a, m, dx, dy, dz, dl)) return r;
+  if (loo && m > 0)
+    k_loo_nonempty<<<blocks_of(m), 256, 0, c->stream>>>(m, (const int32_t*)c->ncount.p, (int32_t*)c->sstatus.p);
+  // Outputs: device arrays in place; host arrays through one scratch block.
+  const int64_t M = std::max<int64_t>(m, 1);
+  const int64_t dbl = (int64_t)K * M * 2 + M * 4 + (int64_t)nc * M * 2 + (int64_t)nq * M, i32 = 3 * M;
+  if (int r = ensure(c, c->mik_out, dbl * sizeof(double) + i32 * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->mik_w, M * stride * sizeof(double))) return r;
+  if (int r = ensure(c, c->mik_st, 2 * M * sizeof(int32_t))) return r;
+  for (Buf* b : {&c->value, &c->variance})
+    if (int r = ensure(c, *b, M * sizeof(double))) return r;
+  double* dpos = (double*)c->mik_out.p;
+  int32_t* ipos = (int32_t*)(dpos + dbl);
+  auto place_d = [&](double* user, int64_t n, bool need) -> double* {
+    double* own = dpos;
+    dpos += n;
+    if (user && is_device(user)) return user;
+    return user || need ? own : nullptr;
+  };
+  auto place_i = [&](int32_t* user, bool need) -> int32_t* {
+    int32_t* own = ipos;
+    ipos += M;
+    if (user && is_device(user)) return user;
+    return user || need ? own : nullptr;
+  };
+  double* d_raw = place_d(out->raw, (int64_t)K * M, true);
+  double* d_cdf = place_d(out->cdf, (int64_t)K * M, true);
+  double* d_corr = place_d(out->correction, M, false);
+  double* d_mean = place_d(out->mean, M, false);
+  double* d_var = place_d(out->variance, M, false);
+  double* d_pit = place_d(pit, M, false);
+  double* d_pa = place_d(out->probability_above, (int64_t)nc * M, false);
+  double* d_ma = place_d(out->mean_above, (int64_t)nc * M, false);
+  double* d_qv = place_d(out->quantile_values, (int64_t)nq * M, false);
+  int32_t* d_viol = place_i(out->violations, false);
+  int32_t* d_nu = place_i(out->n_used, false);
+  int32_t* d_st = place_i(out->status, true);
+  int32_t* st_v = (int32_t*)c->mik_st.p;
+  const int kind = p->simple ? BK_SIMPLE : BK_ORDINARY;
+  if (m > 0) {
+    if (p->simple && !stationary) {
+      // krige.rs:117-124: simple kriging needs a sill; every searched target fails.
+      if (bk::launch_mik_status(c->stream, m, (const int32_t*)c->sstatus.p, BK_INVALID_PARAMETERS, d_st))
+        return fail(c, BK_E_HIP, "status launch failed");
+    } else {
+      for (int v = 0; v < V; ++v) {
+        if (int r = run_solve(c, &vgs[v], nullptr, kind, 0.0, precision, m, dx, dy, dz, (int)stride, blk,
+                              (double*)c->value.p, (double*)c->variance.p, nullptr, nullptr, nullptr, st_v,
+                              (double*)c->mik_w.p))
+          return r;
+        MikRawArgs ra;
+        ra.m = m;
+        ra.v = v;
+        ra.V = V;
+        ra.K = K;
+        ra.simple = p->simple ? 1 : 0;
+        ra.thr = dthr;
+        ra.prop = dthr + K;
+        ra.sv = (const double*)c->v.p;
+        ra.nidx = (const int32_t*)c->nidx.p;
+        ra.ncount = (const int32_t*)c->ncount.p;
+        ra.stride = (int)stride;
+        ra.w = (const double*)c->mik_w.p;
+        ra.st_v = st_v;
+        ra.status = d_st;
+        ra.raw = d_raw;
+        if (bk::launch_mik_raw(c->stream, ra)) return fail(c, BK_E_HIP, "indicator launch failed");
+      }
+    }
+    MikCondArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    ca.m = m;
+    ca.K = K;
+    ca.C = nc;
+    ca.Q = nq;
+    ca.raw = d_raw;
+    ca.status = d_st;
+    ca.ncount = (const int32_t*)c->ncount.p;
+    ca.cls = (const MikClass*)c->mik_cls.p;
+    ca.table = (const double*)c->mik_tab.p;
+    ca.quantiles = dthr + 2 * K;
+    ca.g = MikGlobal{(const double*)c->mik_gv.p, (const double*)c->mik_gs.p, c->n};
+    ca.upper = c->mik_upper;
+    ca.own_values = loo ? (const double*)c->v.p : nullptr;
+    ca.held = dl;
+    ca.cdf = d_cdf;
+    ca.correction = d_corr;
+    ca.mean = d_mean;
+    ca.variance = d_var;
+    ca.pa = d_pa;
+    ca.ma = d_ma;
+    ca.qv = d_qv;
+    ca.violations = d_viol;
+    ca.n_used = d_nu;
+    ca.pit = loo ? d_pit : nullptr;
+    if (bk::launch_mik_conditional(c->stream, ca)) return fail(c, BK_E_HIP, "conditional launch failed");
+  }
+  if (int r = stage_out(c, out->raw, d_raw, (int64_t)K * m)) return r;
+  if (int r = stage_out(c, out->cdf, d_cdf, (int64_t)K * m)) return r;
+  if (int r = stage_out(c, out->correction, d_corr, m)) return r;
+  if (int r = stage_out(c, out->mean, d_mean, m)) return r;
+  if (int r = stage_out(c, out->variance, d_var, m)) return r;
+  if (int r = stage_out(c, pit, d_pit, m)) return r;
+  if (int r = stage_out(c, out->probability_above, d_pa, (int64_t)nc * m)) return r;
+  if (int r = stage_out(c, out->mean_above, d_ma, (int64_t)nc * m)) return r;
+  if (int r = stage_out(c, out->quantile_values, d_qv, (int64_t)nq * m)) return r;
+  if (int r = stage_out(c, out->violations, d_viol, m)) return r;
+  if (int r = stage_out(c, out->n_used, d_nu, m)) return r;
+  if (int r = stage_out(c, out->status, d_st, m)) return r;
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bk_mik_predict(bk_ctx* c, const bk_mik* mik, const double* thresholds, const bk_variogram* variograms,
+                   const double* weights, const bk_search* search, const bk_block* block, int32_t precision,
+                   int64_t m, const double* tx, const double* ty, const double* tz, int32_t n_cutoffs,
+                   const double* cutoffs, int32_t n_quantiles, const double* quantiles, const bk_mik_out* out) {
+  return mik_common(c, mik, thresholds, variograms, weights, search, block, precision, m, tx, ty, tz, false, nullptr,
+                    n_cutoffs, cutoffs, n_quantiles, quantiles, nullptr, out);
+}
+
+int bk_mik_cross_validate(bk_ctx* c, const bk_mik* mik, const double* thresholds, const bk_variogram* variograms,
+                          const double* weights, const bk_search* search, int32_t precision, int64_t m,
+                          const int64_t* which, double* pit, const bk_mik_out* out) {
+  if (!out) return c ? fail(c, BK_E_INVALID, "null argument") : BK_E_INVALID;
+  bk_mik_out o = *out;
+  o.probability_above = o.mean_above = o.quantile_values = nullptr;
+  return mik_common(c, mik, thresholds, variograms, weights, search, nullptr, precision, which ? m : (c ? c->n : 0),
+                    nullptr, nullptr, nullptr, true, which, 0, nullptr, 0, nullptr, pit, &o);
+}
+
 int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor) {
   if (!c) return BK_E_INVALID;
   if (edges3) {

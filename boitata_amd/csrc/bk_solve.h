@@ -48,6 +48,7 @@ struct SolveArgs {
   double* variance;
   double* lagrange;     // may be null
   double* support;      // may be null
+  double* weights;      // may be null: kriging weights per target, in neighbour order at [t * stride + k]
   int32_t* n_used;      // may be null
   int32_t* status;
   int32_t* guard;       // targets handed to the guarded (merge + LU) kernel

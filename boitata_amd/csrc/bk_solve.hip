@@ -113,6 +113,7 @@ __device__ __forceinline__ void finish_outputs(const SolveArgs& a, int t, int r,
   }
   double sum = group_sum<NMAX>(f);
   double swc = group_sum<NMAX>(in ? w * c : 0.0);
+  if (a.weights && in) a.weights[(int64_t)t * a.stride + r] = w;
   if (r == 0) {
     if (ok) {
       double val = sum;
@@ -1747,6 +1748,7 @@ __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, doubl
       double sum = -0.0;
       for (int i = 0; i < n0; ++i) {
         double wi = b[cl[i]] / (double)members[cl[i]];
+        if (a.weights) a.weights[(int64_t)t * a.stride + i] = wi;
         double zz = Vo[i];
         double f = a.kind == BK_SIMPLE ? zz - a.param : (a.kind == BK_INDICATOR ? (zz <= a.param ? 1.0 : 0.0) : zz);
         sum += wi * f;

@@ -230,3 +230,57 @@ class Engine:
         self._check(self._lib.bk_loo(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
                                      int(precision), m, _ptr(which), _ptr(value), _ptr(variance), _ptr(status)))
         return dict(value=value, variance=variance, status=status)
+
+    def _mik_outputs(self, m, k, nc, nq, like):
+        """Output arrays of multiple indicator kriging: entry-major (rows, m)."""
+        if like is not None and hasattr(like, "data_ptr") and like.is_cuda:
+            import torch
+
+            dev = like.device
+            f = lambda *s: torch.empty(*s, dtype=torch.float64, device=dev)  # noqa: E731
+            i = lambda *s: torch.empty(*s, dtype=torch.int32, device=dev)  # noqa: E731
+        else:
+            f = lambda *s: np.empty(s, dtype=np.float64)  # noqa: E731
+            i = lambda *s: np.empty(s, dtype=np.int32)  # noqa: E731
+        out = dict(raw=f(k, m), cdf=f(k, m), correction=f(m), mean=f(m), variance=f(m),
+                   probability_above=f(nc, m), mean_above=f(nc, m), quantile_values=f(nq, m),
+                   violations=i(m), n_used=i(m), status=i(m))
+        ptrs = [_ptr(out[name]) if name in out else None for name, _ in _abi.MikOut._fields_]
+        return out, _abi.MikOut(*[p.value if p is not None else None for p in ptrs])
+
+    def mik_predict(self, mik, thresholds, variograms, search, tx, ty, tz, weights=None, block=None, cutoffs=(),
+                    quantiles=(), precision=_abi.FP64):
+        """Multiple indicator kriging of the targets (one search pass):
+        conditional distributions, outputs (rows, m) as documented for bk_mik_out."""
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+        vs = list(variograms) if isinstance(variograms, (list, tuple)) else [variograms]
+        vgs = (_abi.Variogram * len(vs))(*vs)
+        w = _f64(weights)
+        cut = np.ascontiguousarray(cutoffs, dtype=np.float64)
+        qs = np.ascontiguousarray(quantiles, dtype=np.float64)
+        out, o = self._mik_outputs(m, len(thr), len(cut), len(qs), tx)
+        self._check(self._lib.bk_mik_predict(self._h, C.byref(mik), _ptr(thr), C.cast(vgs, C.c_void_p), _ptr(w),
+                                             C.byref(search), C.byref(block) if block is not None else None,
+                                             int(precision), m, _ptr(tx), _ptr(ty), _ptr(tz), len(cut), _ptr(cut),
+                                             len(qs), _ptr(qs), C.byref(o)))
+        return out
+
+    def mik_cross_validate(self, mik, thresholds, variograms, search, weights=None, which=None,
+                           precision=_abi.FP64):
+        """Leave-one-out multiple indicator kriging at the samples `which`
+        (None: all); adds `pit`, F*(z) at each sample's own value."""
+        which = _i64(which)
+        m = self.n if which is None else int(len(which))
+        thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+        vs = list(variograms) if isinstance(variograms, (list, tuple)) else [variograms]
+        vgs = (_abi.Variogram * len(vs))(*vs)
+        w = _f64(weights)
+        out, o = self._mik_outputs(m, len(thr), 0, 0, None)
+        pit = np.empty(m, dtype=np.float64)
+        self._check(self._lib.bk_mik_cross_validate(self._h, C.byref(mik), _ptr(thr), C.cast(vgs, C.c_void_p),
+                                                    _ptr(w), C.byref(search), int(precision), m, _ptr(which),
+                                                    _ptr(pit), C.byref(o)))
+        out["pit"] = pit
+        return out

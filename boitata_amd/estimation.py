@@ -4,7 +4,8 @@ The classes keep the constructor, ``fit``, ``predict`` and ``cross_validate``
 signatures of the reference's Python API (python/boitata/estimation.py:251-485,
 through ``_Estimator``, crates/py/src/estimation.rs:1213-1866) for the methods
 the engine covers: ``OrdinaryKriging``, ``SimpleKriging``,
-``IndicatorKriging`` and ``BlockKriging``; and of ``Variogram``
+``IndicatorKriging``, ``BlockKriging`` and ``MultipleIndicatorKriging``
+(crates/py/src/indicator.rs:116-420); and of ``Variogram``
 (crates/py/src/variogram.rs:329-362), ``Structure`` (:211) and ``Search``
 (crates/py/src/estimation.rs:415-531).
 
@@ -43,8 +44,11 @@ from .errors import InvalidInput
 __all__ = [
     "BlockKriging",
     "CrossValidation",
+    "IndicatorCrossValidation",
     "IndicatorKriging",
+    "IndicatorSummary",
     "InvalidInput",
+    "MultipleIndicatorKriging",
     "OrdinaryKriging",
     "Search",
     "SimpleKriging",
@@ -513,3 +517,283 @@ class BlockKriging(_Base):
         out = self._engine.block_krige(self._variogram._c, s._c, self._block, T[:, 0].copy(), T[:, 1].copy(),
                                        T[:, 2].copy(), self._kind, self._param, self._precision)
         return out["value"], out["variance"], out["status"] == _abi.OK
+
+
+# --------------------------------------------------------------------------
+# Multiple indicator kriging (crates/py/src/indicator.rs:116-420,
+# crates/estimation/src/indicator.rs).
+
+_INTERPOLATIONS = {"global": _abi.MIK_GLOBAL, "linear": _abi.MIK_LINEAR}
+_TAILS = {"power": _abi.TAIL_POWER, "hyperbolic": _abi.TAIL_HYPERBOLIC}
+DIAGNOSTIC_COLUMNS = ("n_samples", "pass", "n_holes", "mean_distance", "max_samples_reached", "correction",
+                      "n_order_violations")
+
+
+class DiagnosticsTable:
+    """Per-target search diagnostics (crates/py/src/indicator.rs:25-57): named
+    float columns, NaN where a target is unestimated."""
+
+    def __init__(self, columns):
+        self._columns = {k: np.asarray(v, dtype=np.float64) for k, v in columns}
+
+    @property
+    def column_names(self):
+        return list(self._columns)
+
+    def __getitem__(self, name):
+        return self._columns[name]
+
+    def __len__(self):
+        return len(next(iter(self._columns.values()))) if self._columns else 0
+
+
+@dataclass(frozen=True)
+class IndicatorSummary:
+    """Conditional distributions per target (indicator.rs:101-116): arrays
+    with one row per target; per-threshold, per-cutoff and per-quantile
+    arrays have one column each; NaN where unestimated."""
+
+    thresholds: list
+    cdf: np.ndarray
+    correction: np.ndarray
+    mean: np.ndarray
+    variance: np.ndarray
+    cutoffs: list
+    probability_above: np.ndarray
+    mean_above: np.ndarray
+    quantiles: list
+    quantile_values: np.ndarray
+    diagnostics: DiagnosticsTable | None = None
+
+    @property
+    def std(self) -> np.ndarray:
+        return np.sqrt(self.variance)
+
+
+@dataclass(frozen=True)
+class IndicatorCrossValidation(CrossValidation):
+    """Cross-validation of multiple indicator kriging (python/boitata/estimation.py:150-185):
+    `estimate` is the E-type mean, `variance` the conditional variance, `cdf`
+    the (samples, thresholds) corrected probabilities and `pit` F*(actual)."""
+
+    thresholds: list
+    cdf: np.ndarray
+    pit: np.ndarray
+
+    @property
+    def brier(self) -> np.ndarray:
+        indicator = self.actual[:, None] <= np.asarray(self.thresholds)
+        return np.nanmean((self.cdf - indicator) ** 2, axis=0)
+
+    def accuracy(self, p):
+        pit = self.pit[~np.isnan(self.pit)]
+        half = np.asarray(p, dtype=float)[..., None] / 2
+        return np.mean(np.abs(pit - 0.5) <= half, axis=-1)
+
+    @property
+    def goodness(self) -> float:
+        p = (np.arange(100) + 0.5) / 100
+        excess = self.accuracy(p) - p
+        return float(1 - np.mean(np.where(excess >= 0, 1, -2) * excess))
+
+
+class MultipleIndicatorKriging:
+    """Multiple indicator kriging (crates/py/src/indicator.rs:129-192): the
+    conditional distribution P(Z <= t | data) kriged at each of `thresholds`
+    with its own variogram (or one for all), corrected for order relations,
+    completed within classes (`interpolation` "global" or "linear") and above
+    the last threshold (`upper_tail` ("power", w) or ("hyperbolic", w)), then
+    summarised. The search must not be calibrated."""
+
+    def __init__(self, variogram, search, thresholds, *, simple=False, tails=None, interpolation="global",
+                 upper_tail=None, precision: str = "fp64", device: int = 0):
+        if isinstance(variogram, Variogram):
+            vgs = [variogram]
+        elif isinstance(variogram, Sequence) and len(variogram) > 0 and all(isinstance(v, Variogram)
+                                                                            for v in variogram):
+            vgs = list(variogram)
+        else:
+            raise InvalidInput("variogram must be a Variogram or a sequence of Variogram")
+        if interpolation not in _INTERPOLATIONS:
+            raise InvalidInput(f"unknown interpolation {interpolation!r}")
+        kind, omega = _abi.TAIL_NONE, 1.0
+        if upper_tail is not None:
+            name, omega = upper_tail
+            if name not in _TAILS:
+                raise InvalidInput(f"unknown upper tail {name!r}")
+            kind, omega = _TAILS[name], float(omega)
+        if precision not in _PRECISIONS:
+            raise InvalidInput(f"precision is one of {', '.join(_PRECISIONS)}")
+        thr = [_number(t, "threshold") for t in thresholds]
+        self._thresholds = thr
+        self._variograms = vgs
+        self._mik = _abi.make_mik(thr, len(vgs), simple=simple, interpolation=_INTERPOLATIONS[interpolation],
+                                  upper_tail=kind, omega=omega, tails=tails)
+        self._validate()
+        self._passes = _searches(search)
+        if any(p._c.calibration for p in self._passes):
+            raise InvalidInput("MultipleIndicatorKriging takes a search without target_slope or target_efficiency")
+        self._precision = _PRECISIONS[precision]
+        self._device = device
+        self._engine = None
+        self._values = None
+        self._weights = None
+        self._holes = None
+        self._xyz = None
+
+    def _validate(self):
+        # indicator.rs:291-317
+        t, p = self._thresholds, self._mik
+        if not t or not all(math.isfinite(v) for v in t):
+            raise InvalidInput("need at least one finite threshold")
+        if any(a >= b for a, b in zip(t, t[1:])):
+            raise InvalidInput("thresholds must be strictly increasing")
+        if len(self._variograms) not in (1, len(t)):
+            raise InvalidInput("need one variogram, or one per threshold")
+        if p.has_tails and not (math.isfinite(p.tails[0]) and math.isfinite(p.tails[1]) and p.tails[0] <= p.tails[1]):
+            raise InvalidInput("tails must be finite with lower <= upper")
+        if p.upper_tail != _abi.TAIL_NONE and not (math.isfinite(p.omega) and p.omega > 0.0):
+            raise InvalidInput("the upper tail exponent must be > 0")
+        if p.upper_tail == _abi.TAIL_HYPERBOLIC and t[-1] <= 0.0:
+            raise InvalidInput("a hyperbolic upper tail needs a last threshold > 0")
+
+    @property
+    def thresholds(self):
+        return list(self._thresholds)
+
+    def _vgs(self):
+        return [v._c for v in self._variograms]
+
+    def fit(self, coords, values, *, weights=None, holes=None):
+        """Stores the samples; samples sharing a location keep the first one.
+        `weights` decluster the global distribution (the simple kriging means,
+        the class contents and the tails)."""
+        if isinstance(values, str) or isinstance(weights, str):
+            raise NotImplementedError("columns by name need a container; pass arrays")
+        xyz = coords_of(coords)
+        n = len(xyz)
+        v = np.asarray(values, dtype=np.float64).reshape(-1)
+        if len(v) != n:
+            raise InvalidInput(f"values: expected {n} values, got {len(v)}")
+        if not np.isfinite(v).all():
+            raise InvalidInput("values must be finite")
+        w = None
+        if weights is not None:
+            w = np.asarray(weights, dtype=np.float64).reshape(-1)
+            if len(w) != n:
+                raise InvalidInput(f"weights: expected {n} values, got {len(w)}")
+            if not np.isfinite(w).all() or (w < 0).any() or w.sum() <= 0:
+                raise InvalidInput("weights must be >= 0 and not all 0")
+        text, ids = hole_codes(holes, n)
+        keep, note = distinct_rows(xyz, text)
+        if note:
+            warnings.warn(note, UserWarning, stacklevel=2)
+        xyz, v = xyz[keep], v[keep]
+        w = None if w is None else w[keep]
+        ids = None if ids is None else ids[keep]
+        if self._engine is None:
+            from .engine import Engine
+
+            self._engine = Engine(self._device)
+        self._engine.set_samples(xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy(), v, None, ids)
+        self._values, self._weights, self._holes, self._xyz = v, w, ids, xyz
+        return self
+
+    def _fitted(self):
+        if self._values is None:
+            raise InvalidInput("call fit first")
+
+    def predict(self, targets, *, cutoffs=None, quantiles=(), anisotropy=None, diagnostics=False,
+                discretization=None, block_size=None) -> IndicatorSummary:
+        """Conditional distribution at each target, through the search passes.
+        With `discretization` the indicators are kriged over blocks of
+        `block_size` centred on the targets (the reference takes both from a
+        BlockModel)."""
+        if anisotropy is not None:
+            raise NotImplementedError("local anisotropy is not supported by this engine")
+        self._fitted()
+        cut = [] if cutoffs is None else [_number(c, "cutoff") for c in np.atleast_1d(cutoffs)]
+        qs = [float(q) for q in quantiles]
+        block = None
+        if discretization is not None:
+            if block_size is None:
+                raise InvalidInput("discretization needs BlockModel targets: give block_size")
+            block = _abi.make_block([float(s) for s in block_size], tuple(int(k) for k in discretization))
+        T = coords_of(targets)
+        m, K, C, Q = len(T), len(self._thresholds), len(cut), len(qs)
+        res = dict(cdf=np.full((K, m), np.nan), correction=np.full(m, np.nan), mean=np.full(m, np.nan),
+                   variance=np.full(m, np.nan), probability_above=np.full((C, m), np.nan),
+                   mean_above=np.full((C, m), np.nan), quantile_values=np.full((Q, m), np.nan),
+                   violations=np.zeros(m), n_used=np.zeros(m))
+        which = np.full(m, -1, dtype=np.int64)
+        remaining = np.arange(m)
+        for p, s in enumerate(self._passes):
+            if len(remaining) == 0:
+                break
+            R = T[remaining]
+            out = self._engine.mik_predict(self._mik, self._thresholds, self._vgs(), s._c, R[:, 0].copy(),
+                                           R[:, 1].copy(), R[:, 2].copy(), weights=self._weights, block=block,
+                                           cutoffs=cut, quantiles=qs, precision=self._precision)
+            ok = out["status"] == _abi.OK
+            rows = remaining[ok]
+            for k in res:
+                src = out[k]
+                res[k][..., rows] = src[..., ok]
+            which[rows] = p
+            remaining = remaining[~ok]
+        diag = self._diagnostics(T, which, res) if diagnostics else None
+        return IndicatorSummary(list(self._thresholds), res["cdf"].T.copy(), res["correction"], res["mean"],
+                                res["variance"], cut, res["probability_above"].T.copy(), res["mean_above"].T.copy(),
+                                qs, res["quantile_values"].T.copy(), diag)
+
+    def _diagnostics(self, T, which, res):
+        """indicator.rs:793-816 with neighborhood_stats (neighborhood.rs:66-136)
+        over each target's samples: their count, distinct holes (untagged
+        samples count one each), mean Euclidean distance, in ascending order."""
+        m = len(T)
+        n_holes = np.full(m, np.nan)
+        mean_d = np.full(m, np.nan)
+        reached = np.full(m, np.nan)
+        svg = self._variograms[len(self._variograms) // 2]._c
+        for p, s in enumerate(self._passes):
+            rows = np.nonzero(which == p)[0]
+            if len(rows) == 0:
+                continue
+            R = T[rows]
+            idx, cnt, _ = self._engine.neighbors(s._c, svg, R[:, 0].copy(), R[:, 1].copy(), R[:, 2].copy())
+            for r, t in enumerate(rows):
+                sel = idx[r, : cnt[r]]
+                d = self._xyz[sel] - T[t]
+                dist = np.sort(np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]))
+                mean_d[t] = np.cumsum(dist)[-1] / len(dist)
+                n_holes[t] = len(sel) if self._holes is None else len(np.unique(self._holes[sel]))
+                reached[t] = 1.0 if cnt[r] >= s.max_samples else 0.0
+        ok = which >= 0
+        n_samples = np.where(ok, res["n_used"], np.nan)
+        viol = np.where(ok, res["violations"], np.nan)
+        passes = np.where(ok, which + 1.0, np.nan)
+        return DiagnosticsTable(zip(DIAGNOSTIC_COLUMNS, (n_samples, passes, n_holes, mean_d, reached,
+                                                         res["correction"], viol)))
+
+    def cross_validate(self, *, folds=None) -> IndicatorCrossValidation:
+        """Re-estimates each sample's distribution from the others, through the
+        same search passes, against the global distribution of all samples."""
+        if folds is not None:
+            raise NotImplementedError("k-fold cross-validation is not supported by this engine")
+        self._fitted()
+        n, K = len(self._values), len(self._thresholds)
+        cdf = np.full((K, n), np.nan)
+        mean, var, pit = np.full(n, np.nan), np.full(n, np.nan), np.full(n, np.nan)
+        remaining = np.arange(n)
+        for s in self._passes:
+            if len(remaining) == 0:
+                break
+            out = self._engine.mik_cross_validate(self._mik, self._thresholds, self._vgs(), s._c,
+                                                  weights=self._weights, which=remaining.astype(np.int64),
+                                                  precision=self._precision)
+            ok = out["status"] == _abi.OK
+            rows = remaining[ok]
+            cdf[:, rows] = out["cdf"][:, ok]
+            mean[rows], var[rows], pit[rows] = out["mean"][ok], out["variance"][ok], out["pit"][ok]
+            remaining = remaining[~ok]
+        return IndicatorCrossValidation(self._values.copy(), mean, var, list(self._thresholds), cdf.T.copy(), pit)

@@ -209,6 +209,61 @@ int bk_loo_prepare(bk_ctx* ctx, const bk_search* search, const bk_variogram* vg)
 int bk_loo_loss_grad(bk_ctx* ctx, const bk_variogram* vg, int32_t precision, double* loss,
                      double* grad, int64_t* n_used);
 
+/* Multiple indicator kriging, crates/estimation/src/indicator.rs:53-67
+ * (MultipleIndicator), :35-50 (Interpolation, UpperTail) and
+ * crates/py/src/indicator.rs:116-192 (the Python constructor). */
+enum { BK_MIK_GLOBAL = 0, BK_MIK_LINEAR = 1 };
+enum { BK_TAIL_NONE = 0, BK_TAIL_POWER = 1, BK_TAIL_HYPERBOLIC = 2 };
+
+typedef struct {
+  int32_t n_thresholds;   /* K >= 1, strictly increasing */
+  int32_t n_variograms;   /* 1 (one for every threshold) or K (one per threshold) */
+  int32_t simple;         /* simple kriging about the declustered proportions; else ordinary */
+  int32_t interpolation;  /* within a class: BK_MIK_GLOBAL (the declustered data) or BK_MIK_LINEAR */
+  int32_t upper_tail;     /* model of the class above the last threshold: BK_TAIL_* */
+  int32_t has_tails;      /* tails[] bound the distribution; else the data minimum and maximum */
+  double omega;           /* exponent of the upper tail model */
+  double tails[2];        /* lower, upper: widened to cover the data and thresholds */
+} bk_mik;
+
+/* Outputs of multiple indicator kriging; any pointer may be NULL. Arrays with
+ * several entries per target are entry-major: entry k of target t at [k * m + t]. */
+typedef struct {
+  double* raw;                /* K * m: kriged P(Z <= t_k), before the correction */
+  double* cdf;                /* K * m: after the order-relation correction */
+  double* correction;         /* m: sum over k of |cdf - raw| */
+  double* mean;               /* m: E-type estimate */
+  double* variance;           /* m */
+  double* probability_above;  /* n_cutoffs * m */
+  double* mean_above;         /* n_cutoffs * m; NaN where nothing lies above the cutoff */
+  double* quantile_values;    /* n_quantiles * m */
+  int32_t* violations;        /* m: thresholds whose kriged probability the correction changed */
+  int32_t* n_used;            /* m: samples found by the search */
+  int32_t* status;            /* m: BK_OK, or the first failure among the target's search and solves */
+} bk_mik_out;
+
+/* MultipleIndicator::predict for one search pass (indicator.rs:439-487):
+ * per target, the search of variograms[n_variograms / 2] (:319-321); each
+ * variogram's kriging weights on those samples (krige_indicators, :688-749,
+ * with block_krige_points on the discretisation of `block` when it is not
+ * NULL); the indicator estimates; then the conditional distribution
+ * (:380-437) against the declustered global distribution of the fitted
+ * samples (Global::new, :152-200; weights may be NULL: all 1). Values are NaN
+ * where status != BK_OK. The search must not be calibrated. */
+int bk_mik_predict(bk_ctx* ctx, const bk_mik* mik, const double* thresholds, const bk_variogram* variograms,
+                   const double* weights, const bk_search* search, const bk_block* block, int32_t precision,
+                   int64_t m, const double* tx, const double* ty, const double* tz, int32_t n_cutoffs,
+                   const double* cutoffs, int32_t n_quantiles, const double* quantiles, const bk_mik_out* out);
+
+/* MultipleIndicator::cross_validate, leave-one-out form (indicator.rs:489-529)
+ * for one pass: each sample at `which` (m of them; NULL = all, m ignored) is
+ * re-estimated without itself; pit[t] = F*(z) at its own value (NaN where
+ * unestimated). probability_above, mean_above and quantile_values of `out`
+ * are not written. */
+int bk_mik_cross_validate(bk_ctx* ctx, const bk_mik* mik, const double* thresholds,
+                          const bk_variogram* variograms, const double* weights, const bk_search* search,
+                          int32_t precision, int64_t m, const int64_t* which, double* pit, const bk_mik_out* out);
+
 /* Tuning of the device path (no counterpart in the reference): bucket edges of
  * the neighbour counts (three of 16, 32, 64, rising to 64; NULL keeps them),
  * FP32+IR refinement sweeps and relative tolerance, and the grid cell size as

@@ -72,6 +72,14 @@ def lib() -> C.CDLL:
             "or_prefix_quality": (I32, [SP, VP, I32, P, BP, I32, P, P, P]),
             "or_calibrated_count": (I32, [I32, D, I64, I32, P, P, P]),
             "or_balanced": (I32, [SP, SR, VP, P, I32, P, P]),
+            "or_mik_predict": (I32, [SP, C.POINTER(_abi.Mik), P, P, P, SR, P, I64, P, P, P, I32, P, I32, P,
+                                     C.POINTER(_abi.MikOut), I32]),
+            "or_mik_cross_validate": (I32, [SP, C.POINTER(_abi.Mik), P, P, P, SR, I64, P, P,
+                                            C.POINTER(_abi.MikOut), I32]),
+            "or_mik_global": (I32, [I64, P, P, I32, P, P, P, P, P, P, P]),
+            "or_mik_shape": (D, [I64, P, P, I32, P, P, I32, D, I64, I64, D, D, D, I32]),
+            "or_mik_conditional": (I32, [I64, P, P, C.POINTER(_abi.Mik), P, I64, P, I32, P, I32, P,
+                                         C.POINTER(_abi.MikOut)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -296,4 +304,100 @@ def balanced(data: Data, search, vg, target, chosen):
                            len(chosen), _p(chosen), _p(out))
     if rc != 0:
         raise ValueError("invalid search")
+    return out
+
+
+def _variogram_array(variograms):
+    vs = list(variograms) if isinstance(variograms, (list, tuple)) else [variograms]
+    arr = (_abi.Variogram * len(vs))(*vs)
+    return arr, len(vs)
+
+
+def _mik_out(m, k, nc, nq):
+    out = dict(raw=np.zeros((k, m)), cdf=np.zeros((k, m)), correction=np.zeros(m), mean=np.zeros(m),
+               variance=np.zeros(m), probability_above=np.zeros((nc, m)), mean_above=np.zeros((nc, m)),
+               quantile_values=np.zeros((nq, m)), violations=np.zeros(m, np.int32), n_used=np.zeros(m, np.int32),
+               status=np.zeros(m, np.int32))
+    o = _abi.MikOut(*[out[name].ctypes.data for name, _ in _abi.MikOut._fields_])
+    return out, o
+
+
+def mik_predict(data: Data, mik, thresholds, variograms, search, targets, weights=None, block=None, cutoffs=(),
+                quantiles=(), threads=0):
+    """MultipleIndicator::predict for one pass; outputs entry-major (k, m)."""
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    cut = np.ascontiguousarray(cutoffs, dtype=np.float64)
+    qs = np.ascontiguousarray(quantiles, dtype=np.float64)
+    vgs, _ = _variogram_array(variograms)
+    out, o = _mik_out(len(t), len(thr), len(cut), len(qs))
+    rc = lib().or_mik_predict(C.byref(data.s), C.byref(mik), _p(thr), C.cast(vgs, C.c_void_p), _p(w),
+                              C.byref(search), C.byref(block) if block is not None else None, len(t), _p(tx), _p(ty),
+                              _p(tz), len(cut), _p(cut), len(qs), _p(qs), C.byref(o), threads)
+    if rc != 0:
+        raise ValueError(f"invalid parameters ({rc})")
+    return out
+
+
+def mik_cross_validate(data: Data, mik, thresholds, variograms, search, weights=None, which=None, threads=0):
+    thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    wh = None if which is None else np.ascontiguousarray(which, dtype=np.int64)
+    m = data.n if wh is None else len(wh)
+    vgs, _ = _variogram_array(variograms)
+    out, o = _mik_out(m, len(thr), 0, 0)
+    pit = np.zeros(m)
+    rc = lib().or_mik_cross_validate(C.byref(data.s), C.byref(mik), _p(thr), C.cast(vgs, C.c_void_p), _p(w),
+                                     C.byref(search), m, _p(wh), _p(pit), C.byref(o), threads)
+    if rc != 0:
+        raise ValueError(f"invalid parameters ({rc})")
+    out["pit"] = pit
+    return out
+
+
+def mik_global(values, thresholds, weights=None, tails=None):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    tl = None if tails is None else np.ascontiguousarray(tails, dtype=np.float64)
+    n, k = len(v), len(thr)
+    sorted_, sums = np.zeros(n), np.zeros((3, n + 1))
+    classes, bounds, props = np.zeros((k + 1, 2), np.int64), np.zeros(2), np.zeros(k)
+    st = lib().or_mik_global(n, _p(v), _p(w), k, _p(thr), _p(tl), _p(sorted_), _p(sums), _p(classes), _p(bounds),
+                             _p(props))
+    if st != 0:
+        raise ValueError(f"Global::new failed ({st})")
+    return dict(values=sorted_, sums=sums, classes=classes, lower=bounds[0], upper=bounds[1], proportions=props)
+
+
+SHAPES = {"data": 0, "power": 1, "hyperbolic": 2}
+
+
+def mik_shape(values, thresholds, shape, a, b, c, j=None, u=None, w=1.0, cls=(0, 0), weights=None, tails=None):
+    """Shape::above(c, j) or, with u, Shape::quantile(u) of one class."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+    wt = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    tl = None if tails is None else np.ascontiguousarray(tails, dtype=np.float64)
+    jj, cc = (-1, float(u)) if u is not None else (int(j), float(c))
+    return lib().or_mik_shape(len(v), _p(v), _p(wt), len(thr), _p(thr), _p(tl), SHAPES[shape], float(w), int(cls[0]),
+                              int(cls[1]), float(a), float(b), cc, jj)
+
+
+def mik_conditional(values, mik, thresholds, raw, cutoffs=(), quantiles=(), weights=None):
+    """MultipleIndicator::conditional of kriged vectors raw (k, m)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+    r = np.ascontiguousarray(raw, dtype=np.float64).reshape(len(thr), -1)
+    wt = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    cut = np.ascontiguousarray(cutoffs, dtype=np.float64)
+    qs = np.ascontiguousarray(quantiles, dtype=np.float64)
+    m = r.shape[1]
+    out, o = _mik_out(m, len(thr), len(cut), len(qs))
+    rc = lib().or_mik_conditional(len(v), _p(v), _p(wt), C.byref(mik), _p(thr), m, _p(r), len(cut), _p(cut), len(qs),
+                                  _p(qs), C.byref(o))
+    if rc != 0:
+        raise ValueError(f"invalid parameters ({rc})")
     return out

@@ -1393,6 +1393,333 @@ static void write_estimate(int st, const Estimate& e, int64_t t, double* value, 
   if (status) status[t] = st;
 }
 
+
+// ---------------------------------------------------------------------------
+// Multiple indicator kriging, crates/estimation/src/indicator.rs.
+
+// f64::total_cmp as an integer key.
+static inline int64_t total_key(double v) {
+  int64_t b;
+  std::memcpy(&b, &v, sizeof(b));
+  return b ^ (int64_t)((uint64_t)(b >> 63) >> 1);
+}
+
+// indicator.rs:69-81 Global, :152-212 Global::new / weight.
+struct Global {
+  std::vector<double> values;
+  std::vector<double> sums[3];
+  std::vector<std::pair<size_t, size_t>> classes;
+  double lower = 0, upper = 0;
+  std::vector<double> proportions;
+  double weight(std::pair<size_t, size_t> r) const { return sums[0][r.second] - sums[0][r.first]; }
+};
+
+// Returns BK_OK, BK_INSUFFICIENT_DATA (no values) or BK_E_INVALID (weights).
+static int global_new(const std::vector<double>& values, const double* weights, const std::vector<double>& thr,
+                      const double* tails, Global& g) {
+  const size_t n = values.size();
+  if (n == 0) return BK_INSUFFICIENT_DATA;
+  std::vector<double> w(n, 1.0);
+  if (weights) w.assign(weights, weights + n);
+  for (double x : w)
+    if (!std::isfinite(x) || x < 0.0) return BK_E_INVALID;
+  double total = -0.0;
+  for (double x : w) total += x;
+  if (total <= 0.0) return BK_E_INVALID;
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t i, size_t j) {
+    int64_t a = total_key(values[i]), b = total_key(values[j]);
+    return a != b ? a < b : i < j;
+  });
+  g.values.resize(n);
+  for (size_t k = 0; k < n; ++k) g.values[k] = values[order[k]];
+  for (int j = 0; j < 3; ++j) g.sums[j].assign(1, 0.0);
+  for (size_t k = 0; k < n; ++k) {
+    const double wk = w[order[k]] / total, v = g.values[k];
+    for (int j = 0; j < 3; ++j) g.sums[j].push_back(g.sums[j].back() + wk * powi(v, j));
+  }
+  std::vector<size_t> cuts;
+  for (double t : thr)
+    cuts.push_back((size_t)(std::upper_bound(g.values.begin(), g.values.end(), t,
+                                             [](double tt, double v) { return !(v <= tt); }) -
+                            g.values.begin()));
+  std::vector<size_t> bounds{0};
+  bounds.insert(bounds.end(), cuts.begin(), cuts.end());
+  bounds.push_back(n);
+  const double first = thr.front(), last = thr.back();
+  const double lo = tails ? tails[0] : g.values.front(), hi = tails ? tails[1] : g.values.back();
+  g.lower = std::fmin(std::fmin(lo, g.values.front()), first);
+  g.upper = std::fmax(std::fmax(hi, g.values.back()), last);
+  g.proportions.clear();
+  for (size_t c : cuts) g.proportions.push_back(g.sums[0][c]);
+  g.classes.clear();
+  for (size_t k = 0; k + 1 < bounds.size(); ++k) g.classes.push_back({bounds[k], bounds[k + 1]});
+  return BK_OK;
+}
+
+// indicator.rs:137-150
+static double correct_order_relations(const std::vector<double>& raw, std::vector<double>& cdf) {
+  const size_t k = raw.size();
+  std::vector<double> up(k), down(k);
+  for (size_t i = 0; i < k; ++i) {
+    double p = raw[i];
+    p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);  // f64::clamp
+    up[i] = down[i] = p;
+  }
+  for (size_t i = 1; i < k; ++i) up[i] = std::fmax(up[i], up[i - 1]);
+  for (size_t i = k - 1; i-- > 0;) down[i] = std::fmin(down[i], down[i + 1]);
+  cdf.resize(k);
+  for (size_t i = 0; i < k; ++i) cdf[i] = 0.5 * (up[i] + down[i]);
+  double correction = -0.0;
+  for (size_t i = 0; i < k; ++i) correction += std::fabs(cdf[i] - raw[i]);
+  return correction;
+}
+
+// indicator.rs:206-213
+enum ShapeKind { SHAPE_DATA = 0, SHAPE_POWER = 1, SHAPE_HYPERBOLIC = 2 };
+struct Shape {
+  int kind;
+  size_t s = 0, e = 0;
+  double w = 1.0;
+};
+
+// indicator.rs:215-217
+static double binomial(int n, int k) {
+  double acc = 1.0;
+  for (int i = 0; i < k; ++i) acc = acc * (double)(n - i) / (double)(i + 1);
+  return acc;
+}
+
+static inline double clampd(double x, double a, double b) { return x < a ? a : (x > b ? b : x); }
+
+// Number of leading entries of v[lo .. hi) for which pred holds (slice::partition_point).
+template <class F>
+static size_t partition_point(const std::vector<double>& v, size_t lo, size_t hi, F pred) {
+  size_t a = lo, b = hi;
+  while (a < b) {
+    size_t mid = a + (b - a) / 2;
+    if (pred(v[mid]))
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  return a - lo;
+}
+
+// indicator.rs:220-253 Shape::above: E[Z^j 1{Z > c}] under the class distribution on [a, b].
+static double shape_above(const Shape& sh, const Global& g, double a, double b, double c, int j) {
+  if (sh.kind == SHAPE_DATA) {
+    size_t p = sh.s + partition_point(g.values, sh.s, sh.e, [&](double v) { return v <= c; });
+    return (g.sums[j][sh.e] - g.sums[j][p]) / g.weight({sh.s, sh.e});
+  }
+  if (b <= a) return a > c ? powi(a, j) : 0.0;
+  const double w = sh.w;
+  if (sh.kind == SHAPE_POWER) {
+    double x0 = clampd((c - a) / (b - a), 0.0, 1.0);
+    double sum = -0.0;
+    for (int i = 0; i <= j; ++i) {
+      double e = w + (double)i;
+      sum += binomial(j, i) * powi(a, j - i) * powi(b - a, i) * w / e * (1.0 - std::pow(x0, e));
+    }
+    return sum;
+  }
+  double yb = b / a, yc = clampd(c, a, b) / a;
+  double m = (double)j - w;
+  double integral = std::fabs(m) < 1e-12 ? std::log(yb / yc) : (std::pow(yb, m) - std::pow(yc, m)) / m;
+  return powi(a, j) * w / (1.0 - std::pow(yb, -w)) * integral;
+}
+
+// indicator.rs:272-289 Shape::quantile: the value at probability u within the class.
+static double shape_quantile(const Shape& sh, const Global& g, double a, double b, double u) {
+  if (sh.kind == SHAPE_DATA) {
+    double target = g.sums[0][sh.s] + u * g.weight({sh.s, sh.e});
+    size_t i = sh.s + partition_point(g.sums[0], sh.s + 1, sh.e + 1, [&](double c) { return c < target; });
+    return g.values[std::min(i, sh.e - 1)];
+  }
+  if (b <= a) return a;
+  if (sh.kind == SHAPE_POWER) return clampd(a + (b - a) * std::pow(u, 1.0 / sh.w), a, b);
+  double r = std::pow(a / b, sh.w);
+  return clampd(a * std::pow(1.0 - u * (1.0 - r), -1.0 / sh.w), a, b);
+}
+
+struct Mik {
+  std::vector<double> thr;
+  std::vector<Variogram> vgs;
+  bool simple = false;
+  int interpolation = BK_MIK_GLOBAL;
+  int upper_tail = BK_TAIL_NONE;
+  double omega = 1.0;
+  bool has_tails = false;
+  double tails[2] = {0, 0};
+};
+
+// indicator.rs:291-317 MultipleIndicator::validate.
+static bool mik_valid(const bk_mik* p, const double* thr) {
+  const int k = p->n_thresholds;
+  if (k < 1) return false;
+  for (int i = 0; i < k; ++i)
+    if (!std::isfinite(thr[i])) return false;
+  for (int i = 1; i < k; ++i)
+    if (thr[i - 1] >= thr[i]) return false;
+  if (p->n_variograms != 1 && p->n_variograms != k) return false;
+  if (p->has_tails && !(std::isfinite(p->tails[0]) && std::isfinite(p->tails[1]) && p->tails[0] <= p->tails[1]))
+    return false;
+  if (p->upper_tail != BK_TAIL_NONE && !(std::isfinite(p->omega) && p->omega > 0.0)) return false;
+  if (p->upper_tail == BK_TAIL_HYPERBOLIC && thr[k - 1] <= 0.0) return false;
+  if (p->interpolation != BK_MIK_GLOBAL && p->interpolation != BK_MIK_LINEAR) return false;
+  if (p->upper_tail < BK_TAIL_NONE || p->upper_tail > BK_TAIL_HYPERBOLIC) return false;
+  return true;
+}
+
+static Mik make_mik(const bk_mik* p, const double* thr, const bk_variogram* vgs) {
+  Mik m;
+  m.thr.assign(thr, thr + p->n_thresholds);
+  for (int v = 0; v < p->n_variograms; ++v) m.vgs.push_back(make_variogram(&vgs[v]));
+  m.simple = p->simple != 0;
+  m.interpolation = p->interpolation;
+  m.upper_tail = p->upper_tail;
+  m.omega = p->omega;
+  m.has_tails = p->has_tails != 0;
+  m.tails[0] = p->tails[0];
+  m.tails[1] = p->tails[1];
+  return m;
+}
+
+struct ClassRow {
+  double a, b, q;
+  Shape shape;
+};
+
+// indicator.rs:347-378 MultipleIndicator::classes.
+static std::vector<ClassRow> mik_classes(const Mik& mk, const Global& g, const std::vector<double>& cdf) {
+  const size_t k = mk.thr.size();
+  std::vector<ClassRow> out;
+  for (size_t c = 0; c <= k; ++c) {
+    ClassRow r;
+    r.a = c == 0 ? g.lower : mk.thr[c - 1];
+    r.b = c == k ? g.upper : mk.thr[c];
+    double lower = c == 0 ? 0.0 : cdf[c - 1];
+    double upper = c == k ? 1.0 : cdf[c];
+    r.q = upper - lower;
+    auto range = g.classes[c];
+    if (mk.upper_tail == BK_TAIL_POWER && c == k) {
+      r.shape.kind = SHAPE_POWER;
+      r.shape.w = mk.omega;
+    } else if (mk.upper_tail == BK_TAIL_HYPERBOLIC && c == k) {
+      r.shape.kind = SHAPE_HYPERBOLIC;
+      r.shape.w = mk.omega;
+    } else if (mk.interpolation == BK_MIK_GLOBAL && g.weight(range) > 0.0) {
+      r.shape.kind = SHAPE_DATA;
+      r.shape.s = range.first;
+      r.shape.e = range.second;
+    } else {
+      r.shape.kind = SHAPE_POWER;
+      r.shape.w = 1.0;
+    }
+    out.push_back(r);
+  }
+  return out;
+}
+
+// indicator.rs:83-99 Conditional.
+struct Conditional {
+  std::vector<double> cdf;
+  double correction = 0;
+  int violations = 0;
+  double mean = 0, variance = 0;
+  std::vector<double> probability_above, mean_above, quantile_values;
+};
+
+// indicator.rs:380-437 MultipleIndicator::conditional.
+static Conditional mik_conditional(const Mik& mk, const Global& g, const std::vector<double>& raw,
+                                   const std::vector<double>& cutoffs, const std::vector<double>& quantiles) {
+  Conditional out;
+  out.correction = correct_order_relations(raw, out.cdf);
+  for (size_t i = 0; i < raw.size(); ++i) out.violations += out.cdf[i] != raw[i];
+  auto classes = mik_classes(mk, g, out.cdf);
+  auto moment = [&](double c, int j) {
+    double sum = -0.0;
+    for (auto& r : classes)
+      if (r.q > 0.0) sum += r.q * shape_above(r.shape, g, r.a, r.b, c, j);
+    return sum;
+  };
+  const double ninf = -std::numeric_limits<double>::infinity();
+  out.mean = moment(ninf, 1);
+  out.variance = std::fmax(moment(ninf, 2) - out.mean * out.mean, 0.0);
+  for (double c : cutoffs) out.probability_above.push_back(moment(c, 0));
+  for (size_t i = 0; i < cutoffs.size(); ++i) {
+    double p = out.probability_above[i];
+    out.mean_above.push_back(p > 0.0 ? moment(cutoffs[i], 1) / p : NAN);
+  }
+  for (double u : quantiles) {
+    double below = 0.0;
+    size_t last = 0;
+    for (size_t i = classes.size(); i-- > 0;)
+      if (classes[i].q > 0.0) {
+        last = i;
+        break;
+      }
+    double v = g.upper;
+    for (size_t i = 0; i < classes.size(); ++i) {
+      const ClassRow& r = classes[i];
+      if (r.q > 0.0 && (below + r.q >= u || i == last)) {
+        double within = clampd((u - below) / r.q, 0.0, 1.0);
+        v = shape_quantile(r.shape, g, r.a, r.b, within);
+        break;
+      }
+      below += r.q;
+    }
+    out.quantile_values.push_back(v);
+  }
+  return out;
+}
+
+// indicator.rs:688-749 krige_indicators, with `indicator(k, s) = s.value <= t_k`
+// (:327-345 MultipleIndicator::kriged). pts: block discretisation or null.
+static int krige_indicators(const Mik& mk, const P3& target, const std::vector<Sample>& s,
+                            const std::vector<P3>* pts, const std::vector<double>& means, std::vector<double>& raw) {
+  const int kind = mk.simple ? SIMPLE : ORDINARY;
+  std::vector<std::vector<double>> weights;
+  for (const Variogram& vg : mk.vgs) {
+    Estimate e;
+    int st = pts ? block_krige_points(kind, 0.0, *pts, s, vg, e) : krige(kind, 0.0, target, s, vg, e);
+    if (st != BK_OK) return st;
+    weights.push_back(e.weights);
+  }
+  raw.assign(means.size(), 0.0);
+  for (size_t k = 0; k < means.size(); ++k) {
+    const std::vector<double>& w = weights[std::min(k, weights.size() - 1)];
+    const double f = means[k];
+    double sum = -0.0;
+    for (size_t j = 0; j < s.size(); ++j) {
+      double i = s[j].value <= mk.thr[k] ? 1.0 : 0.0;
+      sum += mk.simple ? w[j] * (i - f) : w[j] * i;
+    }
+    raw[k] = mk.simple ? f + sum : sum;
+  }
+  return BK_OK;
+}
+
+static void write_conditional(const Conditional* c, int st, int n, int64_t t, int64_t m, const bk_mik_out* o,
+                              const std::vector<double>* raw, size_t k, size_t nc, size_t nq) {
+  auto put = [&](double* a, size_t rows, const std::vector<double>* v) {
+    if (!a) return;
+    for (size_t r = 0; r < rows; ++r) a[r * m + t] = c ? (*v)[r] : NAN;
+  };
+  put(o->raw, k, raw);
+  put(o->cdf, k, c ? &c->cdf : nullptr);
+  put(o->probability_above, nc, c ? &c->probability_above : nullptr);
+  put(o->mean_above, nc, c ? &c->mean_above : nullptr);
+  put(o->quantile_values, nq, c ? &c->quantile_values : nullptr);
+  if (o->correction) o->correction[t] = c ? c->correction : NAN;
+  if (o->mean) o->mean[t] = c ? c->mean : NAN;
+  if (o->variance) o->variance[t] = c ? c->variance : NAN;
+  if (o->violations) o->violations[t] = c ? c->violations : 0;
+  if (o->n_used) o->n_used[t] = c ? n : 0;
+  if (o->status) o->status[t] = st;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1689,6 +2016,172 @@ int or_krige_lists(const or_samples* sm, const bk_variogram* v, int kind, double
     Estimate e;
     int st = krige(kind, param, q, sel, vg, e);
     write_estimate(st, e, t, value, variance, lagrange, nullptr, nullptr, status);
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multiple indicator kriging exports.
+extern "C" {
+
+// MultipleIndicator::predict (indicator.rs:439-487) for one search pass.
+int or_mik_predict(const or_samples* sm, const bk_mik* p, const double* thr, const bk_variogram* vgs,
+                   const double* weights, const bk_search* sp, const bk_block* blk, int64_t m, const double* tx,
+                   const double* ty, const double* tz, int nc, const double* cutoffs, int nq,
+                   const double* quantiles, const bk_mik_out* o, int nthreads) {
+  if (!mik_valid(p, thr)) return BK_E_INVALID;
+  for (int i = 0; i < nc; ++i)
+    if (!std::isfinite(cutoffs[i])) return BK_E_INVALID;
+  for (int i = 0; i < nq; ++i)
+    if (!(quantiles[i] >= 0.0 && quantiles[i] <= 1.0)) return BK_E_INVALID;
+  Search sr = make_search(sp);
+  if (!check(sr) || sr.calibration) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Mik mk = make_mik(p, thr, vgs);
+  std::vector<double> values(s.size());
+  for (size_t i = 0; i < s.size(); ++i) values[i] = s[i].value;
+  Global g;
+  int gs = global_new(values, weights, mk.thr, p->has_tails ? p->tails : nullptr, g);
+  if (gs == BK_E_INVALID) return BK_E_INVALID;
+  if (gs != BK_OK) return BK_E_STATE;
+  const Variogram& svg = mk.vgs[mk.vgs.size() / 2];
+  SearchTree tree(s, sr, &svg);
+  std::vector<double> cut(cutoffs, cutoffs + nc), qs(quantiles, quantiles + nq);
+  const size_t k = mk.thr.size();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    P3 q{tx[t], ty[t], tz[t]};
+    std::vector<int64_t> chosen;
+    int st = tree.neighbors_around(q, -1, chosen);
+    std::vector<double> raw;
+    std::vector<Sample> sel;
+    if (st == BK_OK) {
+      for (auto i : chosen) sel.push_back(s[i]);
+      if (blk) {
+        std::vector<P3> pts = block_points(q, *blk);
+        st = krige_indicators(mk, q, sel, &pts, g.proportions, raw);
+      } else {
+        st = krige_indicators(mk, q, sel, nullptr, g.proportions, raw);
+      }
+    }
+    if (st == BK_OK) {
+      Conditional c = mik_conditional(mk, g, raw, cut, qs);
+      write_conditional(&c, st, (int)sel.size(), t, m, o, &raw, k, nc, nq);
+    } else {
+      write_conditional(nullptr, st, 0, t, m, o, nullptr, k, nc, nq);
+    }
+  }
+  return 0;
+}
+
+// MultipleIndicator::cross_validate, leave-one-out (indicator.rs:489-529) over
+// the samples at which[0 .. m) (NULL: all).
+int or_mik_cross_validate(const or_samples* sm, const bk_mik* p, const double* thr, const bk_variogram* vgs,
+                          const double* weights, const bk_search* sp, int64_t m, const int64_t* which, double* pit,
+                          const bk_mik_out* o, int nthreads) {
+  if (!mik_valid(p, thr)) return BK_E_INVALID;
+  Search sr = make_search(sp);
+  if (!check(sr) || sr.calibration) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Mik mk = make_mik(p, thr, vgs);
+  std::vector<double> values(s.size());
+  for (size_t i = 0; i < s.size(); ++i) values[i] = s[i].value;
+  Global g;
+  int gs = global_new(values, weights, mk.thr, p->has_tails ? p->tails : nullptr, g);
+  if (gs == BK_E_INVALID) return BK_E_INVALID;
+  if (gs != BK_OK) return BK_E_STATE;
+  const Variogram& svg = mk.vgs[mk.vgs.size() / 2];
+  SearchTree tree(s, sr, &svg);
+  if (!which) m = (int64_t)s.size();
+  const size_t k = mk.thr.size();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    int64_t i = which ? which[t] : t;
+    const P3& q = s[i].loc;
+    std::vector<int64_t> chosen;
+    int st = tree.neighbors_around(q, i, chosen);
+    if (st == BK_OK && chosen.empty()) st = BK_SEARCH_FAILED;
+    std::vector<double> raw;
+    std::vector<Sample> sel;
+    if (st == BK_OK) {
+      for (auto j : chosen) sel.push_back(s[j]);
+      st = krige_indicators(mk, q, sel, nullptr, g.proportions, raw);
+    }
+    if (st == BK_OK) {
+      Conditional c = mik_conditional(mk, g, raw, {s[i].value}, {});
+      if (pit) pit[t] = 1.0 - c.probability_above[0];
+      write_conditional(&c, st, (int)sel.size(), t, m, o, &raw, k, 0, 0);
+    } else {
+      if (pit) pit[t] = NAN;
+      write_conditional(nullptr, st, 0, t, m, o, nullptr, k, 0, 0);
+    }
+  }
+  return 0;
+}
+
+// Global::new (indicator.rs:152-200) of n values: sorted values (n),
+// sums (3 x (n + 1)), class bounds (2 (K + 1)), lower / upper, proportions (K).
+int or_mik_global(int64_t n, const double* values, const double* weights, int k, const double* thr,
+                  const double* tails, double* sorted, double* sums, int64_t* classes, double* bounds,
+                  double* proportions) {
+  Global g;
+  int st = global_new(std::vector<double>(values, values + n), weights, std::vector<double>(thr, thr + k), tails, g);
+  if (st != BK_OK) return st;
+  std::copy(g.values.begin(), g.values.end(), sorted);
+  for (int j = 0; j < 3; ++j) std::copy(g.sums[j].begin(), g.sums[j].end(), sums + j * (n + 1));
+  for (size_t c = 0; c < g.classes.size(); ++c) {
+    classes[2 * c] = (int64_t)g.classes[c].first;
+    classes[2 * c + 1] = (int64_t)g.classes[c].second;
+  }
+  bounds[0] = g.lower;
+  bounds[1] = g.upper;
+  std::copy(g.proportions.begin(), g.proportions.end(), proportions);
+  return 0;
+}
+
+// Shape::above (j = 0, 1, 2) or, with j < 0, Shape::quantile at u = c, of the
+// class [s, e) on [a, b] of the global distribution of the n values (weights
+// may be NULL). shape: 0 data, 1 power, 2 hyperbolic (exponent w).
+double or_mik_shape(int64_t n, const double* values, const double* weights, int k, const double* thr,
+                    const double* tails, int shape, double w, int64_t s, int64_t e, double a, double b, double c,
+                    int j) {
+  Global g;
+  if (global_new(std::vector<double>(values, values + n), weights, std::vector<double>(thr, thr + k), tails, g) !=
+      BK_OK)
+    return NAN;
+  Shape sh;
+  sh.kind = shape;
+  sh.s = (size_t)s;
+  sh.e = (size_t)e;
+  sh.w = w;
+  return j < 0 ? shape_quantile(sh, g, a, b, c) : shape_above(sh, g, a, b, c, j);
+}
+
+// MultipleIndicator::conditional (indicator.rs:380-437) of m given kriged
+// vectors raw[k * m + t] against the global distribution of the n values.
+int or_mik_conditional(int64_t n, const double* values, const double* weights, const bk_mik* p, const double* thr,
+                       int64_t m, const double* raw, int nc, const double* cutoffs, int nq, const double* quantiles,
+                       const bk_mik_out* o) {
+  if (!mik_valid(p, thr)) return BK_E_INVALID;
+  Mik mk;
+  mk.thr.assign(thr, thr + p->n_thresholds);
+  mk.simple = p->simple != 0;
+  mk.interpolation = p->interpolation;
+  mk.upper_tail = p->upper_tail;
+  mk.omega = p->omega;
+  Global g;
+  if (global_new(std::vector<double>(values, values + n), weights, mk.thr, p->has_tails ? p->tails : nullptr, g) !=
+      BK_OK)
+    return BK_E_INVALID;
+  const size_t k = mk.thr.size();
+  std::vector<double> cut(cutoffs, cutoffs + nc), qs(quantiles, quantiles + nq);
+  for (int64_t t = 0; t < m; ++t) {
+    std::vector<double> r(k);
+    for (size_t i = 0; i < k; ++i) r[i] = raw[i * m + t];
+    Conditional c = mik_conditional(mk, g, r, cut, qs);
+    write_conditional(&c, BK_OK, 0, t, m, o, &r, k, nc, nq);
   }
   return 0;
 }

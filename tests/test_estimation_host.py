@@ -189,3 +189,33 @@ def test_simple_kriging_takes_no_slope_calibration():
     with pytest.raises(InvalidInput):
         SimpleKriging(vg, Search(30.0, target_slope=0.9), mean=1.0)
     SimpleKriging(vg, Search(30.0, target_efficiency=0.5), mean=1.0)
+
+
+def test_multiple_indicator_kriging_checks_its_parameters():
+    # crates/py/src/indicator.rs:131-192 and indicator.rs:291-317.
+    from boitata_amd.estimation import MultipleIndicatorKriging
+
+    vg = Variogram([("spherical", 1.0, 30.0)])
+    s = Search(20.0)
+    with pytest.raises(InvalidInput, match="strictly increasing"):
+        MultipleIndicatorKriging(vg, s, [1.0, 1.0])
+    with pytest.raises(InvalidInput, match="one per threshold"):
+        MultipleIndicatorKriging([vg, vg], s, [0.0, 1.0, 2.0])
+    with pytest.raises(InvalidInput, match="finite threshold"):
+        MultipleIndicatorKriging(vg, s, [])
+    with pytest.raises(InvalidInput, match="interpolation"):
+        MultipleIndicatorKriging(vg, s, [1.0], interpolation="cubic")
+    with pytest.raises(InvalidInput, match="upper tail"):
+        MultipleIndicatorKriging(vg, s, [1.0], upper_tail=("lognormal", 1.0))
+    with pytest.raises(InvalidInput, match="exponent"):
+        MultipleIndicatorKriging(vg, s, [1.0], upper_tail=("power", 0.0))
+    with pytest.raises(InvalidInput, match="last threshold > 0"):
+        MultipleIndicatorKriging(vg, s, [-1.0], upper_tail=("hyperbolic", 2.0))
+    with pytest.raises(InvalidInput, match="tails"):
+        MultipleIndicatorKriging(vg, s, [1.0], tails=(2.0, 1.0))
+    with pytest.raises(InvalidInput, match="target_slope"):
+        MultipleIndicatorKriging(vg, Search(20.0, target_efficiency=0.5), [1.0])
+    m = MultipleIndicatorKriging([vg, vg], [s, Search(40.0)], [0.5, 1.5], upper_tail=("power", 2.0))
+    assert m.thresholds == [0.5, 1.5]
+    with pytest.raises(InvalidInput, match="fit"):
+        m.predict([[0.0, 0.0]])
