@@ -170,6 +170,14 @@ def make_block(size, discretization=(4, 4, 1), rotation=(0.0, 0.0, 0.0)) -> Bloc
     return b
 
 
+class DiagOut(C.Structure):
+    """bk_diag_out: output pointers of bk_krige_diagnostics; NULL ones are skipped."""
+
+    _fields_ = [(name, C.c_void_p) for name in ("value", "variance", "lagrange", "support_variance", "n_used",
+                                                 "status", "negative_weight_sum", "n_holes", "mean_distance",
+                                                 "target_met")]
+
+
 MIK_GLOBAL, MIK_LINEAR = 0, 1
 TAIL_NONE, TAIL_POWER, TAIL_HYPERBOLIC = 0, 1, 2
 
@@ -232,6 +240,8 @@ SIGNATURES = [
     ("bk_loo", C.c_int, [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, I32, I64, P, P, P, P]),
     ("bk_loo_prepare", C.c_int, [P, C.POINTER(Search), C.POINTER(Variogram)]),
     ("bk_loo_loss_grad", C.c_int, [P, C.POINTER(Variogram), I32, P, P, P]),
+    ("bk_krige_diagnostics", C.c_int,
+     [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, P, I32, I64, P, P, P, C.POINTER(DiagOut)]),
     ("bk_mik_predict", C.c_int,
      [P, C.POINTER(Mik), P, P, P, C.POINTER(Search), P, I32, I64, P, P, P, I32, P, I32, P, C.POINTER(MikOut)]),
     ("bk_mik_cross_validate", C.c_int,

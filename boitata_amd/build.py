@@ -25,6 +25,7 @@ UNITS = [
     ("bk_solve.hip", "bk_solve_pass.o", ["-DBK_PART=1", "-mllvm", "-disable-machine-licm"]),
     ("bk_solve.hip", "bk_solve_grad.o", ["-DBK_PART=2"]),
     ("bk_mik.hip", "bk_mik.o", []),
+    ("bk_diag.hip", "bk_diag.o", []),
     ("bk_api.hip", "bk_api.o", []),
 ]
 SOURCES = sorted({u[0] for u in UNITS})

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/boitata_gpu.h"
+#include "bk_diag.h"
 #include "bk_mik.h"
 #include "bk_search.h"
 #include "bk_solve.h"
@@ -545,7 +546,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t 
               int64_t m,
               const double* tx, const double* ty, const double* tz, int stride, const bk_block* blk, double* value,
               double* variance, double* lagrange, double* support, int32_t* n_used, int32_t* status,
-              double* weights = nullptr) {
+              double* weights = nullptr, int32_t* met = nullptr) {
   SolveArgs a;
   std::memset(&a, 0, sizeof(a));
   a.weights = weights;
@@ -689,7 +690,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t 
   // the search).
   if (srch && srch->calibration && m > 0) {
     CalibArgs k{(const int32_t*)c->sstatus.p, (int32_t*)c->ncount.p, m, srch->calibration, srch->calibration_target,
-                srch->min_samples, kind != BK_SIMPLE};
+                srch->min_samples, kind != BK_SIMPLE, met};
     if (bk::launch_calibrate(c->stream, a, k, stride <= 16 ? 16 : (stride <= 32 ? 32 : 64)))
       return fail(c, BK_E_HIP, "calibration launch failed");
   }
@@ -1671,9 +1672,131 @@ int mik_common(bk_ctx* c, const bk_mik* p, const double* thresholds, const bk_va
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Diagnostics of a kriging pass (py/estimation.rs:895-904,1034-1105): the
+// ordinary pass with the solves writing their weights and the calibration its
+// outcome, then the statistics of the samples used.
+int diag_common(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                const bk_block* blk, int32_t precision, int64_t m, const double* tx, const double* ty,
+                const double* tz, const bk_diag_out* out) {
+  if (!c) return BK_E_INVALID;
+  if (!vg || !s || !out) return fail(c, BK_E_INVALID, "null argument");
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  if (m > 0 && (!tx || !ty || !tz)) return fail(c, BK_E_INVALID, "targets missing");
+  if (kind < BK_ORDINARY || kind > BK_INDICATOR) return fail(c, BK_E_INVALID, "unknown kind");
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
+  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
+  bool stationary = true;
+  for (int k = 0; k < vg->n_structures; ++k) stationary = stationary && vg->structures[k].model != BK_POWER;
+  if (kind == BK_SIMPLE && !stationary)
+    return fail(c, BK_E_INVALID,
+                "simple kriging needs a variogram with a sill: every structure must be stationary (not power); use "
+                "ordinary kriging");
+  if (s->calibration == BK_CALIBRATE_SLOPE && kind == BK_SIMPLE)
+    return fail(c, BK_E_INVALID, "simple kriging has a slope of regression of 1; calibrate it with target_efficiency");
+  if (s->calibration && s->max_samples > BK_MAX_CHOL)
+    return fail(c, BK_E_INVALID, "calibrated searches select at most 64 samples on the device");
+  if (blk) {
+    if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
+      return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
+    for (int d = 0; d < 3; ++d)
+      if (!(std::isfinite(blk->size[d]) && blk->size[d] >= 0.0))
+        return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
+  }
+  HIPOK(c, hipSetDevice(c->device));
+  SearchArgs a;
+  int64_t stride;
+  if (int r = prepare_search(c, s, vg, a, stride)) return r;
+  const double *dx, *dy, *dz;
+  if (int r = stage_in(c, tx, m, c->tx, &dx)) return r;
+  if (int r = stage_in(c, ty, m, c->ty, &dy)) return r;
+  if (int r = stage_in(c, tz, m, c->tz, &dz)) return r;
+  const int64_t M = std::max<int64_t>(m, 1);
+  double *ov, *ovar, *olag, *osup, *onws, *omd;
+  int32_t *onu, *ost, *oh, *omet;
+  if (int r = out_ptr(c, out->value, m, c->value, &ov)) return r;
+  if (int r = out_ptr(c, out->variance, m, c->variance, &ovar)) return r;
+  if (int r = out_ptr(c, out->lagrange, m, c->lagrange, &olag)) return r;
+  if (int r = out_ptr(c, out->support_variance, m, c->support, &osup)) return r;
+  if (int r = out_ptr(c, out->n_used, m, c->n_used, &onu)) return r;
+  // The value, variance and status always run through device memory; the rest
+  // of the host outputs through one scratch block.
+  if (!ov) {
+    if (int r = ensure(c, c->value, M * sizeof(double))) return r;
+    ov = (double*)c->value.p;
+  }
+  if (!ovar) {
+    if (int r = ensure(c, c->variance, M * sizeof(double))) return r;
+    ovar = (double*)c->variance.p;
+  }
+  if (int r = ensure(c, c->mik_out, 2 * M * sizeof(double) + 3 * M * sizeof(int32_t))) return r;
+  double* dscr = (double*)c->mik_out.p;
+  int32_t* iscr = (int32_t*)(dscr + 2 * M);
+  auto place_d = [&](double* user, double* own) { return user && is_device(user) ? user : (user ? own : nullptr); };
+  auto place_i = [&](int32_t* user, int32_t* own, bool need) {
+    return user && is_device(user) ? user : (user || need ? own : nullptr);
+  };
+  onws = place_d(out->negative_weight_sum, dscr);
+  omd = place_d(out->mean_distance, dscr + M);
+  ost = place_i(out->status, iscr, true);
+  oh = place_i(out->n_holes, iscr + M, false);
+  omet = place_i(out->target_met, iscr + 2 * M, false);
+  if (int r = ensure(c, c->mik_w, M * stride * sizeof(double))) return r;
+  if (int r = run_search(c, a, m, dx, dy, dz, nullptr)) return r;
+  if (s->calibration && (s->octant || s->sectors > 0) && m > 0)
+    if (bk::launch_balance(c->stream, a, m)) return fail(c, BK_E_HIP, "balance launch failed");
+  // -1 where the search is not calibrated or the target is not solved: the
+  // calibration writes 0 / 1 for the targets it sees, the statistics kernel
+  // -1 for those that failed.
+  if (omet && m > 0)
+    if (bk::launch_fill_i32(c->stream, omet, m, -1)) return fail(c, BK_E_HIP, "fill launch failed");
+  if (int r = run_solve(c, vg, s, kind, param, precision, m, dx, dy, dz, (int)stride, blk, ov, ovar, olag, osup, onu,
+                        ost, (double*)c->mik_w.p, s->calibration ? omet : nullptr))
+    return r;
+  if (int r = finalize(c, m, ost, ov, ovar, olag, osup, onu)) return r;
+  DiagArgs da;
+  std::memset(&da, 0, sizeof(da));
+  da.m = m;
+  da.stride = (int)stride;
+  da.nidx = (const int32_t*)c->nidx.p;
+  da.ncount = (const int32_t*)c->ncount.p;
+  da.status = ost;
+  da.tx = dx;
+  da.ty = dy;
+  da.tz = dz;
+  da.sx = (const double*)c->x.p;
+  da.sy = (const double*)c->y.p;
+  da.sz = (const double*)c->z.p;
+  da.hole = c->has_hole ? (const int32_t*)c->hole.p : nullptr;
+  da.w = (const double*)c->mik_w.p;
+  da.negative_weight_sum = onws;
+  da.n_holes = oh;
+  da.mean_distance = omd;
+  da.target_met = omet;
+  if (bk::launch_used_stats(c->stream, da)) return fail(c, BK_E_HIP, "diagnostics launch failed");
+  if (int r = stage_out(c, out->value, ov, m)) return r;
+  if (int r = stage_out(c, out->variance, ovar, m)) return r;
+  if (int r = stage_out(c, out->lagrange, olag, m)) return r;
+  if (int r = stage_out(c, out->support_variance, osup, m)) return r;
+  if (int r = stage_out(c, out->n_used, onu, m)) return r;
+  if (int r = stage_out(c, out->status, ost, m)) return r;
+  if (int r = stage_out(c, out->negative_weight_sum, onws, m)) return r;
+  if (int r = stage_out(c, out->mean_distance, omd, m)) return r;
+  if (int r = stage_out(c, out->n_holes, oh, m)) return r;
+  if (int r = stage_out(c, out->target_met, omet, m)) return r;
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int bk_krige_diagnostics(bk_ctx* c, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
+                         const bk_block* block, int32_t precision, int64_t m, const double* tx, const double* ty,
+                         const double* tz, const bk_diag_out* out) {
+  return diag_common(c, vg, search, kind, param, block, precision, m, tx, ty, tz, out);
+}
 
 int bk_mik_predict(bk_ctx* c, const bk_mik* mik, const double* thresholds, const bk_variogram* variograms,
                    const double* weights, const bk_search* search, const bk_block* block, int32_t precision,

@@ -55,8 +55,8 @@ BK_HD inline double mik_binomial(int n, int k) {
 
 BK_HD inline double mik_clamp(double x, double a, double b) { return x < a ? a : (x > b ? b : x); }
 
-// Entries of v[lo, hi) before the first for which `ge` holds (slices sorted so
-// that `ge` is monotone): slice::partition_point of its negation.
+// Entries of v[lo, hi) before the first for which `below` fails (v sorted so
+// that `below` holds on a prefix): slice::partition_point.
 template <class F>
 BK_HD inline int64_t mik_partition(const double* v, int64_t lo, int64_t hi, F below) {
   int64_t a = lo, b = hi;

@@ -1,8 +1,9 @@
 // bk_mik.hip — multiple indicator kriging kernels (crates/estimation/src/indicator.rs).
 //
-// One thread per target. The solves that give the weights are the ordinary
-// pass kernels (bk_solve.hip) writing their weights out; these kernels turn
-// weights into indicator estimates and those into conditional distributions.
+// The solves that give the weights are the ordinary pass kernels
+// (bk_solve.hip) writing their weights out; these kernels turn weights into
+// indicator estimates (a group of lanes per target) and those into
+// conditional distributions (a thread per target).
 // Everything that does not depend on the target (the class bounds and shapes,
 // and each class's moments above each shared cutoff) is tabulated on the host,
 // so a target's moments are sums of its class probabilities times table rows.
@@ -17,27 +18,49 @@ int64_t blocks_for(int64_t m) { return (m + kThreads - 1) / kThreads; }
 __device__ inline double nan64() { return __longlong_as_double(0x7ff8000000000000LL); }
 
 // krige_indicators (indicator.rs:735-748): with simple kriging about f_k,
-// f_k + sum_j w_j (1{z_j <= t_k} - f_k); else sum_j w_j 1{z_j <= t_k}; the
-// sums run over the samples in neighbour order.
+// f_k + sum_j w_j (1{z_j <= t_k} - f_k); else sum_j w_j 1{z_j <= t_k}.
+// G lanes per target (G >= 16, a power of two): lane r holds the weights and
+// values of neighbours r, r + G, ... (a target's weights are contiguous, so
+// the group's loads coalesce), and each threshold's sum is a shuffle
+// reduction over the group.
+constexpr int kMikPer = 4;  // neighbours per lane: G * kMikPer >= the list stride (<= 256)
+
+template <int G>
 __global__ void __launch_bounds__(kThreads) k_mik_raw(const MikRawArgs a) {
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= a.m) return;
-  int32_t st = a.v == 0 ? BK_OK : a.status[t];
-  if (st == BK_OK) st = a.st_v[t];
-  a.status[t] = st;
-  if (st != BK_OK) return;
-  const int n = a.ncount[t];
-  const int32_t* id = a.nidx + t * a.stride;
-  const double* w = a.w + t * a.stride;
+  const int64_t t = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / G;
+  const int r = threadIdx.x % G;
+  const bool in = t < a.m;
+  int32_t st = BK_SEARCH_FAILED;
+  if (in) {
+    st = a.v == 0 ? BK_OK : a.status[t];
+    if (st == BK_OK) st = a.st_v[t];
+  }
+  const bool ok = in && st == BK_OK;
+  const int n = ok ? a.ncount[t] : 0;
+  double w[kMikPer], z[kMikPer];
+#pragma unroll
+  for (int e = 0; e < kMikPer; ++e) {
+    const int j = r + e * G;
+    w[e] = 0.0;
+    z[e] = 0.0;
+    if (j < n) {
+      w[e] = a.w[t * a.stride + j];
+      z[e] = a.sv[a.nidx[t * a.stride + j]];
+    }
+  }
+  if (in && r == 0) a.status[t] = st;
   const int k0 = a.V == 1 ? 0 : a.v, k1 = a.V == 1 ? a.K : a.v + 1;
   for (int k = k0; k < k1; ++k) {
     const double th = a.thr[k], f = a.prop[k];
-    double sum = -0.0;
-    for (int j = 0; j < n; ++j) {
-      const double ind = a.sv[id[j]] <= th ? 1.0 : 0.0;
-      sum += a.simple ? w[j] * (ind - f) : w[j] * ind;
+    double part = 0.0;
+#pragma unroll
+    for (int e = 0; e < kMikPer; ++e) {
+      const double ind = z[e] <= th ? 1.0 : 0.0;
+      part += a.simple ? w[e] * (ind - f) : w[e] * ind;
     }
-    a.raw[(int64_t)k * a.m + t] = a.simple ? f + sum : sum;
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, G);
+    if (ok && r == 0) a.raw[(int64_t)k * a.m + t] = a.simple ? f + part : part;
   }
 }
 
@@ -161,7 +184,13 @@ namespace bk {
 
 int launch_mik_raw(hipStream_t s, const MikRawArgs& a) {
   if (a.m <= 0) return 0;
-  k_mik_raw<<<blocks_for(a.m), kThreads, 0, s>>>(a);
+  if (a.stride > 64 * kMikPer) return -1;
+  if (a.stride <= 16)
+    k_mik_raw<16><<<blocks_for(a.m * 16), kThreads, 0, s>>>(a);
+  else if (a.stride <= 32)
+    k_mik_raw<32><<<blocks_for(a.m * 32), kThreads, 0, s>>>(a);
+  else
+    k_mik_raw<64><<<blocks_for(a.m * 64), kThreads, 0, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -103,6 +103,7 @@ struct CalibArgs {
   double target;
   int64_t min_samples;
   int ordinary;            // the unbiasedness column of ones (not simple kriging)
+  int32_t* met;            // may be null: 1 where a leading part reaches the target, else 0
 };
 
 namespace bk {

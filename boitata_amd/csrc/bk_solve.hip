@@ -1906,6 +1906,7 @@ __global__ void __launch_bounds__(64) k_calibrate(const SolveArgs a, const Calib
     const int first = __ffsll((unsigned long long)mine) - 1 - g * NMAX;
     if (first + 1 < n) k.count[t] = first + 1;
   }
+  if (active && r == 0 && k.met) k.met[t] = mine ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(64) k_guarded(const SolveArgs a) {

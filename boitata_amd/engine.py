@@ -284,3 +284,19 @@ class Engine:
                                                     _ptr(pit), C.byref(o)))
         out["pit"] = pit
         return out
+
+    def krige_diagnostics(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, block=None,
+                          precision=_abi.FP64):
+        """A kriging pass (point, or block when `block` is given) with the
+        per-target diagnostics of bk_diag_out."""
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        f, i = self._outputs(m, tx)
+        out = {k: f() for k in ("value", "variance", "lagrange", "support_variance", "negative_weight_sum",
+                                "mean_distance")}
+        out.update({k: i() for k in ("n_used", "status", "n_holes", "target_met")})
+        o = _abi.DiagOut(*[_ptr(out[name]).value for name, _ in _abi.DiagOut._fields_])
+        self._check(self._lib.bk_krige_diagnostics(self._h, C.byref(variogram), C.byref(search), int(kind),
+                                                   float(param), C.byref(block) if block is not None else None,
+                                                   int(precision), m, _ptr(tx), _ptr(ty), _ptr(tz), C.byref(o)))
+        return out

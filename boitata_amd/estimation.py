@@ -23,8 +23,8 @@ Behaviour kept from the reference:
   (crates/py/src/containers.rs:15-21).
 
 Not covered (raise NotImplementedError): domains, local anisotropy,
-diagnostics tables, k-fold and block cross-validation, units, soft / high-grade data and
-calibrated searches, and the other estimation methods.
+k-fold and block cross-validation, units, soft / high-grade data, and the
+other estimation methods.
 
 Extension: ``precision="fp32_ir"`` solves in FP32 with FP64 refinement; the
 default ``"fp64"`` is the reference's arithmetic.
@@ -352,6 +352,24 @@ def run_passes(n, searches, one_pass):
     return value, variance, which
 
 
+class DiagnosticsTable:
+    """Per-target search diagnostics (crates/py/src/indicator.rs:25-57): named
+    float columns, NaN where a target is unestimated."""
+
+    def __init__(self, columns):
+        self._columns = {k: np.asarray(v, dtype=np.float64) for k, v in columns}
+
+    @property
+    def column_names(self):
+        return list(self._columns)
+
+    def __getitem__(self, name):
+        return self._columns[name]
+
+    def __len__(self):
+        return len(next(iter(self._columns.values()))) if self._columns else 0
+
+
 def _searches(search):
     if isinstance(search, Search):
         return [search]
@@ -431,18 +449,72 @@ class _Base:
 
     def predict(self, targets, *, return_variance=False, anisotropy=None, diagnostics=False, domains=None,
                 domain_column=None):
-        """Estimates at targets; NaN where the search found too few samples."""
+        """Estimates at targets; NaN where the search found too few samples.
+        With `diagnostics` the result is the per-target table of
+        crates/py/src/estimation.rs:1046-1100 instead."""
         if anisotropy is not None:
             raise NotImplementedError("local anisotropy is not supported by this engine")
-        if diagnostics:
-            raise NotImplementedError("diagnostics tables are not supported by this engine")
         if domains is not None or domain_column is not None:
             raise NotImplementedError("domains are not supported by this engine")
         if self._values is None:
             raise InvalidInput("call fit first")
         T = coords_of(targets)
+        if diagnostics:
+            return self._diagnostics(T)
         value, variance, _ = run_passes(len(T), self._passes, lambda s, rows: self._one_pass(s, T[rows]))
         return (value, variance) if return_variance else value
+
+    _block = None
+
+    def _diagnostics(self, T):
+        """py/estimation.rs:1046-1100: the estimate's fields, derived ratios and
+        the statistics of the samples used, per target; NaN where unestimated.
+        Domains are not supported, so n_other_domain is 0 where estimated."""
+        m = len(T)
+        cols = {k: np.full(m, np.nan) for k in ("value", "variance", "lagrange", "support_variance", "n_samples",
+                                                 "n_holes", "mean_distance", "negative_weight_sum", "target_met",
+                                                 "max_samples_reached")}
+        which = np.full(m, -1, dtype=np.int64)
+        remaining = np.arange(m)
+        for p, s in enumerate(self._passes):
+            if len(remaining) == 0:
+                break
+            R = T[remaining]
+            out = self._engine.krige_diagnostics(self._variogram._c, s._c, R[:, 0].copy(), R[:, 1].copy(),
+                                                 R[:, 2].copy(), self._kind, self._param, self._block,
+                                                 self._precision)
+            ok = out["status"] == _abi.OK
+            rows = remaining[ok]
+            for k in ("value", "variance", "lagrange", "support_variance", "negative_weight_sum", "mean_distance"):
+                cols[k][rows] = out[k][ok]
+            cols["n_samples"][rows] = out["n_used"][ok]
+            cols["n_holes"][rows] = out["n_holes"][ok]
+            met = out["target_met"][ok].astype(np.float64)
+            cols["target_met"][rows] = np.where(met < 0, np.nan, met)
+            cols["max_samples_reached"][rows] = (out["n_used"][ok] >= s.max_samples).astype(np.float64)
+            which[rows] = p
+            remaining = remaining[~ok]
+        sv, var, lag = cols["support_variance"], cols["variance"], cols["lagrange"]
+        cov = sv - var - lag  # krige.rs:58-61, Cov(Z, Z*)
+        est_var = cov - lag   # krige.rs:63-66, Var(Z*)
+        ok = which >= 0
+        return DiagnosticsTable([
+            ("value", cols["value"]),
+            ("variance", var),
+            ("efficiency", (sv - var) / sv),
+            ("slope", cov / est_var),
+            ("n_samples", cols["n_samples"]),
+            ("pass", np.where(ok, which + 1.0, np.nan)),
+            ("n_holes", cols["n_holes"]),
+            ("n_other_domain", np.where(ok, 0.0, np.nan)),
+            ("mean_distance", cols["mean_distance"]),
+            ("negative_weight_sum", cols["negative_weight_sum"]),
+            ("lagrange", lag),
+            ("support_variance", sv),
+            ("estimate_variance", est_var),
+            ("max_samples_reached", cols["max_samples_reached"]),
+            ("target_met", cols["target_met"]),
+        ])
 
     def cross_validate(self, *, folds=None) -> CrossValidation:
         """Re-estimates every sample from the others, through the same passes."""
@@ -527,24 +599,6 @@ _INTERPOLATIONS = {"global": _abi.MIK_GLOBAL, "linear": _abi.MIK_LINEAR}
 _TAILS = {"power": _abi.TAIL_POWER, "hyperbolic": _abi.TAIL_HYPERBOLIC}
 DIAGNOSTIC_COLUMNS = ("n_samples", "pass", "n_holes", "mean_distance", "max_samples_reached", "correction",
                       "n_order_violations")
-
-
-class DiagnosticsTable:
-    """Per-target search diagnostics (crates/py/src/indicator.rs:25-57): named
-    float columns, NaN where a target is unestimated."""
-
-    def __init__(self, columns):
-        self._columns = {k: np.asarray(v, dtype=np.float64) for k, v in columns}
-
-    @property
-    def column_names(self):
-        return list(self._columns)
-
-    def __getitem__(self, name):
-        return self._columns[name]
-
-    def __len__(self):
-        return len(next(iter(self._columns.values()))) if self._columns else 0
 
 
 @dataclass(frozen=True)

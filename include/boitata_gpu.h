@@ -209,6 +209,32 @@ int bk_loo_prepare(bk_ctx* ctx, const bk_search* search, const bk_variogram* vg)
 int bk_loo_loss_grad(bk_ctx* ctx, const bk_variogram* vg, int32_t precision, double* loss,
                      double* grad, int64_t* n_used);
 
+/* Per-target diagnostics of a kriging pass: the estimate's fields and the
+ * statistics of the samples used, as _Estimator.predict(diagnostics=True)
+ * tabulates them (crates/py/src/estimation.rs:1034-1105, with
+ * neighborhood_stats over the samples used, crates/estimation/src/neighborhood.rs:66-136).
+ * Any pointer may be NULL. */
+typedef struct {
+  double* value;
+  double* variance;
+  double* lagrange;
+  double* support_variance;      /* C(v, v): the sill for points, C(B, B) for blocks */
+  int32_t* n_used;
+  int32_t* status;
+  double* negative_weight_sum;   /* sum of the negative kriging weights (krige.rs:64-75) */
+  int32_t* n_holes;              /* distinct holes among the samples used; untagged samples count one each */
+  double* mean_distance;         /* mean Euclidean distance from the target to the samples used */
+  int32_t* target_met;           /* calibrated searches: 1 when the samples kept reach the target, else 0; -1 when
+                                    the search is not calibrated or the target failed */
+} bk_diag_out;
+
+/* estimate_many with the krige or block_krige_points estimator (block NULL:
+ * point support) and the diagnostics of each target (py/estimation.rs:895-904,
+ * 1034-1105). Values are NaN where status != BK_OK. */
+int bk_krige_diagnostics(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
+                         const bk_block* block, int32_t precision, int64_t m, const double* tx, const double* ty,
+                         const double* tz, const bk_diag_out* out);
+
 /* Multiple indicator kriging, crates/estimation/src/indicator.rs:53-67
  * (MultipleIndicator), :35-50 (Interpolation, UpperTail) and
  * crates/py/src/indicator.rs:116-192 (the Python constructor). */

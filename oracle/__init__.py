@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
             "or_prefix_quality": (I32, [SP, VP, I32, P, BP, I32, P, P, P]),
             "or_calibrated_count": (I32, [I32, D, I64, I32, P, P, P]),
             "or_balanced": (I32, [SP, SR, VP, P, I32, P, P]),
+            "or_neighborhood_stats": (None, [SP, P, I32, P, I64, D, P]),
+            "or_krige_diag": (I32, [SP, VP, SR, I32, D, P, I64, P, P, P, C.POINTER(_abi.DiagOut), I32]),
             "or_mik_predict": (I32, [SP, C.POINTER(_abi.Mik), P, P, P, SR, P, I64, P, P, P, I32, P, I32, P,
                                      C.POINTER(_abi.MikOut), I32]),
             "or_mik_cross_validate": (I32, [SP, C.POINTER(_abi.Mik), P, P, P, SR, I64, P, P,
@@ -400,4 +402,37 @@ def mik_conditional(values, mik, thresholds, raw, cutoffs=(), quantiles=(), weig
                                   _p(qs), C.byref(o))
     if rc != 0:
         raise ValueError(f"invalid parameters ({rc})")
+    return out
+
+
+STATS = ("n_within", "n_holes", "nearest_dist", "mean_dist_knn", "max_dist_knn", "value_mean", "value_var",
+         "value_min", "value_max", "value_idw")
+
+
+def neighborhood_stats(data: Data, target, idx, k, radius=float("inf")):
+    """neighborhood_stats of the samples idx at target (Euclidean distance)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    out = np.zeros(10)
+    lib().or_neighborhood_stats(C.byref(data.s), _p(t), len(idx), _p(idx), int(k), float(radius), _p(out))
+    return dict(zip(STATS, out))
+
+
+DIAG_F = ("value", "variance", "lagrange", "support_variance", "negative_weight_sum", "mean_distance")
+DIAG_I = ("n_used", "status", "n_holes", "target_met")
+
+
+def krige_diag(data: Data, vg, search, targets, kind=_abi.ORDINARY, param=0.0, block=None, threads=0):
+    """One pass of point (block None) or block kriging with the per-target diagnostics."""
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    m = len(t)
+    out = {k: np.zeros(m) for k in DIAG_F}
+    out.update({k: np.zeros(m, np.int32) for k in DIAG_I})
+    o = _abi.DiagOut(*[out[name].ctypes.data for name, _ in _abi.DiagOut._fields_])
+    rc = lib().or_krige_diag(C.byref(data.s), C.byref(vg), C.byref(search), kind, param,
+                             C.byref(block) if block is not None else None, m, _p(tx), _p(ty), _p(tz), C.byref(o),
+                             threads)
+    if rc != 0:
+        raise ValueError("invalid search")
     return out

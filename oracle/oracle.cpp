@@ -1348,9 +1348,18 @@ struct SearchTree {
 // The leading samples a calibrated search keeps (py/estimation.rs:846-897 with
 // kriging_algebra.rs:421-441), for point support (pts null) or the block
 // discretised by pts; all of them without calibration.
+static std::pair<int, bool> calibrated_nm(const Search& p, int kind, const P3& t, const std::vector<P3>* pts,
+                                          const std::vector<Sample>& sel, const Variogram& vg);
+
 static int calibrated_n(const Search& p, int kind, const P3& t, const std::vector<P3>* pts,
                         const std::vector<Sample>& sel, const Variogram& vg) {
-  if (p.calibration == 0 || sel.empty()) return (int)sel.size();
+  return calibrated_nm(p, kind, t, pts, sel, vg).first;
+}
+
+// The count kept and whether it meets the target (py/estimation.rs:846-893).
+static std::pair<int, bool> calibrated_nm(const Search& p, int kind, const P3& t, const std::vector<P3>* pts,
+                                          const std::vector<Sample>& sel, const Variogram& vg) {
+  if (p.calibration == 0 || sel.empty()) return {(int)sel.size(), false};
   std::vector<double> rhs;
   double support;
   if (pts) {
@@ -1362,7 +1371,7 @@ static int calibrated_n(const Search& p, int kind, const P3& t, const std::vecto
     support = vg.total_sill();
   }
   auto q = prefix_quality(sel, vg, rhs, kind != SIMPLE, support);
-  return calibrated_count(p.calibration, p.calibration_target, p.min_samples, q).first;
+  return calibrated_count(p.calibration, p.calibration_target, p.min_samples, q);
 }
 
 // Simple kriging has a slope of 1 by construction (kriging_algebra.rs:429-434).
@@ -1718,6 +1727,82 @@ static void write_conditional(const Conditional* c, int st, int n, int64_t t, in
   if (o->violations) o->violations[t] = c ? c->violations : 0;
   if (o->n_used) o->n_used[t] = c ? n : 0;
   if (o->status) o->status[t] = st;
+}
+
+
+// ---------------------------------------------------------------------------
+// neighborhood_stats, crates/estimation/src/neighborhood.rs:18-136 (Euclidean
+// distance; the anisotropic metric is not used by the diagnostics).
+struct NeighborhoodStats {
+  int64_t n_within = 0, n_holes = 0;
+  double nearest_dist = INFINITY, mean_dist_knn = INFINITY, max_dist_knn = INFINITY;
+  double value_mean = NAN, value_var = NAN, value_min = NAN, value_max = NAN, value_idw = NAN;
+};
+
+static NeighborhoodStats neighborhood_stats(const P3& target, const std::vector<Sample>& s, size_t k, double radius) {
+  NeighborhoodStats st;
+  if (s.empty() || k == 0) return st;
+  struct Row {
+    double d, v;
+    int32_t hole;
+  };
+  std::vector<Row> d;
+  for (const Sample& x : s) d.push_back({euclidean(target, x.loc), x.value, x.hole});
+  std::stable_sort(d.begin(), d.end(), [](const Row& a, const Row& b) { return a.d < b.d; });
+  std::vector<int32_t> seen;
+  for (const Row& r : d) {
+    if (!(r.d <= radius)) continue;
+    st.n_within++;
+    if (r.hole < 0) {
+      st.n_holes++;
+    } else if (std::find(seen.begin(), seen.end(), r.hole) == seen.end()) {
+      seen.push_back(r.hole);
+      st.n_holes++;
+    }
+  }
+  st.nearest_dist = d[0].d;
+  const size_t kk = std::min(k, d.size());
+  double sd = -0.0;
+  for (size_t i = 0; i < kk; ++i) sd += d[i].d;
+  st.mean_dist_knn = sd / (double)kk;
+  st.max_dist_knn = d[kk - 1].d;
+  double sv = -0.0;
+  for (size_t i = 0; i < kk; ++i) sv += d[i].v;
+  st.value_mean = sv / (double)kk;
+  double ss = -0.0;
+  for (size_t i = 0; i < kk; ++i) ss += powi(d[i].v - st.value_mean, 2);
+  st.value_var = ss / (double)kk;
+  double lo = INFINITY, hi = -INFINITY;
+  for (size_t i = 0; i < kk; ++i) {
+    lo = std::fmin(lo, d[i].v);
+    hi = std::fmax(hi, d[i].v);
+  }
+  st.value_min = lo;
+  st.value_max = hi;
+  double num = 0.0, den = 0.0;
+  bool exact = false;
+  double hit = 0.0;
+  for (size_t i = 0; i < kk; ++i) {
+    if (d[i].d < 1e-12) {
+      exact = true;
+      hit = d[i].v;
+      break;
+    }
+    double w = 1.0 / (d[i].d * d[i].d);
+    num += w * d[i].v;
+    den += w;
+  }
+  st.value_idw = exact ? hit : (den > 0.0 ? num / den : st.value_mean);
+  return st;
+}
+
+// krige.rs:64-75 Estimate::negative_weight_sum (NaN without weights).
+static double negative_weight_sum(const std::vector<double>& w) {
+  if (w.empty()) return NAN;
+  double s = -0.0;
+  for (double x : w)
+    if (x < 0.0) s += x;
+  return s;
 }
 
 }  // namespace orc
@@ -2182,6 +2267,74 @@ int or_mik_conditional(int64_t n, const double* values, const double* weights, c
     for (size_t i = 0; i < k; ++i) r[i] = raw[i * m + t];
     Conditional c = mik_conditional(mk, g, r, cut, qs);
     write_conditional(&c, BK_OK, 0, t, m, o, &r, k, nc, nq);
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Diagnostics exports.
+extern "C" {
+
+// neighborhood_stats (neighborhood.rs:66-136) of the samples idx[0 .. n) at
+// target t with k nearest and radius; out: n_within, n_holes, nearest,
+// mean_dist_knn, max_dist_knn, value_mean, value_var, value_min, value_max,
+// value_idw.
+void or_neighborhood_stats(const or_samples* sm, const double* t, int n, const int64_t* idx, int64_t k,
+                           double radius, double* out) {
+  auto s = samples_of(sm);
+  std::vector<Sample> sel;
+  for (int i = 0; i < n; ++i) sel.push_back(s[idx[i]]);
+  NeighborhoodStats st = neighborhood_stats(P3{t[0], t[1], t[2]}, sel, (size_t)k, radius);
+  const double v[10] = {(double)st.n_within, (double)st.n_holes, st.nearest_dist, st.mean_dist_knn,
+                        st.max_dist_knn, st.value_mean, st.value_var, st.value_min, st.value_max, st.value_idw};
+  std::copy(v, v + 10, out);
+}
+
+// The per-target diagnostics of _Estimator.predict(diagnostics=True)
+// (py/estimation.rs:895-904,1034-1105) for one pass of point (blk NULL) or
+// block kriging: the estimate, the statistics of the samples used and, for
+// calibrated searches, whether they meet the target (-1 otherwise).
+int or_krige_diag(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int kind, double param,
+                  const bk_block* blk, int64_t m, const double* tx, const double* ty, const double* tz,
+                  const bk_diag_out* o, int nthreads) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  if (!calibration_fits(p, kind)) return BK_E_INVALID;
+  SearchTree tree(s, p, &vg);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    P3 q{tx[t], ty[t], tz[t]};
+    std::vector<int64_t> chosen;
+    Estimate e;
+    std::vector<Sample> sel;
+    int met = -1;
+    int st = tree.neighbors_around(q, -1, chosen);
+    if (st == BK_OK) {
+      for (auto i : tree.balanced(q, chosen)) sel.push_back(s[i]);
+      std::vector<P3> pts;
+      if (blk) pts = block_points(q, *blk);
+      auto nm = calibrated_nm(p, kind, q, blk ? &pts : nullptr, sel, vg);
+      sel.resize(nm.first);
+      if (p.calibration) met = nm.second ? 1 : 0;
+      st = blk ? block_krige_points(kind, param, pts, sel, vg, e) : krige(kind, param, q, sel, vg, e);
+    }
+    const bool ok = st == BK_OK;
+    NeighborhoodStats ns;
+    if (ok) ns = neighborhood_stats(q, sel, sel.size(), INFINITY);
+    if (o->value) o->value[t] = ok ? e.value : NAN;
+    if (o->variance) o->variance[t] = ok ? e.variance : NAN;
+    if (o->lagrange) o->lagrange[t] = ok ? e.lagrange : NAN;
+    if (o->support_variance) o->support_variance[t] = ok ? e.support : NAN;
+    if (o->n_used) o->n_used[t] = ok ? e.n_used : 0;
+    if (o->status) o->status[t] = st;
+    if (o->negative_weight_sum) o->negative_weight_sum[t] = ok ? negative_weight_sum(e.weights) : NAN;
+    if (o->n_holes) o->n_holes[t] = ok ? (int32_t)ns.n_holes : 0;
+    if (o->mean_distance) o->mean_distance[t] = ok ? ns.mean_dist_knn : NAN;
+    if (o->target_met) o->target_met[t] = ok ? met : -1;
   }
   return 0;
 }

@@ -181,7 +181,7 @@ def test_fit_warns_on_shared_locations_before_any_device_call(monkeypatch):
     assert fake.got[3].tolist() == [1.0, 3.0] and fake.got[5].tolist() == [0, 2]
     assert est.values.tolist() == [1.0, 3.0]
     with pytest.raises(NotImplementedError):
-        est.predict([[0, 0, 0]], diagnostics=True)
+        est.predict([[0, 0, 0]], anisotropy=object())
 
 
 def test_simple_kriging_takes_no_slope_calibration():

@@ -1,0 +1,97 @@
+"""Per-target diagnostics on the device (bk_krige_diagnostics) against the
+oracle (crates/py/src/estimation.rs:1034-1105 with neighborhood_stats over the
+samples used): the 5,000-composite drill-hole workload with holes, every
+kind, both precisions, a calibrated octant search and block support; and the
+estimators' predict(diagnostics=True) table."""
+import numpy as np
+import pytest
+
+import oracle
+from boitata_amd import _abi, datasets
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    w = datasets.config1()
+    return w, oracle.Data(w.x, w.y, w.z, w.value, None, w.hole), w.targets[::23].copy()
+
+
+def check(g, o, tol, c0=1.0):
+    assert np.array_equal(g["status"], o["status"])
+    ok = o["status"] == 0
+    assert ok.mean() > 0.5
+    for k in ("value", "variance", "lagrange", "support_variance"):
+        a, b = g[k][ok], o[k][ok]
+        err = np.abs(a - b) / np.maximum(np.abs(b), 1e-3 * c0)
+        assert err.max() <= tol, (k, err.max())
+        assert np.isnan(g[k][~ok]).all(), k
+    for k in ("n_used", "n_holes", "target_met"):
+        assert np.array_equal(g[k][ok], o[k][ok]), k
+    # Same distances, the same sorted order and sequential sum as the reference.
+    assert np.abs(g["mean_distance"][ok] - o["mean_distance"][ok]).max() <= 1e-13 * o["mean_distance"][ok].max()
+    # Weights agree to the solve's tolerance; a weight within rounding of 0 may
+    # change sign, so the bound is absolute.
+    assert np.abs(g["negative_weight_sum"][ok] - o["negative_weight_sum"][ok]).max() <= max(tol, 1e-12) * 10
+    assert np.isnan(g["mean_distance"][~ok]).all() and np.all(g["target_met"][~ok] == -1)
+
+
+@pytest.mark.parametrize("kind,param", [(_abi.ORDINARY, 0.0), (_abi.SIMPLE, 1.0), (_abi.INDICATOR, 1.2)])
+@pytest.mark.parametrize("precision,tol", [(_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)])
+def test_point_diagnostics(engine, c1, kind, param, precision, tol):
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    g = engine.krige_diagnostics(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], kind=kind, param=param,
+                                 precision=precision)
+    o = oracle.krige_diag(d, w.variogram, w.search, T, kind=kind, param=param)
+    check(g, o, tol)
+    assert np.all(g["target_met"] == -1)
+
+
+def test_calibrated_octant_search(engine, c1):
+    w, d, T = c1
+    cal = _abi.make_search(min_samples=4, max_samples=24, radius=120.0, rotation=(30, 0, 0),
+                           ratios=(80 / 120, 30 / 120), octant=True, target_efficiency=0.35)
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    g = engine.krige_diagnostics(w.variogram, cal, T[:, 0], T[:, 1], T[:, 2])
+    o = oracle.krige_diag(d, w.variogram, cal, T)
+    check(g, o, 1e-10)
+    ok = o["status"] == 0
+    assert (o["target_met"][ok] == 1).any() and (o["target_met"][ok] == 0).any()
+
+
+def test_block_diagnostics(engine, c1):
+    w, d, T = c1
+    blk = _abi.make_block((10.0, 10.0, 10.0), (3, 3, 2))
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    g = engine.krige_diagnostics(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], block=blk)
+    o = oracle.krige_diag(d, w.variogram, w.search, T, block=blk)
+    check(g, o, 1e-9)
+    ok = o["status"] == 0
+    assert np.all(g["support_variance"][ok] < 1.0)  # C(B, B) below the sill
+
+
+def test_estimator_diagnostics_table(c1):
+    from boitata_amd.estimation import OrdinaryKriging, Search, Variogram
+
+    w, d, T = c1
+    vg = Variogram([("spherical", 0.8, 120)], nugget=0.2, rotation=(30, 0, 0), ratios=(80 / 120, 30 / 120))
+    passes = [Search(ranges=(40, 30, 10), rotation=(30, 0, 0), min_samples=8, max_samples=16),
+              Search(ranges=(120, 80, 30), rotation=(30, 0, 0), min_samples=4, max_samples=24)]
+    est = OrdinaryKriging(vg, passes).fit(np.c_[w.x, w.y, w.z], w.value, holes=w.hole)
+    tab = est.predict(T, diagnostics=True)
+    assert tab.column_names == ["value", "variance", "efficiency", "slope", "n_samples", "pass", "n_holes",
+                                "n_other_domain", "mean_distance", "negative_weight_sum", "lagrange",
+                                "support_variance", "estimate_variance", "max_samples_reached", "target_met"]
+    v, q = est.predict(T, return_variance=True)
+    assert np.array_equal(tab["value"], v, equal_nan=True) and np.array_equal(tab["variance"], q, equal_nan=True)
+    ok = ~np.isnan(v)
+    assert set(np.unique(tab["pass"][ok])) == {1.0, 2.0}
+    full = np.where(tab["pass"] == 1, tab["n_samples"] == 16, tab["n_samples"] == 24)
+    assert np.array_equal(tab["max_samples_reached"][ok] == 1.0, full[ok])
+    assert np.all(tab["n_holes"][ok] <= tab["n_samples"][ok]) and np.all(tab["n_other_domain"][ok] == 0)
+    sv, var, lag = tab["support_variance"], tab["variance"], tab["lagrange"]
+    np.testing.assert_array_equal(tab["efficiency"][ok], ((sv - var) / sv)[ok])
+    np.testing.assert_array_equal(tab["estimate_variance"][ok], ((sv - var - lag) - lag)[ok])
+    assert np.isnan(tab["target_met"]).all()
