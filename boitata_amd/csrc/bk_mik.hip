@@ -72,68 +72,81 @@ __global__ void __launch_bounds__(kThreads) k_mik_status(int64_t m, const int32_
 
 __device__ inline double clip01(double p) { return p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p); }
 
-// Class c's probability under the corrected cdf (indicator.rs:357-358).
-__device__ inline double class_q(const MikCondArgs& a, int64_t t, int c) {
-  const double lo = c == 0 ? 0.0 : a.cdf[(int64_t)(c - 1) * a.m + t];
-  const double hi = c == a.K ? 1.0 : a.cdf[(int64_t)c * a.m + t];
-  return hi - lo;
-}
-
-// MultipleIndicator::conditional (indicator.rs:380-437).
-__global__ void __launch_bounds__(kThreads) k_mik_conditional(const MikCondArgs a) {
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= a.m) return;
+// MultipleIndicator::conditional (indicator.rs:380-437) of target t. With
+// KR > 0 (K <= KR) the corrected cdf lives in registers (every loop over the
+// thresholds unrolled to KR); with KR = 0 it lives in the cdf output, read back
+// per class.
+template <int KR>
+__device__ __forceinline__ void conditional_of(const MikCondArgs& a, int64_t t) {
+  constexpr int KN = KR > 0 ? KR : 1;
+  constexpr int KC = KR > 0 ? KR + 1 : 1;  // unroll counts (none for the memory form)
   const int64_t m = a.m;
   const int K = a.K;
-  if (a.status[t] != BK_OK) {
-    const double nan = nan64();
-    for (int k = 0; k < K; ++k) a.cdf[(int64_t)k * m + t] = a.raw[(int64_t)k * m + t] = nan;
-    if (a.correction) a.correction[t] = nan;
-    if (a.mean) a.mean[t] = nan;
-    if (a.variance) a.variance[t] = nan;
-    for (int c = 0; c < a.C; ++c) {
-      if (a.pa) a.pa[(int64_t)c * m + t] = nan;
-      if (a.ma) a.ma[(int64_t)c * m + t] = nan;
-    }
-    if (a.qv)
-      for (int q = 0; q < a.Q; ++q) a.qv[(int64_t)q * m + t] = nan;
-    if (a.violations) a.violations[t] = 0;
-    if (a.n_used) a.n_used[t] = 0;
-    if (a.pit) a.pit[t] = nan;
-    return;
-  }
+  const int KL = KR > 0 ? KR : K;  // loop bound (guarded by K when unrolled)
+  double reg[KN];
+  auto put = [&](int k, double v) {
+    if constexpr (KR > 0)
+      reg[k] = v;
+    else
+      a.cdf[(int64_t)k * m + t] = v;
+  };
+  auto get = [&](int k) -> double {
+    if constexpr (KR > 0)
+      return reg[k];
+    else
+      return a.cdf[(int64_t)k * m + t];
+  };
   // correct_order_relations (indicator.rs:137-150): the upward running
-  // maximum goes to cdf, then the downward running minimum is averaged in.
+  // maximum, then the downward running minimum averaged in.
   double run = 0.0;
-  for (int k = 0; k < K; ++k) {
-    const double p = clip01(a.raw[(int64_t)k * m + t]);
-    run = k == 0 ? p : fmax(p, run);
-    a.cdf[(int64_t)k * m + t] = run;
+#pragma unroll KN
+  for (int k = 0; k < KL; ++k) {
+    if (k < K) {
+      const double p = clip01(a.raw[(int64_t)k * m + t]);
+      run = k == 0 ? p : fmax(p, run);
+      put(k, run);
+    }
   }
-  for (int k = K - 1; k >= 0; --k) {
-    const double p = clip01(a.raw[(int64_t)k * m + t]);
-    run = k == K - 1 ? p : fmin(p, run);
-    const int64_t o = (int64_t)k * m + t;
-    a.cdf[o] = 0.5 * (a.cdf[o] + run);
+#pragma unroll KN
+  for (int kk = 0; kk < KL; ++kk) {
+    const int k = KL - 1 - kk;
+    if (k < K) {
+      const double p = clip01(a.raw[(int64_t)k * m + t]);
+      run = k == K - 1 ? p : fmin(p, run);
+      put(k, 0.5 * (get(k) + run));
+    }
   }
   double corr = -0.0;
   int viol = 0;
-  for (int k = 0; k < K; ++k) {
-    const double c = a.cdf[(int64_t)k * m + t], r = a.raw[(int64_t)k * m + t];
-    corr += fabs(c - r);
-    viol += c != r;
+#pragma unroll KN
+  for (int k = 0; k < KL; ++k) {
+    if (k < K) {
+      const double c = get(k), r = a.raw[(int64_t)k * m + t];
+      corr += fabs(c - r);
+      viol += c != r;
+      if constexpr (KR > 0) a.cdf[(int64_t)k * m + t] = c;
+    }
   }
   if (a.correction) a.correction[t] = corr;
   if (a.violations) a.violations[t] = viol;
   if (a.n_used) a.n_used[t] = a.ncount[t];
+  // Class c's probability under the corrected cdf (indicator.rs:357-358).
+  auto class_q = [&](int c) -> double {
+    const double lo = c == 0 ? 0.0 : get(c - 1);
+    const double hi = c == K ? 1.0 : get(c);
+    return hi - lo;
+  };
   // Moments: sum over the classes with q > 0 of q * Shape::above, one sum per
   // (cutoff, power), in class order.
   const int CC = a.C + 1;
   auto moment = [&](int ci, int j) {
     double sum = -0.0;
-    for (int c = 0; c <= K; ++c) {
-      const double q = class_q(a, t, c);
-      if (q > 0.0) sum += q * a.table[((int64_t)c * CC + ci) * 3 + j];
+#pragma unroll KC
+    for (int c = 0; c <= KL; ++c) {
+      if (c <= K) {
+        const double q = class_q(c);
+        if (q > 0.0) sum += q * a.table[((int64_t)c * CC + ci) * 3 + j];
+      }
     }
     return sum;
   };
@@ -149,33 +162,64 @@ __global__ void __launch_bounds__(kThreads) k_mik_conditional(const MikCondArgs 
     // The held-out value as the one cutoff: F*(z) = 1 - P(Z > z).
     const double z = a.own_values[a.held[t]];
     double sum = -0.0;
-    for (int c = 0; c <= K; ++c) {
-      const double q = class_q(a, t, c);
-      if (q > 0.0) sum += q * mik_above(a.g, a.cls[c], z, 0);
+#pragma unroll KC
+    for (int c = 0; c <= KL; ++c) {
+      if (c <= K) {
+        const double q = class_q(c);
+        if (q > 0.0) sum += q * mik_above(a.g, a.cls[c], z, 0);
+      }
     }
     a.pit[t] = 1.0 - sum;
   }
   if (a.qv && a.Q > 0) {
     int last = 0;
-    for (int c = K; c >= 0; --c)
-      if (class_q(a, t, c) > 0.0) {
-        last = c;
-        break;
-      }
+#pragma unroll KC
+    for (int c = 0; c <= KL; ++c)
+      if (c <= K && class_q(c) > 0.0) last = c;
     for (int qi = 0; qi < a.Q; ++qi) {
       const double u = a.quantiles[qi];
       double below = 0.0, v = a.upper;
-      for (int c = 0; c <= K; ++c) {
-        const double q = class_q(a, t, c);
-        if (q > 0.0 && (below + q >= u || c == last)) {
-          v = mik_quantile(a.g, a.cls[c], mik_clamp((u - below) / q, 0.0, 1.0));
-          break;
+      bool found = false;
+#pragma unroll KC
+      for (int c = 0; c <= KL; ++c) {
+        if (c <= K && !found) {
+          const double q = class_q(c);
+          if (q > 0.0 && (below + q >= u || c == last)) {
+            v = mik_quantile(a.g, a.cls[c], mik_clamp((u - below) / q, 0.0, 1.0));
+            found = true;
+          } else {
+            below += q;
+          }
         }
-        below += q;
       }
       a.qv[(int64_t)qi * m + t] = v;
     }
   }
+}
+
+template <int KR>
+__global__ void __launch_bounds__(kThreads) k_mik_conditional(const MikCondArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= a.m) return;
+  const int64_t m = a.m;
+  if (a.status[t] != BK_OK) {
+    const double nan = nan64();
+    for (int k = 0; k < a.K; ++k) a.cdf[(int64_t)k * m + t] = a.raw[(int64_t)k * m + t] = nan;
+    if (a.correction) a.correction[t] = nan;
+    if (a.mean) a.mean[t] = nan;
+    if (a.variance) a.variance[t] = nan;
+    for (int c = 0; c < a.C; ++c) {
+      if (a.pa) a.pa[(int64_t)c * m + t] = nan;
+      if (a.ma) a.ma[(int64_t)c * m + t] = nan;
+    }
+    if (a.qv)
+      for (int q = 0; q < a.Q; ++q) a.qv[(int64_t)q * m + t] = nan;
+    if (a.violations) a.violations[t] = 0;
+    if (a.n_used) a.n_used[t] = 0;
+    if (a.pit) a.pit[t] = nan;
+    return;
+  }
+  conditional_of<KR>(a, t);
 }
 
 }  // namespace
@@ -196,7 +240,10 @@ int launch_mik_raw(hipStream_t s, const MikRawArgs& a) {
 
 int launch_mik_conditional(hipStream_t s, const MikCondArgs& a) {
   if (a.m <= 0) return 0;
-  k_mik_conditional<<<blocks_for(a.m), kThreads, 0, s>>>(a);
+  if (a.K <= 16)
+    k_mik_conditional<16><<<blocks_for(a.m), kThreads, 0, s>>>(a);
+  else
+    k_mik_conditional<0><<<blocks_for(a.m), kThreads, 0, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

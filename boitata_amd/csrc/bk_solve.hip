@@ -776,60 +776,86 @@ __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, 
       RowL[k * BK_WAVE + lane] = v;
     x[k] = (float)(v * ic0);
   }
-  // Gauss-Jordan: after step k, column k holds the inverse's column. Steps go
-  // in pairs (k, k + 1) to halve the LDS round trips: lanes k and k + 1
-  // publish their rows, and every lane applies step k to row k + 1 itself
-  // (the same operations lane k + 1 performs), which gives the pivot row of
-  // step k + 1 without a second round trip.
+  // Gauss-Jordan (the sweep operator): after sweeping pivot k, column k holds
+  // the inverse's column. Pivots go in blocks K = {k .. k + 3}: lanes k .. k + 3
+  // publish their rows, every lane inverts the 4 x 4 pivot block P by four
+  // single sweeps (its pivots are those of the four single sweeps of the whole
+  // matrix, so the small-pivot test is the same) and applies the block sweep,
+  // x_j -= sum_q s_q row_q[j] with s = x_K P^-1, one float4 chunk of columns
+  // at a time. One LDS round trip per four pivots. The four rows take P2 and
+  // the right-hand-side buffer after it (4 NMAX floats), which the solves use
+  // only after the elimination.
   bad = false;
-  float* rowA = P2;
-  float* rowB = P2 + NMAX;
+  float* rows = P2;
 #pragma unroll
-  for (int k = 0; k < NMAX; k += 2) {
-    if (r == k || r == k + 1) {
-      float* dst = (r == k) ? rowA : rowB;
+  for (int k = 0; k < NMAX; k += 4) {
+    const int pr = r - k;  // 0 .. 3 on the pivot rows
+    const bool piv_row = pr >= 0 && pr < 4;
+    if (piv_row) {
+      float* dst = rows + pr * NMAX;
 #pragma unroll
       for (int c4 = 0; c4 < NMAX / 4; ++c4)
         reinterpret_cast<float4*>(dst)[c4] = make_float4(x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]);
     }
     wave_sync();
-    float pa[NMAX], pb[NMAX];
+    float P[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 q = reinterpret_cast<const float4*>(rows + i * NMAX)[k / 4];
+      P[i][0] = q.x;
+      P[i][1] = q.y;
+      P[i][2] = q.z;
+      P[i][3] = q.w;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const float pv = P[p][p];
+      bad |= !(pv > 1e-5f);
+      const float ip = __builtin_amdgcn_rcpf(pv);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j != p) P[p][j] *= ip;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i == p) continue;
+        const float f = P[i][p];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j != p) P[i][j] = fmaf(-f, P[p][j], P[i][j]);
+        P[i][p] = -f * ip;
+      }
+      P[p][p] = ip;
+    }
+    // s = x_K P^-1; on pivot row i, s_q = [q == i] - (P^-1)_iq, which turns
+    // the row (equal to row_i) into row i of P^-1 [row_0; ..; row_3]. The
+    // pivot row's own row of P^-1 is blended with one-hot factors (a select
+    // on the lane's row index would index the block dynamically).
+    const float e0 = pr == 0 ? 1.0f : 0.0f, e1 = pr == 1 ? 1.0f : 0.0f, e2 = pr == 2 ? 1.0f : 0.0f,
+                e3 = pr == 3 ? 1.0f : 0.0f;
+    float own[4], s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      own[q] = fmaf(P[3][q], e3, fmaf(P[2][q], e2, fmaf(P[1][q], e1, P[0][q] * e0)));
+      float acc = x[k] * P[0][q];
+#pragma unroll
+      for (int i = 1; i < 4; ++i) acc = fmaf(x[k + i], P[i][q], acc);
+      s[q] = piv_row ? ((pr == q ? 1.0f : 0.0f) - own[q]) : acc;
+    }
 #pragma unroll
     for (int c4 = 0; c4 < NMAX / 4; ++c4) {
-      const float4 qa = reinterpret_cast<const float4*>(rowA)[c4];
-      const float4 qb = reinterpret_cast<const float4*>(rowB)[c4];
-      pa[4 * c4] = qa.x;
-      pa[4 * c4 + 1] = qa.y;
-      pa[4 * c4 + 2] = qa.z;
-      pa[4 * c4 + 3] = qa.w;
-      pb[4 * c4] = qb.x;
-      pb[4 * c4 + 1] = qb.y;
-      pb[4 * c4 + 2] = qb.z;
-      pb[4 * c4 + 3] = qb.w;
+      if (c4 == k / 4) continue;
+      float4 q[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[i] = reinterpret_cast<const float4*>(rows + i * NMAX)[c4];
+      float* xc = x + 4 * c4;
+      xc[0] = fmaf(-s[3], q[3].x, fmaf(-s[2], q[2].x, fmaf(-s[1], q[1].x, fmaf(-s[0], q[0].x, xc[0]))));
+      xc[1] = fmaf(-s[3], q[3].y, fmaf(-s[2], q[2].y, fmaf(-s[1], q[1].y, fmaf(-s[0], q[0].y, xc[1]))));
+      xc[2] = fmaf(-s[3], q[3].z, fmaf(-s[2], q[2].z, fmaf(-s[1], q[1].z, fmaf(-s[0], q[0].z, xc[2]))));
+      xc[3] = fmaf(-s[3], q[3].w, fmaf(-s[2], q[2].w, fmaf(-s[1], q[1].w, fmaf(-s[0], q[0].w, xc[3]))));
+      if (c4 % 2 == 1) asm volatile("" ::: "memory");
     }
-    // Step k.
-    const float p = pa[k];
-    bad |= !(p > 1e-5f);
-    const float ip = __builtin_amdgcn_rcpf(p);
-    const float f1 = pb[k] * ip;
 #pragma unroll
-    for (int cc = 0; cc < NMAX; ++cc)
-      if (cc != k) pb[cc] = fmaf(-f1, pa[cc], pb[cc]);
-    pb[k] = -f1;
-    const float sk = (r == k) ? 1.0f - ip : x[k] * ip;
-#pragma unroll
-    for (int cc = 0; cc < NMAX; ++cc)
-      if (cc != k) x[cc] = fmaf(-sk, pa[cc], x[cc]);
-    x[k] = (r == k) ? ip : -sk;
-    // Step k + 1, pivot row pb.
-    const float p2 = pb[k + 1];
-    bad |= !(p2 > 1e-5f);
-    const float ip2 = __builtin_amdgcn_rcpf(p2);
-    const float sk2 = (r == k + 1) ? 1.0f - ip2 : x[k + 1] * ip2;
-#pragma unroll
-    for (int cc = 0; cc < NMAX; ++cc)
-      if (cc != k + 1) x[cc] = fmaf(-sk2, pb[cc], x[cc]);
-    x[k + 1] = (r == k + 1) ? ip2 : -sk2;
+    for (int q = 0; q < 4; ++q) x[k + q] = piv_row ? own[q] : -s[q];
     wave_sync();
   }
   return guard;
@@ -861,7 +887,8 @@ __device__ __forceinline__ void krige32_blk(const SolveArgs& a, int64_t nl, int 
   double crow[ROWREG ? NMAX : 1];
   float x[NMAX];
   bool bad;
-  const bool guard = refined_front<NMAX>(a, t, n, r, g, lane, active, true, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
+  const bool guard =
+      refined_front<NMAX>(a, t, n, r, g, lane, active, true, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
   const bool go = active && !guard;
   const bool live = go && r < n;
   const double c0 = a.v.c0;
@@ -1381,7 +1408,8 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
   double crow[ROWREG ? NMAX : 1];
   float x[NMAX];
   bool bad;
-  const bool guard = refined_front<NMAX>(a, t, n, r, g, lane, active, false, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
+  const bool guard =
+      refined_front<NMAX>(a, t, n, r, g, lane, active, false, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
   const bool go = active && !guard;
   const bool live = go && r < n;
   const double c0 = a.v.c0;
