@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tuning")
     ap.add_argument("--mik-steps", type=int, default=3,
                     help="timed passes of the multiple indicator kriging leg (0: skip)")
+    ap.add_argument("--config-steps", type=int, default=2,
+                    help="timed passes of the uneven (4M targets, octants) and block (2M blocks) legs (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="finish with a device-to-device copy of this many MiB (byte calibration of PMC runs)")
     args = ap.parse_args()
@@ -265,6 +267,57 @@ def main():
         print(f"[bench] MIK: {mik_line['value']:.0f} targets/s FP32+IR, {mik_line['fp64']['value']:.0f} FP64",
               file=sys.stderr, flush=True)
 
+    # The other workloads, measured beside the headline: uneven neighbourhoods
+    # (4M targets among clustered samples, octant search, 4-64 neighbours:
+    # size buckets) and block kriging (2M blocks discretised 4 x 4 x 2, 64
+    # neighbours, Gaussian covariance with a 1e-4 nugget: ill-conditioned
+    # systems that the refined form hands back to FP64).
+    configs = None
+    if args.config_steps > 0:
+        configs = {}
+        for name, gen in (("uneven_4m_octants", datasets.config3), ("block_2m_gaussian", datasets.config5)):
+            wc = gen()
+            Tc = wc.targets
+            mc = len(Tc)
+            holes = None if wc.hole is None else tt(wc.hole.astype(np.int32))
+            e.set_samples(tt(wc.x), tt(wc.y), tt(wc.z), tt(wc.value), None, holes)
+            cx, cy, cz = tt(Tc[:, 0]), tt(Tc[:, 1]), tt(Tc[:, 2])
+            blk = getattr(wc, "block", None)
+            res = {}
+            for prec in (_abi.FP32_IR, _abi.FP64):
+                def one():
+                    if blk is None:
+                        return e.krige(wc.variogram, wc.search, cx, cy, cz, precision=prec, out=res.get(prec))
+                    return e.block_krige(wc.variogram, wc.search, blk, cx, cy, cz, precision=prec)
+                res[prec] = one()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(args.config_steps):
+                    res[prec] = one()
+                torch.cuda.synchronize()
+                res[prec]["dt"] = (time.perf_counter() - t0) / args.config_steps
+            a32, a64 = res[_abi.FP32_IR], res[_abi.FP64]
+            s32, s64 = a32["status"].cpu().numpy(), a64["status"].cpu().numpy()
+            okc = (s32 == 0) & (s64 == 0)
+            c0c = wc.variogram.nugget + sum(wc.variogram.structures[k].sill for k in range(wc.variogram.n_structures))
+            errs = {}
+            for k in ("value", "variance"):
+                x, y = a32[k].cpu().numpy()[okc], a64[k].cpu().numpy()[okc]
+                errs[k] = float(np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-3 * c0c))) if okc.any() else None
+            nu = a64["n_used"].cpu().numpy()[s64 == 0]
+            configs[name] = {
+                "targets": mc, "samples": wc.n, "value": mc / a32["dt"], "unit": "targets/s",
+                "ms_per_step": a32["dt"] * 1e3, "fp64": {"value": mc / a64["dt"], "ms_per_step": a64["dt"] * 1e3},
+                "speedup_fp32ir_over_fp64": a64["dt"] / a32["dt"], "max_rel_err_vs_fp64": errs,
+                "status_equal": bool((s32 == s64).all()), "estimated": int((s64 == 0).sum()),
+                "n_used": {"min": int(nu.min()) if len(nu) else 0, "max": int(nu.max()) if len(nu) else 0,
+                           "le16": int((nu <= 16).sum()), "le32": int(((nu > 16) & (nu <= 32)).sum()),
+                           "le64": int((nu > 32).sum())},
+                "support": "block 4x4x2" if blk is not None else "point"}
+            print(f"[bench] {name}: {configs[name]['value']:.0f} targets/s FP32+IR, "
+                  f"{configs[name]['fp64']['value']:.0f} FP64", file=sys.stderr, flush=True)
+        e.set_samples(sx, sy, sz, sv, None, hole)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -334,6 +387,7 @@ def main():
         "cpu_baseline": cpu,
         "loo_cv_grad": loo_line,
         "mik": mik_line,
+        "other_workloads": configs,
         "autotune": tuned,
         "graph_launches": e.stat(_abi.STAT_GRAPH_LAUNCHES),
     }

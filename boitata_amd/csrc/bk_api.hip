@@ -1296,14 +1296,17 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
   const int nb = 3;
   if (int r = ensure(c, c->counts, 8 * sizeof(int32_t))) return r;
   if (int r = ensure(c, c->guard, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
-  if (int r = ensure(c, c->work, nb * BK_CTR_INTS * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->redo, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->work, (nb + 1) * BK_CTR_INTS * sizeof(int32_t))) return r;
   HIPOK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(int32_t), c->stream));
-  HIPOK(c, hipMemsetAsync(c->work.p, 0, nb * BK_CTR_INTS * sizeof(int32_t), c->stream));
+  HIPOK(c, hipMemsetAsync(c->work.p, 0, (nb + 1) * BK_CTR_INTS * sizeof(int32_t), c->stream));
   // Rows of samples outside every bucket stay zero (they are out of the loss).
   if (m > 0) HIPOK(c, hipMemsetAsync(c->support.p, 0, m * cols * sizeof(double), c->stream));
   int32_t* counts = (int32_t*)c->counts.p;
   a.guard = (int32_t*)c->guard.p;
   a.guard_count = counts + 4;
+  a.redo = (int32_t*)c->redo.p;
+  a.redo_count = counts + 5;
   // The buckets of bk_loo_prepare, with the edges they were made with.
   for (int b = 0; b < nb; ++b) {
     if (c->loo_hc[b] == 0) continue;
@@ -1313,6 +1316,16 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     const int r = precision == BK_FP32_IR ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b])
                                           : bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b]);
     if (r) return fail(c, BK_E_HIP, "gradient launch failed");
+  }
+  // Systems the refined kernel handed back: the FP64 gradient over the redo
+  // list (its length stays on the device).
+  if (precision == BK_FP32_IR) {
+    SolveArgs b = a;
+    b.list = a.redo;
+    b.n_list = m;
+    b.n_list_dev = a.redo_count;
+    b.work = (int32_t*)c->work.p + nb * BK_CTR_INTS;
+    if (bk::launch_loo_grad(c->stream, b, gp, 64)) return fail(c, BK_E_HIP, "gradient launch failed");
   }
   // Systems beyond the Cholesky kernels (n > 64) stay out of the loss.
   const int64_t rb = bk::reduce_blocks(m);

@@ -920,7 +920,9 @@ __device__ __forceinline__ void krige32_blk(const SolveArgs& a, int64_t nl, int 
   const float ma = group_maxf<NMAX>(fabsf(fa)), mb = group_maxf<NMAX>(fabsf(fb));
   int it = 0;
   bool conv = false;
-  for (; it < a.max_iter; ++it) {
+  // A flagged elimination (bad is the same on the group's lanes) goes to the
+  // FP64 kernel without refinement sweeps.
+  for (; it < a.max_iter && !bad; ++it) {
     reinterpret_cast<double2*>(X2)[r] = make_double2(xa, xb);
     wave_sync();
     double ra = ba, rb = bb;
@@ -1439,7 +1441,7 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
   const float tolf = (float)a.tol;
   const float ma = group_maxf<NMAX>(fabsf(fa)), mb = group_maxf<NMAX>(fabsf(fb)), mz = group_maxf<NMAX>(fabsf(fz));
   bool conv = false;
-  for (int it = 0; it < a.max_iter; ++it) {
+  for (int it = 0; it < a.max_iter && !bad; ++it) {
     double2* X = reinterpret_cast<double2*>(X4);
     X[2 * r] = make_double2(xa, xb);
     X[2 * r + 1] = make_double2(xz, 0.0);
@@ -1470,6 +1472,10 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
     }
   }
   const bool okf = go && !bad && conv;
+  // A system the FP32 elimination cannot serve (a small pivot, or no
+  // convergence) goes to the FP64 gradient kernel, which writes its row.
+  const bool redo = go && !okf && a.redo;
+  if (redo && r == 0) a.redo[atomicAdd(a.redo_count, 1)] = t;
   // Forward quantities in FP64 (kernel.rs:86-97), adjoint vector v = K^-1 [z; 0].
   const bool in = okf && r < n;
   const double sa = group_sum<NMAX>(in ? xa : 0.0);
@@ -1526,7 +1532,7 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
 #pragma unroll
   for (int p = 0; p < BK_MAX_GRAD; ++p)
     if (g32_used(p, g32.S)) acc[p] = group_sumf<NMAX>(acc[p]);
-  if (active && r == 0) {
+  if (active && r == 0 && !redo) {
     double* out = a.grad_part + t;
     const int64_t m = gp.m;
     if (ok) {

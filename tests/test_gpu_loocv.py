@@ -196,3 +196,33 @@ def test_refined_gradient_on_larger_neighbourhoods(engine, setup, max_samples):
         assert (np.abs(g32 - g64) / np.maximum(np.abs(g64), floor)).max() <= 1e-5
     finally:
         engine.loo_prepare(SEARCH, None)
+
+
+def test_refined_gradient_hands_ill_conditioned_systems_to_fp64(engine, setup):
+    # A Gaussian covariance with a 1e-4 nugget and samples duplicated 1 cm
+    # away: most FP32 eliminations meet a pivot below their threshold. The
+    # refined form must then take the FP64 gradient for those systems, so the
+    # loss runs over the same held-out samples as the FP64 form.
+    x, y, z, v, hole = small_set()
+    rng = np.random.default_rng(7)
+    dup = rng.choice(len(x), 12, replace=False)
+    jit = rng.normal(size=(12, 3))
+    jit *= 0.01 / np.linalg.norm(jit, axis=1, keepdims=True)
+    X, Y, Z = np.r_[x, x[dup] + jit[:, 0]], np.r_[y, y[dup] + jit[:, 1]], np.r_[z, z[dup] + jit[:, 2]]
+    V, H = np.r_[v, v[dup]], np.r_[hole, hole[dup]]
+    s = _abi.make_search(min_samples=1, max_samples=24, radius=400.0)
+    theta = np.array([1e-4, 1.0, 100.0, 100.0, 100.0, 0.0, 0.0, 0.0])
+    try:
+        engine.set_samples(X, Y, Z, V, None, H)
+        engine.loo_prepare(s, None)
+        vg = loocv.variogram_of(theta, ["gaussian"])
+        l64, g64, u64 = engine.loo_loss_grad(vg, _abi.FP64)
+        l32, g32, u32 = engine.loo_loss_grad(vg, _abi.FP32_IR)
+        assert u32 == u64 and u64 > 0.9 * len(X)
+        assert abs(l32 - l64) <= 1e-9 * abs(l64)
+        floor = 1e-3 * np.linalg.norm(g64)
+        assert (np.abs(g32 - g64) / np.maximum(np.abs(g64), floor)).max() <= 1e-5
+    finally:
+        x, y, z, v, hole = small_set()
+        engine.set_samples(x, y, z, v, None, hole)
+        engine.loo_prepare(SEARCH, None)
