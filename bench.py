@@ -283,6 +283,9 @@ def main():
             e.set_samples(tt(wc.x), tt(wc.y), tt(wc.z), tt(wc.value), None, holes)
             cx, cy, cz = tt(Tc[:, 0]), tt(Tc[:, 1]), tt(Tc[:, 2])
             blk = getattr(wc, "block", None)
+            tuned_c = None
+            if not args.no_autotune and blk is None:
+                tuned_c = e.autotune(wc.variogram, wc.search, cx, cy, cz, precision=_abi.FP32_IR, sample=65536)
             res = {}
             for prec in (_abi.FP32_IR, _abi.FP64):
                 def one():
@@ -313,7 +316,8 @@ def main():
                 "n_used": {"min": int(nu.min()) if len(nu) else 0, "max": int(nu.max()) if len(nu) else 0,
                            "le16": int((nu <= 16).sum()), "le32": int(((nu > 16) & (nu <= 32)).sum()),
                            "le64": int((nu > 32).sum())},
-                "support": "block 4x4x2" if blk is not None else "point"}
+                "support": "block 4x4x2" if blk is not None else "point",
+                "refine_limit": tuned_c["refine_limit"] if tuned_c else None}
             print(f"[bench] {name}: {configs[name]['value']:.0f} targets/s FP32+IR, "
                   f"{configs[name]['fp64']['value']:.0f} FP64", file=sys.stderr, flush=True)
         e.set_samples(sx, sy, sz, sv, None, hole)

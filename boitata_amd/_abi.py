@@ -258,10 +258,11 @@ EXTRA_SIGNATURES = [
     ("bk_set_solve_tuning", C.c_int, [P, I32]),
     ("bk_autotune", C.c_int, [P, P, P, I32, D, I32, I64, P, P, P, P]),
     ("bk_set_graphs", C.c_int, [P, I32]),
+    ("bk_set_refine_limit", C.c_int, [P, I32]),
 ]
 
 AUTOTUNE_REPORT = ("cell_factor", "search_mode", "waves", "edge0", "edge1", "edge2", "search_ms_before",
-                   "solve_ms_before", "search_ms_after", "solve_ms_after")
+                   "solve_ms_before", "search_ms_after", "solve_ms_after", "refine_limit")
 
 STAT_SEARCH_SPILL = 0
 STAT_GRAPH_LAUNCHES = 1

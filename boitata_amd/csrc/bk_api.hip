@@ -93,6 +93,10 @@ struct bk_ctx {
   bk::TileBuffers tiles;
   int search_mode = 0;  // uncapped searches: 0 wave per target, 1..8 tiles of that reach, -1 lane per target
   int solve_waves = 2;  // wavefronts per workgroup of the solve kernels
+  // FP32+IR solves buckets of at most this many samples; larger systems take
+  // the FP64 kernel (at n <= 64 the refined kernel's LDS allows ~3 waves per
+  // CU and measured 2x slower than the FP64 Cholesky kernel on gfx950).
+  int refine_max = 32;
   GridDev grid{};
   bool grid_valid = false;
   double grid_F[9] = {0};
@@ -735,7 +739,8 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t 
     a.n_list = m;
     a.n_list_dev = counts + b;
     a.work = work + b * BK_CTR_INTS;
-    int r = refined ? bk::launch_krige32(c->stream, a, c->edges[b]) : bk::launch_krige64(c->stream, a, c->edges[b]);
+    const bool ref_b = refined && c->edges[b] <= c->refine_max;
+    int r = ref_b ? bk::launch_krige32(c->stream, a, c->edges[b]) : bk::launch_krige64(c->stream, a, c->edges[b]);
     if (r) return fail(c, BK_E_HIP, "solve launch failed");
   }
   mark(c, 4);
@@ -1313,8 +1318,9 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     a.list = (const int32_t*)c->loo_lists.p + (int64_t)b * m;
     a.n_list = c->loo_hc[b];
     a.work = (int32_t*)c->work.p + b * BK_CTR_INTS;
-    const int r = precision == BK_FP32_IR ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b])
-                                          : bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b]);
+    const bool ref_b = precision == BK_FP32_IR && c->loo_edges[b] <= c->refine_max;
+    const int r = ref_b ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b])
+                        : bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b]);
     if (r) return fail(c, BK_E_HIP, "gradient launch failed");
   }
   // Systems the refined kernel handed back: the FP64 gradient over the redo
@@ -1857,6 +1863,15 @@ int bk_set_search_tuning(bk_ctx* c, int32_t mode) {
   return 0;
 }
 
+int bk_set_refine_limit(bk_ctx* c, int32_t max_n) {
+  if (!c) return BK_E_INVALID;
+  if (max_n <= 0) return 0;
+  if (max_n != 16 && max_n != 32 && max_n != 64) return fail(c, BK_E_INVALID, "refine limit must be 16, 32 or 64");
+  c->refine_max = max_n;
+  c->gen++;
+  return 0;
+}
+
 int bk_set_solve_tuning(bk_ctx* c, int32_t waves) {
   if (!c) return BK_E_INVALID;
   if (waves != 1 && waves != 2 && waves != 4) return fail(c, BK_E_INVALID, "waves per workgroup is 1, 2 or 4");
@@ -1979,31 +1994,41 @@ int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t k
   if (best_cf != c->cell_factor) c->grid_valid = false;
   c->cell_factor = best_cf;
   c->search_mode = best_mode;
-  // The solves: workgroup size and bucket edges.
+  // The solves: workgroup size, bucket edges and, for FP32+IR, the largest
+  // bucket the refined kernel takes (the rest go to the FP64 kernel).
   const int ws[] = {1, 2, 4};
   const int edge_sets[4][3] = {{16, 32, 64}, {32, 32, 64}, {16, 64, 64}, {64, 64, 64}};
-  int best_w = c->solve_waves, best_e[3] = {c->edges[0], c->edges[1], c->edges[2]};
+  const int limits[2] = {32, 64};
+  const int n_limits = precision == BK_FP32_IR ? 2 : 1;
+  int best_w = c->solve_waves, best_e[3] = {c->edges[0], c->edges[1], c->edges[2]}, best_l = c->refine_max;
   double best_solve = def_solve;
   for (int w : ws)
-    for (const auto& es : edge_sets) {
-      c->solve_waves = w;
-      for (int k = 0; k < 3; ++k) c->edges[k] = es[k];
-      double sm = 0.0, so = 0.0;
-      if (int r = trial(sm, so)) return r;
-      if (so < best_solve) {
-        best_solve = so;
-        best_w = w;
-        for (int k = 0; k < 3; ++k) best_e[k] = es[k];
+    for (const auto& es : edge_sets)
+      for (int li = 0; li < n_limits; ++li) {
+        c->solve_waves = w;
+        for (int k = 0; k < 3; ++k) c->edges[k] = es[k];
+        if (n_limits > 1) c->refine_max = limits[li];
+        double sm = 0.0, so = 0.0;
+        if (int r = trial(sm, so)) return r;
+        // The refine limit moves off its default only for a clear gain (a
+        // workload without large systems times the same either way).
+        if (c->refine_max != 32) so *= 1.05;
+        if (so < best_solve) {
+          best_solve = so;
+          best_w = w;
+          best_l = c->refine_max;
+          for (int k = 0; k < 3; ++k) best_e[k] = es[k];
+        }
       }
-    }
   c->solve_waves = best_w;
+  c->refine_max = best_l;
   for (int k = 0; k < 3; ++k) c->edges[k] = best_e[k];
   c->gen++;
   if (!was_prof) (void)bk_profile(c, 0);
   if (report) {
     const double rep[BK_AUTOTUNE_REPORT] = {best_cf,     (double)best_mode, (double)best_w, (double)best_e[0],
                                             (double)best_e[1], (double)best_e[2], def_search,  def_solve,
-                                            best_search, best_solve};
+                                            best_search, best_solve,      (double)best_l};
     std::memcpy(report, rep, sizeof(rep));
   }
   return 0;

@@ -124,6 +124,10 @@ class Engine:
         """Wavefronts per workgroup of the solve kernels (1, 2 or 4)."""
         self._check(self._lib.bk_set_solve_tuning(self._h, int(waves)))
 
+    def refine_limit(self, max_n: int):
+        """Largest size bucket (16, 32, 64) FP32+IR solves by refinement; larger take FP64."""
+        self._check(self._lib.bk_set_refine_limit(self._h, int(max_n)))
+
     def graphs(self, enable: bool = True):
         """Repeated krige passes over device arrays run as a captured HIP graph (default on)."""
         self._check(self._lib.bk_set_graphs(self._h, 1 if enable else 0))

@@ -311,6 +311,14 @@ int bk_set_search_tuning(bk_ctx* ctx, int32_t mode);
  * (default) or 4 wavefronts. Results do not depend on it. */
 int bk_set_solve_tuning(bk_ctx* ctx, int32_t waves);
 
+/* Largest size bucket (16, 32 or 64 samples; default 32) that BK_FP32_IR
+ * solves by FP32 elimination with FP64 refinement (no counterpart in the
+ * reference). Systems in larger buckets take the FP64 kernel, whose results
+ * FP32+IR matches anyway: on gfx950 the refined kernel for up to 64 samples is
+ * LDS-bound and slower than the FP64 Cholesky kernel. Values <= 0 keep the
+ * setting. Applies to kriging passes and to the leave-one-out gradient. */
+int bk_set_refine_limit(bk_ctx* ctx, int32_t max_n);
+
 /* Autotuner (no counterpart in the reference): times krige passes over the m
  * targets given (a sample of a workload) with HIP events and keeps, in the
  * context, the fastest grid cell factor and uncapped search mode (by the
@@ -318,8 +326,9 @@ int bk_set_solve_tuning(bk_ctx* ctx, int32_t waves);
  * the bucket, solve and deferred stages). None of them changes results.
  * report (may be NULL) receives BK_AUTOTUNE_REPORT doubles: cell factor,
  * search mode, waves, edges[3], then search and solve milliseconds per pass
- * before and after tuning. */
-#define BK_AUTOTUNE_REPORT 10
+ * before and after tuning, then the refine limit (bk_set_refine_limit; for
+ * FP32+IR it is tuned too). */
+#define BK_AUTOTUNE_REPORT 11
 int bk_autotune(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
                 int32_t precision, int64_t m, const double* tx, const double* ty, const double* tz,
                 double* report);

@@ -142,6 +142,14 @@ def test_c3_clustered_octants(engine):
         assert_estimates(g, o, w.variogram, tol)
         counts = np.bincount(g["n_used"], minlength=65)
         assert counts[:16].sum() > 0 and counts[33:].sum() > 0  # uneven neighbourhoods
+    # Every refine limit: the buckets above it take the FP64 kernel.
+    try:
+        for limit in (16, 64, 32):
+            engine.refine_limit(limit)
+            g = engine.krige(w.variogram, w.search, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
+            assert_estimates(g, o, w.variogram, 1e-6)
+    finally:
+        engine.refine_limit(32)
 
 
 def test_edge_cases(engine):
