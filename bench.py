@@ -280,18 +280,19 @@ def main():
             Tc = wc.targets
             mc = len(Tc)
             holes = None if wc.hole is None else tt(wc.hole.astype(np.int32))
-            e.set_samples(tt(wc.x), tt(wc.y), tt(wc.z), tt(wc.value), None, holes)
+            ec = Engine(0)  # each workload on a context of its own (its own tuning)
+            ec.set_samples(tt(wc.x), tt(wc.y), tt(wc.z), tt(wc.value), None, holes)
             cx, cy, cz = tt(Tc[:, 0]), tt(Tc[:, 1]), tt(Tc[:, 2])
             blk = getattr(wc, "block", None)
             tuned_c = None
             if not args.no_autotune and blk is None:
-                tuned_c = e.autotune(wc.variogram, wc.search, cx, cy, cz, precision=_abi.FP32_IR, sample=65536)
+                tuned_c = ec.autotune(wc.variogram, wc.search, cx, cy, cz, precision=_abi.FP32_IR, sample=65536)
             res = {}
             for prec in (_abi.FP32_IR, _abi.FP64):
                 def one():
                     if blk is None:
-                        return e.krige(wc.variogram, wc.search, cx, cy, cz, precision=prec, out=res.get(prec))
-                    return e.block_krige(wc.variogram, wc.search, blk, cx, cy, cz, precision=prec)
+                        return ec.krige(wc.variogram, wc.search, cx, cy, cz, precision=prec, out=res.get(prec))
+                    return ec.block_krige(wc.variogram, wc.search, blk, cx, cy, cz, precision=prec)
                 res[prec] = one()
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -320,7 +321,8 @@ def main():
                 "refine_limit": tuned_c["refine_limit"] if tuned_c else None}
             print(f"[bench] {name}: {configs[name]['value']:.0f} targets/s FP32+IR, "
                   f"{configs[name]['fp64']['value']:.0f} FP64", file=sys.stderr, flush=True)
-        e.set_samples(sx, sy, sz, sv, None, hole)
+            ec.close()
+            del res
 
     cpu = None
     if not args.no_cpu:
