@@ -95,3 +95,10 @@ def test_estimator_diagnostics_table(c1):
     np.testing.assert_array_equal(tab["efficiency"][ok], ((sv - var) / sv)[ok])
     np.testing.assert_array_equal(tab["estimate_variance"][ok], ((sv - var - lag) - lag)[ok])
     assert np.isnan(tab["target_met"]).all()
+
+
+def test_no_targets(engine, c1):
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    g = engine.krige_diagnostics(w.variogram, w.search, [], [], [])
+    assert all(len(v) == 0 for v in g.values())

@@ -272,3 +272,14 @@ def test_estimator_passes_diagnostics_and_cross_validation():
     ok = ~np.isnan(cv.pit)
     assert ok.all() and np.all((cv.pit >= 0) & (cv.pit <= 1))
     assert cv.brier.shape == (6,) and np.all((cv.brier >= 0) & (cv.brier <= 1)) and 0.0 < cv.goodness <= 1.0
+
+
+def test_no_targets(engine):
+    d = data(50, 2)
+    thr = [0.5, 1.0]
+    vg = [single("spherical", 1.0, 30.0)]
+    e = engine_with(engine, d)
+    out = e.mik_predict(mik(thr, 1), thr, vg, search(25.0), [], [], [], cutoffs=[1.0], quantiles=[0.5])
+    assert out["cdf"].shape == (2, 0) and out["status"].shape == (0,)
+    cv = e.mik_cross_validate(mik(thr, 1), thr, vg, search(25.0), which=np.zeros(0, np.int64))
+    assert cv["pit"].shape == (0,)
