@@ -39,11 +39,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from .errors import InvalidInput
+from .errors import FileError, InvalidInput
 
 __all__ = [
     "BlockKriging",
     "CrossValidation",
+    "FileError",
     "IndicatorCrossValidation",
     "IndicatorKriging",
     "IndicatorSummary",
@@ -152,6 +153,23 @@ class Variogram:
     def __repr__(self):
         return f"Variogram({self.structures!r}, nugget={self.nugget})"
 
+    def to_json(self) -> str:
+        """JSON of the model (crates/py/src/variogram.rs:569-579)."""
+        from . import persist
+
+        return persist.envelope("Variogram", persist.variogram_fields(self))
+
+    @staticmethod
+    def from_json(text: str) -> "Variogram":
+        """Reads `to_json` output; raises InvalidInput on another class's JSON
+        or a newer format."""
+        from . import persist
+
+        return persist.variogram_from(persist.open_envelope("Variogram", text))
+
+    def __reduce__(self):
+        return Variogram.from_json, (self.to_json(),)
+
 
 class Search:
     """Search neighbourhood (crates/py/src/estimation.rs:415-531): `radius`, or
@@ -219,6 +237,23 @@ class Search:
 
     def __repr__(self):
         return f"Search(radius={self.radius}, max_samples={self.max_samples}, min_samples={self.min_samples})"
+
+    def to_json(self) -> str:
+        """JSON of the parameters (crates/py/src/estimation.rs:398-412)."""
+        from . import persist
+
+        return persist.envelope("Search", persist.search_fields(self))
+
+    @staticmethod
+    def from_json(text: str) -> "Search":
+        """Reads `to_json` output; raises InvalidInput on another class's JSON
+        or a newer format."""
+        from . import persist
+
+        return persist.search_from(persist.open_envelope("Search", text))
+
+    def __reduce__(self):
+        return Search.from_json, (self.to_json(),)
 
 
 @dataclass(frozen=True)
@@ -392,6 +427,8 @@ class _Base:
         self._device = device
         self._engine = None
         self._values = None
+        self._samples = None  # the fitted samples on the host: x, y, z, value, hole, error_variance
+        self._uploaded = False
         self._param = 0.0
 
     @property
@@ -411,6 +448,27 @@ class _Base:
 
             self._engine = Engine(self._device)
         return self._engine
+
+    def _ready(self):
+        """The device engine holding the fitted samples (uploaded on first use
+        after a restore from a file or a pickle)."""
+        eng = self._device_engine()
+        if not self._uploaded:
+            s = self._samples
+            hole = s["hole"]
+            ids = None
+            if not np.isnan(hole).all():
+                # A sample without a code (null) counts as a hole of its own
+                # (crates/estimation/src/neighborhood.rs:85-92).
+                ids = np.empty(len(hole), dtype=np.int32)
+                tagged = ~np.isnan(hole)
+                ids[tagged] = hole[tagged].astype(np.int32)
+                top = int(hole[tagged].max()) + 1 if tagged.any() else 0
+                ids[~tagged] = top + np.arange(int((~tagged).sum()), dtype=np.int32)
+            err = s["error_variance"]
+            eng.set_samples(s["x"], s["y"], s["z"], s["value"], err if (err != 0.0).any() else None, ids)
+            self._uploaded = True
+        return eng
 
     def fit(self, coords, values, *, holes=None, error_variance=None, domains=None, domain_column=None):
         """Stores the samples; samples sharing a location keep the first one."""
@@ -437,9 +495,15 @@ class _Base:
         xyz, v = xyz[keep], v[keep]
         err = None if err is None else err[keep]
         ids = None if ids is None else ids[keep]
-        eng = self._device_engine()
-        eng.set_samples(xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy(), v, err, ids)
+        # Host copy of the fitted samples (crates/py/src/estimation.rs:1110-1121):
+        # hole codes (an unlabelled sample has a code of its own), null without
+        # holes; error variance 0 by default.
+        hole = np.full(len(v), np.nan) if ids is None else ids.astype(np.float64)
+        self._samples = dict(x=xyz[:, 0].copy(), y=xyz[:, 1].copy(), z=xyz[:, 2].copy(), value=v, hole=hole,
+                             error_variance=np.zeros(len(v)) if err is None else err)
         self._values = v
+        self._uploaded = False
+        self._ready()
         return self
 
     def _one_pass(self, s, T):
@@ -459,6 +523,7 @@ class _Base:
         if self._values is None:
             raise InvalidInput("call fit first")
         T = coords_of(targets)
+        self._ready()
         if diagnostics:
             return self._diagnostics(T)
         value, variance, _ = run_passes(len(T), self._passes, lambda s, rows: self._one_pass(s, T[rows]))
@@ -522,6 +587,7 @@ class _Base:
             raise NotImplementedError("k-fold cross-validation is not supported by this engine")
         if self._values is None:
             raise InvalidInput("call fit first")
+        self._ready()
 
         def one(s, rows):
             out = self._engine.loo(self._variogram._c, s._c, rows.astype(np.int64), self._kind, self._param,
@@ -530,6 +596,40 @@ class _Base:
 
         value, variance, _ = run_passes(len(self._values), self._passes, one)
         return CrossValidation(self._values.copy(), value, variance)
+
+
+    def to_parquet(self, path) -> None:
+        """Writes the samples as Parquet columns and the parameters as JSON in
+        the file metadata (python/boitata/estimation.py:377-384)."""
+        from . import persist
+
+        persist.to_parquet(self, path, type(self).__name__)
+
+    @classmethod
+    def from_parquet(cls, path, **engine):
+        """Reads `to_parquet` output; predictions match the saved estimator's
+        bit for bit. Raises InvalidInput if the file holds another class or a
+        newer format, FileError if it cannot be read
+        (python/boitata/estimation.py:386-401). `engine` takes this engine's
+        own options (precision, device)."""
+        from . import persist
+
+        return persist.from_parquet(cls, path, **engine)
+
+    def __reduce__(self):
+        # The pickle carries the file's JSON and columns (crates/py/src/persist.rs:90-112)
+        # and this engine's own options.
+        from . import persist
+
+        meta, cols = persist.state(self, type(self).__name__)
+        precision = next(k for k, v in _PRECISIONS.items() if v == self._precision)
+        return _unpickle, (type(self), meta, cols, {"precision": precision, "device": self._device})
+
+
+def _unpickle(cls, meta, columns, engine):
+    from . import persist
+
+    return persist.from_state(cls, meta, columns, **engine)
 
 
 class OrdinaryKriging(_Base):
