@@ -794,6 +794,7 @@ class MultipleIndicatorKriging:
         self._weights = None
         self._holes = None
         self._xyz = None
+        self._uploaded = False
 
     def _validate(self):
         # indicator.rs:291-317
@@ -845,17 +846,47 @@ class MultipleIndicatorKriging:
         xyz, v = xyz[keep], v[keep]
         w = None if w is None else w[keep]
         ids = None if ids is None else ids[keep]
-        if self._engine is None:
-            from .engine import Engine
-
-            self._engine = Engine(self._device)
-        self._engine.set_samples(xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy(), v, None, ids)
-        self._values, self._weights, self._holes, self._xyz = v, w, ids, xyz
+        self._values, self._weights, self._holes, self._xyz = v, w, ids, np.ascontiguousarray(xyz)
+        self._uploaded = False
+        self._fitted()
         return self
 
     def _fitted(self):
+        """Raises before fit; uploads the fitted samples on first use (after a
+        restore from a file or a pickle)."""
         if self._values is None:
             raise InvalidInput("call fit first")
+        if not self._uploaded:
+            if self._engine is None:
+                from .engine import Engine
+
+                self._engine = Engine(self._device)
+            x = self._xyz
+            self._engine.set_samples(x[:, 0].copy(), x[:, 1].copy(), x[:, 2].copy(), self._values, None, self._holes)
+            self._uploaded = True
+
+    def to_parquet(self, path) -> None:
+        """Writes the samples (and declustering weights) as Parquet columns and
+        the parameters as JSON in the file metadata (crates/py/src/indicator.rs:505-525)."""
+        from . import persist
+
+        persist.to_parquet(self, path, type(self).__name__)
+
+    @classmethod
+    def from_parquet(cls, path, **engine):
+        """Reads `to_parquet` output; predictions match the saved estimator's
+        bit for bit. `engine` takes this engine's own options (precision,
+        device)."""
+        from . import persist
+
+        return persist.from_parquet(cls, path, **engine)
+
+    def __reduce__(self):
+        from . import persist
+
+        meta, cols = persist.state(self, type(self).__name__)
+        precision = next(k for k, v in _PRECISIONS.items() if v == self._precision)
+        return _unpickle, (type(self), meta, cols, {"precision": precision, "device": self._device})
 
     def predict(self, targets, *, cutoffs=None, quantiles=(), anisotropy=None, diagnostics=False,
                 discretization=None, block_size=None) -> IndicatorSummary:

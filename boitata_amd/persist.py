@@ -18,6 +18,10 @@ reference's formats (crates/py/src/persist.rs):
   crates/io/src/parquet.rs:487-497).
 * A pickle carries the same JSON and columns (persist.rs:90-112).
 
+Multiple indicator kriging is written as crates/py/src/indicator.rs:115-125
+lays it out (the MultipleIndicator model, crates/estimation/src/indicator.rs:
+34-65, and the passes), its declustering weights as a ``weight`` column.
+
 Extensions of this engine with no reference field are written only when
 used: a structure's own ``ratios`` (the per-structure anisotropy ratios).
 """
@@ -317,8 +321,100 @@ def estimator_from(cls, fields: dict, **engine):
     raise InvalidInput(f"method {json.dumps(m)} is not that of a {cls.__name__}")
 
 
+# --- multiple indicator kriging (crates/py/src/indicator.rs:115-125,505-525;
+# crates/estimation/src/indicator.rs:34-65)
+
+_INTERP_NAMES = {"global": "Global", "linear": "Linear"}
+_TAIL_NAMES = {"power": "Power", "hyperbolic": "Hyperbolic"}
+
+
+def mik_fields(est) -> dict:
+    """The MultipleIndicator model and the search passes."""
+    from . import _abi
+
+    p = est._mik
+    interp = next(k for k, v in _abi_interp().items() if v == p.interpolation)
+    tail = None
+    if p.upper_tail != _abi.TAIL_NONE:
+        name = next(k for k, v in _abi_tails().items() if v == p.upper_tail)
+        tail = {_TAIL_NAMES[name]: float(p.omega)}
+    model = {"thresholds": [float(t) for t in est._thresholds],
+             "variograms": [variogram_fields(v) for v in est._variograms], "simple": bool(p.simple),
+             "tails": [float(p.tails[0]), float(p.tails[1])] if p.has_tails else None,
+             "interpolation": _INTERP_NAMES[interp], "upper_tail": tail}
+    return {"model": model, "search": [search_fields(s) for s in est._passes]}
+
+
+def _abi_interp():
+    from .estimation import _INTERPOLATIONS
+
+    return _INTERPOLATIONS
+
+
+def _abi_tails():
+    from .estimation import _TAILS
+
+    return _TAILS
+
+
+def mik_from(cls, fields: dict, **engine):
+    if fields.get("unit") is not None:
+        raise NotImplementedError("units are not supported by this engine")
+    m = fields.get("model")
+    if not isinstance(m, dict):
+        raise InvalidInput("model: expected an object")
+    thr = m.get("thresholds")
+    if not isinstance(thr, list):
+        raise InvalidInput("thresholds: expected a list")
+    vgs = m.get("variograms")
+    if not isinstance(vgs, list) or not vgs:
+        raise InvalidInput("variograms: expected a list")
+    interp = {v: k for k, v in _INTERP_NAMES.items()}.get(m.get("interpolation"))
+    if interp is None:
+        raise InvalidInput(f"unknown interpolation {json.dumps(m.get('interpolation'))}")
+    tail = m.get("upper_tail")
+    upper = None
+    if tail is not None:
+        names = {v: k for k, v in _TAIL_NAMES.items()}
+        if not isinstance(tail, dict) or len(tail) != 1 or next(iter(tail)) not in names:
+            raise InvalidInput(f"unknown upper tail {json.dumps(tail)}")
+        k, w = next(iter(tail.items()))
+        upper = (names[k], _parse_float(w, "upper_tail"))
+    tails = m.get("tails")
+    if tails is not None:
+        if not isinstance(tails, list) or len(tails) != 2:
+            raise InvalidInput("tails: expected a pair")
+        tails = (_parse_float(tails[0], "tails"), _parse_float(tails[1], "tails"))
+    simple = m.get("simple", False)
+    if not isinstance(simple, bool):
+        raise InvalidInput("simple: expected true or false")
+    passes = fields.get("search")
+    if not isinstance(passes, list) or not passes:
+        raise InvalidInput("search: expected a list of searches")
+    return cls([variogram_from(v) for v in vgs], [search_from(s) for s in passes],
+               [_parse_float(t, "thresholds") for t in thr], simple=simple, tails=tails, interpolation=interp,
+               upper_tail=upper, **engine)
+
+
+def _is_mik(est) -> bool:
+    return hasattr(est, "_mik")
+
+
 def sample_columns(est) -> dict | None:
-    """crates/py/src/estimation.rs:1110-1121; None before fit."""
+    """crates/py/src/estimation.rs:1110-1121 (and the declustering weights of
+    multiple indicator kriging, crates/py/src/indicator.rs:510-518); None
+    before fit."""
+    if _is_mik(est):
+        if est._values is None:
+            return None
+        n = len(est._values)
+        x = est._xyz
+        cols = {"x": x[:, 0].copy(), "y": x[:, 1].copy(), "z": x[:, 2].copy(), "value": est._values,
+                "hole": np.full(n, np.nan) if est._holes is None else est._holes.astype(np.float64),
+                "error_variance": np.zeros(n), "domain": np.full(n, np.nan)}
+        if est._weights is not None:
+            cols["weight"] = est._weights
+        return cols
     s = est._samples
     if s is None:
         return None
@@ -351,15 +447,33 @@ def restore(est, columns: dict) -> None:
         raise InvalidInput("hole codes must be indices")
     if not np.isnan(dom).all():
         raise NotImplementedError("domains are not supported by this engine")
-    est._samples = dict(x=x, y=y, z=z, value=v, hole=hole, error_variance=err)
-    est._values = v
+    if _is_mik(est):
+        if (err != 0.0).any():
+            raise InvalidInput("multiple indicator kriging takes no error_variance")
+        ids = None
+        if not np.isnan(hole).all():
+            # A sample without a code counts as a hole of its own.
+            ids = np.empty(n, dtype=np.int32)
+            tagged = ~np.isnan(hole)
+            ids[tagged] = hole[tagged].astype(np.int32)
+            top = int(hole[tagged].max()) + 1 if tagged.any() else 0
+            ids[~tagged] = top + np.arange(int((~tagged).sum()), dtype=np.int32)
+        w = _values(columns, "weight") if "weight" in columns else None
+        if w is not None and (len(w) != n or (w < 0).any() or not w.sum() > 0):
+            raise InvalidInput("weights must be >= 0 and not all 0")
+        est._values, est._weights, est._holes = v, w, ids
+        est._xyz = np.ascontiguousarray(np.stack([x, y, z], 1))
+    else:
+        est._samples = dict(x=x, y=y, z=z, value=v, hole=hole, error_variance=err)
+        est._values = v
     est._uploaded = False
 
 
 def state(est, name: str):
     """persist.rs:90-95: the JSON with `fitted`, and the columns (None unfitted)."""
     cols = sample_columns(est)
-    return envelope(name, estimator_fields(est), {"fitted": cols is not None}), cols
+    fields = mik_fields(est) if _is_mik(est) else estimator_fields(est)
+    return envelope(name, fields, {"fitted": cols is not None}), cols
 
 
 def from_state(cls, meta: str, columns, **engine):
@@ -370,7 +484,7 @@ def from_state(cls, meta: str, columns, **engine):
             raise InvalidInput("columns differ in length")
     fields = open_envelope(cls.__name__, meta)
     fitted = fields.pop("fitted", None) is True
-    est = estimator_from(cls, fields, **engine)
+    est = mik_from(cls, fields, **engine) if hasattr(cls, "_validate") else estimator_from(cls, fields, **engine)
     if fitted and columns is not None:
         restore(est, columns)
     return est
@@ -384,10 +498,11 @@ def to_parquet(est, path, name: str) -> None:
 
     meta, cols = state(est, name)
     cols = cols or {k: np.zeros(0) for k in SAMPLE_COLUMNS}
+    names = list(cols)
     arrays = [pa.array(np.asarray(cols[k], dtype=np.float64), type=pa.float64(),
                        mask=np.isnan(np.asarray(cols[k], dtype=np.float64)) if k in ("hole", "domain") else None)
-              for k in SAMPLE_COLUMNS]
-    schema = pa.schema([pa.field(k, pa.float64(), nullable=True) for k in SAMPLE_COLUMNS], metadata={KEY: meta})
+              for k in names]
+    schema = pa.schema([pa.field(k, pa.float64(), nullable=True) for k in names], metadata={KEY: meta})
     try:
         pq.write_table(pa.Table.from_arrays(arrays, schema=schema), os.fspath(path))
     except OSError as e:

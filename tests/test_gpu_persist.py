@@ -75,3 +75,36 @@ def test_round_trip_predicts_bit_identically(c1, tmp_path, precision, k):
     for back in (type(est).from_parquet(path, precision=precision), pickle.loads(pickle.dumps(est))):
         assert type(back) is type(est)
         same(outputs(back, T, cv), ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_multiple_indicator_round_trip(c1, tmp_path, precision):
+    # python/tests/test_persist.py:180-190: the conditional distributions, the
+    # diagnostics table and leave-one-out with the PIT, after a reload.
+    from boitata_amd.estimation import MultipleIndicatorKriging
+
+    w, P, holes, _, T = c1
+    print(f"[persist] MultipleIndicatorKriging {precision}", flush=True)
+    thr = list(np.quantile(w.value, [0.2, 0.4, 0.6, 0.8]))
+    vgs = [Variogram([("spherical", 0.2, 100.0 + 20 * k)], nugget=0.05) for k in range(4)]
+    weights = 1.0 + (np.arange(len(w.x)) % 3)
+    est = MultipleIndicatorKriging(vgs, PASSES, thr, tails=(0.0, float(w.value.max()) * 1.5),
+                                   upper_tail=("hyperbolic", 1.5), precision=precision)
+
+    def run(e):
+        s = e.predict(T, cutoffs=[thr[1], thr[3]], quantiles=[0.1, 0.5, 0.9])
+        d = e.predict(T, cutoffs=[thr[1]], diagnostics=True).diagnostics
+        c = e.cross_validate()
+        return ([s.cdf, s.correction, s.mean, s.variance, s.probability_above, s.mean_above, s.quantile_values]
+                + [d[k] for k in d.column_names] + [c.estimate, c.variance, c.cdf, c.pit])
+
+    path = tmp_path / "mik.parquet"
+    est.to_parquet(path)
+    unfitted = MultipleIndicatorKriging.from_parquet(path, precision=precision)
+    est.fit(P, w.value, weights=weights, holes=holes)
+    ref = run(est)
+    assert np.isfinite(ref[2]).mean() > 0.9
+    same(run(unfitted.fit(P, w.value, weights=weights, holes=holes)), ref)
+    est.to_parquet(path)
+    for back in (MultipleIndicatorKriging.from_parquet(path, precision=precision), pickle.loads(pickle.dumps(est))):
+        same(run(back), ref)

@@ -190,3 +190,42 @@ def test_file_errors(tmp_path):
     pq.write_table(t, path)
     with pytest.raises(InvalidInput, match="null in column value"):
         OrdinaryKriging.from_parquet(path)
+
+
+def test_multiple_indicator_layout_and_reference_file(tmp_path):
+    from boitata_amd.estimation import MultipleIndicatorKriging
+
+    vg = Variogram([("spherical", 1.0, 30.0)])
+    m = MultipleIndicatorKriging(vg, Search(25.0, max_samples=12), [0.5, 1.0], simple=True, tails=(0.0, 10.0),
+                                 upper_tail=("power", 0.5))
+    path = tmp_path / "mik.parquet"
+    m.to_parquet(path)
+    text = pq.read_table(path).schema.metadata[b"boitata"].decode()
+    # crates/py/src/indicator.rs:115-125 with crates/estimation/src/indicator.rs:34-65.
+    assert text == ('{"type":"MultipleIndicatorKriging","format":1,"fitted":false,"model":{"thresholds":[0.5,1.0],'
+                    '"variograms":[{"nugget":0.0,"structures":[{"model":"Spherical","sill":1.0,"range":30.0}],'
+                    '"anisotropy":null}],"simple":true,"tails":[0.0,10.0],"interpolation":"Global",'
+                    '"upper_tail":{"Power":0.5}},"search":[{"min_samples":1,"max_samples":12,"radius":25.0,'
+                    '"max_per_hole":null,"octant":false,"anisotropy":null,"high_grade":null}]}')
+    # A fitted file with declustering weights, read and written again.
+    meta = json.loads(text)
+    meta["fitted"] = True
+    n = 30
+    rng = np.random.default_rng(5)
+    cols = {"x": rng.uniform(0, 50, n), "y": rng.uniform(0, 50, n), "z": np.zeros(n), "value": rng.lognormal(0, 1, n),
+            "hole": (np.arange(n) // 3).astype(np.float64), "error_variance": np.zeros(n),
+            "domain": np.full(n, np.nan), "weight": rng.uniform(0.5, 2.0, n)}
+    arrays = [pa.array(v, mask=np.isnan(v) if k == "domain" else None) for k, v in cols.items()]
+    schema = pa.schema([pa.field(k, pa.float64()) for k in cols], metadata={"boitata": json.dumps(meta)})
+    pq.write_table(pa.Table.from_arrays(arrays, schema=schema), path)
+    back = MultipleIndicatorKriging.from_parquet(path)
+    assert np.array_equal(back._weights, cols["weight"]) and np.array_equal(back._holes, np.arange(n) // 3)
+    again = tmp_path / "again.parquet"
+    back.to_parquet(again)
+    a, b = pq.read_table(path), pq.read_table(again)
+    assert a.column_names == b.column_names
+    for k in a.column_names:
+        assert a[k].equals(b[k]), k
+    assert json.loads(b.schema.metadata[b"boitata"]) == meta
+    p = pickle.loads(pickle.dumps(back))
+    assert np.array_equal(p._xyz, back._xyz) and bytes(p._mik) == bytes(m._mik)
