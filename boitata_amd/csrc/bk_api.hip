@@ -100,7 +100,14 @@ struct bk_ctx {
   GridDev grid{};
   bool grid_valid = false;
   double grid_F[9] = {0};
-  double cell_factor = 0.65;  // grid side as a factor of the reference's rule (measured best for the wave search)
+  // Grid side as a factor of the reference's rule, for searches without caps
+  // (measured best for the wave search) and for capped searches (octants,
+  // sectors or a hole cap: whose walk to the radius around targets with an
+  // unfilled octant costs per cell visited, measured best at 1.0 - 1.4 on the
+  // uneven 4M-target octant workload). grid_cf: the factor the grid was built with.
+  double cell_factor = 0.65;
+  double cell_factor_capped = 1.0;
+  double grid_cf = 0.0;
   ShellTable tab{};
   void* tab_storage = nullptr;
   // Scratch (grow-only).
@@ -502,13 +509,16 @@ int prepare_search(bk_ctx* c, const bk_search* s, const bk_variogram* vg, Search
   a.sz = (const double*)c->z.p;
   a.hole = c->has_hole ? (const int32_t*)c->hole.p : nullptr;
   a.tab = c->tab;
-  // Grid over the projected samples, rebuilt when the frame changes.
+  // Grid over the projected samples, rebuilt when the frame or the cell
+  // factor of the search's kind changes.
   if (c->n > 0) {
-    bool same = c->grid_valid && std::memcmp(c->grid_F, F.m, sizeof(F.m)) == 0;
+    const double cf = (a.balances || a.max_per_hole >= 0) ? c->cell_factor_capped : c->cell_factor;
+    bool same = c->grid_valid && c->grid_cf == cf && std::memcmp(c->grid_F, F.m, sizeof(F.m)) == 0;
     if (!same) {
-      int r = bk::build_grid(c->stream, c->n, a.sx, a.sy, a.sz, F.m, a.origin, c->cell_factor, c->gb, c->grid);
+      int r = bk::build_grid(c->stream, c->n, a.sx, a.sy, a.sz, F.m, a.origin, cf, c->gb, c->grid);
       if (r) return fail(c, BK_E_HIP, "grid build failed");
       std::memcpy(c->grid_F, F.m, sizeof(F.m));
+      c->grid_cf = cf;
       c->grid_valid = true;
       c->gen++;
     }
@@ -1848,8 +1858,8 @@ int bk_set_tuning(bk_ctx* c, const int32_t* edges3, int32_t max_iter, double tol
   if (max_iter > 0) c->max_iter = max_iter;
   if (tol > 0) c->tol = tol;
   c->gen++;
-  if (cell_factor > 0 && cell_factor != c->cell_factor) {
-    c->cell_factor = cell_factor;
+  if (cell_factor > 0) {
+    c->cell_factor = c->cell_factor_capped = cell_factor;
     c->grid_valid = false;
   }
   return 0;
@@ -1971,28 +1981,30 @@ int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t k
   };
   double def_search = 0.0, def_solve = 0.0;
   if (int r = trial(def_search, def_solve)) return r;
-  // The search: grid cell size and the uncapped search's mode.
-  const double cfs[] = {0.45, 0.55, 0.65, 0.8, 1.0};
+  // The search: the grid cell size of this search's kind (capped or not)
+  // and, without caps, the search's mode.
+  const bool capped = s->octant || s->sectors > 0 || s->max_per_hole >= 0;
+  double& knob = capped ? c->cell_factor_capped : c->cell_factor;
+  const double cfs[] = {0.45, 0.55, 0.65, 0.8, 1.0, 1.4};
   const int modes[] = {0, -1, 2};
-  double best_cf = c->cell_factor, best_search = def_search;
+  const int n_modes = capped ? 1 : 3;
+  double best_cf = knob, best_search = def_search;
   int best_mode = c->search_mode;
   for (double cf : cfs)
-    for (int mode : modes) {
-      if (cf != c->cell_factor) {
-        c->cell_factor = cf;
-        c->grid_valid = false;
-      }
-      c->search_mode = mode;
+    for (int mi = 0; mi < n_modes; ++mi) {
+      knob = cf;
+      c->grid_valid = false;
+      if (!capped) c->search_mode = modes[mi];
       double sm = 0.0, so = 0.0;
       if (int r = trial(sm, so)) return r;
       if (sm < best_search) {
         best_search = sm;
         best_cf = cf;
-        best_mode = mode;
+        best_mode = c->search_mode;
       }
     }
-  if (best_cf != c->cell_factor) c->grid_valid = false;
-  c->cell_factor = best_cf;
+  knob = best_cf;
+  c->grid_valid = false;
   c->search_mode = best_mode;
   // The solves: workgroup size, bucket edges and, for FP32+IR, the largest
   // bucket the refined kernel takes (the rest go to the FP64 kernel).

@@ -83,7 +83,7 @@ __device__ __forceinline__ int clamp_cell(double v, int dim) {
 struct SearchLds {
   uint64_t pk[BK_POOL];
   uint32_t pi[BK_POOL];
-  int ps[BK_POOL];
+  int16_t ps[BK_POOL];  // sector (< 360)
   int ph[BK_POOL];
   uint64_t bk[BK_WAVE];
   uint32_t bi[BK_WAVE];
@@ -92,7 +92,7 @@ struct SearchLds {
   int chosen[BK_MAX_SEARCH];
   int hole_id[BK_MAX_SEARCH];
   int hole_cnt[BK_MAX_SEARCH];
-  int per_sector[360];
+  int16_t per_sector[360];  // selected per sector (<= max_samples)
   uint64_t thr_sec[8];
   int scalar[4];
 };
@@ -159,7 +159,7 @@ __device__ __forceinline__ int pool_compact(SearchLds& L, int P, int lane, Keep 
     if (kp) {
       L.pk[pos] = k;
       L.pi[pos] = ii;
-      L.ps[pos] = s;
+      L.ps[pos] = (int16_t)s;
       L.ph[pos] = h;
     }
     wave_sync();
@@ -225,9 +225,15 @@ struct Ctx {
       for (int o = 0; o < 8; ++o) running[o] = 0;
       int sure = 0;
       const uint64_t lt = (1ull << lane) - 1ull;
-      const SearchLds* Lc = L;
+      SearchLds* Lc = L;
       const int cap = a->sector_cap;
-      st.P = pool_compact(*L, st.P, lane, [&](bool in, int, int s) {
+      // Thresholds: an octant holding all it may still take makes larger keys
+      // of that octant useless; its threshold is the key of its last kept
+      // entry, whose rank is what is left of its cap minus one (the kept
+      // entries of an octant are its ranks 0, 1, ... in pool order).
+      if (lane < 8) L->thr_sec[lane] = ~0ull;
+      wave_sync();
+      st.P = pool_compact(*L, st.P, lane, [&](bool in, int j, int s) {
         int rank = 0;
         for (int o = 0; o < nsec; ++o) {
           uint64_t mask = __ballot(in && s == o);
@@ -239,23 +245,12 @@ struct Ctx {
         uint64_t km = __ballot(k);
         int before = sure + __popcll(km & lt);
         sure += __popcll(km);
-        return k && before < rem;
+        bool kept = k && before < rem;
+        if (kept && rank == cap - Lc->per_sector[s] - 1) Lc->thr_sec[s] = Lc->pk[j];
+        return kept;
       });
-      // Thresholds: an octant holding all it may still take, or a pool of
-      // `rem` sure picks, makes larger keys useless.
-      if (lane == 0) {
-        int cnt[8];
-        for (int o = 0; o < 8; ++o) {
-          cnt[o] = 0;
-          L->thr_sec[o] = ~0ull;
-        }
-        for (int j = 0; j < st.P; ++j) {
-          int s = L->ps[j];
-          cnt[s]++;
-          if (cnt[s] == a->sector_cap - L->per_sector[s]) L->thr_sec[s] = L->pk[j];
-        }
-      }
       wave_sync();
+      // A pool of `rem` sure picks makes larger keys useless.
       if (st.P == rem && rem > 0) {
         st.thr = L->pk[st.P - 1];
         st.thr_i = L->pi[st.P - 1];
@@ -303,7 +298,7 @@ struct Ctx {
       if (pos >= 0) {
         L->pk[pos] = k;
         L->pi[pos] = ii;
-        L->ps[pos] = s;
+        L->ps[pos] = (int16_t)s;
         L->ph[pos] = h;
       }
       wave_sync();
@@ -311,7 +306,7 @@ struct Ctx {
     if (bpos >= 0) {
       L->pk[bpos] = key;
       L->pi[bpos] = idx;
-      L->ps[bpos] = sec;
+      L->ps[bpos] = (int16_t)sec;
       L->ph[bpos] = hl;
     }
     wave_sync();
@@ -382,6 +377,25 @@ struct Ctx {
       int take = min(r, a->max_samples - st.n);
       for (int k = lane; k < take; k += BK_WAVE) L->chosen[st.n + k] = (int)L->pi[k];
       st.n += take;
+    } else if (a->octant_rank) {
+      // Octants without a hole cap: after prune every pooled entry is a sure
+      // pick (its rank in its octant is below what is left of the octant's
+      // cap), so the released entries are taken in order up to max_samples,
+      // as the one-by-one Selector::offer would take them.
+      int take = min(r, a->max_samples - st.n);
+      const int nsec = a->n_sectors;
+      for (int base = 0; base < take; base += BK_WAVE) {
+        int e = base + lane;
+        bool in = e < take;
+        int s = in ? L->ps[e] : -1;
+        if (in) L->chosen[st.n + e] = (int)L->pi[e];
+        for (int o = 0; o < nsec; ++o) {
+          int add = __popcll(__ballot(in && s == o));
+          if (lane == 0 && add) L->per_sector[o] = (int16_t)(L->per_sector[o] + add);
+        }
+      }
+      wave_sync();
+      st.n += take;
     } else {
       if (lane == 0) {
         int n = st.n, nh = st.n_holes, un = st.untagged;
@@ -398,7 +412,7 @@ struct Ctx {
           if (a->balances) {
             int s = L->ps[e];
             if (L->per_sector[s] >= a->sector_cap) continue;
-            L->per_sector[s] += 1;
+            L->per_sector[s] = (int16_t)(L->per_sector[s] + 1);
           }
           if (hl >= 0) {
             if (slot >= 0)

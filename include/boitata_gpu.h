@@ -293,8 +293,11 @@ int bk_mik_cross_validate(bk_ctx* ctx, const bk_mik* mik, const double* threshol
 /* Tuning of the device path (no counterpart in the reference): bucket edges of
  * the neighbour counts (three of 16, 32, 64, rising to 64; NULL keeps them),
  * FP32+IR refinement sweeps and relative tolerance, and the grid cell size as
- * a factor of the reference's side rule (search/grid.rs:83-106). Values <= 0
- * keep the current setting. Results do not depend on the cell factor. */
+ * a factor of the reference's side rule (search/grid.rs:83-106), set for both
+ * kinds of search (the defaults are 0.65 without caps and 1.0 for searches
+ * with octants, sectors or a hole cap; bk_autotune tunes the kind of the
+ * search it is given). Values <= 0 keep the current setting. Results do not
+ * depend on the cell factor. */
 int bk_set_tuning(bk_ctx* ctx, const int32_t* edges3, int32_t max_iter, double tol, double cell_factor);
 
 /* Search tuning (no counterpart in the reference) for searches without caps
