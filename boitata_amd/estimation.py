@@ -598,6 +598,26 @@ class _Base:
         return CrossValidation(self._values.copy(), value, variance)
 
 
+    @property
+    def unit(self):
+        """Unit of the fitted values (python/boitata/estimation.py:220-223);
+        None, as this engine takes no units."""
+        return None
+
+    def with_search(self, search):
+        """A copy of the estimator, fitted samples included, with `search` (one
+        search or passes) in place of its searches
+        (python/boitata/estimation.py:357-375)."""
+        passes = _searches(search)
+        if self._kind == _abi.SIMPLE and any(p._c.calibration == _abi.CALIBRATE_SLOPE for p in passes):
+            raise InvalidInput("simple kriging has a slope of regression of 1; calibrate it with target_efficiency")
+        est = type(self).__new__(type(self))
+        est.__dict__.update(self.__dict__)
+        est._passes = passes
+        est._engine = None  # a context of its own; the samples upload on first use
+        est._uploaded = False
+        return est
+
     def to_parquet(self, path) -> None:
         """Writes the samples as Parquet columns and the parameters as JSON in
         the file metadata (python/boitata/estimation.py:377-384)."""

@@ -108,3 +108,11 @@ def test_multiple_indicator_round_trip(c1, tmp_path, precision):
     est.to_parquet(path)
     for back in (MultipleIndicatorKriging.from_parquet(path, precision=precision), pickle.loads(pickle.dumps(est))):
         same(run(back), ref)
+
+
+def test_with_search_predicts_as_a_fresh_estimator(c1):
+    w, P, holes, err, T = c1
+    est = OrdinaryKriging(VG, PASSES).fit(P, w.value, holes=holes, error_variance=err)
+    s = Search(200.0, max_samples=10, min_samples=2)
+    fresh = OrdinaryKriging(VG, s).fit(P, w.value, holes=holes, error_variance=err)
+    same(est.with_search(s).predict(T, return_variance=True), fresh.predict(T, return_variance=True))

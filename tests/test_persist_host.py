@@ -229,3 +229,16 @@ def test_multiple_indicator_layout_and_reference_file(tmp_path):
     assert json.loads(b.schema.metadata[b"boitata"]) == meta
     p = pickle.loads(pickle.dumps(back))
     assert np.array_equal(p._xyz, back._xyz) and bytes(p._mik) == bytes(m._mik)
+
+
+def test_with_search_keeps_the_samples(tmp_path):
+    # python/boitata/estimation.py:357-375: a copy with other passes.
+    path = tmp_path / "model.parquet"
+    _, cols = reference_file(path, "SimpleKriging", {"Kriging": {"Simple": {"mean": 1.5}}})
+    est = SimpleKriging.from_parquet(path)
+    other = est.with_search(Search(55.0, max_samples=6))
+    assert type(other) is SimpleKriging and other._param == 1.5 and len(other._passes) == 1
+    assert other._samples is est._samples and other._engine is None and est.unit is None
+    assert len(est._passes) == 2
+    with pytest.raises(InvalidInput, match="calibrate it with target_efficiency"):
+        est.with_search(Search(55.0, target_slope=0.9))
