@@ -186,6 +186,10 @@ __device__ __forceinline__ int sector_at(const SearchArgs& a, double t0, double 
   return a.planar ? (bx << 1) | by : (bx << 2) | (by << 1) | bz;
 }
 
+// Batches up to this size are ranked by reading the others' keys lane by lane
+// rather than sorted across the wave.
+constexpr int kSmallBatch = 16;
+
 struct Ctx {
   const SearchArgs* a;
   SearchLds* L;
@@ -263,22 +267,49 @@ struct Ctx {
     uint64_t mask = __ballot(have);
     if (mask == 0) return;
     int nb = __popcll(mask);
-    if (!have) {
-      key = ~0ull;
-      idx = 0xffffffffu;
-      sec = 0;
-      hl = -1;
-    }
-    wave_sort(key, idx, sec, hl, lane);
-    if (st.P + nb > BK_POOL) {
-      st.overflow = 1;
-      return;
-    }
-    L->bk[lane] = key;
-    L->bi[lane] = idx;
-    wave_sync();
     // Final slot of each batch entry, against the pool before it moves.
-    int bpos = lane < nb ? lane + pool_lower(*L, st.P, key, idx) : -1;
+    int bpos = -1;
+    if (nb <= kSmallBatch) {
+      // Few candidates: each counts the batch entries below it (the others'
+      // keys read lane by lane) instead of a 64-lane sort; the ranks give the
+      // same sorted batch, entries staying in their lanes.
+      int rank = 0;
+      uint64_t m = mask;
+      for (int q = 0; q < nb; ++q) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, src);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(key >> 32), src);
+        const uint32_t iq = (uint32_t)__builtin_amdgcn_readlane((int)idx, src);
+        rank += key_less(((uint64_t)hi << 32) | lo, iq, key, idx) ? 1 : 0;
+      }
+      if (st.P + nb > BK_POOL) {
+        st.overflow = 1;
+        return;
+      }
+      if (have) {
+        L->bk[rank] = key;
+        L->bi[rank] = idx;
+      }
+      wave_sync();
+      if (have) bpos = rank + pool_lower(*L, st.P, key, idx);
+    } else {
+      if (!have) {
+        key = ~0ull;
+        idx = 0xffffffffu;
+        sec = 0;
+        hl = -1;
+      }
+      wave_sort(key, idx, sec, hl, lane);
+      if (st.P + nb > BK_POOL) {
+        st.overflow = 1;
+        return;
+      }
+      L->bk[lane] = key;
+      L->bi[lane] = idx;
+      wave_sync();
+      bpos = lane < nb ? lane + pool_lower(*L, st.P, key, idx) : -1;
+    }
     // Pool entries move up by the batch entries below them; chunks from the
     // top, so a destination never holds an entry not yet read.
     for (int base = ((st.P - 1) / BK_WAVE) * BK_WAVE; base >= 0 && st.P > 0; base -= BK_WAVE) {
