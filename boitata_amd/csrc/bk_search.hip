@@ -905,6 +905,41 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
     const uint32_t n_off = a.tab.end[a.tab.n - 1];
     int nbuf = 0;
     auto merge = [&]() {
+      if (nbuf <= kSmallBatch) {
+        // Few entries: every list entry and every batch entry counts the
+        // entries below it (the batch's keys read lane by lane) and is
+        // written at its rank in the merged list, in the buffer's upper half;
+        // the 64 smallest of the list and the batch, sorted, as the networks
+        // below give them.
+        const uint64_t mk = lane < nbuf ? bk[lane] : KEY_INF;
+        const uint32_t mi = lane < nbuf ? bi[lane] : 0xffffffffu;
+        int below = 0, posb = 0;
+        for (int q = 0; q < nbuf; ++q) {
+          const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, q);
+          const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mk >> 32), q);
+          const uint32_t iq = (uint32_t)__builtin_amdgcn_readlane((int)mi, q);
+          const uint64_t kq = ((uint64_t)hi << 32) | lo;
+          below += key_less(kq, iq, ak, ai) ? 1 : 0;
+          const int na = __popcll(__ballot(key_less(ak, ai, kq, iq)));
+          if (lane == q) posb += na;
+          if (lane < nbuf && key_less(kq, iq, mk, mi)) posb += 1;
+        }
+        const int posa = lane + below;
+        if (posa < BK_WAVE) {
+          bk[BK_WAVE + posa] = ak;
+          bi[BK_WAVE + posa] = ai;
+        }
+        if (lane < nbuf && posb < BK_WAVE) {
+          bk[BK_WAVE + posb] = mk;
+          bi[BK_WAVE + posb] = mi;
+        }
+        wave_sync();
+        ak = bk[BK_WAVE + lane];
+        ai = bi[BK_WAVE + lane];
+        wave_sync();
+        nbuf = 0;
+        return;
+      }
       uint64_t k = lane < nbuf ? bk[lane] : KEY_INF;
       uint32_t i = lane < nbuf ? bi[lane] : 0xffffffffu;
       wave_sort_keys(k, i, lane);
