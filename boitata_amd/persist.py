@@ -331,30 +331,19 @@ _TAIL_NAMES = {"power": "Power", "hyperbolic": "Hyperbolic"}
 def mik_fields(est) -> dict:
     """The MultipleIndicator model and the search passes."""
     from . import _abi
+    from .estimation import _INTERPOLATIONS, _TAILS
 
     p = est._mik
-    interp = next(k for k, v in _abi_interp().items() if v == p.interpolation)
+    interp = next(k for k, v in _INTERPOLATIONS.items() if v == p.interpolation)
     tail = None
     if p.upper_tail != _abi.TAIL_NONE:
-        name = next(k for k, v in _abi_tails().items() if v == p.upper_tail)
+        name = next(k for k, v in _TAILS.items() if v == p.upper_tail)
         tail = {_TAIL_NAMES[name]: float(p.omega)}
     model = {"thresholds": [float(t) for t in est._thresholds],
              "variograms": [variogram_fields(v) for v in est._variograms], "simple": bool(p.simple),
              "tails": [float(p.tails[0]), float(p.tails[1])] if p.has_tails else None,
              "interpolation": _INTERP_NAMES[interp], "upper_tail": tail}
     return {"model": model, "search": [search_fields(s) for s in est._passes]}
-
-
-def _abi_interp():
-    from .estimation import _INTERPOLATIONS
-
-    return _INTERPOLATIONS
-
-
-def _abi_tails():
-    from .estimation import _TAILS
-
-    return _TAILS
 
 
 def mik_from(cls, fields: dict, **engine):
@@ -527,7 +516,7 @@ def from_parquet(cls, path, **engine):
         raise InvalidInput("no boitata metadata")
     columns = {}
     for f, col in zip(table.schema, table.columns):
-        if not pa.types.is_floating(f.type) or f.type != pa.float64():
+        if f.type != pa.float64():
             raise InvalidInput(f"column {f.name} is not float64")
         columns[f.name] = col.to_numpy(zero_copy_only=False).astype(np.float64)
     return from_state(cls, meta.decode(), columns, **engine)
