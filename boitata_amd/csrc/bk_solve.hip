@@ -418,14 +418,14 @@ __device__ __forceinline__ void krige64_blk(const SolveArgs& a, int64_t nl, int 
     double d = group_bcast<NMAX>(s, j, g);
     if (act && (isnan(d) || d <= PIVOT * c0)) sing = true;
     if (act && !sing) {
+      // Every lane divides (the division costs the same issued for one lane
+      // or all); the pivot lane keeps the root, the lanes below the quotient.
       d = sqrt(d);
-      if (r == j) {
-        reg[j] = d;
-        L[tri(j) + j] = d;
-      } else if (r > j && r < n) {
-        double x = s / d;
-        reg[j] = x;
-        L[tri(r) + j] = x;
+      const double x = s / d;
+      if (r >= j && r < n) {
+        const double v = r == j ? d : x;
+        reg[j] = v;
+        L[tri(r) + j] = v;
       }
     }
     wave_sync();
