@@ -42,18 +42,29 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> Path:
-    """Compiles every source and links the shared library; returns its path."""
+    """Compiles every source (the units in parallel) and links the shared
+    library; returns its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
     out_dir = CSRC / "_obj"
     out_dir.mkdir(exist_ok=True)
+    cmds = []
+    objs = []
     for src, name, extra in UNITS:
         obj = out_dir / name
-        cmd = [hipcc, *FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
+        cmds.append([hipcc, *FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
     if verbose:

@@ -113,6 +113,9 @@ struct bk_ctx {
   // Scratch (grow-only).
   Buf tx, ty, tz, left, nidx, ncount, sstatus, lists, counts, status, value, variance, lagrange, support, n_used,
       guard, redo, offs, which, iters, work;
+  // Workspace of the LU for systems above BK_MAX_LU samples (searches with
+  // max_samples > BK_MAX_LU only).
+  Buf lu_big;
   // Tuning.
   int edges[3] = {16, 32, 64};
   int max_iter = 8;
@@ -764,18 +767,24 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t 
     if (bk::launch_krige64(c->stream, b, 64)) return fail(c, BK_E_HIP, "solve launch failed");
   }
   // The LU list (n > 64, power models), then the guarded systems (merged
-  // coincident samples) of every solve kernel.
+  // coincident samples, block systems the Cholesky rejects) of every solve
+  // kernel. Systems above BK_MAX_LU samples take the LU in global memory.
+  double* lu_ws = nullptr;
+  if (stride > BK_MAX_LU) {
+    if (int r = ensure(c, c->lu_big, bk::lu_big_workspace_bytes())) return r;
+    lu_ws = (double*)c->lu_big.p;
+  }
   if (!a.v.stationary || stride > BK_MAX_CHOL) {
     SolveArgs b = a;
     b.guard = lists + (int64_t)nb * m;
     b.guard_count = counts + nb;
     b.n_list = m;
-    if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
+    if (bk::launch_guarded(c->stream, b, lu_ws)) return fail(c, BK_E_HIP, "guarded launch failed");
   }
   {
     SolveArgs b = a;
     b.n_list = m;
-    if (bk::launch_guarded(c->stream, b)) return fail(c, BK_E_HIP, "guarded launch failed");
+    if (bk::launch_guarded(c->stream, b, lu_ws)) return fail(c, BK_E_HIP, "guarded launch failed");
   }
   mark(c, 5);
   return 0;
@@ -825,7 +834,7 @@ void bk_destroy(bk_ctx* c) {
                  &c->sstatus, &c->lists, &c->counts, &c->status, &c->value, &c->variance, &c->lagrange, &c->support,
                  &c->n_used, &c->guard, &c->redo, &c->offs, &c->which, &c->iters, &c->work, &c->loo_idx, &c->loo_count,
                  &c->loo_status, &c->loo_lists, &c->mik_gv, &c->mik_gs, &c->mik_par, &c->mik_cls, &c->mik_tab,
-                 &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out})
+                 &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out, &c->lu_big})
     release(*b);
   bk::free_grid(c->gb);
   bk::free_tiles(c->tiles);

@@ -15,7 +15,7 @@
 namespace {
 
 constexpr int kWaves = 4;
-constexpr int kMaxUsed = 128;  // > BK_MAX_LU, the largest system the device solves
+constexpr int kMaxUsed = 256;  // the most samples a search selects (BK_MAX_SEARCH), all solved on the device
 
 __global__ void __launch_bounds__(64 * kWaves) k_used_stats(const DiagArgs a) {
   __shared__ double dist[kWaves][kMaxUsed];

@@ -13,7 +13,6 @@
 #include "bk_device.h"
 
 #define BK_POOL 512       // candidates held per target between releases
-#define BK_MAX_SEARCH 256 // most samples one search selects on the device
 #define BK_MAX_KNN 64     // selections without caps up to this size run one target per lane
 #define BK_REACH 12
 

@@ -20,6 +20,9 @@
 #define BK_ST_REDO64 102
 #define BK_ST_LU 103
 
+// Workgroups of the LU kernel for systems above BK_MAX_LU samples.
+#define BK_LU_BIG_GROUPS 32
+
 struct SolveArgs {
   DevVariogram v;
   DevVariogram32 v32;
@@ -111,7 +114,12 @@ namespace bk {
 int launch_calibrate(hipStream_t s, const SolveArgs& a, const CalibArgs& k, int nmax);
 int launch_krige64(hipStream_t s, const SolveArgs& a, int nmax);
 int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax);
-int launch_guarded(hipStream_t s, const SolveArgs& a);
+// Solves the systems of a.guard[0 .. *a.guard_count) by bordered LU (merged
+// coincident samples, n > 64, power models): up to BK_MAX_LU samples with the
+// matrix in LDS, larger ones (up to BK_MAX_SEARCH) in lu_ws, which must hold
+// lu_big_workspace_bytes() when a list may hold such systems (else NULL).
+int launch_guarded(hipStream_t s, const SolveArgs& a, double* lu_ws);
+size_t lu_big_workspace_bytes();
 int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int nmax);
 // The refined form: FP32 Gauss-Jordan forward with FP64 refinement, FP32
 // adjoint accumulation.

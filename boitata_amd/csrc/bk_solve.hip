@@ -470,8 +470,16 @@ __device__ __forceinline__ void krige64_blk(const SolveArgs& a, int64_t nl, int 
   double sb = group_sum<NMAX>(in ? xb : 0.0);
   double mu = ordinary ? (sa - 1.0) / sb : 0.0;
   double w = xa - (ordinary ? mu * xb : 0.0);
-  if (active && !guard)
+  if (go && sing && a.block) {
+    // block_krige_points solves by LU (block.rs:167-170), which has no pivot
+    // test: a block system the Cholesky rejects goes to the LU kernel.
+    if (r == 0) {
+      a.status[t] = BK_ST_GUARD;
+      a.guard[atomicAdd(a.guard_count, 1)] = t;
+    }
+  } else if (active && !guard) {
     finish_outputs<NMAX>(a, t, r, n, !sing, BK_SINGULAR, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
+  }
 }
 
 template <int NMAX>
@@ -1595,25 +1603,33 @@ __global__ void __launch_bounds__(RED_T) k_reduce_final(int64_t nb, const double
 // merged system is small and rare), weights spread back to the members.
 // Also takes systems beyond the Cholesky kernels (n > 64, power models).
 constexpr int LU_MAX = BK_MAX_LU + 1;
+// Systems above BK_MAX_LU samples (up to the search's BK_MAX_SEARCH) take the
+// same LU with its matrix in a global-memory workspace.
+constexpr int LU_BIG = BK_MAX_SEARCH + 1;
 
+// Doubles and ints of one LU system's workspace with rows of CAP entries.
+__host__ __device__ constexpr int64_t lu_doubles(int cap) { return (int64_t)cap * cap + 8 * (int64_t)cap; }
+__host__ __device__ constexpr int64_t lu_ints(int cap) { return 3 * (int64_t)cap + 4; }
+
+template <int CAP>
 __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, double* ws, int* cl, int lane) {
-  // Runs on one wavefront. A: LU_MAX * LU_MAX, b: LU_MAX, ws: scratch.
+  // Runs on one wavefront. A: CAP * CAP, b: CAP, ws: 7 CAP doubles, cl: lu_ints(CAP).
   const int n0 = a.ncount[t];
   double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
   double* X = ws;                 // merged samples
-  double* Y = X + LU_MAX;
-  double* Z = Y + LU_MAX;
-  double* Vv = Z + LU_MAX;
-  double* Ev = Vv + LU_MAX;
-  double* Vo = Ev + LU_MAX;       // original values
-  double* rhs = Vo + LU_MAX;
-  int* members = cl + LU_MAX;     // cluster sizes
+  double* Y = X + CAP;
+  double* Z = Y + CAP;
+  double* Vv = Z + CAP;
+  double* Ev = Vv + CAP;
+  double* Vo = Ev + CAP;          // original values
+  double* rhs = Vo + CAP;
+  int* members = cl + CAP;        // cluster sizes
+  int* lead = members + CAP;      // first member of each cluster
   __shared__ int sh_n, sh_fail;
   const double c0 = a.v.c0;
   if (lane == 0) {
     // Clusters::find + merge (coincident.rs:27-82), serial.
     int nc = 0;
-    int lead[LU_MAX];
     for (int i = 0; i < n0; ++i) {
       int id = a.nidx[(int64_t)t * a.stride + i];
       double x = a.sx[id], y = a.sy[id], z = a.sz[id];
@@ -1684,7 +1700,7 @@ __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, doubl
     } else {
       v = 1.0;
     }
-    A[i * LU_MAX + j] = v;
+    A[i * CAP + j] = v;
   }
   for (int i = lane; i < dim; i += BK_WAVE) {
     double v = 1.0;
@@ -1711,28 +1727,28 @@ __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, doubl
   for (int i = 0; i < dim; ++i) {
     if (lane == 0) {
       int piv = i;
-      double best = fabs(A[i * LU_MAX + i]);
+      double best = fabs(A[i * CAP + i]);
       for (int q = i + 1; q < dim; ++q) {
-        double v = fabs(A[q * LU_MAX + i]);
+        double v = fabs(A[q * CAP + i]);
         if (v > best) {
           best = v;
           piv = q;
         }
       }
-      cl[2 * LU_MAX] = piv;
+      cl[2 * CAP] = piv;
     }
     wave_sync();
-    int piv = cl[2 * LU_MAX];
-    double diag = A[piv * LU_MAX + i];
+    int piv = cl[2 * CAP];
+    double diag = A[piv * CAP + i];
     if (diag == 0.0) {
       wave_sync();
       continue;
     }
     if (piv != i) {
       for (int k = lane; k < dim; k += BK_WAVE) {
-        double u = A[i * LU_MAX + k];
-        A[i * LU_MAX + k] = A[piv * LU_MAX + k];
-        A[piv * LU_MAX + k] = u;
+        double u = A[i * CAP + k];
+        A[i * CAP + k] = A[piv * CAP + k];
+        A[piv * CAP + k] = u;
       }
       if (lane == 0) {
         double u = b[i];
@@ -1742,28 +1758,28 @@ __device__ void lu_system(const SolveArgs& a, int t, double* A, double* b, doubl
     }
     wave_sync();
     double inv = 1.0 / diag;
-    for (int q = i + 1 + lane; q < dim; q += BK_WAVE) A[q * LU_MAX + i] *= inv;
+    for (int q = i + 1 + lane; q < dim; q += BK_WAVE) A[q * CAP + i] *= inv;
     wave_sync();
     for (int q = lane; q < (dim - i - 1) * (dim - i - 1); q += BK_WAVE) {
       int rr = i + 1 + q / (dim - i - 1), k = i + 1 + q % (dim - i - 1);
-      A[rr * LU_MAX + k] = A[rr * LU_MAX + k] + (-A[i * LU_MAX + k]) * A[rr * LU_MAX + i];
+      A[rr * CAP + k] = A[rr * CAP + k] + (-A[i * CAP + k]) * A[rr * CAP + i];
     }
     wave_sync();
   }
   if (lane == 0) {
     for (int i = 0; i < dim; ++i) {
       double co = b[i];
-      for (int q = i + 1; q < dim; ++q) b[q] = b[q] + (-co) * A[q * LU_MAX + i];
+      for (int q = i + 1; q < dim; ++q) b[q] = b[q] + (-co) * A[q * CAP + i];
     }
     for (int i = dim - 1; i >= 0; --i) {
-      double d = A[i * LU_MAX + i];
+      double d = A[i * CAP + i];
       if (d == 0.0) {
         sh_fail = 1;
         break;
       }
       b[i] = b[i] / d;
       double co = b[i];
-      for (int q = 0; q < i; ++q) b[q] = b[q] + (-co) * A[q * LU_MAX + i];
+      for (int q = 0; q < i; ++q) b[q] = b[q] + (-co) * A[q * CAP + i];
     }
     double mu = ordinary ? b[n] : 0.0;
     double sup = a.block ? a.cbb : c0;
@@ -1952,16 +1968,25 @@ __global__ void __launch_bounds__(64) k_guarded(const SolveArgs a) {
   const int64_t count = *a.guard_count;
   for (int64_t sys = blockIdx.x; sys < count; sys += gridDim.x) {
     const int t = a.guard[sys];
-    if (a.ncount[t] > BK_MAX_LU) {
-      if (threadIdx.x == 0) {
-        const double nan = __longlong_as_double(0x7ff8000000000000LL);
-        a.value[t] = nan;
-        a.variance[t] = nan;
-        a.status[t] = BK_UNSUPPORTED;
-      }
-      continue;
-    }
-    lu_system(a, t, A, b, ws, cl, threadIdx.x);
+    if (a.ncount[t] > BK_MAX_LU) continue;  // k_guarded_big
+    lu_system<LU_MAX>(a, t, A, b, ws, cl, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// The systems of the same list above BK_MAX_LU samples, each workgroup on its
+// own slice of the global workspace lu_ws (lu_doubles(LU_BIG) doubles, then
+// lu_ints(LU_BIG) ints per workgroup).
+__global__ void __launch_bounds__(64) k_guarded_big(const SolveArgs a, double* lu_ws) {
+  double* A = lu_ws + (int64_t)blockIdx.x * (lu_doubles(LU_BIG) + (lu_ints(LU_BIG) + 1) / 2);
+  double* b = A + (int64_t)LU_BIG * LU_BIG;
+  double* ws = b + LU_BIG;
+  int* cl = (int*)(ws + 7 * LU_BIG);
+  const int64_t count = *a.guard_count;
+  for (int64_t sys = blockIdx.x; sys < count; sys += gridDim.x) {
+    const int t = a.guard[sys];
+    if (a.ncount[t] <= BK_MAX_LU) continue;  // k_guarded
+    lu_system<LU_BIG>(a, t, A, b, ws, cl, threadIdx.x);
     __syncthreads();
   }
 }
@@ -2209,13 +2234,22 @@ int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax) {
 }
 
 // Handles a.guard[0 .. *a.guard_count); a.n_list bounds that count.
-int launch_guarded(hipStream_t s, const SolveArgs& a) {
+int launch_guarded(hipStream_t s, const SolveArgs& a, double* lu_ws) {
   if (a.n_list <= 0) return 0;
-  size_t sh = (size_t)(LU_MAX * LU_MAX + LU_MAX + 7 * LU_MAX) * sizeof(double) + (2 * LU_MAX + 4) * sizeof(int);
+  size_t sh = (size_t)lu_doubles(LU_MAX) * sizeof(double) + (size_t)lu_ints(LU_MAX) * sizeof(int);
   const void* fn = (const void*)k_guarded;
   allow_lds(fn, sh);
   k_guarded<<<dim3(persistent_grid(fn, 64, sh, a.n_list)), dim3(64), sh, s>>>(a);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  if (hipGetLastError() != hipSuccess) return -2;
+  if (lu_ws) {
+    k_guarded_big<<<dim3(BK_LU_BIG_GROUPS), dim3(64), 0, s>>>(a, lu_ws);
+    if (hipGetLastError() != hipSuccess) return -2;
+  }
+  return 0;
+}
+
+size_t lu_big_workspace_bytes() {
+  return (size_t)BK_LU_BIG_GROUPS * (size_t)(lu_doubles(LU_BIG) + (lu_ints(LU_BIG) + 1) / 2) * sizeof(double);
 }
 
 int launch_bucket(hipStream_t s, int64_t m, const int32_t* count, const int32_t* sstatus, int32_t* status,

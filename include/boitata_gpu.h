@@ -75,7 +75,9 @@ enum {
 
 #define BK_MAX_STRUCTURES 4
 #define BK_MAX_CHOL 64     /* systems solved by Cholesky on the device (kernel.rs:11 MAX) */
-#define BK_MAX_LU 79       /* larger systems and power models: bordered LU on the device */
+#define BK_MAX_LU 79       /* larger systems and power models: bordered LU with the matrix in LDS */
+#define BK_MAX_SEARCH 256  /* most samples one search selects (bk_search.max_samples); systems above
+                              BK_MAX_LU take the same LU with the matrix in device memory */
 
 /* One nested structure, crates/variogram/src/model.rs:34-41. `range` runs along
  * the major axis. ratio_semi / ratio_minor > 0 give this structure its own

@@ -69,6 +69,9 @@ def lib() -> C.CDLL:
             "or_block_krige": (I32, [SP, VP, SR, I32, D, BP, I64, P, P, P, P, P, P, P, P, P, I32]),
             "or_loo": (I32, [SP, VP, SR, I32, D, I64, P, P, P, P, I32]),
             "or_krige_lists": (I32, [SP, VP, I32, D, I64, P, P, P, P, P, I64, P, P, P, P, I32]),
+            "or_block_krige_lists": (I32, [SP, VP, I32, D, BP, I64, P, P, P, P, P, I64, I32, P, P, P, P, P, P,
+                                           I32]),
+            "or_system_matrix": (None, [SP, VP, I32, P, P]),
             "or_prefix_quality": (I32, [SP, VP, I32, P, BP, I32, P, P, P]),
             "or_calibrated_count": (I32, [I32, D, I64, I32, P, P, P]),
             "or_balanced": (I32, [SP, SR, VP, P, I32, P, P]),
@@ -275,6 +278,37 @@ def krige_lists(data: Data, vg, targets, idx, count, kind=_abi.ORDINARY, param=0
     lib().or_krige_lists(C.byref(data.s), C.byref(vg), kind, param, m, _p(tx), _p(ty), _p(tz), _p(idx), _p(count),
                          idx.shape[1], _p(value), _p(variance), _p(lagrange), _p(status), threads)
     return dict(value=value, variance=variance, lagrange=lagrange, status=status)
+
+
+def block_krige_lists(data: Data, vg, blk, targets, idx, count, solver=0, lanes=None, kind=_abi.ORDINARY,
+                      param=0.0, threads=0):
+    """Block kriging of `targets` from the given neighbour lists: solver 0 is
+    the reference's LU (block_krige_points), 1 the engine's FP64 order
+    (Cholesky, Schur scalar, butterfly sums over lanes[t] lanes) with the LU
+    for what that order leaves to it."""
+    t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+    tx, ty, tz = (np.ascontiguousarray(t[:, k]) for k in range(3))
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    count = np.ascontiguousarray(count, dtype=np.int32)
+    m = len(t)
+    ln = None if lanes is None else np.ascontiguousarray(lanes, dtype=np.int32)
+    if solver == 1 and ln is None:
+        raise ValueError("the engine's order needs the lane group of each system")
+    out = {k: np.zeros(m) for k in ("value", "variance", "lagrange", "support")}
+    status = np.zeros(m, np.int32)
+    lib().or_block_krige_lists(C.byref(data.s), C.byref(vg), kind, param, C.byref(blk), m, _p(tx), _p(ty), _p(tz),
+                               _p(idx), _p(count), idx.shape[1], solver, _p(ln), _p(out["value"]),
+                               _p(out["variance"]), _p(out["lagrange"]), _p(out["support"]), _p(status), threads)
+    out["status"] = status
+    return out
+
+
+def system_matrix(data: Data, vg, idx):
+    """The covariance matrix (cov_points, error variances on the diagonal) of the samples idx."""
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    out = np.zeros((len(idx), len(idx)))
+    lib().or_system_matrix(C.byref(data.s), C.byref(vg), len(idx), _p(idx), _p(out))
+    return out
 
 
 def prefix_quality(data: Data, vg, target, idx, kind=_abi.ORDINARY, block=None):

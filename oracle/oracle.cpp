@@ -720,6 +720,105 @@ int block_krige_points(int kind, double param, const std::vector<P3>& pts, const
   return BK_OK;
 }
 
+// Sum of v[0 .. n) zero-padded to `lanes` (a power of two) as a butterfly:
+// at each stride o = lanes/2 .. 1 every entry l becomes v[l] + v[l ^ o]; the
+// sum is entry 0. This is the order of a sum across a group of lanes by xor
+// shuffles.
+static double butterfly_sum(std::vector<double> v, int lanes) {
+  v.resize(lanes, 0.0);
+  for (int o = lanes / 2; o > 0; o >>= 1) {
+    std::vector<double> w(lanes);
+    for (int l = 0; l < lanes; ++l) w[l] = v[l] + v[l ^ o];
+    v.swap(w);
+  }
+  return v[0];
+}
+
+// Block kriging in the engine's FP64 order (not the reference's), kept to pin
+// the engine's arithmetic where the reference's LU and this path differ by
+// more than 1e-10 (ill-conditioned block systems): the right-hand side, C(B,B)
+// and matrix of block_krige_points (block.rs:100-158); the Cholesky of
+// kernel.rs:58-72 with its pivot test; the forward solves of kernel.rs:73-78;
+// back solves that subtract the rows below from the last one up (kernel.rs:79-84
+// subtracts them first to last); the Schur scalar, weights, value and variance
+// of kernel.rs:86-122 with every sum a butterfly over `lanes` (a power of two,
+// at least n). Returns -1 for systems with coincident samples or a failed
+// pivot test: the engine solves those by LU, as block_krige_points does.
+int block_krige_chol(int kind, double param, const std::vector<P3>& pts, const std::vector<Sample>& s,
+                     const Variogram& vg, int lanes, Estimate& out) {
+  int n = (int)s.size();
+  if (n == 0) return BK_INSUFFICIENT_DATA;
+  if (n > lanes || n > MAXK || !vg.is_stationary() || pts.empty()) return -1;
+  Clusters cl;
+  if (Clusters::find(s, cl)) return -1;
+  const double c0 = vg.total_sill();
+  const double np = (double)pts.size();
+  std::vector<double> c(n);
+  for (int i = 0; i < n; ++i) {
+    double acc = -0.0;
+    for (auto& p : pts) acc += pts.size() == 1 ? vg.cov_points(s[i].loc, p) : vg.block_cov_points(s[i].loc, p);
+    c[i] = acc / np;
+  }
+  const double cbb = vg.block_mean_cov(pts);
+  std::vector<double> l(n * (n + 1) / 2);
+  for (int i = 0; i < n; ++i) {
+    int row = i * (i + 1) / 2;
+    for (int j = 0; j < i; ++j) l[row + j] = vg.cov_points(s[i].loc, s[j].loc);
+    l[row + i] = c0 + s[i].err;
+  }
+  auto dot = [](const double* a, const double* b, int len) {
+    double acc = -0.0;
+    for (int k = 0; k < len; ++k) acc += a[k] * b[k];
+    return acc;
+  };
+  for (int j = 0; j < n; ++j) {
+    int rj = j * (j + 1) / 2;
+    double d = l[rj + j] - dot(&l[rj], &l[rj], j);
+    if (std::isnan(d) || d <= PIVOT * c0) return -1;
+    d = std::sqrt(d);
+    l[rj + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      int ri = i * (i + 1) / 2;
+      l[ri + j] = (l[ri + j] - dot(&l[ri], &l[rj], j)) / d;
+    }
+  }
+  auto solve = [&](std::vector<double>& v) {
+    for (int i = 0; i < n; ++i) {
+      int ri = i * (i + 1) / 2;
+      v[i] = (v[i] - dot(&l[ri], v.data(), i)) / l[ri + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double acc = v[i];
+      for (int k = n - 1; k > i; --k) acc -= l[k * (k + 1) / 2 + i] * v[k];
+      v[i] = acc / l[i * (i + 1) / 2 + i];
+    }
+  };
+  std::vector<double> a = c;
+  solve(a);
+  const bool ordinary = kind != SIMPLE;
+  std::vector<double> b(n, 1.0);
+  double mu = 0.0;
+  if (ordinary) {
+    solve(b);
+    mu = (butterfly_sum(a, lanes) - 1.0) / butterfly_sum(b, lanes);
+  }
+  out.weights.resize(n);
+  for (int i = 0; i < n; ++i) out.weights[i] = a[i] - (ordinary ? mu * b[i] : 0.0);
+  std::vector<double> f(n), wc(n);
+  for (int i = 0; i < n; ++i) {
+    const double w = out.weights[i], z = s[i].value;
+    f[i] = kind == SIMPLE ? w * (z - param) : (kind == INDICATOR ? w * (z <= param ? 1.0 : 0.0) : w * z);
+    wc[i] = w * c[i];
+  }
+  const double sum = butterfly_sum(f, lanes);
+  out.value = kind == SIMPLE ? param + sum : (kind == INDICATOR ? std::min(std::max(sum, 0.0), 1.0) : sum);
+  out.variance = std::max(cbb - butterfly_sum(wc, lanes) - mu, 0.0);
+  out.n_used = n;
+  out.lagrange = mu;
+  out.support = cbb;
+  return BK_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Calibrated kriging, crates/estimation/src/kriging_algebra.rs:320-441.
 
@@ -2103,6 +2202,41 @@ int or_krige_lists(const or_samples* sm, const bk_variogram* v, int kind, double
     write_estimate(st, e, t, value, variance, lagrange, nullptr, nullptr, status);
   }
   return 0;
+}
+
+// Block kriging with given neighbour lists on the discretisation of `blk`
+// centred on each target: solver 0 = block_krige_points (block.rs:122-198, the
+// reference's LU), 1 = block_krige_chol (the engine's FP64 order; lanes[t] is
+// the lane group of the target's system) with the LU taking what it returns
+// -1 for.
+int or_block_krige_lists(const or_samples* sm, const bk_variogram* v, int kind, double param, const bk_block* blk,
+                         int64_t m, const double* tx, const double* ty, const double* tz, const int32_t* idx,
+                         const int32_t* count, int64_t stride, int solver, const int32_t* lanes, double* value,
+                         double* variance, double* lagrange, double* support, int32_t* status, int nthreads) {
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t t = 0; t < m; ++t) {
+    P3 q{tx[t], ty[t], tz[t]};
+    std::vector<Sample> sel;
+    for (int32_t k = 0; k < count[t]; ++k) sel.push_back(s[idx[t * stride + k]]);
+    const std::vector<P3> pts = block_points(q, *blk);
+    Estimate e;
+    int st = solver == 1 ? block_krige_chol(kind, param, pts, sel, vg, lanes[t], e) : -1;
+    if (st == -1) st = block_krige_points(kind, param, pts, sel, vg, e);
+    write_estimate(st, e, t, value, variance, lagrange, support, nullptr, status);
+  }
+  return 0;
+}
+
+// The covariance matrix of the samples idx[0 .. n) (cov_points, error
+// variances on the diagonal; block.rs:140-158), row-major into out (n * n).
+void or_system_matrix(const or_samples* sm, const bk_variogram* v, int n, const int32_t* idx, double* out) {
+  auto s = samples_of(sm);
+  Variogram vg = make_variogram(v);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      out[i * n + j] = vg.cov_points(s[idx[i]].loc, s[idx[j]].loc) + (i == j ? s[idx[i]].err : 0.0);
 }
 
 }  // extern "C"

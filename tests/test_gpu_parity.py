@@ -191,17 +191,58 @@ def test_no_samples_and_invalid_parameters(engine):
                      np.zeros(1), np.zeros(1), kind=_abi.SIMPLE)
 
 
+def lanes_of(count, edges=(16, 32, 64)):
+    """The lane group (size bucket) of each system under the default edges."""
+    count = np.asarray(count)
+    return np.where(count <= edges[0], edges[0], np.where(count <= edges[1], edges[1], edges[2]))
+
+
 def test_block_kriging(engine):
     w = datasets.config5(target_counts=(40, 40, 10), holes=(16, 16))
+    engine.tuning(edges=(16, 32, 64))
     engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
     d = oracle.Data(w.x, w.y, w.z, w.value, None, w.hole)
     T = w.targets[::7].copy()
     o = oracle.block_krige(d, w.variogram, w.search, w.block, T)
-    # Gaussian covariance with a 1e-4 nugget and 1 cm twins: the LU of the
-    # reference and the Cholesky of the device agree to kappa * eps only.
-    for prec in (_abi.FP64, _abi.FP32_IR):
-        g = engine.block_krige(w.variogram, w.search, w.block, T[:, 0], T[:, 1], T[:, 2], precision=prec)
-        assert_estimates(g, o, w.variogram, 1e-6, ("value", "variance", "support"))
+    # Gaussian covariance with a 1e-4 nugget and 1 cm twins. The device's FP64
+    # blocks follow the oracle's engine-order path (Cholesky, Schur scalar) to
+    # 1e-10; against the reference's LU the gap is bounded by kappa * eps.
+    idx, cnt, st = oracle.neighbors(d, w.search, w.variogram, T)
+    eng = oracle.block_krige_lists(d, w.variogram, w.block, T, idx, cnt, solver=1, lanes=lanes_of(cnt))
+    eng["status"] = np.where(st != 0, st, eng["status"])
+    g = engine.block_krige(w.variogram, w.search, w.block, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP64)
+    assert_estimates(g, eng, w.variogram, 1e-10, ("value", "variance", "support"))
+    ok = o["status"] == 0
+    kappa = np.array([np.linalg.cond(oracle.system_matrix(d, w.variogram, idx[i, :cnt[i]])) if ok[i] else 1.0
+                      for i in range(len(T))])
+    floor = 1e-3 * sill_of(w.variogram)
+    for k in ("value", "variance"):
+        gap = np.abs(g[k][ok] - o[k][ok]) / np.maximum(np.abs(o[k][ok]), floor)
+        assert (gap <= 1e-10 + 64 * kappa[ok] * np.finfo(float).eps).all(), (k, gap.max(), kappa.max())
+    g32 = engine.block_krige(w.variogram, w.search, w.block, T[:, 0], T[:, 1], T[:, 2], precision=_abi.FP32_IR)
+    assert_estimates(g32, o, w.variogram, 1e-6, ("value", "variance", "support"))
+
+
+@pytest.mark.parametrize("max_samples", [80, 128, 256])
+def test_systems_beyond_the_cholesky_kernel(engine, c1, max_samples):
+    # Searches of up to 256 samples: systems above 64 take the bordered LU
+    # (krige.rs:127-201), above 79 with its matrix in device memory. Point and
+    # block support, both precisions, ordinary and simple kriging.
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    s = _abi.make_search(4, max_samples, 400.0, rotation=(30, 0, 0), ratios=(80 / 120, 30 / 120))
+    S = T[::6]
+    for kind, param in ((_abi.ORDINARY, 0.0), (_abi.SIMPLE, 1.0)):
+        o = oracle.krige(d, w.variogram, s, S, kind, param)
+        assert (o["n_used"][o["status"] == 0] > 64).mean() > 0.5
+        for prec in (_abi.FP64, _abi.FP32_IR):
+            g = engine.krige(w.variogram, s, S[:, 0], S[:, 1], S[:, 2], kind, param, prec)
+            assert not (g["status"] == _abi.UNSUPPORTED).any()
+            assert_estimates(g, o, w.variogram, 1e-10, ("value", "variance", "lagrange"))
+    blk = _abi.make_block((10, 10, 10), (2, 2, 2))
+    o = oracle.block_krige(d, w.variogram, s, blk, S[::4])
+    g = engine.block_krige(w.variogram, s, blk, S[::4, 0], S[::4, 1], S[::4, 2])
+    assert_estimates(g, o, w.variogram, 1e-10, ("value", "variance", "support"))
 
 
 def test_block_kriging_well_conditioned(engine, c1):
@@ -296,3 +337,27 @@ def test_graph_replay_matches_eager(engine, c1):
         for k in keys:
             assert (got[k] == ref[k]).all(), k
     engine.solve_tuning(2)
+
+
+@pytest.mark.parametrize("model,param", [("cubic", 0.0), ("pentaspherical", 0.0), ("circular", 0.0),
+                                         ("sinehole", 0.0), ("matern", 0.5), ("matern", 1.5), ("matern", 2.5)])
+def test_every_stationary_model(engine, c1, model, param):
+    # model.rs:76-126: each stationary shape, nested with a spherical
+    # structure under the workload's anisotropy, point and block support, in
+    # both precisions (the refined form takes cubic and pentaspherical; the
+    # others run the FP64 kernel in either precision).
+    w, d, T = c1
+    engine.set_samples(w.x, w.y, w.z, w.value, None, w.hole)
+    vg = _abi.make_variogram([(model, 0.5, 90.0, param), ("spherical", 0.3, 200.0)], nugget=0.2,
+                             rotation=(30, 0, 0), ratios=(80 / 120, 30 / 120))
+    S = T[::3]
+    o = oracle.krige(d, vg, w.search, S)
+    assert (o["status"] == 0).mean() > 0.9
+    for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+        g = engine.krige(vg, w.search, S[:, 0], S[:, 1], S[:, 2], precision=prec)
+        assert_estimates(g, o, vg, tol, ("value", "variance", "lagrange"))
+    blk = _abi.make_block((10, 10, 10), (2, 2, 2))
+    o = oracle.block_krige(d, vg, w.search, blk, S[::5])
+    for prec, tol in ((_abi.FP64, 1e-10), (_abi.FP32_IR, 1e-6)):
+        g = engine.block_krige(vg, w.search, blk, S[::5, 0], S[::5, 1], S[::5, 2], precision=prec)
+        assert_estimates(g, o, vg, tol, ("value", "variance", "support"))
