@@ -689,21 +689,18 @@ __device__ __forceinline__ void pairs_cov_fast(const DevVariogram& v, const doub
   }
 }
 
-// The front of the refined kernels for the system of target t (n samples;
-// lane r of group g): loads the samples as records in LDS (world coordinates
-// and the projection on every frame relative to the target, kernel.rs:36-42),
-// tests coincident samples (coincident.rs:13-15, 27-35; with report_guard the
-// system is handed to the guarded kernel), computes the right-hand side c of
-// sample r (the covariance to the target, or to the block), the FP64 matrix
-// (pairs once, packed triangle T), keeps row r of it in crow (NMAX <= 32) or
-// RowL, and leaves row r of (C / c0)^-1 in x (FP32 Gauss-Jordan; bad is set
-// where a pivot falls below 1e-5 of the scaled diagonal). Returns the guard.
+// The pair phase of the refined kernels for the system of target t (n
+// samples; lane r of group g): loads the samples as records in LDS (world
+// coordinates and the projection on every frame relative to the target,
+// kernel.rs:36-42), tests coincident samples (coincident.rs:13-15, 27-35; with
+// report_guard the system is handed to the guarded kernel), computes the
+// right-hand side c of sample r (the covariance to the target, or to the
+// block) and the strict lower triangle of the FP64 matrix into T (each pair
+// once, packed). Returns the guard.
 template <int NMAX>
-__device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, int r, int g, int lane, bool active,
-                                              bool report_guard, double* PJ, int SJ, float* P2, double* T,
-                                              double* RowL, double& z, double& e, double& c, float (&x)[NMAX],
-                                              double (&crow)[NMAX <= 32 ? NMAX : 1], bool& bad) {
-  constexpr bool ROWREG = NMAX <= 32;
+__device__ __forceinline__ bool refined_pairs(const SolveArgs& a, int t, int n, int r, int g, bool active,
+                                              bool report_guard, double* PJ, int SJ, double* T, double& z,
+                                              double& e, double& c) {
   const int nf = a.v.nf;
   const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
   // Sample r of the system: world coordinates and its projection on every
@@ -766,11 +763,26 @@ __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, 
       c = cov_fast64(a.v, h);
     }
   }
-  // Row r of C (c0 + e on the diagonal; c0 on the diagonal and 0 elsewhere
-  // beyond n) and its FP32 copy scaled by 1/c0.
-  const double ic0 = 1.0 / c0;
   if (go) pairs_cov_fast<NMAX>(a.v, PJ, SJ, r, n, T);
   wave_sync();
+  return guard;
+}
+
+// The refined front for one system per NMAX lanes: refined_pairs, then row r
+// of C (c0 + e on the diagonal; c0 on the diagonal and 0 elsewhere beyond n)
+// in crow (NMAX <= 32) or RowL, and row r of (C / c0)^-1 in x by FP32
+// Gauss-Jordan.
+template <int NMAX>
+__device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, int r, int g, int lane, bool active,
+                                              bool report_guard, double* PJ, int SJ, float* P2, double* T,
+                                              double* RowL, double& z, double& e, double& c, float (&x)[NMAX],
+                                              double (&crow)[NMAX <= 32 ? NMAX : 1], bool& bad) {
+  constexpr bool ROWREG = NMAX <= 32;
+  const bool guard = refined_pairs<NMAX>(a, t, n, r, g, active, report_guard, PJ, SJ, T, z, e, c);
+  const bool go = active && !guard;
+  const bool live = go && r < n;
+  const double c0 = a.v.c0;
+  const double ic0 = 1.0 / c0;
   // Row r of the FP64 matrix: in registers for NMAX <= 32, else in LDS
   // ([k][lane], lane-private); the FP32 copy scaled by 1/c0 starts the
   // elimination.
@@ -989,6 +1001,244 @@ k_krige32(const SolveArgs a) {
     q.unit = next;
   }
 }
+
+#if BK_PART != 2
+// ---------------------------------------------------------------------------
+// FP32 + FP64 refinement with the elimination on the matrix cores: two
+// systems of at most 32 samples per wavefront.
+//
+// The pair phase is refined_pairs (lane r of group g on sample r of system g).
+// The matrices then live in the accumulator of v_mfma_f32_32x32x1_2b_f32,
+// whose two blocks are the two systems: block b takes entry i of its
+// operands from lane 32 b + i, and lane l = j + 32 h holds column j of block b
+// at the rows i = 8 q + 4 h + p (q, p < 4) in accumulator register 16 b + 4 q + p.
+// The lane also keeps the FP64 entries of those rows and columns for the
+// residual (C is symmetric, so lane j's column is row j as well). Gauss-Jordan
+// pivot k is the rank-one update X -= u v' with u_i = X_ik / d and v_j = X_kj
+// off the pivot, u_k = 1 - 1/d and v_k = d + 1 (d = X_kk): it leaves 1/d on
+// the diagonal, row k / d, column k / -d and the Schur complement elsewhere,
+// the in-place Gauss-Jordan inverse (no pivoting: the matrix is SPD and the
+// pivots are its Cholesky pivots squared). Row k of both systems is one
+// accumulator register of the half-wave (k / 4) % 2; one permlane32 swap puts
+// system b's row in lanes 32 b + j. Column k follows from row k: after pivots
+// 0 .. k-1 of a symmetric matrix, X_ik = -X_ki for i < k and X_ik = X_ki for
+// i > k. A pivot costs about a dozen VALU instructions besides the matrix
+// instruction, where a lane-per-row sweep issues 32 fused multiply-adds per
+// row.
+__host__ __device__ constexpr int sysm_doubles(int nf) { return 2 * 32 * rec_doubles(nf) + 2 * 528; }
+
+typedef float bk_floatx32 __attribute__((ext_vector_type(32)));
+
+// v of lane l plus v of lane l ^ 32, the same bits in both lanes.
+__device__ __forceinline__ float halves_sum(float v) {
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+
+__device__ __forceinline__ double halves_sum(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x0 = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double x1 = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return x0 + x1;
+}
+
+// The value of lane l ^ 32 (h = l / 32).
+__device__ __forceinline__ double other_half(double v, int h) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const unsigned l = h ? lo[0] : lo[1], u = h ? hi[0] : hi[1];
+  return __longlong_as_double((long long)(((unsigned long long)u << 32) | l));
+}
+
+__device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int unit) {
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int SJ = rec_doubles(nf);
+  const int tid = opaque_tid();
+  const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
+  const int g = lane >> 5, r = lane & 31;
+  double* S = dyn + wave * sysm_doubles(nf);
+  double* T = S + 2 * 32 * SJ;  // two packed strict lower triangles of 528 doubles
+  const int64_t sys = (int64_t)unit * 2 + g;
+  const bool active = sys < nl;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z, e, c;
+  const bool guard = refined_pairs<32>(a, t, n, r, g, active, true, S + g * 32 * SJ, SJ, T + g * 528, z, e, c);
+  const bool go = active && !guard;
+  const double c0 = a.v.c0;
+  const double ic0 = 1.0 / c0;
+  const bool go0 = __builtin_amdgcn_readlane(go ? 1 : 0, 0) != 0, go1 = __builtin_amdgcn_readlane(go ? 1 : 0, 32) != 0;
+  const int n0 = go0 ? __builtin_amdgcn_readlane(n, 0) : 0, n1 = go1 ? __builtin_amdgcn_readlane(n, 32) : 0;
+  // Column r of both systems' matrices at the lane's rows (identity rows and
+  // columns beyond n).
+  const double e_o = other_half(e, g);
+  double cr[32];
+  bk_floatx32 X;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const double* Tb = T + b * 528;
+    const int nb = b ? n1 : n0;
+    const double eb = (b == g) ? e : e_o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int j = 8 * q + 4 * g + p;
+        double v = 0.0;
+        if (j == r)
+          v = r < nb ? c0 + eb : c0;
+        else if (r < nb && j < nb)
+          v = j < r ? Tb[r * (r + 1) / 2 + j] : Tb[j * (j + 1) / 2 + r];
+        cr[16 * b + 4 * q + p] = v;
+        X[16 * b + 4 * q + p] = (float)(v * ic0);
+      }
+  }
+  wave_sync();  // the triangles are read; their space holds the vectors below
+  // The smallest pivot of each system as an integer over the float bits: for
+  // floats >= 0 the order is the same, a negative pivot is a negative integer,
+  // and a NaN pivot fails the convergence test later.
+  int dmin0 = 0x7f7fffff, dmin1 = 0x7f7fffff;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    const int rk = 4 * (k / 8) + (k % 4);
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[rk]), __float_as_uint(X[16 + rk]), false, false);
+    const float row = __uint_as_float(((k / 4) % 2) ? p[1] : p[0]);
+    const int b0 = __builtin_amdgcn_readlane(__float_as_int(row), k);
+    const int b1 = __builtin_amdgcn_readlane(__float_as_int(row), 32 + k);
+    dmin0 = min(dmin0, b0);
+    dmin1 = min(dmin1, b1);
+    const float inv = __builtin_amdgcn_rcpf(__int_as_float(g ? b1 : b0));
+    // Operand entry i of lane 32 b + i: -u_i (from row k by the sign rule)
+    // and v_i.
+    const float u = r == k ? inv - 1.0f : (r < k ? row * inv : -row * inv);
+    const float v = r == k ? row + 1.0f : row;
+    X = __builtin_amdgcn_mfma_f32_32x32x1f32(u, v, X, 0, 0, 0);
+  }
+  const bool bad0 = !(dmin0 > __float_as_int(1e-5f)), bad1 = !(dmin1 > __float_as_int(1e-5f));
+  // Solves. Lane (r, g) publishes its own system g's entry r of a vector; the
+  // products take the lane's 16 rows of column r of both systems and the two
+  // halves' partial sums: C x exactly (C is symmetric), and X' v for X v (the
+  // FP32 inverse is symmetric to rounding, which the refinement absorbs).
+  double* X2 = T;                    // iterates, double2 [b][j]
+  float* V2 = (float*)(T + 2 * 64);  // product inputs, float2 [b][j]
+  auto apply = [&](float va, float vb, float (&ya)[2], float (&yb)[2]) {
+    reinterpret_cast<float2*>(V2)[g * 32 + r] = make_float2(va, vb);
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 w0 = reinterpret_cast<const float4*>(V2)[16 * b + 4 * q + 2 * g];
+        const float4 w1 = reinterpret_cast<const float4*>(V2)[16 * b + 4 * q + 2 * g + 1];
+        const int o = 16 * b + 4 * q;
+        sa = fmaf(X[o], w0.x, sa);
+        sb = fmaf(X[o], w0.y, sb);
+        sa = fmaf(X[o + 1], w0.z, sa);
+        sb = fmaf(X[o + 1], w0.w, sb);
+        sa = fmaf(X[o + 2], w1.x, sa);
+        sb = fmaf(X[o + 2], w1.y, sb);
+        sa = fmaf(X[o + 3], w1.z, sa);
+        sb = fmaf(X[o + 3], w1.w, sb);
+      }
+      ya[b] = halves_sum(sa);
+      yb[b] = halves_sum(sb);
+    }
+    wave_sync();
+  };
+  const bool live = go && r < n;
+  const bool ordinary = a.kind != BK_SIMPLE;
+  const double ba = live ? c : 0.0, bb = (live && ordinary) ? 1.0 : 0.0;
+  const double ba_o = other_half(ba, g), bb_o = other_half(bb, g);
+  const double rha[2] = {g ? ba_o : ba, g ? ba : ba_o}, rhb[2] = {g ? bb_o : bb, g ? bb : bb_o};
+  float fa[2], fb[2];
+  apply((float)(ba * ic0), (float)(bb * ic0), fa, fb);
+  double xa[2] = {fa[0], fa[1]}, xb[2] = {fb[0], fb[1]};
+  const float tolf = (float)a.tol;
+  float ma[2], mb[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    ma[b] = group_maxf<32>(fabsf(fa[b]));
+    mb[b] = group_maxf<32>(fabsf(fb[b]));
+  }
+  bool conv0 = false, conv1 = false;
+  int it0 = 0, it1 = 0;
+  for (int it = 0; it < a.max_iter; ++it) {
+    const bool more0 = go0 && !bad0 && !conv0, more1 = go1 && !bad1 && !conv1;
+    if (!more0 && !more1) break;
+    reinterpret_cast<double2*>(X2)[g * 32 + r] = make_double2(g ? xa[1] : xa[0], g ? xb[1] : xb[0]);
+    wave_sync();
+    double ra[2], rb[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      double sa = 0.0, sb = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const double2 xv = reinterpret_cast<const double2*>(X2)[32 * b + 8 * q + 4 * g + p];
+          sa = fma(cr[16 * b + 4 * q + p], xv.x, sa);
+          sb = fma(cr[16 * b + 4 * q + p], xv.y, sb);
+        }
+      ra[b] = rha[b] - halves_sum(sa);
+      rb[b] = rhb[b] - halves_sum(sb);
+    }
+    wave_sync();
+    float da[2], db[2];
+    apply((float)((g ? ra[1] : ra[0]) * ic0), (float)((g ? rb[1] : rb[0]) * ic0), da, db);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      xa[b] += (double)da[b];
+      xb[b] += (double)db[b];
+    }
+    const bool small0 = __ballot(!(fabsf(da[0]) <= tolf * ma[0] && fabsf(db[0]) <= tolf * mb[0])) == 0;
+    const bool small1 = __ballot(!(fabsf(da[1]) <= tolf * ma[1] && fabsf(db[1]) <= tolf * mb[1])) == 0;
+    if (more0) {
+      it0 = it + 1;
+      conv0 = small0;
+    }
+    if (more1) {
+      it1 = it + 1;
+      conv1 = small1;
+    }
+  }
+  const bool bad = g ? bad1 : bad0, conv = g ? conv1 : conv0;
+  if (active && !guard && (bad || !conv)) {
+    if (r == 0) {
+      a.status[t] = BK_ST_REDO64;
+      a.redo[atomicAdd(a.redo_count, 1)] = t;
+    }
+  }
+  const bool ok = go && !bad && conv;
+  // The Schur scalar 1'C^-1 1 and the weights in FP64 (kernel.rs:86-97).
+  const double xag = g ? xa[1] : xa[0], xbg = g ? xb[1] : xb[0];
+  const bool in = ok && r < n;
+  const double sa = group_sum<32>(in ? xag : 0.0);
+  const double sb = group_sum<32>(in ? xbg : 0.0);
+  const double mu = ordinary ? (sa - 1.0) / sb : 0.0;
+  const double w = xag - (ordinary ? mu * xbg : 0.0);
+  if (ok) {
+    finish_outputs<32>(a, t, r, n, true, BK_OK, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
+    if (a.iters && r == 0) a.iters[t] = g ? it1 : it0;
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_krige32m(const SolveArgs a) {
+  const int64_t nl = list_len(a);
+  UnitLoop q = unit_loop((nl + 1) / 2, a.work);
+  while (q.unit < q.end) {
+    // The next unit is taken before this one is solved, hiding the atomic.
+    const int next = unit_next(q);
+    krige32m_blk(kernarg_first<SolveArgs>(), nl, q.unit);
+    wave_sync();
+    q.unit = next;
+  }
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // Leave-one-out cross-validation loss and its adjoint gradient (no
@@ -2127,6 +2377,16 @@ static int run32(hipStream_t s, const SolveArgs& a) {
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+static int run32m(hipStream_t s, const SolveArgs& a) {
+  const SolveShape q = solve_shape<32>(a, sysm_doubles(a.v.nf) / 2);
+  if (q.work == 0) return 0;
+  if (!a.work) return -1;  // persistent kernels take their units from this counter
+  const void* fn = (const void*)k_krige32m;
+  allow_lds(fn, q.sh);
+  k_krige32m<<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 template <int NMAX>
 static int run_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp) {
   const SolveShape q = solve_shape<NMAX>(a, sys_doubles(NMAX, a.v.nf));
@@ -2226,7 +2486,7 @@ int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax) {
     case 16:
       return run32<16>(s, a);
     case 32:
-      return run32<32>(s, a);
+      return run32m(s, a);
     case 64:
       return run32<64>(s, a);
   }
