@@ -387,6 +387,13 @@ void dev_variogram(const bk_variogram* v, DevVariogram& d, DevVariogram32& f) {
   }
   d.nf = nf;
   d.nug_frame = d.S > 0 ? d.frame[0] : 0;
+  d.inv_c0 = 1.0 / d.c0;
+  {
+    double fro = 0.0;
+    for (int q = 0; q < 9; ++q) fro += d.F[d.nug_frame][q] * d.F[d.nug_frame][q];
+    if (d.ident[d.nug_frame] || nf == 0) fro = 3.0;
+    d.near2 = 1.21e-12 * fro;
+  }
   f.S = d.S;
   f.nf = d.nf;
   f.nug_frame = d.nug_frame;

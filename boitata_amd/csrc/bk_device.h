@@ -123,6 +123,10 @@ struct DevVariogram {
   int stationary;
   double nugget;
   double c0;          // total_sill in the reference's order
+  double inv_c0;      // 1 / c0
+  // Squared lag in the frame of structure 0 below which two samples may lie
+  // within 1e-6 m of each other: (1.1e-6 |F|_F)^2, since |F d| <= |F|_F |d|.
+  double near2;
   int model[BK_MAX_STRUCTURES];
   int frame[BK_MAX_STRUCTURES];
   double param[BK_MAX_STRUCTURES];

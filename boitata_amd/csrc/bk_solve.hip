@@ -608,8 +608,8 @@ __device__ __forceinline__ void row_cov_fast(const DevVariogram& v, const double
 // acc[d - 1] receives C(s_r, s_j) for valid d.
 template <int NMAX, class Shape>
 __device__ __forceinline__ void half_pass(const double* PJ, int SJ, int off, const double* me, int r, int n,
-                                          double sill, double ir, bool first, uint64_t& apart, double* acc,
-                                          Shape shape) {
+                                          double sill, double ir, bool first, uint64_t& apart, uint64_t& near,
+                                          double near2, double* acc, Shape shape) {
 #pragma clang fp contract(fast)
   const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
 #pragma unroll
@@ -621,8 +621,10 @@ __device__ __forceinline__ void half_pass(const double* PJ, int SJ, int off, con
     j = j < n ? j : 0;
     const double* pj = PJ + j * SJ + off;
     const double d0 = q0 - pj[0], d1 = q1 - pj[1], d2 = q2 - pj[2];
-    const double h = sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2);
+    const double l2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const double h = sqrt_fast(l2);
     if (first && h > 0.0) apart |= 1ull << d;
+    if (first && l2 <= near2) near |= 1ull << d;
     acc[d - 1] += sill * shape(h * ir);
     if (d % 4 == 0) asm volatile("" ::: "memory");
   }
@@ -630,51 +632,55 @@ __device__ __forceinline__ void half_pass(const double* PJ, int SJ, int off, con
 
 // The strict lower triangle of the group's covariance matrix into T (packed,
 // T[i (i + 1) / 2 + j], j < i), each pair evaluated once (see half_pass).
+// Returns whether one of the lane's pairs is near (within v.near2 in the
+// frame of structure 0): only then can two of the samples coincide.
 template <int NMAX>
-__device__ __forceinline__ void pairs_cov_fast(const DevVariogram& v, const double* PJ, int SJ, int r, int n,
+__device__ __forceinline__ bool pairs_cov_fast(const DevVariogram& v, const double* PJ, int SJ, int r, int n,
                                                double* T) {
 #pragma clang fp contract(fast)
-  if (r >= n) return;
+  if (r >= n) return false;
   const double* me = PJ + r * SJ;
   double acc[NMAX / 2];
 #pragma unroll
   for (int d = 0; d < NMAX / 2; ++d) acc[d] = 0.0;
-  uint64_t apart = 0;
+  uint64_t apart = 0, near = 0;
+  const double near2 = v.near2;
   for (int s = 0; s < v.S; ++s) {
     const int off = 3 + 3 * v.frame[s];
     const double sill = v.sill[s], ir = v.inv_range[s];
     const bool first = s == 0;
     switch (v.model[s]) {
       case BK_SPHERICAL:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
                         [](double t) { return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t); });
         break;
       case BK_EXPONENTIAL:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
                         [](double t) { return 1.0 - exp_fast(-3.0 * t); });
         break;
       case BK_GAUSSIAN:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
                         [](double t) { return 1.0 - exp_fast(-3.0 * t * t); });
         break;
       case BK_CUBIC:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc, [](double t) {
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc, [](double t) {
           const double t2 = t * t;
           return t >= 1.0 ? 1.0 : t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
         });
         break;
       case BK_PENTASPHERICAL:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc, [](double t) {
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc, [](double t) {
           const double t2 = t * t;
           return t >= 1.0 ? 1.0 : t * (1.875 + t2 * (-1.25 + 0.375 * t2));
         });
         break;
       default:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, acc,
+        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
                         [](double) { return __builtin_nan(""); });
     }
   }
   const double c0 = v.c0, nug = v.nugget;
+  uint64_t nearv = 0;
   const int half = (n - 1) / 2;
   const bool even = (n & 1) == 0;
 #pragma unroll
@@ -685,8 +691,10 @@ __device__ __forceinline__ void pairs_cov_fast(const DevVariogram& v, const doub
       j = j >= n ? j - n : j;
       const double c = ((apart >> d) & 1) ? c0 - (nug + acc[d - 1]) : c0;
       T[j < r ? r * (r + 1) / 2 + j : j * (j + 1) / 2 + r] = c;
+      nearv |= (near >> d) & 1;
     }
   }
+  return nearv != 0;
 }
 
 // The pair phase of the refined kernels for the system of target t (n
@@ -697,7 +705,11 @@ __device__ __forceinline__ void pairs_cov_fast(const DevVariogram& v, const doub
 // right-hand side c of sample r (the covariance to the target, or to the
 // block) and the strict lower triangle of the FP64 matrix into T (each pair
 // once, packed). Returns the guard.
-template <int NMAX>
+// With NEAR the coincidence test is left to the FP64 kernel: the function
+// returns whether a pair of the system lies within v.near2 in the frame of
+// structure 0 (a superset of the coincident pairs), and the caller hands such
+// a system to the FP64 kernel, which applies the exact test.
+template <int NMAX, bool NEAR = false>
 __device__ __forceinline__ bool refined_pairs(const SolveArgs& a, int t, int n, int r, int g, bool active,
                                               bool report_guard, double* PJ, int SJ, double* T, double& z,
                                               double& e, double& c) {
@@ -734,36 +746,48 @@ __device__ __forceinline__ bool refined_pairs(const SolveArgs& a, int t, int n, 
   wave_sync();
   // Coincident samples (coincident.rs:13-15, 27-35) send the system to the
   // guarded kernel.
-  bool close = false;
+  auto coincident = [&]() {
+    bool close = false;
 #pragma unroll
-  for (int j = 0; j < NMAX; ++j) {
-    if (j < r && r < n) {
-      const double* pj = PJ + j * SJ;
-      const double u = wx - pj[0], vv = wy - pj[1], w = wz - pj[2];
-      close |= (u * u + vv * vv) + w * w <= EPS2;
+    for (int j = 0; j < NMAX; ++j) {
+      if (j < r && r < n) {
+        const double* pj = PJ + j * SJ;
+        const double u = wx - pj[0], vv = wy - pj[1], w = wz - pj[2];
+        close |= (u * u + vv * vv) + w * w <= EPS2;
+      }
+      if (j % 4 == 3) asm volatile("" ::: "memory");
     }
-    if (j % 4 == 3) asm volatile("" ::: "memory");
-  }
-  const bool guard = group_any<NMAX>(close, g);
-  if (report_guard && guard && r == 0 && active) {
-    a.status[t] = BK_ST_GUARD;
-    a.guard[atomicAdd(a.guard_count, 1)] = t;
-  }
-  const bool go = active && !guard;
-  const bool live = go && r < n;
-  const double c0 = a.v.c0;
-  c = 0.0;
-  if (live) {
+    return group_any<NMAX>(close, g);
+  };
+  // The right-hand side of sample r.
+  auto rhs = [&]() {
+    double cc = 0.0;
     if (a.block) {
-      c = rhs_fast_world(a, t0, t1, t2, wx, wy, wz);
+      cc = rhs_fast_world(a, t0, t1, t2, wx, wy, wz);
     } else {
       double h[BK_MAX_FRAMES];
       const double* me = PJ + r * SJ;
       for (int f = 0; f < nf; ++f) h[f] = norm3_fast(me[3 + 3 * f], me[4 + 3 * f], me[5 + 3 * f]);
-      c = cov_fast64(a.v, h);
+      cc = cov_fast64(a.v, h);
     }
+    return cc;
+  };
+  bool guard;
+  c = 0.0;
+  if constexpr (NEAR) {
+    if (active && r < n) c = rhs();
+    const bool near = active && pairs_cov_fast<NMAX>(a.v, PJ, SJ, r, n, T);
+    guard = group_any<NMAX>(near, g);
+  } else {
+    guard = coincident();
+    const bool go = active && !guard;
+    if (go && r < n) c = rhs();
+    if (go) pairs_cov_fast<NMAX>(a.v, PJ, SJ, r, n, T);
   }
-  if (go) pairs_cov_fast<NMAX>(a.v, PJ, SJ, r, n, T);
+  if (report_guard && guard && r == 0 && active) {
+    a.status[t] = BK_ST_GUARD;
+    a.guard[atomicAdd(a.guard_count, 1)] = t;
+  }
   wave_sync();
   return guard;
 }
@@ -1012,8 +1036,8 @@ k_krige32(const SolveArgs a) {
 // whose two blocks are the two systems: block b takes entry i of its
 // operands from lane 32 b + i, and lane l = j + 32 h holds column j of block b
 // at the rows i = 8 q + 4 h + p (q, p < 4) in accumulator register 16 b + 4 q + p.
-// The lane also keeps the FP64 entries of those rows and columns for the
-// residual (C is symmetric, so lane j's column is row j as well). Gauss-Jordan
+// The FP64 matrices stay in LDS as packed triangles for the residuals (C is
+// symmetric, so lane j's column is row j as well). Gauss-Jordan
 // pivot k is the rank-one update X -= u v' with u_i = X_ik / d and v_j = X_kj
 // off the pivot, u_k = 1 - 1/d and v_k = d + 1 (d = X_kk): it leaves 1/d on
 // the diagonal, row k / d, column k / -d and the Schur complement elsewhere,
@@ -1025,7 +1049,11 @@ k_krige32(const SolveArgs a) {
 // i > k. A pivot costs about a dozen VALU instructions besides the matrix
 // instruction, where a lane-per-row sweep issues 32 fused multiply-adds per
 // row.
-__host__ __device__ constexpr int sysm_doubles(int nf) { return 2 * 32 * rec_doubles(nf) + 2 * 528; }
+__host__ __device__ constexpr int sysm_doubles(int nf) {
+  // Records of both systems (later the iterates and product vectors: 160
+  // doubles), then the two packed triangles.
+  return (2 * 32 * rec_doubles(nf) > 160 ? 2 * 32 * rec_doubles(nf) : 160) + 2 * 528;
+}
 
 typedef float bk_floatx32 __attribute__((ext_vector_type(32)));
 
@@ -1067,37 +1095,46 @@ __device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
   double z, e, c;
-  const bool guard = refined_pairs<32>(a, t, n, r, g, active, true, S + g * 32 * SJ, SJ, T + g * 528, z, e, c);
-  const bool go = active && !guard;
+  // A system with a near pair goes to the FP64 kernel (exact coincidence test).
+  const bool near =
+      refined_pairs<32, true>(a, t, n, r, g, active, false, S + g * 32 * SJ, SJ, T + g * 528, z, e, c);
+  const bool guard = false;
+  const bool go = active && !near;
   const double c0 = a.v.c0;
-  const double ic0 = 1.0 / c0;
+  const double ic0 = a.v.inv_c0;
   const bool go0 = __builtin_amdgcn_readlane(go ? 1 : 0, 0) != 0, go1 = __builtin_amdgcn_readlane(go ? 1 : 0, 32) != 0;
-  const int n0 = go0 ? __builtin_amdgcn_readlane(n, 0) : 0, n1 = go1 ? __builtin_amdgcn_readlane(n, 32) : 0;
-  // Column r of both systems' matrices at the lane's rows (identity rows and
-  // columns beyond n).
-  const double e_o = other_half(e, g);
-  double cr[32];
-  bk_floatx32 X;
+  // The diagonal, and identity rows beyond n (all rows of a system not solved
+  // here), complete the packed triangle of each system.
+  {
+    double* Tg = T + g * 528;
+    const int tr = r * (r + 1) / 2;
+    if (go && r < n) {
+      Tg[tr + r] = c0 + e;
+    } else {
 #pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const double* Tb = T + b * 528;
-    const int nb = b ? n1 : n0;
-    const double eb = (b == g) ? e : e_o;
+      for (int j = 0; j < 32; ++j)
+        if (j <= r) Tg[tr + j] = j == r ? c0 : 0.0;
+    }
+  }
+  wave_sync();
+  // Column r of both systems' matrices at the lane's rows i = i0 + 4 g
+  // (i0 = 8 q + p): entry (i, r) of the packed triangle, at tri(i) + r for
+  // i >= r and tri(r) + i otherwise, with tri(i0 + 4 g) = tri(i0) + g (4 i0 + 10).
+  // The triangles stay in LDS for the residuals, read through these offsets.
+  int idx[16];
+  bk_floatx32 X;
+  {
+    const int tr = r * (r + 1) / 2 + 4 * g, rg = r - 4 * g;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int p = 0; p < 4; ++p) {
-        const int j = 8 * q + 4 * g + p;
-        double v = 0.0;
-        if (j == r)
-          v = r < nb ? c0 + eb : c0;
-        else if (r < nb && j < nb)
-          v = j < r ? Tb[r * (r + 1) / 2 + j] : Tb[j * (j + 1) / 2 + r];
-        cr[16 * b + 4 * q + p] = v;
-        X[16 * b + 4 * q + p] = (float)(v * ic0);
+        const int i0 = 8 * q + p;
+        idx[4 * q + p] = i0 >= rg ? r + g * (4 * i0 + 10) + i0 * (i0 + 1) / 2 : tr + i0;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) X[16 * b + 4 * q + p] = (float)(T[b * 528 + idx[4 * q + p]] * ic0);
       }
   }
-  wave_sync();  // the triangles are read; their space holds the vectors below
   // The smallest pivot of each system as an integer over the float bits: for
   // floats >= 0 the order is the same, a negative pivot is a negative integer,
   // and a NaN pivot fails the convergence test later.
@@ -1123,8 +1160,8 @@ __device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int
   // products take the lane's 16 rows of column r of both systems and the two
   // halves' partial sums: C x exactly (C is symmetric), and X' v for X v (the
   // FP32 inverse is symmetric to rounding, which the refinement absorbs).
-  double* X2 = T;                    // iterates, double2 [b][j]
-  float* V2 = (float*)(T + 2 * 64);  // product inputs, float2 [b][j]
+  double* X2 = S;                    // iterates, double2 [b][j] (the records are no longer read)
+  float* V2 = (float*)(S + 2 * 64);  // product inputs, float2 [b][j]
   auto apply = [&](float va, float vb, float (&ya)[2], float (&yb)[2]) {
     reinterpret_cast<float2*>(V2)[g * 32 + r] = make_float2(va, vb);
     wave_sync();
@@ -1181,8 +1218,9 @@ __device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           const double2 xv = reinterpret_cast<const double2*>(X2)[32 * b + 8 * q + 4 * g + p];
-          sa = fma(cr[16 * b + 4 * q + p], xv.x, sa);
-          sb = fma(cr[16 * b + 4 * q + p], xv.y, sb);
+          const double cv = T[b * 528 + idx[4 * q + p]];
+          sa = fma(cv, xv.x, sa);
+          sb = fma(cv, xv.y, sb);
         }
       ra[b] = rha[b] - halves_sum(sa);
       rb[b] = rhb[b] - halves_sum(sb);
@@ -1207,7 +1245,7 @@ __device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int
     }
   }
   const bool bad = g ? bad1 : bad0, conv = g ? conv1 : conv0;
-  if (active && !guard && (bad || !conv)) {
+  if (active && (near || bad || !conv)) {
     if (r == 0) {
       a.status[t] = BK_ST_REDO64;
       a.redo[atomicAdd(a.redo_count, 1)] = t;
@@ -1227,7 +1265,7 @@ __device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int
   }
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_krige32m(const SolveArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_krige32m(const SolveArgs a) {
   const int64_t nl = list_len(a);
   UnitLoop q = unit_loop((nl + 1) / 2, a.work);
   while (q.unit < q.end) {
