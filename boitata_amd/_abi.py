@@ -266,5 +266,8 @@ AUTOTUNE_REPORT = ("cell_factor", "search_mode", "waves", "edge0", "edge1", "edg
 
 STAT_SEARCH_SPILL = 0
 STAT_GRAPH_LAUNCHES = 1
+STAT_REFINED = 2
+STAT_REFINE_SWEEPS = 3
+STAT_REFINE_REDO = 4
 
 STAGES = ("grid", "search", "bucket", "solve", "deferred", "total")

@@ -93,10 +93,9 @@ struct bk_ctx {
   bk::TileBuffers tiles;
   int search_mode = 0;  // uncapped searches: 0 wave per target, 1..8 tiles of that reach, -1 lane per target
   int solve_waves = 2;  // wavefronts per workgroup of the solve kernels
-  // FP32+IR solves buckets of at most this many samples; larger systems take
-  // the FP64 kernel (at n <= 64 the refined kernel's LDS allows ~3 waves per
-  // CU and measured 2x slower than the FP64 Cholesky kernel on gfx950).
-  int refine_max = 32;
+  // FP32+IR solves buckets of at most this many samples by FP32 elimination
+  // on the matrix cores and FP64 refinement; larger buckets take the FP64 kernel.
+  int refine_max = 64;
   GridDev grid{};
   bool grid_valid = false;
   double grid_F[9] = {0};
@@ -741,6 +740,7 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t 
   a.redo = (int32_t*)c->redo.p;
   a.guard_count = counts + 4;
   a.redo_count = counts + 5;
+  a.refine_stats = counts + 6;
   // The refined form covers spherical, exponential, Gaussian, cubic and
   // pentaspherical structures (fast FP64 covariances); a variogram with any
   // other model takes the FP64 kernel in either precision.
@@ -1922,6 +1922,19 @@ int bk_get_stat(bk_ctx* c, int32_t which, int64_t* value) {
     case BK_STAT_GRAPH_LAUNCHES:
       *value = c->graph_launches;
       return 0;
+    case BK_STAT_REFINED:
+    case BK_STAT_REFINE_SWEEPS:
+    case BK_STAT_REFINE_REDO: {
+      // Pass totals kept on the device with the bucket counts.
+      *value = 0;
+      if (!c->counts.p) return 0;
+      HIPOK(c, hipSetDevice(c->device));
+      int32_t h[8] = {0};
+      HIPOK(c, hipMemcpyAsync(h, c->counts.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      HIPOK(c, hipStreamSynchronize(c->stream));
+      *value = which == BK_STAT_REFINED ? h[6] : (which == BK_STAT_REFINE_SWEEPS ? h[7] : h[5]);
+      return 0;
+    }
     case BK_STAT_SEARCH_SPILL: {
       // The count stays on the device during the pass; read it now.
       *value = 0;

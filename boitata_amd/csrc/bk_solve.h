@@ -63,6 +63,9 @@ struct SolveArgs {
   double tol;           // FP32+IR: stop when the correction is below tol * |x|
   // Per-system diagnostics of the refined form (may be null): sweeps taken.
   int32_t* iters;
+  // Pass totals of the refined form (may be null): [0] systems it solved,
+  // [1] their refinement sweeps (zeroed before the pass).
+  int32_t* refine_stats;
   // Leave-one-out gradient: the sample held out by each system, and per
   // system (loss, used, gradient[P]).
   const int64_t* left_out_ids;

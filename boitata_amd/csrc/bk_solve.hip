@@ -605,33 +605,36 @@ __device__ __forceinline__ void row_cov_fast(const DevVariogram& v, const double
 // Each unordered pair of the group's n samples once: lane r takes the pairs
 // (r, (r + d) mod n) for d = 1 .. NMAX/2 (d = n/2 only from r < n/2 when n is
 // even, d <= (n - 1)/2 otherwise), so every lane evaluates about n/2 pairs.
-// acc[d - 1] receives C(s_r, s_j) for valid d.
-template <int NMAX, class Shape>
-__device__ __forceinline__ void half_pass(const double* PJ, int SJ, int off, const double* me, int r, int n,
+// The offsets d0 .. d0 + CH - 1 of one chunk: acc[d - d0] receives
+// C(s_r, s_j) for valid d.
+template <int CH, class Shape>
+__device__ __forceinline__ void half_pass(const double* PJ, int SJ, int off, const double* me, int r, int n, int d0,
                                           double sill, double ir, bool first, uint64_t& apart, uint64_t& near,
                                           double near2, double* acc, Shape shape) {
 #pragma clang fp contract(fast)
   const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
 #pragma unroll
-  for (int d = 1; d <= NMAX / 2; ++d) {
+  for (int dd = 0; dd < CH; ++dd) {
+    const int d = d0 + dd;
     // For small n, r + d can pass n twice; such d are not valid pairs and
     // read sample 0 instead of beyond the records.
     int j = r + d;
     j = j >= n ? j - n : j;
     j = j < n ? j : 0;
     const double* pj = PJ + j * SJ + off;
-    const double d0 = q0 - pj[0], d1 = q1 - pj[1], d2 = q2 - pj[2];
-    const double l2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const double d0v = q0 - pj[0], d1 = q1 - pj[1], d2 = q2 - pj[2];
+    const double l2 = d0v * d0v + d1 * d1 + d2 * d2;
     const double h = sqrt_fast(l2);
     if (first && h > 0.0) apart |= 1ull << d;
     if (first && l2 <= near2) near |= 1ull << d;
-    acc[d - 1] += sill * shape(h * ir);
-    if (d % 4 == 0) asm volatile("" ::: "memory");
+    acc[dd] += sill * shape(h * ir);
+    if (dd % 4 == 3) asm volatile("" ::: "memory");
   }
 }
 
 // The strict lower triangle of the group's covariance matrix into T (packed,
-// T[i (i + 1) / 2 + j], j < i), each pair evaluated once (see half_pass).
+// T[i (i + 1) / 2 + j], j < i), each pair evaluated once (see half_pass), in
+// chunks of at most 16 offsets so that a lane holds at most 16 partial sums.
 // Returns whether one of the lane's pairs is near (within v.near2 in the
 // frame of structure 0): only then can two of the samples coincide.
 template <int NMAX>
@@ -639,59 +642,63 @@ __device__ __forceinline__ bool pairs_cov_fast(const DevVariogram& v, const doub
                                                double* T) {
 #pragma clang fp contract(fast)
   if (r >= n) return false;
+  constexpr int CH = NMAX / 2 < 16 ? NMAX / 2 : 16;
   const double* me = PJ + r * SJ;
-  double acc[NMAX / 2];
-#pragma unroll
-  for (int d = 0; d < NMAX / 2; ++d) acc[d] = 0.0;
-  uint64_t apart = 0, near = 0;
+  uint64_t apart = 0, near = 0, nearv = 0;
   const double near2 = v.near2;
-  for (int s = 0; s < v.S; ++s) {
-    const int off = 3 + 3 * v.frame[s];
-    const double sill = v.sill[s], ir = v.inv_range[s];
-    const bool first = s == 0;
-    switch (v.model[s]) {
-      case BK_SPHERICAL:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
-                        [](double t) { return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t); });
-        break;
-      case BK_EXPONENTIAL:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
-                        [](double t) { return 1.0 - exp_fast(-3.0 * t); });
-        break;
-      case BK_GAUSSIAN:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
-                        [](double t) { return 1.0 - exp_fast(-3.0 * t * t); });
-        break;
-      case BK_CUBIC:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc, [](double t) {
-          const double t2 = t * t;
-          return t >= 1.0 ? 1.0 : t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
-        });
-        break;
-      case BK_PENTASPHERICAL:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc, [](double t) {
-          const double t2 = t * t;
-          return t >= 1.0 ? 1.0 : t * (1.875 + t2 * (-1.25 + 0.375 * t2));
-        });
-        break;
-      default:
-        half_pass<NMAX>(PJ, SJ, off, me, r, n, sill, ir, first, apart, near, near2, acc,
-                        [](double) { return __builtin_nan(""); });
-    }
-  }
   const double c0 = v.c0, nug = v.nugget;
-  uint64_t nearv = 0;
   const int half = (n - 1) / 2;
   const bool even = (n & 1) == 0;
 #pragma unroll
-  for (int d = 1; d <= NMAX / 2; ++d) {
-    const bool valid = d <= half || (even && d == n / 2 && r < n / 2);
-    if (valid) {
-      int j = r + d;
-      j = j >= n ? j - n : j;
-      const double c = ((apart >> d) & 1) ? c0 - (nug + acc[d - 1]) : c0;
-      T[j < r ? r * (r + 1) / 2 + j : j * (j + 1) / 2 + r] = c;
-      nearv |= (near >> d) & 1;
+  for (int d0 = 1; d0 <= NMAX / 2; d0 += CH) {
+    double acc[CH];
+#pragma unroll
+    for (int dd = 0; dd < CH; ++dd) acc[dd] = 0.0;
+    for (int s = 0; s < v.S; ++s) {
+      const int off = 3 + 3 * v.frame[s];
+      const double sill = v.sill[s], ir = v.inv_range[s];
+      const bool first = s == 0;
+      switch (v.model[s]) {
+        case BK_SPHERICAL:
+          half_pass<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, apart, near, near2, acc,
+                        [](double t) { return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t); });
+          break;
+        case BK_EXPONENTIAL:
+          half_pass<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, apart, near, near2, acc,
+                        [](double t) { return 1.0 - exp_fast(-3.0 * t); });
+          break;
+        case BK_GAUSSIAN:
+          half_pass<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, apart, near, near2, acc,
+                        [](double t) { return 1.0 - exp_fast(-3.0 * t * t); });
+          break;
+        case BK_CUBIC:
+          half_pass<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, apart, near, near2, acc, [](double t) {
+            const double t2 = t * t;
+            return t >= 1.0 ? 1.0 : t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
+          });
+          break;
+        case BK_PENTASPHERICAL:
+          half_pass<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, apart, near, near2, acc, [](double t) {
+            const double t2 = t * t;
+            return t >= 1.0 ? 1.0 : t * (1.875 + t2 * (-1.25 + 0.375 * t2));
+          });
+          break;
+        default:
+          half_pass<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, apart, near, near2, acc,
+                        [](double) { return __builtin_nan(""); });
+      }
+    }
+#pragma unroll
+    for (int dd = 0; dd < CH; ++dd) {
+      const int d = d0 + dd;
+      const bool valid = d <= half || (even && d == n / 2 && r < n / 2);
+      if (valid) {
+        int j = r + d;
+        j = j >= n ? j - n : j;
+        const double c = ((apart >> d) & 1) ? c0 - (nug + acc[dd]) : c0;
+        T[j < r ? r * (r + 1) / 2 + j : j * (j + 1) / 2 + r] = c;
+        nearv |= (near >> d) & 1;
+      }
     }
   }
   return nearv != 0;
@@ -1081,7 +1088,9 @@ __device__ __forceinline__ double other_half(double v, int h) {
   return __longlong_as_double((long long)(((unsigned long long)u << 32) | l));
 }
 
-__device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int unit) {
+// Returns the systems this unit solved by refinement and their sweeps (x, y),
+// the same in every lane.
+__device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int unit) {
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
   const int SJ = rec_doubles(nf);
@@ -1263,18 +1272,229 @@ __device__ __forceinline__ void krige32m_blk(const SolveArgs& a, int64_t nl, int
     finish_outputs<32>(a, t, r, n, true, BK_OK, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
     if (a.iters && r == 0) a.iters[t] = g ? it1 : it0;
   }
+  const bool ok0 = go0 && !bad0 && conv0, ok1 = go1 && !bad1 && conv1;
+  return make_int2((int)ok0 + (int)ok1, (ok0 ? it0 : 0) + (ok1 ? it1 : 0));
+}
+
+// Adds a wave's refinement totals to the pass totals.
+__device__ __forceinline__ void add_refine_stats(const SolveArgs& a, int2 st) {
+  if (a.refine_stats && (threadIdx.x & (BK_WAVE - 1)) == 0 && st.x > 0) {
+    atomicAdd(a.refine_stats, st.x);
+    atomicAdd(a.refine_stats + 1, st.y);
+  }
+}
+
+// One system of 33 to 64 samples per wavefront, in the same arithmetic. The
+// 64 x 64 matrix is four 32 x 32 tiles held by two accumulators of
+// v_mfma_f32_32x32x1_2b_f32: A1 holds tiles (0, 0) and (1, 1) as its blocks 0
+// and 1, A2 tiles (0, 1) and (1, 0). Lane l = j + 32 h thus holds columns j
+// and 32 + j at the rows i = 32 I + 8 q + 4 h + p. Pivot k updates A1 with u
+// and v (entry l of each from lane l) and A2 with u and v's halves swapped
+// (entry l ^ 32 of v from lane l). Row k lies in the registers of row block
+// k / 32 at the half-wave (k / 4) % 2, one register per column block; a
+// permlane32 swap of the two gives v, another the swapped v.
+__host__ __device__ constexpr int sysm64_doubles(int nf) {
+  // Records (later the iterates and product vectors: 192 doubles), then the
+  // packed triangle with the diagonal.
+  return (64 * rec_doubles(nf) > 192 ? 64 * rec_doubles(nf) : 192) + 2080;
+}
+
+__device__ __forceinline__ int2 krige64m_blk(const SolveArgs& a, int64_t nl, int unit) {
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int SJ = rec_doubles(nf);
+  const int tid = opaque_tid();
+  const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
+  const int j = lane & 31, h = lane >> 5;
+  double* S = dyn + wave * sysm64_doubles(nf);
+  double* T = S + 64 * SJ;  // packed lower triangle with the diagonal, 2080 doubles
+  const bool active = unit < nl;
+  const int t = active ? a.list[unit] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  double z, e, c;
+  // A system with a near pair goes to the FP64 kernel (exact coincidence test).
+  const bool near = refined_pairs<64, true>(a, t, n, lane, 0, active, false, S, SJ, T, z, e, c);
+  const bool go = active && !near;
+  const double c0 = a.v.c0;
+  const double ic0 = a.v.inv_c0;
+  {
+    const int tr = lane * (lane + 1) / 2;
+    if (go && lane < n) {
+      T[tr + lane] = c0 + e;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 64; ++q)
+        if (q <= lane) T[tr + q] = q == lane ? c0 : 0.0;
+    }
+  }
+  wave_sync();
+  // Entry (i, col) of the packed triangle for the lane's rows i = 32 I + i0 +
+  // 4 h (i0 = 8 q + p) and columns col = 32 J + j: tri(i) + col for i >= col,
+  // else tri(col) + i. The triangle stays in LDS for the residuals.
+  auto entry = [&](int I, int J, int q, int p) {
+    const int i = 32 * I + 8 * q + 4 * h + p, col = 32 * J + j;
+    if (I > J) return T[i * (i + 1) / 2 + col];
+    if (I < J) return T[col * (col + 1) / 2 + i];
+    return T[i >= col ? i * (i + 1) / 2 + col : col * (col + 1) / 2 + i];
+  };
+  bk_floatx32 A1, A2;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      A1[4 * q + p] = (float)(entry(0, 0, q, p) * ic0);
+      A1[16 + 4 * q + p] = (float)(entry(1, 1, q, p) * ic0);
+      A2[4 * q + p] = (float)(entry(0, 1, q, p) * ic0);
+      A2[16 + 4 * q + p] = (float)(entry(1, 0, q, p) * ic0);
+    }
+  int dmin = 0x7f7fffff;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    const int K = k / 32, kk = k % 32;
+    const int rk = 4 * (kk / 8) + (kk % 4);
+    const float R0 = K == 0 ? A1[rk] : A2[16 + rk];  // row k, columns 0..31
+    const float R1 = K == 0 ? A2[rk] : A1[16 + rk];  // row k, columns 32..63
+    const auto pv = __builtin_amdgcn_permlane32_swap(__float_as_uint(R0), __float_as_uint(R1), false, false);
+    const auto ps = __builtin_amdgcn_permlane32_swap(__float_as_uint(R1), __float_as_uint(R0), false, false);
+    const bool hi = (kk / 4) % 2;
+    const float row = __uint_as_float(hi ? pv[1] : pv[0]);   // X_k,l
+    const float rows = __uint_as_float(hi ? ps[1] : ps[0]);  // X_k,l^32
+    const int db = __builtin_amdgcn_readlane(__float_as_int(row), k);
+    dmin = min(dmin, db);
+    const float inv = __builtin_amdgcn_rcpf(__int_as_float(db));
+    // The lane's offset from the pivot, opaque so that the compiler compares
+    // here rather than holding 64 pivots' lane masks.
+    int dk = lane - k, dks = (lane ^ 32) - k;
+    asm volatile("" : "+v"(dk), "+v"(dks));
+    const float sg = dk < 0 ? inv : -inv;
+    float u = row * sg;
+    u = dk == 0 ? inv - 1.0f : u;
+    const float v = dk == 0 ? row + 1.0f : row;
+    const float vs = dks == 0 ? rows + 1.0f : rows;
+    A1 = __builtin_amdgcn_mfma_f32_32x32x1f32(u, v, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_32x32x1f32(u, vs, A2, 0, 0, 0);
+  }
+  const bool bad = !(dmin > __float_as_int(1e-5f));
+  // Solves: lane r publishes entry r of a vector; the products take the
+  // lane's 32 rows of columns j and 32 + j and the two halves' partial sums.
+  double* X2 = S;                   // iterates, double2 [64] (the records are no longer read)
+  float* V2 = (float*)(S + 128);    // product inputs, float2 [64]
+  auto apply = [&](float va, float vb, float& ya, float& yb) {
+    reinterpret_cast<float2*>(V2)[lane] = make_float2(va, vb);
+    wave_sync();
+    float sa0 = 0.0f, sb0 = 0.0f, sa1 = 0.0f, sb1 = 0.0f;  // columns j, 32 + j
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 w0 = reinterpret_cast<const float4*>(V2)[16 * I + 4 * q + 2 * h];
+        const float4 w1 = reinterpret_cast<const float4*>(V2)[16 * I + 4 * q + 2 * h + 1];
+        const float wa[4] = {w0.x, w0.z, w1.x, w1.z}, wb[4] = {w0.y, w0.w, w1.y, w1.w};
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const float x0 = I == 0 ? A1[4 * q + p] : A2[16 + 4 * q + p];  // column j
+          const float x1 = I == 0 ? A2[4 * q + p] : A1[16 + 4 * q + p];  // column 32 + j
+          sa0 = fmaf(x0, wa[p], sa0);
+          sb0 = fmaf(x0, wb[p], sb0);
+          sa1 = fmaf(x1, wa[p], sa1);
+          sb1 = fmaf(x1, wb[p], sb1);
+        }
+      }
+    const float ya0 = halves_sum(sa0), yb0 = halves_sum(sb0), ya1 = halves_sum(sa1), yb1 = halves_sum(sb1);
+    ya = h ? ya1 : ya0;
+    yb = h ? yb1 : yb0;
+    wave_sync();
+  };
+  const bool live = go && lane < n;
+  const bool ordinary = a.kind != BK_SIMPLE;
+  const double ba = live ? c : 0.0, bb = (live && ordinary) ? 1.0 : 0.0;
+  float fa, fb;
+  apply((float)(ba * ic0), (float)(bb * ic0), fa, fb);
+  double xa = fa, xb = fb;
+  const float tolf = (float)a.tol;
+  const float ma = group_maxf<64>(fabsf(fa)), mb = group_maxf<64>(fabsf(fb));
+  bool conv = false;
+  int it = 0;
+  for (; it < a.max_iter && go && !bad; ++it) {
+    reinterpret_cast<double2*>(X2)[lane] = make_double2(xa, xb);
+    wave_sync();
+    double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0;
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const double2 xv = reinterpret_cast<const double2*>(X2)[32 * I + 8 * q + 4 * h + p];
+          const double c0v = entry(I, 0, q, p), c1v = entry(I, 1, q, p);
+          sa0 = fma(c0v, xv.x, sa0);
+          sb0 = fma(c0v, xv.y, sb0);
+          sa1 = fma(c1v, xv.x, sa1);
+          sb1 = fma(c1v, xv.y, sb1);
+        }
+    // Each column's partial sums over the two halves' rows: lane l's entry
+    // is column l (j for h = 0, 32 + j for h = 1).
+    const double ca0 = halves_sum(sa0), ca1 = halves_sum(sa1), cb0 = halves_sum(sb0), cb1 = halves_sum(sb1);
+    const double ca = h ? ca1 : ca0, cb = h ? cb1 : cb0;
+    wave_sync();
+    float da, db;
+    apply((float)((ba - ca) * ic0), (float)((bb - cb) * ic0), da, db);
+    xa += (double)da;
+    xb += (double)db;
+    if (__ballot(!(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb)) == 0) {
+      conv = true;
+      ++it;
+      break;
+    }
+  }
+  if (active && (near || bad || !conv)) {
+    if (lane == 0) {
+      a.status[t] = BK_ST_REDO64;
+      a.redo[atomicAdd(a.redo_count, 1)] = t;
+    }
+  }
+  const bool ok = go && !bad && conv;
+  const bool in = ok && lane < n;
+  const double sa = group_sum<64>(in ? xa : 0.0);
+  const double sb = group_sum<64>(in ? xb : 0.0);
+  const double mu = ordinary ? (sa - 1.0) / sb : 0.0;
+  const double w = xa - (ordinary ? mu * xb : 0.0);
+  if (ok) {
+    finish_outputs<64>(a, t, lane, n, true, BK_OK, w, z, c, mu, a.block ? a.cbb : c0, a.block ? a.cbb : c0);
+    if (a.iters && lane == 0) a.iters[t] = it;
+  }
+  return make_int2(ok ? 1 : 0, ok ? it : 0);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_krige64m(const SolveArgs a) {
+  const int64_t nl = list_len(a);
+  UnitLoop q = unit_loop(nl, a.work);
+  int2 st = make_int2(0, 0);
+  while (q.unit < q.end) {
+    const int next = unit_next(q);
+    const int2 u = krige64m_blk(kernarg_first<SolveArgs>(), nl, q.unit);
+    st.x += u.x;
+    st.y += u.y;
+    wave_sync();
+    q.unit = next;
+  }
+  add_refine_stats(a, st);
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_krige32m(const SolveArgs a) {
   const int64_t nl = list_len(a);
   UnitLoop q = unit_loop((nl + 1) / 2, a.work);
+  int2 st = make_int2(0, 0);
   while (q.unit < q.end) {
     // The next unit is taken before this one is solved, hiding the atomic.
     const int next = unit_next(q);
-    krige32m_blk(kernarg_first<SolveArgs>(), nl, q.unit);
+    const int2 u = krige32m_blk(kernarg_first<SolveArgs>(), nl, q.unit);
+    st.x += u.x;
+    st.y += u.y;
     wave_sync();
     q.unit = next;
   }
+  add_refine_stats(a, st);
 }
 #endif
 
@@ -2415,6 +2635,16 @@ static int run32(hipStream_t s, const SolveArgs& a) {
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+static int run64m(hipStream_t s, const SolveArgs& a) {
+  const SolveShape q = solve_shape<64>(a, sysm64_doubles(a.v.nf));
+  if (q.work == 0) return 0;
+  if (!a.work) return -1;  // persistent kernels take their units from this counter
+  const void* fn = (const void*)k_krige64m;
+  allow_lds(fn, q.sh);
+  k_krige64m<<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 static int run32m(hipStream_t s, const SolveArgs& a) {
   const SolveShape q = solve_shape<32>(a, sysm_doubles(a.v.nf) / 2);
   if (q.work == 0) return 0;
@@ -2526,7 +2756,7 @@ int launch_krige32(hipStream_t s, const SolveArgs& a, int nmax) {
     case 32:
       return run32m(s, a);
     case 64:
-      return run32<64>(s, a);
+      return run64m(s, a);
   }
   return -1;
 }

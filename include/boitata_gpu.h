@@ -40,7 +40,7 @@ enum {
   BK_SINGULAR = 2,           /* pivot <= 1e-12 * sill (kernel.rs:58-65) or LU breakdown */
   BK_INSUFFICIENT_DATA = 3,  /* no samples */
   BK_INVALID_PARAMETERS = 4,
-  BK_UNSUPPORTED = 5         /* a system beyond what the device path holds (n > BK_MAX_LU) */
+  BK_UNSUPPORTED = 5         /* a capped search whose pending candidates outgrew the device pool */
 };
 
 /* Return codes of the API calls. */
@@ -54,10 +54,23 @@ enum {
 /* Kriging kind, crates/estimation/src/krige.rs:20-28. */
 enum { BK_ORDINARY = 0, BK_SIMPLE = 1, BK_INDICATOR = 2 };
 
-/* Arithmetic of the solve. */
+/* Arithmetic of the solve.
+ * BK_FP64: FP64 throughout. Point systems follow kernel.rs:24-123 (covariances
+ *   in its operation order, left-looking Cholesky with its pivot test; the back
+ *   substitution subtracts from the last row up). Block systems take the same
+ *   Cholesky and Schur steps where block.rs:167-170 uses a bordered LU, and go
+ *   to that LU when the Cholesky pivot test fails; merged coincident samples,
+ *   systems above 64 samples and power models take the bordered LU.
+ * BK_FP32_IR: covariances in FP64 (a fast exp and square root), an FP32
+ *   Gauss-Jordan inverse of C / c0 on the matrix cores (MFMA), FP64 residual
+ *   refinement until the correction is below the tolerance, and the Schur
+ *   scalar and weights in FP64. Size buckets above the refine limit
+ *   (bk_set_refine_limit), variograms with a circular, sine-hole, Matern or
+ *   power structure, and systems whose FP32 pivot falls below 1e-5 of the
+ *   scaled sill or that do not converge take the FP64 path. */
 enum {
-  BK_FP64 = 0,     /* plain FP64, the reference's operation order */
-  BK_FP32_IR = 1   /* FP32 covariance and Cholesky, FP64 residual refinement and Schur scalar */
+  BK_FP64 = 0,
+  BK_FP32_IR = 1
 };
 
 /* Variogram model, crates/variogram/src/model.rs:11-30. */
@@ -316,12 +329,11 @@ int bk_set_search_tuning(bk_ctx* ctx, int32_t mode);
  * (default) or 4 wavefronts. Results do not depend on it. */
 int bk_set_solve_tuning(bk_ctx* ctx, int32_t waves);
 
-/* Largest size bucket (16, 32 or 64 samples; default 32) that BK_FP32_IR
+/* Largest size bucket (16, 32 or 64 samples; default 64) that BK_FP32_IR
  * solves by FP32 elimination with FP64 refinement (no counterpart in the
- * reference). Systems in larger buckets take the FP64 kernel, whose results
- * FP32+IR matches anyway: on gfx950 the refined kernel for up to 64 samples is
- * LDS-bound and slower than the FP64 Cholesky kernel. Values <= 0 keep the
- * setting. Applies to kriging passes and to the leave-one-out gradient. */
+ * reference). Systems in larger buckets take the FP64 kernel. Values <= 0
+ * keep the setting. Applies to kriging passes and to the leave-one-out
+ * gradient. */
 int bk_set_refine_limit(bk_ctx* ctx, int32_t max_n);
 
 /* Autotuner (no counterpart in the reference): times krige passes over the m
@@ -349,9 +361,16 @@ int bk_set_graphs(bk_ctx* ctx, int32_t enable);
 
 /* Counters (no counterpart in the reference): BK_STAT_SEARCH_SPILL, targets
  * the last wave or tiled search handed back to the lane-per-target search;
- * BK_STAT_GRAPH_LAUNCHES, bk_krige calls run as a graph so far. */
+ * BK_STAT_GRAPH_LAUNCHES, bk_krige calls run as a graph so far. Of the last
+ * kriging pass: BK_STAT_REFINED, systems solved by FP32 elimination and FP64
+ * refinement; BK_STAT_REFINE_SWEEPS, their refinement sweeps in total;
+ * BK_STAT_REFINE_REDO, systems the refined form handed to the FP64 kernel
+ * (a small pivot, no convergence, or two samples within a few micrometres). */
 #define BK_STAT_SEARCH_SPILL 0
 #define BK_STAT_GRAPH_LAUNCHES 1
+#define BK_STAT_REFINED 2
+#define BK_STAT_REFINE_SWEEPS 3
+#define BK_STAT_REFINE_REDO 4
 int bk_get_stat(bk_ctx* ctx, int32_t which, int64_t* value);
 
 /* Stage timing with HIP events recorded on the context's stream (no
