@@ -235,6 +235,8 @@ SIGNATURES = [
     ("bk_set_samples", C.c_int, [P, I64, P, P, P, P, P, P]),
     ("bk_neighbors", C.c_int, [P, C.POINTER(Search), C.POINTER(Variogram), I64, P, P, P, P, P, P, P]),
     ("bk_krige", C.c_int, [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, I32, I64, P, P, P, P, P, P, P, P]),
+    ("bk_krige_noalloc", C.c_int,
+     [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, I32, I64, P, P, P, P, P, P, P, P]),
     ("bk_block_krige", C.c_int,
      [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, C.POINTER(Block), I32, I64, P, P, P, P, P, P, P, P, P]),
     ("bk_loo", C.c_int, [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, I32, I64, P, P, P, P]),
@@ -264,6 +266,9 @@ EXTRA_SIGNATURES = [
 AUTOTUNE_REPORT = ("cell_factor", "search_mode", "waves", "edge0", "edge1", "edge2", "search_ms_before",
                    "solve_ms_before", "search_ms_after", "solve_ms_after", "refine_limit")
 
+STATIC_MAX_TARGETS = 65536
+STATIC_MAX_SAMPLES = 262144
+STATIC_MAX_NEIGHBOURS = 64
 STAT_SEARCH_SPILL = 0
 STAT_GRAPH_LAUNCHES = 1
 STAT_REFINED = 2

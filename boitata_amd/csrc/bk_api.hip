@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -115,6 +116,9 @@ struct bk_ctx {
   // Workspace of the LU for systems above BK_MAX_LU samples (searches with
   // max_samples > BK_MAX_LU only).
   Buf lu_big;
+  // Set by bk_krige_noalloc while its scratch views stand in for the buffers
+  // above: a buffer that would have to grow is an error, not an allocation.
+  bool fixed_buffers = false;
   // Tuning.
   int edges[3] = {16, 32, 64};
   int max_iter = 8;
@@ -197,6 +201,7 @@ int fail(bk_ctx* c, int code, const std::string& m) {
 int ensure(bk_ctx* c, Buf& b, size_t bytes) {
   if (bytes <= b.cap) return 0;
   if (c->capturing) return fail(c, BK_E_STATE, "a buffer would move while a pass is captured");
+  if (c->fixed_buffers) return fail(c, BK_E_STATE, "bk_krige_noalloc: a buffer beyond its static scratch");
   c->gen++;
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
@@ -807,6 +812,72 @@ int finalize(bk_ctx* c, int64_t m, int32_t* status, double* value, double* varia
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of bk_krige_noalloc: one static device array per GPU (each device
+// holds its own copy until the process ends), carved into the buffers of one
+// pass of at most BK_STATIC_MAX_TARGETS targets, BK_STATIC_MAX_SAMPLES
+// samples and BK_STATIC_MAX_NEIGHBOURS neighbours per target. Host code never
+// uses the array's name as an address: bk_create takes each device's address
+// once with hipGetSymbolAddress.
+namespace {
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr int64_t kStM = BK_STATIC_MAX_TARGETS, kStN = BK_STATIC_MAX_SAMPLES, kStK = BK_STATIC_MAX_NEIGHBOURS;
+constexpr int64_t kStCells = 4 * kStN + 64;  // the grid's side rule at cell factor 1 (grid.rs:83-106)
+constexpr size_t kStScan = 1 << 20;          // hipCUB scan storage over kStCells + 1 counts
+struct StaticPlan {
+  size_t nidx, ncount, sstatus, status, lists, counts, guard, redo, work, ids, entries, cell_of, px, bounds, start,
+      fill, scan, total;
+};
+constexpr StaticPlan static_plan() {
+  StaticPlan p{};
+  size_t o = 0;
+  auto take = [&o](size_t bytes) {
+    const size_t at = o;
+    o += al256(bytes);
+    return at;
+  };
+  p.nidx = take(kStM * kStK * 4);
+  p.ncount = take(kStM * 4);
+  p.sstatus = take(kStM * 4);
+  p.status = take(kStM * 4);
+  p.lists = take(kStM * 4 * 4);
+  p.counts = take(8 * 4);
+  p.guard = take(kStM * 4);
+  p.redo = take(kStM * 4);
+  p.work = take(4 * BK_CTR_INTS * 4);
+  p.ids = take((3 * kStM + 1) * 4);
+  p.entries = take(kStN * 32);
+  p.cell_of = take(kStN * 4);
+  p.px = take(3 * kStN * 8);
+  p.bounds = take((1024 * 6 + 16) * 8);
+  p.start = take((kStCells + 1) * 4);
+  p.fill = take((kStCells + 1) * 4);
+  p.scan = take(kStScan);
+  p.total = o;
+  return p;
+}
+constexpr StaticPlan kStPlan = static_plan();
+static_assert(kStPlan.total <= 48u * 1024 * 1024, "static scratch above 48 MiB");
+
+__device__ __align__(256) unsigned char scratch[kStPlan.total];
+
+std::mutex g_scratch_mu;         // guards g_scratch
+void* g_scratch[64] = {};        // scratch's address on each device
+std::mutex g_scratch_calls[64];  // one bk_krige_noalloc at a time per device (they share its scratch)
+
+// Takes the device's address of scratch once (the device current).
+void scratch_setup(int device) {
+  if (device < 0 || device >= 64) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p)
+    g_scratch[device] = p;
+  else
+    (void)hipGetLastError();
+}
+}  // namespace
+
 extern "C" {
 
 const char* bk_version(void) { return "boitata-mi355x 0.1.0 (gfx950)"; }
@@ -829,6 +900,7 @@ int bk_create(bk_ctx** out, int device) {
     delete c;
     return fail(nullptr, BK_E_HIP, "shell table upload failed");
   }
+  scratch_setup(device);
   *out = c;
   return 0;
 }
@@ -1163,6 +1235,85 @@ int bk_krige(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind
              double* lagrange, int32_t* n_used, int32_t* status) {
   return krige_common(c, vg, s, kind, param, nullptr, precision, m, tx, ty, tz, nullptr, false, value, variance,
                       lagrange, nullptr, n_used, status);
+}
+
+int bk_krige_noalloc(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                     int32_t precision, int64_t m, const double* tx, const double* ty, const double* tz, double* value,
+                     double* variance, double* lagrange, int32_t* n_used, int32_t* status) {
+  if (!c || !vg || !s) return BK_E_INVALID;
+  if (m < 0 || m > BK_STATIC_MAX_TARGETS)
+    return fail(c, BK_E_INVALID, "bk_krige_noalloc: m must be in [0, BK_STATIC_MAX_TARGETS = 65536]");
+  if (c->n > BK_STATIC_MAX_SAMPLES)
+    return fail(c, BK_E_INVALID, "bk_krige_noalloc: at most BK_STATIC_MAX_SAMPLES = 262144 samples");
+  if (std::min<int64_t>(s->max_samples, std::max<int64_t>(c->n, 1)) > BK_STATIC_MAX_NEIGHBOURS)
+    return fail(c, BK_E_INVALID, "bk_krige_noalloc: max_samples above BK_STATIC_MAX_NEIGHBOURS = 64");
+  if (s->calibration) return fail(c, BK_E_INVALID, "bk_krige_noalloc: calibrated searches are not served");
+  HIPOK(c, hipSetDevice(c->device));
+  if (!is_device(tx) || !is_device(ty) || !is_device(tz) || !is_device(value) || !is_device(variance) ||
+      (lagrange && !is_device(lagrange)) || (n_used && !is_device(n_used)) || (status && !is_device(status)))
+    return fail(c, BK_E_INVALID, "bk_krige_noalloc: targets and outputs must be device arrays");
+  if (c->device < 0 || c->device >= 64 || !g_scratch[c->device])
+    return fail(c, BK_E_STATE, "bk_krige_noalloc: the static scratch of this device is not available");
+  if (bk::grid_scan_bytes(kStCells) > kStScan)
+    return fail(c, BK_E_STATE, "bk_krige_noalloc: the grid scan needs more storage than its scratch holds");
+  std::lock_guard<std::mutex> lk(g_scratch_calls[c->device]);
+  unsigned char* base = (unsigned char*)g_scratch[c->device];
+  // The context's buffers and grid give way to views of the scratch for
+  // this call, and come back afterwards.
+  Buf* bufs[9] = {&c->nidx, &c->ncount, &c->sstatus, &c->status, &c->lists, &c->counts, &c->guard, &c->redo, &c->work};
+  const size_t offs[9] = {kStPlan.nidx,  kStPlan.ncount, kStPlan.sstatus, kStPlan.status, kStPlan.lists,
+                          kStPlan.counts, kStPlan.guard, kStPlan.redo,   kStPlan.work};
+  const size_t caps[9] = {(size_t)kStM * kStK * 4, (size_t)kStM * 4, (size_t)kStM * 4, (size_t)kStM * 4,
+                          (size_t)kStM * 16,       32,               (size_t)kStM * 4, (size_t)kStM * 4,
+                          4 * BK_CTR_INTS * 4};
+  Buf saved[9];
+  for (int k = 0; k < 9; ++k) {
+    saved[k] = *bufs[k];
+    bufs[k]->p = base + offs[k];
+    bufs[k]->cap = caps[k];
+  }
+  bk::GridBuffers gb_saved = c->gb;
+  GridDev grid_saved = c->grid;
+  const bool grid_valid_saved = c->grid_valid;
+  double grid_F_saved[9];
+  std::memcpy(grid_F_saved, c->grid_F, sizeof(grid_F_saved));
+  const double grid_cf_saved = c->grid_cf;
+  bk::TileBuffers tiles_saved = c->tiles;
+  const bool graphs_saved = c->graphs;
+  bk::GridBuffers& g = c->gb;
+  g = bk::GridBuffers();
+  g.entries = (double4*)(base + kStPlan.entries);
+  g.cell_of = (uint32_t*)(base + kStPlan.cell_of);
+  g.px = (double*)(base + kStPlan.px);
+  g.bounds = (double*)(base + kStPlan.bounds);
+  g.start = (uint32_t*)(base + kStPlan.start);
+  g.fill = (uint32_t*)(base + kStPlan.fill);
+  g.scan_tmp = base + kStPlan.scan;
+  g.cap_n = kStN;
+  g.cap_cells = kStCells;
+  g.scan_tmp_bytes = kStScan;
+  g.fixed = true;
+  c->tiles = bk::TileBuffers();
+  c->tiles.ids = (int32_t*)(base + kStPlan.ids);
+  c->tiles.cap_ids = 3 * kStM + 1;
+  c->tiles.fixed = true;
+  c->grid_valid = false;
+  c->graphs = false;
+  c->fixed_buffers = true;
+  int r = krige_common(c, vg, s, kind, param, nullptr, precision, m, tx, ty, tz, nullptr, false, value, variance,
+                       lagrange, nullptr, n_used, status);
+  if (r == 0 && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(c, BK_E_HIP, "bk_krige_noalloc: sync failed");
+  c->fixed_buffers = false;
+  for (int k = 0; k < 9; ++k) *bufs[k] = saved[k];
+  c->gb = gb_saved;
+  c->grid = grid_saved;
+  c->grid_valid = grid_valid_saved;
+  std::memcpy(c->grid_F, grid_F_saved, sizeof(grid_F_saved));
+  c->grid_cf = grid_cf_saved;
+  c->tiles = tiles_saved;
+  c->graphs = graphs_saved;
+  c->gen++;
+  return r;
 }
 
 int bk_block_krige(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,

@@ -82,9 +82,15 @@ struct GridBuffers {
   int64_t cap_n = 0, cap_cells = 0, cap_blocks = 0;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
+  // Buffers of fixed capacity owned by the caller: the build allocates
+  // nothing, widens its cells until they fit cap_cells (the result does not
+  // depend on the cell size) and fails where n or the scan's storage does not fit.
+  bool fixed = false;
 };
 int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const double* z, const double F[9],
                const double origin[3], double cell_factor, GridBuffers& gb, GridDev& out);
+// Bytes of hipCUB scan storage the grid build needs for up to `cells` cells.
+size_t grid_scan_bytes(int64_t cells);
 void free_grid(GridBuffers& gb);
 int upload_shell_table(ShellTable& tab, void** storage);
 // Scratch of the tiled search: Morton keys and target ids (sorted copies),
@@ -96,6 +102,7 @@ struct TileBuffers {
   size_t tmp_bytes = 0;
   int64_t cap_keys = 0, cap_ids = 0;
   const int32_t* spill_count = nullptr;  // device count of the targets the last wave or tile search left to k_knn
+  bool fixed = false;                    // caller-owned ids of fixed capacity: the searches allocate nothing
 };
 void free_tiles(TileBuffers& tb);
 // SearchTree::balanced order of each target's selection (search.rs:909-936),

@@ -1378,6 +1378,7 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
   const int T = 256;
   int64_t blocks = (n + T - 1) / T;
   int64_t red_blocks = std::min<int64_t>(std::max<int64_t>(blocks, 1), 1024);
+  if (gb.fixed && (n > gb.cap_n || !gb.bounds)) return -3;
   if (n > gb.cap_n) {
     (void)hipFree(gb.entries);
     (void)hipFree(gb.cell_of);
@@ -1396,8 +1397,8 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
   HIPCHK(hipMemcpyAsync(dF, F, 9 * sizeof(double), hipMemcpyHostToDevice, s));
   k_project<<<dim3((unsigned)blocks), dim3(T), 0, s>>>(n, x, y, z, dF, origin[0], origin[1], origin[2], px, py, pz);
   k_bounds<<<dim3((unsigned)red_blocks), dim3(256), 0, s>>>(n, px, py, pz, gb.bounds);
-  std::vector<double> part(red_blocks * 6);
-  HIPCHK(hipMemcpyAsync(part.data(), gb.bounds, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  double part[1024 * 6];
+  HIPCHK(hipMemcpyAsync(part, gb.bounds, red_blocks * 6 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t b = 0; b < red_blocks; ++b)
@@ -1411,10 +1412,15 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
   double h = reference_side(lo, hi, n) * cell_factor;
   int dims[3];
   int64_t cells = 1;
-  for (int d = 0; d < 3; ++d) {
-    double k = std::floor((hi[d] - lo[d]) / h);
-    dims[d] = (int)std::min(k, 1e9) + 1;
-    cells *= dims[d];
+  for (;;) {
+    cells = 1;
+    for (int d = 0; d < 3; ++d) {
+      double k = std::floor((hi[d] - lo[d]) / h);
+      dims[d] = (int)std::min(k, 1e9) + 1;
+      cells *= dims[d];
+    }
+    if (!gb.fixed || cells <= gb.cap_cells) break;
+    h *= 2.0;
   }
   if (cells > gb.cap_cells) {
     (void)hipFree(gb.start);
@@ -1429,6 +1435,7 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
   size_t need = 0;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, gb.fill, gb.start, (int)(cells + 1), s));
   if (need > gb.scan_tmp_bytes) {
+    if (gb.fixed) return -3;
     (void)hipFree(gb.scan_tmp);
     HIPCHK(hipMalloc(&gb.scan_tmp, need));
     gb.scan_tmp_bytes = need;
@@ -1445,6 +1452,16 @@ int build_grid(hipStream_t s, int64_t n, const double* x, const double* y, const
   out.start = gb.start;
   out.entries = gb.entries;
   return 0;
+}
+
+size_t grid_scan_bytes(int64_t cells) {
+  size_t need = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, need, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(cells + 1),
+                                       (hipStream_t)0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return need;
 }
 
 // grid.rs:39-64: offsets within REACH sorted by m = sum max(0, |o| - 1)^2.
@@ -1596,6 +1613,7 @@ void free_tiles(TileBuffers& tb) {
 // k_knn.
 static int launch_waves(hipStream_t s, const SearchArgs& a, int64_t m, TileBuffers& tb) {
   if (m > 2147483647LL) return -1;
+  if (tb.fixed && 3 * m + 1 > tb.cap_ids) return -1;
   if (grow(tb.ids, tb.cap_ids, 3 * m + 1)) return -1;
   int32_t* spill = tb.ids + 2 * m;
   int32_t* count = tb.ids + 3 * m;
@@ -1612,7 +1630,8 @@ int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mod
   if (n_targets <= 0) return 0;
   tb.spill_count = nullptr;
   if (knn_path(a)) {
-    if (mode > 0 && a.list == nullptr) return launch_tiles(s, a, n_targets, mode, tb);
+    if (mode > 0 && a.list == nullptr && !tb.fixed) return launch_tiles(s, a, n_targets, mode, tb);
+    if (mode > 0 && tb.fixed) mode = 0;
     if (mode == 0) return launch_waves(s, a, n_targets, tb);
     // Selections without caps: one target per lane.
     SearchArgs c = a;

@@ -197,6 +197,19 @@ class Engine:
                                        _ptr(lagrange), _ptr(n_used), _ptr(status)))
         return dict(value=value, variance=variance, lagrange=lagrange, n_used=n_used, status=status)
 
+    def krige_noalloc(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64):
+        """bk_krige_noalloc: as krige, with the pass's scratch in the library's
+        static device array (no allocation by the library). Targets must be
+        device tensors; at most 65536 targets, 262144 samples, 64 neighbours."""
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        f, i = self._outputs(m, tx)
+        value, variance, lagrange, n_used, status = f(), f(), f(), i(), i()
+        self._check(self._lib.bk_krige_noalloc(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                               int(precision), m, _ptr(tx), _ptr(ty), _ptr(tz), _ptr(value),
+                                               _ptr(variance), _ptr(lagrange), _ptr(n_used), _ptr(status)))
+        return dict(value=value, variance=variance, lagrange=lagrange, n_used=n_used, status=status)
+
     def block_krige(self, variogram, search, block, tx, ty, tz, kind=_abi.ORDINARY, param=0.0,
                     precision=_abi.FP64):
         tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)

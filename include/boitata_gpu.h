@@ -197,6 +197,26 @@ int bk_krige(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32
              const double* tz, double* value, double* variance, double* lagrange,
              int32_t* n_used, int32_t* status);
 
+/* bk_krige with no device allocation (no counterpart in the reference), for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc. The pass's scratch (neighbour lists,
+ * buckets, counters, the cell grid of the search) lives in one static device
+ * array per GPU of about 44 MiB, which each GPU the library runs on holds until
+ * the process ends; the call allocates nothing and calls no allocating
+ * routine. Limits: m <= BK_STATIC_MAX_TARGETS, at most BK_STATIC_MAX_SAMPLES
+ * samples set on the context, max_samples <= BK_STATIC_MAX_NEIGHBOURS, no
+ * calibrated search; targets and outputs must be device arrays (lagrange,
+ * n_used and status may be NULL). Beyond a limit the call returns BK_E_INVALID
+ * and computes nothing. Results equal bk_krige's. Calls on one GPU share its
+ * array: the library serialises them within a process; calls that overlap on
+ * one GPU from several processes race. */
+#define BK_STATIC_MAX_TARGETS 65536
+#define BK_STATIC_MAX_SAMPLES 262144
+#define BK_STATIC_MAX_NEIGHBOURS 64
+int bk_krige_noalloc(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
+                     int32_t precision, int64_t m, const double* tx, const double* ty, const double* tz,
+                     double* value, double* variance, double* lagrange, int32_t* n_used, int32_t* status);
+
 /* estimate_many with block_krige_points (block.rs:122-198) on the discretisation
  * of `block` centred on each target (py/estimation.rs:947-958). support_variance
  * (C(B,B)) may be NULL. */
