@@ -240,6 +240,10 @@ SIGNATURES = [
     ("bk_block_krige", C.c_int,
      [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, C.POINTER(Block), I32, I64, P, P, P, P, P, P, P, P, P]),
     ("bk_loo", C.c_int, [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, I32, I64, P, P, P, P]),
+    ("bk_block_loo", C.c_int,
+     [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, C.POINTER(Block), I32, I64, P, P, P, P]),
+    ("bk_krige_weights", C.c_int,
+     [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, P, I32, I64, P, P, P, P, P, P, P]),
     ("bk_loo_prepare", C.c_int, [P, C.POINTER(Search), C.POINTER(Variogram)]),
     ("bk_loo_loss_grad", C.c_int, [P, C.POINTER(Variogram), I32, P, P, P]),
     ("bk_krige_diagnostics", C.c_int,

@@ -1331,6 +1331,14 @@ int bk_loo(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, 
                       which, true, value, variance, nullptr, nullptr, nullptr, status);
 }
 
+int bk_block_loo(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                 const bk_block* block, int32_t precision, int64_t m, const int64_t* which, double* value,
+                 double* variance, int32_t* status) {
+  if (!block) return c ? fail(c, BK_E_INVALID, "no block") : BK_E_INVALID;
+  return krige_common(c, vg, s, kind, param, block, precision, which ? m : (c ? c->n : 0), nullptr, nullptr, nullptr,
+                      which, true, value, variance, nullptr, nullptr, nullptr, status);
+}
+
 int bk_loo_prepare(bk_ctx* c, const bk_search* s, const bk_variogram* vg) {
   if (!c || !s) return BK_E_INVALID;
   if (s->calibration)
@@ -1992,6 +2000,69 @@ int bk_krige_diagnostics(bk_ctx* c, const bk_variogram* vg, const bk_search* sea
                          const bk_block* block, int32_t precision, int64_t m, const double* tx, const double* ty,
                          const double* tz, const bk_diag_out* out) {
   return diag_common(c, vg, search, kind, param, block, precision, m, tx, ty, tz, out);
+}
+
+int bk_krige_weights(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t kind, double param,
+                     const bk_block* blk, int32_t precision, int64_t m, const double* tx, const double* ty,
+                     const double* tz, int32_t* idx, double* weights, int32_t* count, int32_t* status) {
+  if (!c) return BK_E_INVALID;
+  if (!vg || !s || !idx || !weights || !count) return fail(c, BK_E_INVALID, "null argument");
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  if (m > 0 && (!tx || !ty || !tz)) return fail(c, BK_E_INVALID, "targets missing");
+  if (kind < BK_ORDINARY || kind > BK_INDICATOR) return fail(c, BK_E_INVALID, "unknown kind");
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
+  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
+  for (int k = 0; k < vg->n_structures; ++k)
+    if (kind == BK_SIMPLE && vg->structures[k].model == BK_POWER)
+      return fail(c, BK_E_INVALID, "simple kriging needs a variogram with a sill: every structure must be stationary");
+  if (s->calibration == BK_CALIBRATE_SLOPE && kind == BK_SIMPLE)
+    return fail(c, BK_E_INVALID, "simple kriging has a slope of regression of 1; calibrate it with target_efficiency");
+  if (s->calibration && s->max_samples > BK_MAX_CHOL)
+    return fail(c, BK_E_INVALID, "calibrated searches select at most 64 samples on the device");
+  if (blk) {
+    if (blk->nx <= 0 || blk->ny <= 0 || blk->nz <= 0)
+      return fail(c, BK_E_INVALID, "block discretization must be at least 1 per axis");
+    for (int d = 0; d < 3; ++d)
+      if (!(std::isfinite(blk->size[d]) && blk->size[d] >= 0.0))
+        return fail(c, BK_E_INVALID, "block size must be finite and >= 0");
+  }
+  HIPOK(c, hipSetDevice(c->device));
+  SearchArgs a;
+  int64_t stride;
+  if (int r = prepare_search(c, s, vg, a, stride)) return r;
+  const double *dx, *dy, *dz;
+  if (int r = stage_in(c, tx, m, c->tx, &dx)) return r;
+  if (int r = stage_in(c, ty, m, c->ty, &dy)) return r;
+  if (int r = stage_in(c, tz, m, c->tz, &dz)) return r;
+  const int64_t M = std::max<int64_t>(m, 1);
+  for (Buf* b : {&c->value, &c->variance})
+    if (int r = ensure(c, *b, M * sizeof(double))) return r;
+  double *ow, *ov = (double*)c->value.p, *ovar = (double*)c->variance.p;
+  int32_t *oc, *ost;
+  if (int r = out_ptr(c, weights, m * stride, c->mik_w, &ow)) return r;
+  if (int r = out_ptr(c, count, m, c->n_used, &oc)) return r;
+  if (status) {
+    if (int r = out_ptr(c, status, m, c->status, &ost)) return r;
+  } else {
+    if (int r = ensure(c, c->status, M * sizeof(int32_t))) return r;
+    ost = (int32_t*)c->status.p;
+  }
+  if (int r = run_search(c, a, m, dx, dy, dz, nullptr)) return r;
+  if (s->calibration && (s->octant || s->sectors > 0) && m > 0)
+    if (bk::launch_balance(c->stream, a, m)) return fail(c, BK_E_HIP, "balance launch failed");
+  if (int r = run_solve(c, vg, s, kind, param, precision, m, dx, dy, dz, (int)stride, blk, ov, ovar, nullptr, nullptr,
+                        oc, ost, ow))
+    return r;
+  if (int r = finalize(c, m, ost, ov, ovar, nullptr, nullptr, oc)) return r;
+  // The samples of each target in the order the estimator received them.
+  if (m > 0)
+    HIPOK(c, hipMemcpyAsync(idx, c->nidx.p, m * stride * sizeof(int32_t), hipMemcpyDefault, c->stream));
+  if (int r = stage_out(c, weights, ow, m * stride)) return r;
+  if (int r = stage_out(c, count, oc, m)) return r;
+  if (status)
+    if (int r = stage_out(c, status, ost, m)) return r;
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int bk_mik_predict(bk_ctx* c, const bk_mik* mik, const double* thresholds, const bk_variogram* variograms,

@@ -248,6 +248,36 @@ class Engine:
                                      int(precision), m, _ptr(which), _ptr(value), _ptr(variance), _ptr(status)))
         return dict(value=value, variance=variance, status=status)
 
+    def block_loo(self, variogram, search, block, which=None, kind=_abi.ORDINARY, param=0.0, precision=_abi.FP64):
+        """bk_block_loo: leave-one-out block kriging at the samples `which` (all by default)."""
+        which = _i64(which)
+        m = self.n if which is None else int(len(which))
+        value = np.empty(m, dtype=np.float64)
+        variance = np.empty(m, dtype=np.float64)
+        status = np.empty(m, dtype=np.int32)
+        self._check(self._lib.bk_block_loo(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                           C.byref(block), int(precision), m, _ptr(which), _ptr(value),
+                                           _ptr(variance), _ptr(status)))
+        return dict(value=value, variance=variance, status=status)
+
+    def krige_weights(self, variogram, search, tx, ty, tz, kind=_abi.ORDINARY, param=0.0, block=None,
+                      precision=_abi.FP64):
+        """bk_krige_weights (weights_many): per target the samples used, in the
+        estimator's order, and their kriging weights; rows of S = min(max_samples,
+        samples) entries, the first count[t] of them valid."""
+        tx, ty, tz = _f64(tx), _f64(ty), _f64(tz)
+        m = int(len(tx))
+        S = int(max(min(search.max_samples, self.n), 1))
+        idx = np.zeros((m, S), dtype=np.int32)
+        weights = np.zeros((m, S), dtype=np.float64)
+        count = np.zeros(m, dtype=np.int32)
+        status = np.zeros(m, dtype=np.int32)
+        self._check(self._lib.bk_krige_weights(self._h, C.byref(variogram), C.byref(search), int(kind), float(param),
+                                               C.byref(block) if block is not None else None, int(precision), m,
+                                               _ptr(tx), _ptr(ty), _ptr(tz), _ptr(idx), _ptr(weights), _ptr(count),
+                                               _ptr(status)))
+        return dict(idx=idx, weights=weights, count=count, status=status)
+
     def _mik_outputs(self, m, k, nc, nq, like):
         """Output arrays of multiple indicator kriging: entry-major (rows, m)."""
         if like is not None and hasattr(like, "data_ptr") and like.is_cuda:

@@ -23,7 +23,7 @@ Behaviour kept from the reference:
   (crates/py/src/containers.rs:15-21).
 
 Not covered (raise NotImplementedError): domains, local anisotropy,
-k-fold and block cross-validation, units, soft / high-grade data, and the
+units, soft / high-grade data, and the
 other estimation methods.
 
 Extension: ``precision="fp32_ir"`` solves in FP32 with FP64 refinement; the
@@ -449,24 +449,35 @@ class _Base:
             self._engine = Engine(self._device)
         return self._engine
 
+    def _device_holes(self):
+        """Hole codes for the device: a sample without a code (null) counts as a
+        hole of its own (crates/estimation/src/neighborhood.rs:85-92); None
+        without holes."""
+        hole = self._samples["hole"]
+        if np.isnan(hole).all():
+            return None
+        ids = np.empty(len(hole), dtype=np.int32)
+        tagged = ~np.isnan(hole)
+        ids[tagged] = hole[tagged].astype(np.int32)
+        top = int(hole[tagged].max()) + 1 if tagged.any() else 0
+        ids[~tagged] = top + np.arange(int((~tagged).sum()), dtype=np.int32)
+        return ids
+
+    def _upload(self, eng, rows=None):
+        """Sets the fitted samples (all, or those at `rows`) on an engine."""
+        s = self._samples
+        ids = self._device_holes()
+        pick = (lambda a: a) if rows is None else (lambda a: np.ascontiguousarray(a[rows]))
+        err = s["error_variance"]
+        eng.set_samples(pick(s["x"]), pick(s["y"]), pick(s["z"]), pick(s["value"]),
+                        pick(err) if (err != 0.0).any() else None, None if ids is None else pick(ids))
+
     def _ready(self):
         """The device engine holding the fitted samples (uploaded on first use
         after a restore from a file or a pickle)."""
         eng = self._device_engine()
         if not self._uploaded:
-            s = self._samples
-            hole = s["hole"]
-            ids = None
-            if not np.isnan(hole).all():
-                # A sample without a code (null) counts as a hole of its own
-                # (crates/estimation/src/neighborhood.rs:85-92).
-                ids = np.empty(len(hole), dtype=np.int32)
-                tagged = ~np.isnan(hole)
-                ids[tagged] = hole[tagged].astype(np.int32)
-                top = int(hole[tagged].max()) + 1 if tagged.any() else 0
-                ids[~tagged] = top + np.arange(int((~tagged).sum()), dtype=np.int32)
-            err = s["error_variance"]
-            eng.set_samples(s["x"], s["y"], s["z"], s["value"], err if (err != 0.0).any() else None, ids)
+            self._upload(eng)
             self._uploaded = True
         return eng
 
@@ -581,21 +592,92 @@ class _Base:
             ("target_met", cols["target_met"]),
         ])
 
+    def _folds(self, k):
+        """k_fold_at's folds (crates/estimation/src/batch.rs:217-226): the j-th
+        of the sorted distinct hole codes goes to fold j % k, a sample without
+        a hole to fold (its index) % k."""
+        hole = self._samples["hole"]
+        fold = np.arange(len(hole), dtype=np.int64) % k
+        tagged = ~np.isnan(hole)
+        if tagged.any():
+            codes = np.unique(hole[tagged])
+            fold[tagged] = np.searchsorted(codes, hole[tagged]) % k
+        return fold
+
     def cross_validate(self, *, folds=None) -> CrossValidation:
-        """Re-estimates every sample from the others, through the same passes."""
-        if folds is not None:
-            raise NotImplementedError("k-fold cross-validation is not supported by this engine")
+        """Estimates and variances at the fitted samples through the same
+        passes: leave-one-out, each sample from its neighbours without itself
+        (batch.rs:165-190), or with `folds` = k the k-fold form, each sample
+        from the samples outside its fold, holes kept whole (batch.rs:197-243)."""
         if self._values is None:
             raise InvalidInput("call fit first")
         self._ready()
+        vg = self._variogram._c
+        n = len(self._values)
+        if folds is None:
+            def one(s, rows):
+                if self._block is None:
+                    out = self._engine.loo(vg, s._c, rows.astype(np.int64), self._kind, self._param,
+                                           self._precision)
+                else:
+                    out = self._engine.block_loo(vg, s._c, self._block, rows.astype(np.int64), self._kind,
+                                                 self._param, self._precision)
+                return out["value"], out["variance"], out["status"] == _abi.OK
+        else:
+            k = int(folds)
+            if k < 2:
+                raise InvalidInput("k must be ≥ 2")
+            fold = self._folds(k)
+            S = self._samples
+            P = np.stack([S["x"], S["y"], S["z"]], axis=1)
+            from .engine import Engine
 
-        def one(s, rows):
-            out = self._engine.loo(self._variogram._c, s._c, rows.astype(np.int64), self._kind, self._param,
-                                   self._precision)
-            return out["value"], out["variance"], out["status"] == _abi.OK
+            eng = Engine(self._device)  # holds one fold's training samples at a time
 
-        value, variance, _ = run_passes(len(self._values), self._passes, one)
+            def one(s, rows):
+                value, variance = np.full(len(rows), np.nan), np.full(len(rows), np.nan)
+                ok = np.zeros(len(rows), dtype=bool)
+                for f in range(k):
+                    test = np.nonzero(fold[rows] == f)[0]
+                    if len(test) == 0:
+                        continue
+                    self._upload(eng, np.nonzero(fold != f)[0])
+                    T = P[rows[test]]
+                    if self._block is None:
+                        out = eng.krige(vg, s._c, T[:, 0].copy(), T[:, 1].copy(), T[:, 2].copy(), self._kind,
+                                        self._param, self._precision)
+                    else:
+                        out = eng.block_krige(vg, s._c, self._block, T[:, 0].copy(), T[:, 1].copy(),
+                                              T[:, 2].copy(), self._kind, self._param, self._precision)
+                    value[test], variance[test] = out["value"], out["variance"]
+                    ok[test] = out["status"] == _abi.OK
+                return value, variance, ok
+
+            try:
+                value, variance, _ = run_passes(n, self._passes, one)
+            finally:
+                eng.close()
+            return CrossValidation(self._values.copy(), value, variance)
+        value, variance, _ = run_passes(n, self._passes, one)
         return CrossValidation(self._values.copy(), value, variance)
+
+    def weights(self, targets):
+        """weights_many (crates/estimation/src/batch.rs:76-95) with the first
+        search: per target, the indices of the fitted samples it is kriged
+        from, in the estimator's order, and their kriging weights; None where
+        the target is not estimated."""
+        if self._values is None:
+            raise InvalidInput("call fit first")
+        T = coords_of(targets)
+        eng = self._ready()
+        out = eng.krige_weights(self._variogram._c, self._passes[0]._c, T[:, 0].copy(), T[:, 1].copy(),
+                                T[:, 2].copy(), self._kind, self._param, self._block, self._precision)
+        res = []
+        for t in range(len(T)):
+            c = int(out["count"][t])
+            res.append(None if out["status"][t] != _abi.OK else
+                       (out["idx"][t, :c].astype(np.int64), out["weights"][t, :c].copy()))
+        return res
 
 
     @property
@@ -653,7 +735,8 @@ def _unpickle(cls, meta, columns, engine):
 
 
 class OrdinaryKriging(_Base):
-    """Kriging with an unknown, locally constant mean (weights sum to 1)."""
+    """Ordinary kriging: the mean is unknown and taken as constant within each
+    neighbourhood, so the kriging weights are constrained to add up to one."""
 
     def __init__(self, variogram: Variogram, search, **engine):
         super().__init__(variogram, search, **engine)
@@ -676,7 +759,9 @@ class SimpleKriging(_Base):
 
 
 class IndicatorKriging(_Base):
-    """Ordinary kriging of the indicator `value <= threshold`; estimates are probabilities."""
+    """Ordinary kriging applied to the indicator 1[value <= threshold]: each
+    estimate is the probability, clamped to [0, 1], that the value at the
+    target does not exceed the threshold."""
 
     _kind = _abi.INDICATOR
 
@@ -701,9 +786,6 @@ class BlockKriging(_Base):
         if len(disc) != 3 or min(disc) < 1:
             raise InvalidInput("discretization must be at least 1 per axis")
         self._block = _abi.make_block(size, disc)
-
-    def cross_validate(self, *, folds=None) -> CrossValidation:
-        raise NotImplementedError("block cross-validation is not supported by this engine")
 
     def _one_pass(self, s, T):
         out = self._engine.block_krige(self._variogram._c, s._c, self._block, T[:, 0].copy(), T[:, 1].copy(),

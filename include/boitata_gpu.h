@@ -232,6 +232,25 @@ int bk_loo(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t
            double param, int32_t precision, int64_t m, const int64_t* which, double* value,
            double* variance, int32_t* status);
 
+/* leave_one_out_at (batch.rs:165-190) with block_krige_points as the
+ * estimator (crates/py/src/estimation.rs:1826-1834, BlockKriging.cross_validate):
+ * each sample at `which` (m of them; NULL = all) is kriged on the block of
+ * `block` centred on its location, from its neighbours without itself. */
+int bk_block_loo(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
+                 const bk_block* block, int32_t precision, int64_t m, const int64_t* which, double* value,
+                 double* variance, int32_t* status);
+
+/* weights_many (batch.rs:76-95) with the krige or block_krige_points estimator
+ * (block NULL: point support): per target, the samples it was kriged from in
+ * the order the estimator received them, idx[t * S + k] for k < count[t], and
+ * their kriging weights weights[t * S + k] (merged coincident samples share
+ * their cluster's weight, coincident.rs:102-112), where S = min(max_samples,
+ * the number of samples), at least 1. count[t] is 0 and status[t] (may be
+ * NULL) not BK_OK where the reference gives None. */
+int bk_krige_weights(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t kind, double param,
+                     const bk_block* block, int32_t precision, int64_t m, const double* tx, const double* ty,
+                     const double* tz, int32_t* idx, double* weights, int32_t* count, int32_t* status);
+
 /* Leave-one-out cross-validation loss and its gradient (no counterpart in the
  * reference). The neighbour sets are those of bk_loo with `search`, fixed before
  * the call by bk_loo_prepare, so they do not move with the parameters.
