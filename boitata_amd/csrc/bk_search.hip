@@ -850,52 +850,69 @@ __device__ __forceinline__ void cx_keys(uint64_t& k, uint32_t& i, int lane, int 
   }
 }
 
-// Ascending bitonic sort of one key per lane.
-__device__ __forceinline__ void wave_sort_keys(uint64_t& k, uint32_t& i, int lane) {
+// Ascending bitonic sort of one key per lane over groups of G lanes (lane is
+// the index within the group).
+template <int G>
+__device__ __forceinline__ void group_sort_keys(uint64_t& k, uint32_t& i, int lane) {
 #pragma unroll
-  for (int size = 2; size <= BK_WAVE; size <<= 1) {
+  for (int size = 2; size <= G; size <<= 1) {
 #pragma unroll
     for (int stride = size >> 1; stride > 0; stride >>= 1) cx_keys(k, i, lane, stride, (lane & size) == 0);
   }
 }
 
-// A (sorted ascending across lanes) becomes the 64 smallest of A and B (B
+// A (sorted ascending across the group) becomes the G smallest of A and B (B
 // sorted ascending), sorted.
-__device__ __forceinline__ void wave_merge_keys(uint64_t& ak, uint32_t& ai, uint64_t bk, uint32_t bi, int lane) {
-  // Reverse B, take the pairwise minimum: a bitonic sequence of the 64 smallest.
-  const int src = BK_WAVE - 1 - lane;
-  const uint32_t lo = __shfl((unsigned)(bk & 0xffffffffu), src, BK_WAVE);
-  const uint32_t hi = __shfl((unsigned)(bk >> 32), src, BK_WAVE);
+template <int G>
+__device__ __forceinline__ void group_merge_keys(uint64_t& ak, uint32_t& ai, uint64_t bk, uint32_t bi, int lane) {
+  // Reverse B, take the pairwise minimum: a bitonic sequence of the G smallest.
+  const int src = G - 1 - lane;
+  const uint32_t lo = __shfl((unsigned)(bk & 0xffffffffu), src, G);
+  const uint32_t hi = __shfl((unsigned)(bk >> 32), src, G);
   const uint64_t rk = ((uint64_t)hi << 32) | lo;
-  const uint32_t ri = __shfl(bi, src, BK_WAVE);
+  const uint32_t ri = __shfl(bi, src, G);
   if (key_less(rk, ri, ak, ai)) {
     ak = rk;
     ai = ri;
   }
 #pragma unroll
-  for (int stride = BK_WAVE / 2; stride > 0; stride >>= 1) cx_keys(ak, ai, lane, stride, true);
+  for (int stride = G / 2; stride > 0; stride >>= 1) cx_keys(ak, ai, lane, stride, true);
 }
 
 constexpr uint64_t KEY_INF = ~0ull;
 
+// The bits of a wave ballot that belong to this lane's group of G lanes.
+template <int G>
+__device__ __forceinline__ uint64_t group_ballot(bool p, int h) {
+  const uint64_t m = __ballot(p);
+  return G == BK_WAVE ? m : (m >> (h * G)) & ((1ull << G) - 1ull);
+}
+
+// One target per group of G lanes (G = 64, or 32 when max_samples <= 32: two
+// targets per wavefront, whose networks, ballots and broadcasts then serve
+// both at once).
+template <int G>
 __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
+  constexpr int PER = BK_WAVE / G;  // targets per wavefront
   __shared__ uint64_t buf_k[4][2 * BK_WAVE];
   __shared__ uint32_t buf_i[4][2 * BK_WAVE];
-  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
-  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
-  if (b >= a.m) return;  // whole wave
-  const int64_t t = a.list ? a.list[b] : b;
-  const int64_t left_out = a.left_out ? a.left_out[t] : -1;
+  const int wave = threadIdx.x / BK_WAVE, wl = threadIdx.x % BK_WAVE;
+  const int h = wl / G, lane = wl % G;
+  const int64_t b = ((int64_t)blockIdx.x * 4 + wave) * PER + h;
+  if (((int64_t)blockIdx.x * 4 + wave) * PER >= a.m) return;  // whole wave
+  const bool live = b < a.m;
+  const int64_t t = live ? (a.list ? a.list[b] : b) : 0;
+  const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
   const int K = a.max_samples;
   const GridDev& g = a.g;
   double q0, q1, q2;
   mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
   uint64_t ak = KEY_INF;
   uint32_t ai = 0xffffffffu;
-  uint64_t* bk = buf_k[wave];
-  uint32_t* bi = buf_i[wave];
-  bool open = g.start != nullptr && K > 0;
-  if (open) {
+  uint64_t* bk = buf_k[wave] + h * 2 * G;
+  uint32_t* bi = buf_i[wave] + h * 2 * G;
+  bool open = live && g.start != nullptr && K > 0;
+  if (__any(open)) {
     const int c0 = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
     const int c1 = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
     const int c2 = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
@@ -903,54 +920,54 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
     const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
     const double slack = 1.0 - 1e-9;
     const uint32_t n_off = a.tab.end[a.tab.n - 1];
-    int nbuf = 0;
+    int nbuf = 0;  // the same in the group's lanes
     auto merge = [&]() {
       if (nbuf <= kSmallBatch) {
         // Few entries: every list entry and every batch entry counts the
-        // entries below it (the batch's keys read lane by lane) and is
-        // written at its rank in the merged list, in the buffer's upper half;
-        // the 64 smallest of the list and the batch, sorted, as the networks
-        // below give them.
+        // entries below it (the batch's keys read from the group's lanes)
+        // and is written at its rank in the merged list, in the buffer's
+        // upper half; the G smallest of the list and the batch, sorted, as
+        // the networks below give them.
         const uint64_t mk = lane < nbuf ? bk[lane] : KEY_INF;
         const uint32_t mi = lane < nbuf ? bi[lane] : 0xffffffffu;
         int below = 0, posb = 0;
         for (int q = 0; q < nbuf; ++q) {
-          const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, q);
-          const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mk >> 32), q);
-          const uint32_t iq = (uint32_t)__builtin_amdgcn_readlane((int)mi, q);
+          const uint32_t lo = __shfl((unsigned)(uint32_t)mk, q, G);
+          const uint32_t hi = __shfl((unsigned)(uint32_t)(mk >> 32), q, G);
+          const uint32_t iq = __shfl(mi, q, G);
           const uint64_t kq = ((uint64_t)hi << 32) | lo;
           below += key_less(kq, iq, ak, ai) ? 1 : 0;
-          const int na = __popcll(__ballot(key_less(ak, ai, kq, iq)));
+          const int na = __popcll(group_ballot<G>(key_less(ak, ai, kq, iq), h));
           if (lane == q) posb += na;
           if (lane < nbuf && key_less(kq, iq, mk, mi)) posb += 1;
         }
         const int posa = lane + below;
-        if (posa < BK_WAVE) {
-          bk[BK_WAVE + posa] = ak;
-          bi[BK_WAVE + posa] = ai;
+        if (posa < G) {
+          bk[G + posa] = ak;
+          bi[G + posa] = ai;
         }
-        if (lane < nbuf && posb < BK_WAVE) {
-          bk[BK_WAVE + posb] = mk;
-          bi[BK_WAVE + posb] = mi;
+        if (lane < nbuf && posb < G) {
+          bk[G + posb] = mk;
+          bi[G + posb] = mi;
         }
         wave_sync();
-        ak = bk[BK_WAVE + lane];
-        ai = bi[BK_WAVE + lane];
+        ak = bk[G + lane];
+        ai = bi[G + lane];
         wave_sync();
         nbuf = 0;
         return;
       }
       uint64_t k = lane < nbuf ? bk[lane] : KEY_INF;
       uint32_t i = lane < nbuf ? bi[lane] : 0xffffffffu;
-      wave_sort_keys(k, i, lane);
-      wave_merge_keys(ak, ai, k, i, lane);
+      group_sort_keys<G>(k, i, lane);
+      group_merge_keys<G>(ak, ai, k, i, lane);
       // Keep what did not fit in this merge.
-      const int rest = nbuf > BK_WAVE ? nbuf - BK_WAVE : 0;
+      const int rest = nbuf > G ? nbuf - G : 0;
       uint64_t kk = 0;
       uint32_t ii = 0;
       if (lane < rest) {
-        kk = bk[BK_WAVE + lane];
-        ii = bi[BK_WAVE + lane];
+        kk = bk[G + lane];
+        ii = bi[G + lane];
       }
       wave_sync();
       if (lane < rest) {
@@ -960,25 +977,23 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
       wave_sync();
       nbuf = rest;
     };
-    for (uint32_t o0 = 0; o0 < n_off; o0 += BK_WAVE) {
+    for (uint32_t o0 = 0; o0 < n_off; o0 += G) {
       // The threshold: the K-th key (rank K - 1), +inf until K are held.
-      const uint32_t tlo = __shfl((unsigned)(ak & 0xffffffffu), K - 1, BK_WAVE);
-      const uint32_t thi = __shfl((unsigned)(ak >> 32), K - 1, BK_WAVE);
+      const uint32_t tlo = __shfl((unsigned)(ak & 0xffffffffu), K - 1, G);
+      const uint32_t thi = __shfl((unsigned)(ak >> 32), K - 1, G);
       const uint64_t tk = ((uint64_t)thi << 32) | tlo;
-      const uint32_t ti = __shfl(ai, K - 1, BK_WAVE);
+      const uint32_t ti = __shfl(ai, K - 1, G);
       // Lower bound of every offset from o0 on (sorted by m).
       const char4 f0 = a.tab.offsets[o0];
       const int m0 = max(abs(f0.x) - 1, 0) * max(abs(f0.x) - 1, 0) + max(abs(f0.y) - 1, 0) * max(abs(f0.y) - 1, 0) +
                      max(abs(f0.z) - 1, 0) * max(abs(f0.z) - 1, 0);
       const double bound = fmin(h2 * (double)m0, beyond) * slack;
-      if (bound > a.radius2 || (tk != KEY_INF && bound > __longlong_as_double((long long)tk))) {
-        open = false;
-        break;
-      }
+      if (open && (bound > a.radius2 || (tk != KEY_INF && bound > __longlong_as_double((long long)tk)))) open = false;
+      if (!__any(open)) break;
       // Lane l: cell o0 + l.
       uint32_t e = 0, e1 = 0;
       const uint32_t o = o0 + lane;
-      if (o < n_off) {
+      if (open && o < n_off) {
         const char4 f = a.tab.offsets[o];
         const int x = c0 + f.x, y = c1 + f.y, z = c2 + f.z;
         if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2]) {
@@ -1000,8 +1015,8 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
           kk = dbits(d2);
           keep = d2 <= a.radius2 && (int64_t)ii != left_out && key_less(kk, ii, tk, ti);
         }
-        const uint64_t mask = __ballot(keep);
-        if (mask) {
+        const uint64_t mask = group_ballot<G>(keep, h);
+        if (__any(mask != 0)) {
           const int pos = nbuf + __popcll(mask & ((1ull << lane) - 1ull));
           if (keep) {
             bk[pos] = kk;
@@ -1009,19 +1024,24 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
           }
           nbuf += __popcll(mask);
           wave_sync();
-          if (nbuf >= BK_WAVE) merge();
+          if (__any(nbuf >= G)) {
+            if (nbuf >= G) merge();
+          }
         }
       }
-      if (nbuf > 0) merge();
+      if (__any(nbuf > 0)) {
+        if (nbuf > 0) merge();
+      }
     }
   }
+  if (!live) return;
   if (open) {
     if (lane == 0) spill[atomicAdd(spill_count, 1)] = (int32_t)t;
     return;
   }
   // The selection: ranks below K with a finite key, in order.
   const bool have = lane < K && ak != KEY_INF;
-  const int n = __popcll(__ballot(have));
+  const int n = __popcll(group_ballot<G>(have, h));
   int* out = a.idx + t * a.stride;
   if (have) out[lane] = (int)ai;
   bi[lane] = ai;
@@ -1620,7 +1640,10 @@ static int launch_waves(hipStream_t s, const SearchArgs& a, int64_t m, TileBuffe
   k_zero32<<<1, 64, 0, s>>>(count, 1);
   SearchArgs c = a;
   c.m = m;
-  k_knn_wave<<<dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s>>>(c, spill, count);
+  if (a.max_samples <= 32)
+    k_knn_wave<32><<<dim3((unsigned)((m + 7) / 8)), dim3(256), 0, s>>>(c, spill, count);
+  else
+    k_knn_wave<64><<<dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s>>>(c, spill, count);
   HIPCHK(hipGetLastError());
   tb.spill_count = count;
   return launch_spill(s, a, m, spill, count);
