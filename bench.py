@@ -1,21 +1,41 @@
 #!/usr/bin/env python
-"""Kriged targets per second of the MI355X engine on the 1M-target workload.
+"""Kriged targets per second of the MI355X engine on the 1M-target workload,
+with every other workload of the estimation path reported beside it.
 
-Workload (synthetic, seeded): 100,000 drill-hole composites (400 holes, 2 m
-composites to -500 m), nested variogram nugget 0.1 + exponential (0.5;
-150/100/40 m) + spherical (0.4; 400/250/80 m) rotated (30, 20, 0), ellipsoid
-search 400/250/80 m with at most 32 neighbours, 200 x 200 x 25 block
+Headline workload (synthetic, seeded): 100,000 drill-hole composites (400
+holes, 2 m composites to -500 m), nested variogram nugget 0.1 + exponential
+(0.5; 150/100/40 m) + spherical (0.4; 400/250/80 m) rotated (30, 20, 0),
+ellipsoid search 400/250/80 m with at most 32 neighbours, 200 x 200 x 25 block
 centroids (1,000,000 targets). One step = one pass of the hot path over all
-targets with inputs resident in HBM: device grid build, neighbour search,
-bucketing, batched solves (and the deferred systems). The headline `value`
-is the FP32 + FP64-refinement form; the plain FP64 form runs beside it for the
-speed-up and the error of the refined form against it.
+targets with samples and targets resident in HBM: neighbour search,
+bucketing, batched solves and the deferred systems. The sample grid is built
+once per sample set and kept across passes (the reference builds its search
+tree per predict call; the device build takes about 0.1 ms). The headline
+`value` is the FP32 + FP64-refinement form; the plain FP64 form runs beside it
+for the speed-up and the error of the refined form against it.
+
+The other workloads, each a leg of the report (`configs`):
+- C1: 5,000 samples, 50,000 targets, at most 24 neighbours (CPU-runnable case);
+- C3: 4,000,000 targets among 60,000 clustered samples, octant search
+  (8 per octant), 4-64 neighbours in size buckets;
+- C4: leave-one-out cross-validation gradient on 200,000 samples, 12
+  variogram parameters, 200 Adam steps;
+- C5: 2,000,000 blocks discretised 4 x 4 x 2, 64 neighbours, Gaussian
+  covariance with a 1e-4 nugget and 1 cm twins (ill-conditioned systems);
+- MIK: multiple indicator kriging of the headline targets, nine thresholds.
+
+Every leg reports the GPU rate of both forms, the stage times from HIP events
+on the library's stream, the refined form's sweeps and hand-backs, errors of
+the refined form against FP64 (max, p50, p99), and against the oracle (the
+C++ restatement of the reference's CPU path; the reference itself cannot be
+built here) on a strided sample of the targets, which is also timed as the CPU
+baseline on every host thread and on one, with the neighbour-set mismatch
+count on that sample.
 
 Prints one JSON line. `roofline` is for the kernel with the most time in the
-headline form, timed live with HIP events on the library's stream;
-`cpu_baseline` times the oracle (the C++ restatement of the reference's CPU
-path, the reference itself cannot be built here) on a bounded sample of the
-same targets with every host thread it may use.
+headline form: its algorithmic flops over its launch time from HIP events,
+against the FP32 peak (the refined kernels compute in FP32 on the VALU and the
+matrix cores; FP64 only for residuals).
 """
 from __future__ import annotations
 
@@ -31,11 +51,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-PEAK_FP32_TFLOPS = 157.3  # MI355X FP32
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 # Per-structure covariance cost in flops (a transcendental counts as one).
 SHAPE_COST = {0: 8, 1: 6, 2: 7, 3: 14, 4: 11, 5: 12, 6: 6, 7: 8, 8: 6}
+
+# The solve kernel of a size bucket (upper edge) in each form.
+KERNEL_FP32IR = {16: "k_krige32<16>", 32: "k_krige32m", 64: "k_krige64m"}
+KERNEL_FP64 = {16: "k_krige64<16>", 32: "k_krige64<32>", 64: "k_krige64<64>"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def flops_per_system(n, shape_sum, simple=False):
@@ -50,6 +78,14 @@ def flops_per_system(n, shape_sum, simple=False):
     return np.where(n > 0, f, 0.0)
 
 
+def block_flops_per_system(n, points, shape_sum):
+    """A block system adds the n x P point-to-block covariances. The P x P
+    block-to-block average is one number per call for a stationary model and
+    is not credited per block."""
+    n = np.asarray(n, dtype=np.float64)
+    return flops_per_system(n, shape_sum) + np.where(n > 0, n * points * (11 + shape_sum), 0.0)
+
+
 def search_bytes_per_target(n):
     """Least HBM traffic of one search: the target (24 B), the selected
     samples' projected coordinates and index (32 B each), the list written (4 B each)."""
@@ -59,7 +95,7 @@ def search_bytes_per_target(n):
 def measured_traffic(key):
     """HBM bytes per launch recorded by the PMC summary in profiles/ (the
     FETCH_SIZE and WRITE_SIZE passes scaled by the copy calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "k_krige32_c2_pmc.md")
+    path = os.path.join(ROOT, "profiles", "k_krige32m_c2_pmc.md")
     try:
         with open(path) as f:
             for ln in f:
@@ -70,20 +106,401 @@ def measured_traffic(key):
     return None
 
 
+def total_sill(vg):
+    return vg.nugget + sum(vg.structures[k].sill for k in range(vg.n_structures))
+
+
+def rel_stats(x, y, floor):
+    """Max, median and 99th percentile of |x - y| / max(|y|, floor)."""
+    x, y = np.asarray(x), np.asarray(y)
+    if len(x) == 0:
+        return None
+    r = np.abs(x - y) / np.maximum(np.abs(y), floor)
+    return {"max": float(r.max()), "p50": float(np.percentile(r, 50)), "p99": float(np.percentile(r, 99))}
+
+
+def strided(m, k):
+    """k evenly strided indices of range(m) (all of them when k >= m)."""
+    k = int(max(1, min(m, k)))
+    return np.arange(0, m, max(m // k, 1), dtype=np.int64)[:k]
+
+
+def host(a):
+    return a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)
+
+
+class Leg:
+    """One workload on a context of its own: the timed passes of both forms,
+    the stage times, the error of the refined form against FP64, and the
+    oracle's CPU baseline and error on a strided sample of the targets."""
+
+    def __init__(self, name, w, args, threads, dev, tuned_sample=65536):
+        import torch
+
+        from boitata_amd.engine import Engine
+
+        self.name, self.w, self.args, self.threads = name, w, args, threads
+        self.tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        self.e = Engine(0)
+        holes = None if w.hole is None else self.tt(w.hole.astype(np.int32))
+        self.e.set_samples(self.tt(w.x), self.tt(w.y), self.tt(w.z), self.tt(w.value), None, holes)
+        self.T = w.targets
+        self.m = len(self.T)
+        self.tdev = tuple(self.tt(self.T[:, k]) for k in range(3))
+        self.block = w.block
+        self.outs = {}
+        self.tuned = None
+        if not args.no_autotune and self.block is None:
+            from boitata_amd import _abi
+
+            t0 = time.perf_counter()
+            self.tuned = self.e.autotune(w.variogram, w.search, *self.tdev, precision=_abi.FP32_IR,
+                                         sample=min(tuned_sample, self.m))
+            self.tuned["seconds"] = time.perf_counter() - t0
+
+    def run(self, prec):
+        vg, s = self.w.variogram, self.w.search
+        if self.block is None:
+            # Each form writes into its own output arrays, so a repeated pass
+            # has the same arguments and the library runs it as a captured graph.
+            self.outs[prec] = self.e.krige(vg, s, *self.tdev, precision=prec, out=self.outs.get(prec))
+        else:
+            self.outs[prec] = self.e.block_krige(vg, s, self.block, *self.tdev, precision=prec)
+        return self.outs[prec]
+
+    def gpu(self, steps, warmup):
+        import torch
+
+        from boitata_amd import _abi
+
+        for _ in range(max(warmup, 1)):
+            self.run(_abi.FP32_IR)
+            self.run(_abi.FP64)
+        dt = {}
+        for prec in (_abi.FP32_IR, _abi.FP64):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                self.run(prec)
+            torch.cuda.synchronize()
+            dt[prec] = (time.perf_counter() - t0) / steps
+        self.dt32, self.dt64 = dt[_abi.FP32_IR], dt[_abi.FP64]
+        # Refinement counters of the last refined pass.
+        self.run(_abi.FP32_IR)
+        refined = self.e.stat(_abi.STAT_REFINED)
+        sweeps = self.e.stat(_abi.STAT_REFINE_SWEEPS)
+        redo = self.e.stat(_abi.STAT_REFINE_REDO)
+        self.refine = {"refined": refined, "sweeps_per_system": sweeps / refined if refined else None,
+                       "handed_to_fp64": redo}
+        # Stage times: the same passes again with HIP events on the library's
+        # stream (a profiled pass runs its launches one by one, not as a graph).
+        self.stages = {}
+        for prec in (_abi.FP32_IR, _abi.FP64):
+            self.e.profile(True)
+            for _ in range(steps):
+                self.run(prec)
+            st, calls = self.e.profile_read()
+            self.e.profile(False)
+            self.stages[prec] = {k: v / max(calls, 1) for k, v in st.items()}
+        a32, a64 = self.outs[_abi.FP32_IR], self.outs[_abi.FP64]
+        self.s32, self.s64 = host(a32["status"]), host(a64["status"])
+        self.h32 = {k: host(a32[k]) for k in ("value", "variance")}
+        self.h64 = {k: host(a64[k]) for k in ("value", "variance")}
+        self.n_used = host(a64["n_used"])
+
+    def report(self):
+        from boitata_amd import _abi
+
+        vg = self.w.variogram
+        c0 = total_sill(vg)
+        floor = 1e-3 * c0
+        ok = (self.s32 == 0) & (self.s64 == 0)
+        est = self.s64 == 0
+        nu = self.n_used[est]
+        shape_sum = sum(SHAPE_COST[vg.structures[k].model] for k in range(vg.n_structures))
+        if self.block is None:
+            f = flops_per_system(nu, shape_sum)
+        else:
+            pts = int(self.block.nx) * int(self.block.ny) * int(self.block.nz)
+            f = block_flops_per_system(nu, pts, shape_sum)
+        flops = float(f.sum())
+        sbytes = float(search_bytes_per_target(self.n_used).sum())
+        st32, st64 = self.stages[_abi.FP32_IR], self.stages[_abi.FP64]
+        # Systems handed back to FP64 are not credited to the refined kernels.
+        redo = self.refine["handed_to_fp64"]
+        f32 = flops * (1.0 - redo / max(int(est.sum()), 1))
+        buckets = sorted({16 if n <= 16 else (32 if n <= 32 else 64) for n in np.unique(nu)})
+        solve32 = f32 / (st32["solve"] * 1e-3) / 1e12 if st32["solve"] > 0 else None
+        solve64 = flops / (st64["solve"] * 1e-3) / 1e12 if st64["solve"] > 0 else None
+        search = sbytes / (st32["search"] * 1e-3) / 1e9 if st32["search"] > 0 else None
+        return {
+            "workload": self.w.name, "targets": self.m, "samples": self.w.n,
+            "support": "point" if self.block is None else
+                       f"block {int(self.block.nx)}x{int(self.block.ny)}x{int(self.block.nz)}",
+            "value": self.m / self.dt32, "unit": "targets/s", "ms_per_step": self.dt32 * 1e3,
+            "fp64": {"value": self.m / self.dt64, "ms_per_step": self.dt64 * 1e3},
+            "speedup_fp32ir_over_fp64": self.dt64 / self.dt32,
+            "stages_ms_fp32ir": st32, "stages_ms_fp64": st64,
+            "refinement": self.refine,
+            "rel_err_fp32ir_vs_fp64": {k: rel_stats(self.h32[k][ok], self.h64[k][ok], floor)
+                                       for k in ("value", "variance")},
+            "rel_err_floor": floor,
+            "status_equal_fp32ir_fp64": bool((self.s32 == self.s64).all()),
+            "estimated": int(est.sum()),
+            "n_used": {"min": int(nu.min()) if len(nu) else 0, "max": int(nu.max()) if len(nu) else 0,
+                       "le16": int((nu <= 16).sum()), "le32": int(((nu > 16) & (nu <= 32)).sum()),
+                       "le64": int(((nu > 32) & (nu <= 64)).sum()), "gt64": int((nu > 64).sum())},
+            "kernels": {"fp32ir": [KERNEL_FP32IR[b] for b in buckets], "fp64": [KERNEL_FP64[b] for b in buckets]},
+            "solve_fp32ir": {"tflops": solve32, "frac_fp32_peak": solve32 / PEAK_FP32_TFLOPS if solve32 else None},
+            "solve_fp64": {"tflops": solve64, "frac_fp64_peak": solve64 / PEAK_FP64_TFLOPS if solve64 else None},
+            "search": {"gbs": search, "frac_hbm_peak": search / PEAK_HBM_GBS if search else None},
+            "autotune": self.tuned,
+        }
+
+    def oracle_leg(self, seconds, neighbour_sample=20000, kappa_sample=0):
+        """Times the oracle on a strided sample (all host threads, then one
+        thread on a smaller sample) and compares both GPU forms and the GPU
+        neighbour lists with it there."""
+        import oracle
+        from boitata_amd import _abi
+
+        w = self.w
+        d = oracle.Data(w.x, w.y, w.z, w.value, None, w.hole)
+
+        def solve(idx, threads):
+            P = self.T[idx]
+            if self.block is None:
+                return oracle.krige(d, w.variogram, w.search, P, threads=threads)
+            return oracle.block_krige(d, w.variogram, w.search, self.block, P, threads=threads)
+
+        probe = strided(self.m, 1000)
+        t0 = time.perf_counter()
+        solve(probe, self.threads)
+        rate = len(probe) / (time.perf_counter() - t0)
+        idx = strided(self.m, rate * seconds)
+        t0 = time.perf_counter()
+        o = solve(idx, self.threads)
+        dt = time.perf_counter() - t0
+        idx1 = strided(self.m, rate / self.threads * seconds / 3)
+        t0 = time.perf_counter()
+        solve(idx1, 1)
+        dt1 = time.perf_counter() - t0
+        stride = max(self.m // len(idx), 1)
+        what = "neighbour search + FP64 " + ("ordinary kriging" if self.block is None else "block kriging (LU)")
+        cpu = {"value": len(idx) / dt, "unit": "targets/s", "cores": self.threads, "kind": "port",
+               "sample": f"{len(idx)} of the {self.m} targets (every {stride}th), {what}, {dt:.1f} s",
+               "single_thread": {"value": len(idx1) / dt1, "unit": "targets/s", "cores": 1,
+                                 "sample": f"{len(idx1)} of the {self.m} targets, {dt1:.1f} s"}}
+        floor = 1e-3 * total_sill(w.variogram)
+        ok = (o["status"] == 0) & (self.s64[idx] == 0) & (self.s32[idx] == 0)
+        err = {}
+        for form, h in (("fp64", self.h64), ("fp32ir", self.h32)):
+            err[form] = {k: rel_stats(h[k][idx][ok], o[k][ok], floor) for k in ("value", "variance")}
+        # Neighbour sets on (a part of) the same sample.
+        nidx = idx[strided(len(idx), neighbour_sample)]
+        P = self.T[nidx]
+        oi, oc, os_ = oracle.neighbors(d, w.search, w.variogram, P, threads=self.threads)
+        gi, gc, gs = self.e.neighbors(w.search, w.variogram, P[:, 0], P[:, 1], P[:, 2])
+        bad = (oc != gc) | (os_ != gs)
+        cols = np.arange(oi.shape[1])[None, :] < oc[:, None]
+        bad |= ((oi != gi) & cols).any(1)
+        out = {"cpu_baseline": cpu, "oracle_sample": len(idx), "rel_err_vs_oracle": err,
+               "status_mismatch_vs_oracle": {"fp64": int((self.s64[idx] != o["status"]).sum()),
+                                             "fp32ir": int((self.s32[idx] != o["status"]).sum())},
+               "neighbour_sets_compared": len(nidx), "neighbour_set_mismatches": int(bad.sum())}
+        if kappa_sample > 0:
+            # 2-norm condition numbers of the covariance matrices of a strided
+            # subset of the sample's systems.
+            kk = nidx[strided(len(nidx), kappa_sample)]
+            sel = np.searchsorted(nidx, kk)
+            kap = np.array([np.linalg.cond(oracle.system_matrix(d, w.variogram, oi[j, :oc[j]]))
+                            for j in sel if oc[j] > 0])
+            out["kappa"] = {"systems": len(kap), "p50": float(np.median(kap)),
+                            "p99": float(np.percentile(kap, 99)), "max": float(kap.max())}
+        return out
+
+    def close(self):
+        self.e.close()
+
+
+def roofline_of(rep, traffic):
+    """The roofline object of the headline: the kernel with the most time."""
+    st = rep["stages_ms_fp32ir"]
+    if st["solve"] >= st["search"]:
+        ach = rep["solve_fp32ir"]["tflops"]
+        return dict(bound="valu", achieved=ach, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=ach / PEAK_FP32_TFLOPS, traffic=traffic, kernel=" + ".join(rep["kernels"]["fp32ir"]),
+                    note="compute-bound batched solves: F(n) algorithmic flops of every system the refined "
+                         "kernels keep, over the solve stage's HIP-event time, against the FP32 peak (the "
+                         "kernels compute in FP32 on the VALU, elimination on FP32 MFMA, residuals in FP64)")
+    ach = rep["search"]["gbs"]
+    return dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+                traffic=None, kernel="k_knn_wave")
+
+
+def loo_leg(args, dev, threads):
+    """C4: Adam on the leave-one-out loss through the adjoint kernels."""
+    import torch
+
+    from boitata_amd import _abi, datasets, loocv
+    from boitata_amd.engine import Engine
+
+    tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    w4 = datasets.config4()
+    models = ["exponential", "spherical"]
+    e = Engine(0)
+    e.set_samples(tt(w4.x), tt(w4.y), tt(w4.z), tt(w4.value), None, tt(w4.hole.astype(np.int32)))
+    t0 = time.perf_counter()
+    e.loo_prepare(w4.search, None)
+    torch.cuda.synchronize()
+    prep = time.perf_counter() - t0
+    th_true = loocv.theta_of(w4.variogram)
+    sign = np.random.default_rng(4).choice([-1.0, 1.0], len(th_true))
+    th0 = th_true * (1.0 + 0.3 * sign)
+    th0[-3:] = th_true[-3:] + 0.3 * sign[-3:] * np.array([30.0, 20.0, 10.0])  # angles: degrees off
+
+    def adam_rate(prec, steps):
+        loocv.fit_adam(e, th0, models, steps=2, lr=0.02, precision=prec)  # warm-up (first optimizer step)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        th, hist = loocv.fit_adam(e, th0, models, steps=steps, lr=0.02, precision=prec)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, th, hist
+
+    dtl, th, hist = adam_rate(_abi.FP32_IR, args.loo_steps)
+    log(f"LOO-CV: {args.loo_steps / dtl:.1f} Adam steps/s on {w4.n} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    s64 = max(args.loo_steps // 4, 1)
+    dtl64, _, _ = adam_rate(_abi.FP64, s64)
+    loss, grad, used = loocv.loss_and_grad(e, th, models, _abi.FP32_IR)
+    loss64, grad64, _ = loocv.loss_and_grad(e, th, models, _abi.FP64)
+    gfloor = 1e-3 * float(np.linalg.norm(grad64))
+    # Central differences of the FP64 device loss at the starting parameters
+    # (Adam drives the nugget towards 0, where a central step leaves the
+    # domain). The device FP64 loss equals the oracle's to 1e-9 relative on
+    # these fixed sets; the gradient tests difference the oracle's loss itself.
+    fd = np.zeros_like(th0)
+    for p in range(len(th0)):
+        h = 1e-5 * max(abs(th0[p]), 1.0)
+        tp, tm = th0.copy(), th0.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd[p] = (loocv.loss_and_grad(e, tp, models, _abi.FP64)[0] -
+                 loocv.loss_and_grad(e, tm, models, _abi.FP64)[0]) / (2 * h)
+    ffloor = 1e-3 * float(np.linalg.norm(fd))
+    g0_32 = loocv.loss_and_grad(e, th0, models, _abi.FP32_IR)[1]
+    g0_64 = loocv.loss_and_grad(e, th0, models, _abi.FP64)[1]
+
+    def rel(g):
+        return float(np.max(np.abs(g - fd) / np.maximum(np.abs(fd), ffloor)))
+
+    line = {"workload": w4.name, "value": args.loo_steps / dtl, "unit": "Adam steps/s", "steps": args.loo_steps,
+            "ms_per_step": dtl / args.loo_steps * 1e3, "samples": w4.n, "systems_per_step": used,
+            "parameters": len(th0), "lr": 0.02, "loss_first": hist[0], "loss_last": hist[-1],
+            "neighbour_sets_fixed_s": prep,
+            "precision": "FP32 elimination forward with FP64 refinement, FP32 adjoint, FP64 reduction",
+            "fp64": {"value": s64 / dtl64, "unit": "Adam steps/s"},
+            "speedup_fp32ir_over_fp64": (args.loo_steps / dtl) / (s64 / dtl64),
+            "rel_err_vs_fp64": {"loss": abs(loss - loss64) / abs(loss64),
+                                "grad": float(np.max(np.abs(grad - grad64) / np.maximum(np.abs(grad64), gfloor)))},
+            "grad_rel_err_vs_central_differences": {"fp32ir": rel(g0_32), "fp64": rel(g0_64),
+                                                    "at": "starting parameters", "floor": "1e-3 |fd|",
+                                                    "step": "1e-5 max(|theta|, 1)"}}
+    e.close()
+    if threads:
+        import oracle
+
+        d = oracle.Data(w4.x, w4.y, w4.z, w4.value, None, w4.hole)
+        vg = loocv.variogram_of(th, models)
+        which = strided(w4.n, 2000)
+        t0 = time.perf_counter()
+        oracle.loo(d, vg, w4.search, which=which, threads=threads)
+        rate = len(which) / (time.perf_counter() - t0)
+        which = strided(w4.n, rate * args.cpu_seconds)
+        t0 = time.perf_counter()
+        oracle.loo(d, vg, w4.search, which=which, threads=threads)
+        dt = time.perf_counter() - t0
+        rate = len(which) / dt
+        # A CPU optimiser without an adjoint needs 2 P + 1 loss passes per step.
+        line["cpu_baseline"] = {
+            "value": rate / w4.n / (2 * len(th) + 1), "unit": "Adam steps/s (central differences)",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle leave-one-out (search + FP64 solve) of {len(which)} of the {w4.n} samples in "
+                      f"{dt:.1f} s: {rate:.0f} systems/s; a step of central differences is {2 * len(th) + 1} "
+                      "passes over all samples"}
+    return line
+
+
+def mik_leg(args, w, dev, threads):
+    """Multiple indicator kriging of the headline targets: the nine deciles of
+    the data as thresholds with the one variogram (median indicator kriging),
+    three cutoffs and three quantiles; one pass = search, solves writing
+    weights, indicator estimates, conditional distributions."""
+    import torch
+
+    from boitata_amd import _abi
+    from boitata_amd.engine import Engine
+
+    tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    e = Engine(0)
+    e.set_samples(tt(w.x), tt(w.y), tt(w.z), tt(w.value), None, tt(w.hole.astype(np.int32)))
+    T = w.targets
+    m = len(T)
+    tx, ty, tz = tt(T[:, 0]), tt(T[:, 1]), tt(T[:, 2])
+    vg, search = w.variogram, w.search
+    thr = np.quantile(w.value, np.linspace(0.1, 0.9, 9))
+    mk = _abi.make_mik(thr, 1)
+    kw = dict(cutoffs=[0.5, 1.0, 2.0], quantiles=[0.1, 0.5, 0.9])
+    mres = {}
+    for prec in (_abi.FP32_IR, _abi.FP64):
+        mres[prec] = e.mik_predict(mk, thr, [vg], search, tx, ty, tz, precision=prec, **kw)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.mik_steps):
+            mres[prec] = e.mik_predict(mk, thr, [vg], search, tx, ty, tz, precision=prec, **kw)
+        torch.cuda.synchronize()
+        mres[prec]["dt"] = (time.perf_counter() - t0) / args.mik_steps
+    a32, a64 = mres[_abi.FP32_IR], mres[_abi.FP64]
+    okm = ((a32["status"] == 0) & (a64["status"] == 0)).cpu().numpy()
+
+    def rel(k):
+        x, y = a32[k].cpu().numpy()[..., okm], a64[k].cpu().numpy()[..., okm]
+        return rel_stats(x.ravel(), y.ravel(), 1e-3) if okm.any() else None
+
+    line = {"value": m / a32["dt"], "unit": "targets/s", "ms_per_step": a32["dt"] * 1e3,
+            "steps": args.mik_steps, "thresholds": len(thr), "variograms": 1, "cutoffs": 3, "quantiles": 3,
+            "precision": "FP32 elimination weights with FP64 refinement",
+            "fp64": {"value": m / a64["dt"], "ms_per_step": a64["dt"] * 1e3},
+            "rel_err_fp32ir_vs_fp64": {"cdf": rel("cdf"), "mean": rel("mean"), "variance": rel("variance")},
+            "estimated": int(okm.sum())}
+    log(f"MIK: {line['value']:.0f} targets/s FP32+IR, {line['fp64']['value']:.0f} FP64")
+    e.close()
+    if threads:
+        import oracle
+
+        d = oracle.Data(w.x, w.y, w.z, w.value, None, w.hole)
+        few = T[strided(m, 20000)]
+        t0 = time.perf_counter()
+        oracle.mik_predict(d, mk, thr, [vg], search, few, threads=threads, **kw)
+        dtm = time.perf_counter() - t0
+        line["cpu_baseline"] = dict(value=len(few) / dtm, unit="targets/s", cores=threads, kind="port",
+                                    sample=f"{len(few)} of the {m} targets, {dtm:.1f} s")
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--targets", type=int, default=0, help="use the first N targets only (0: all)")
-    ap.add_argument("--loo-steps", type=int, default=200, help="Adam steps of the LOO-CV gradient leg (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and oracle comparisons")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="all-thread oracle time per leg")
+    ap.add_argument("--targets", type=int, default=0, help="headline: use the first N targets only (0: all)")
+    ap.add_argument("--loo-steps", type=int, default=200, help="Adam steps of the C4 leg (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tuning")
-    ap.add_argument("--mik-steps", type=int, default=3,
-                    help="timed passes of the multiple indicator kriging leg (0: skip)")
+    ap.add_argument("--mik-steps", type=int, default=3, help="timed passes of the MIK leg (0: skip)")
     ap.add_argument("--config-steps", type=int, default=2,
-                    help="timed passes of the uneven (4M targets, octants) and block (2M blocks) legs (0: skip)")
+                    help="timed passes of the C1, C3 and C5 legs (0: skip them)")
+    ap.add_argument("--legs", default="c1,c3,c4,c5,mik", help="comma-separated legs beside the headline")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="finish with a device-to-device copy of this many MiB (byte calibration of PMC runs)")
     args = ap.parse_args()
@@ -95,285 +512,68 @@ def main():
     import torch
 
     from boitata_amd import _abi, datasets
-    from boitata_amd.engine import Engine
 
     if not torch.cuda.is_available():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+    legs = {s.strip() for s in args.legs.split(",") if s.strip()}
+    threads = 0 if args.no_cpu else int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
     w = datasets.config2()
-    T = w.targets if args.targets <= 0 else w.targets[: args.targets]
-    m = len(T)
-    print(f"[bench] workload: {w.n} samples, {m} targets", file=sys.stderr, flush=True)
+    if args.targets > 0:
+        w.targets = w.targets[: args.targets]
+    m = len(w.targets)
+    log(f"headline: {w.n} samples, {m} targets")
+    head = Leg("C2", w, args, threads, dev)
+    if head.tuned:
+        log(f"autotune: {head.tuned}")
+    head.gpu(args.steps, args.warmup)
+    log(f"FP32+IR {head.dt32 * 1e3:.2f} ms/step, FP64 {head.dt64 * 1e3:.2f} ms/step")
+    rep = head.report()
+    graph_launches = head.e.stat(_abi.STAT_GRAPH_LAUNCHES)
+    if threads:
+        rep.update(head.oracle_leg(args.cpu_seconds))
+        log(f"C2 CPU oracle {rep['cpu_baseline']['value']:.0f} targets/s on {threads} threads; "
+            f"neighbour mismatches {rep['neighbour_set_mismatches']}")
+    head.close()
+    traffic = measured_traffic("traffic_bytes_k_krige32m_c2") if m == 1_000_000 else None
+    roof = roofline_of(rep, traffic)
 
-    tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
-    sx, sy, sz, sv = tt(w.x), tt(w.y), tt(w.z), tt(w.value)
-    hole = tt(w.hole.astype(np.int32))
-    tx, ty, tz = tt(T[:, 0]), tt(T[:, 1]), tt(T[:, 2])
-    e = Engine(0)
-    e.set_samples(sx, sy, sz, sv, None, hole)
-    vg, search = w.variogram, w.search
-    # Each form writes into its own output arrays, so a repeated pass has the
-    # same arguments and the library runs it as a captured HIP graph.
-    outs = {}
-
-    def run(prec):
-        outs[prec] = e.krige(vg, search, tx, ty, tz, precision=prec, out=outs.get(prec))
-        return outs[prec]
-
-    # The autotuner picks grid, search mode, workgroup size and bucket edges on
-    # a strided sample of 64k targets (results do not depend on them).
-    tuned = None
-    if not args.no_autotune:
-        t0 = time.perf_counter()
-        tuned = e.autotune(vg, search, tx, ty, tz, precision=_abi.FP32_IR, sample=65536)
-        tuned["seconds"] = time.perf_counter() - t0
-        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
-    for _ in range(max(args.warmup, 1)):
-        run(_abi.FP32_IR)
-        run(_abi.FP64)
-    torch.cuda.synchronize()
-
-    def timed(prec):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        out = None
-        for _ in range(args.steps):
-            out = run(prec)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / args.steps, out
-
-    dt32, out32 = timed(_abi.FP32_IR)
-    dt64, out64 = timed(_abi.FP64)
-    print(f"[bench] FP32+IR {dt32 * 1e3:.2f} ms/step, FP64 {dt64 * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
-
-    # Stage times from HIP events on the library's stream.
-    e.profile(True)
-    run(_abi.FP32_IR)
-    st32, _ = e.profile_read()
-    run(_abi.FP64)
-    st64, _ = e.profile_read()
-    e.profile(False)
-
-    # Error of the refined form against plain FP64, floor 1e-3 * sill.
-    c0 = vg.nugget + sum(vg.structures[k].sill for k in range(vg.n_structures))
-    s32, s64 = out32["status"].cpu().numpy(), out64["status"].cpu().numpy()
-    ok = (s32 == 0) & (s64 == 0)
-    v32, v64 = out32["value"].cpu().numpy()[ok], out64["value"].cpu().numpy()[ok]
-    q32, q64 = out32["variance"].cpu().numpy()[ok], out64["variance"].cpu().numpy()[ok]
-    floor = 1e-3 * c0
-    err_v = float(np.max(np.abs(v32 - v64) / np.maximum(np.abs(v64), floor))) if ok.any() else float("nan")
-    err_q = float(np.max(np.abs(q32 - q64) / np.maximum(np.abs(q64), floor))) if ok.any() else float("nan")
-    status_equal = bool((s32 == s64).all())
-
-    n_used = out64["n_used"].cpu().numpy()
-    shape_sum = sum(SHAPE_COST[vg.structures[k].model] for k in range(vg.n_structures))
-    flops = float(flops_per_system(n_used[s64 == 0], shape_sum).sum())
-    sbytes = float(search_bytes_per_target(n_used).sum())
-
-    # The dominant kernel of the headline form. Its HBM traffic per launch comes
-    # from the committed FETCH_SIZE / WRITE_SIZE passes on this workload (only
-    # quoted when the full 1M-target workload runs).
-    solve_ms = st32["solve"]
-    search_ms = st32["search"]
-    traffic = measured_traffic("traffic_bytes_k_krige32_c2") if m == len(w.targets) else None
-    if solve_ms >= search_ms:
-        ach = flops / (solve_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP64_TFLOPS,
-                    traffic=traffic, kernel="k_krige32<32> (FP32 Gauss-Jordan inverse, FP64 refinement)",
-                    note="compute (VALU) bound batched solves: algorithmic F(n) flops per system over the kernel's "
-                         "time against the FP64 vector peak (= the FP64 matrix peak on MI355X); packed FP32 is off")
-    else:
-        ach = sbytes / (search_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
-                    traffic=None, kernel="k_knn_wave")
-    other = dict(search_gbs=sbytes / (search_ms * 1e-3) / 1e9 if search_ms > 0 else None,
-                 solve32_tflops=flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else None,
-                 solve64_tflops=flops / (st64["solve"] * 1e-3) / 1e12 if st64["solve"] > 0 else None)
-
-    # Second half of the metric: LOO-CV gradient steps/s on 200,000 samples of
-    # the same generator (800 holes), 12 parameters, fixed neighbour sets of
-    # the explicit 400/250/80 ellipsoid (<= 32 neighbours), Adam from
-    # parameters 30% off the generating values.
-    from boitata_amd import loocv
-
-    loo_line = None
-    if args.loo_steps > 0:
-        w4 = datasets.config4()
-        models = ["exponential", "spherical"]
-        e.set_samples(tt(w4.x), tt(w4.y), tt(w4.z), tt(w4.value), None, tt(w4.hole.astype(np.int32)))
-        e.loo_prepare(w4.search, None)
-        th_true = loocv.theta_of(w4.variogram)
-        sign = np.random.default_rng(4).choice([-1.0, 1.0], len(th_true))
-        th0 = th_true * (1.0 + 0.3 * sign)
-        th0[-3:] = th_true[-3:] + 0.3 * sign[-3:] * np.array([30.0, 20.0, 10.0])  # angles: degrees off
-
-        def adam_rate(prec, steps):
-            loocv.fit_adam(e, th0, models, steps=2, lr=0.02, precision=prec)  # warm-up (first optimizer step)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            th, hist = loocv.fit_adam(e, th0, models, steps=steps, lr=0.02, precision=prec)
-            torch.cuda.synchronize()
-            return time.perf_counter() - t0, th, hist
-
-        dtl, th, hist = adam_rate(_abi.FP32_IR, args.loo_steps)
-        # The same steps with the FP64 forward and adjoint, and the agreement of
-        # the two at the final parameters.
-        dtl64, _, hist64 = adam_rate(_abi.FP64, max(args.loo_steps // 4, 1))
-        loss, grad, used = loocv.loss_and_grad(e, th, models, _abi.FP32_IR)
-        loss64, grad64, _ = loocv.loss_and_grad(e, th, models, _abi.FP64)
-        gfloor = 1e-3 * float(np.linalg.norm(grad64))
-        loo_line = {"value": args.loo_steps / dtl, "unit": "Adam steps/s", "steps": args.loo_steps,
-                    "ms_per_step": dtl / args.loo_steps * 1e3, "samples": w4.n, "systems_per_step": used,
-                    "parameters": len(th0), "lr": 0.02, "loss_first": hist[0], "loss_last": hist[-1],
-                    "precision": "FP32 Gauss-Jordan forward with FP64 refinement, FP32 adjoint, FP64 reduction",
-                    "fp64": {"value": max(args.loo_steps // 4, 1) / dtl64, "unit": "Adam steps/s"},
-                    "rel_err_vs_fp64": {"loss": abs(loss - loss64) / abs(loss64),
-                                        "grad": float(np.max(np.abs(grad - grad64) /
-                                                             np.maximum(np.abs(grad64), gfloor)))}}
-        print(f"[bench] LOO-CV: {loo_line['value']:.1f} Adam steps/s on {w4.n} samples, loss {hist[0]:.4f} -> "
-              f"{hist[-1]:.4f}", file=sys.stderr, flush=True)
-        e.set_samples(sx, sy, sz, sv, None, hole)
-
-    # Multiple indicator kriging on the same samples and targets: the nine
-    # deciles of the data as thresholds with the one variogram (median
-    # indicator kriging), three cutoffs and three quantiles; one pass = search,
-    # solves writing weights, indicator estimates, conditional distributions.
-    mik_line = None
-    if args.mik_steps > 0:
-        thr = np.quantile(w.value, np.linspace(0.1, 0.9, 9))
-        mk = _abi.make_mik(thr, 1)
-        kw = dict(cutoffs=[0.5, 1.0, 2.0], quantiles=[0.1, 0.5, 0.9])
-        mres = {}
-        for prec in (_abi.FP32_IR, _abi.FP64):
-            mres[prec] = e.mik_predict(mk, thr, [vg], search, tx, ty, tz, precision=prec, **kw)  # warm-up
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(args.mik_steps):
-                mres[prec] = e.mik_predict(mk, thr, [vg], search, tx, ty, tz, precision=prec, **kw)
-            torch.cuda.synchronize()
-            mres[prec]["dt"] = (time.perf_counter() - t0) / args.mik_steps
-        a32, a64 = mres[_abi.FP32_IR], mres[_abi.FP64]
-        okm = ((a32["status"] == 0) & (a64["status"] == 0)).cpu().numpy()
-        def rel(k):
-            x, y = a32[k].cpu().numpy()[..., okm], a64[k].cpu().numpy()[..., okm]
-            return float(np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-3))) if okm.any() else float("nan")
-        mik_line = {"value": m / a32["dt"], "unit": "targets/s", "ms_per_step": a32["dt"] * 1e3,
-                    "steps": args.mik_steps, "thresholds": len(thr), "variograms": 1, "cutoffs": 3, "quantiles": 3,
-                    "precision": "FP32 Gauss-Jordan weights with FP64 refinement",
-                    "fp64": {"value": m / a64["dt"], "ms_per_step": a64["dt"] * 1e3},
-                    "rel_err_vs_fp64": {"cdf": rel("cdf"), "mean": rel("mean"), "variance": rel("variance")},
-                    "estimated": int(okm.sum())}
-        print(f"[bench] MIK: {mik_line['value']:.0f} targets/s FP32+IR, {mik_line['fp64']['value']:.0f} FP64",
-              file=sys.stderr, flush=True)
-
-    # The other workloads, measured beside the headline: uneven neighbourhoods
-    # (4M targets among clustered samples, octant search, 4-64 neighbours:
-    # size buckets) and block kriging (2M blocks discretised 4 x 4 x 2, 64
-    # neighbours, Gaussian covariance with a 1e-4 nugget: ill-conditioned
-    # systems that the refined form hands back to FP64).
-    configs = None
+    configs = {"C2": rep}
     if args.config_steps > 0:
-        configs = {}
-        for name, gen in (("uneven_4m_octants", datasets.config3), ("block_2m_gaussian", datasets.config5)):
-            wc = gen()
-            Tc = wc.targets
-            mc = len(Tc)
-            holes = None if wc.hole is None else tt(wc.hole.astype(np.int32))
-            ec = Engine(0)  # each workload on a context of its own (its own tuning)
-            ec.set_samples(tt(wc.x), tt(wc.y), tt(wc.z), tt(wc.value), None, holes)
-            cx, cy, cz = tt(Tc[:, 0]), tt(Tc[:, 1]), tt(Tc[:, 2])
-            blk = getattr(wc, "block", None)
-            tuned_c = None
-            if not args.no_autotune and blk is None:
-                tuned_c = ec.autotune(wc.variogram, wc.search, cx, cy, cz, precision=_abi.FP32_IR, sample=65536)
-            res = {}
-            for prec in (_abi.FP32_IR, _abi.FP64):
-                def one():
-                    if blk is None:
-                        return ec.krige(wc.variogram, wc.search, cx, cy, cz, precision=prec, out=res.get(prec))
-                    return ec.block_krige(wc.variogram, wc.search, blk, cx, cy, cz, precision=prec)
-                res[prec] = one()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(args.config_steps):
-                    res[prec] = one()
-                torch.cuda.synchronize()
-                res[prec]["dt"] = (time.perf_counter() - t0) / args.config_steps
-            a32, a64 = res[_abi.FP32_IR], res[_abi.FP64]
-            s32, s64 = a32["status"].cpu().numpy(), a64["status"].cpu().numpy()
-            okc = (s32 == 0) & (s64 == 0)
-            c0c = wc.variogram.nugget + sum(wc.variogram.structures[k].sill for k in range(wc.variogram.n_structures))
-            errs = {}
-            for k in ("value", "variance"):
-                x, y = a32[k].cpu().numpy()[okc], a64[k].cpu().numpy()[okc]
-                errs[k] = float(np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-3 * c0c))) if okc.any() else None
-            nu = a64["n_used"].cpu().numpy()[s64 == 0]
-            configs[name] = {
-                "targets": mc, "samples": wc.n, "value": mc / a32["dt"], "unit": "targets/s",
-                "ms_per_step": a32["dt"] * 1e3, "fp64": {"value": mc / a64["dt"], "ms_per_step": a64["dt"] * 1e3},
-                "speedup_fp32ir_over_fp64": a64["dt"] / a32["dt"], "max_rel_err_vs_fp64": errs,
-                "status_equal": bool((s32 == s64).all()), "estimated": int((s64 == 0).sum()),
-                "n_used": {"min": int(nu.min()) if len(nu) else 0, "max": int(nu.max()) if len(nu) else 0,
-                           "le16": int((nu <= 16).sum()), "le32": int(((nu > 16) & (nu <= 32)).sum()),
-                           "le64": int((nu > 32).sum())},
-                "support": "block 4x4x2" if blk is not None else "point",
-                "refine_limit": tuned_c["refine_limit"] if tuned_c else None}
-            print(f"[bench] {name}: {configs[name]['value']:.0f} targets/s FP32+IR, "
-                  f"{configs[name]['fp64']['value']:.0f} FP64", file=sys.stderr, flush=True)
-            ec.close()
-            del res
+        for key, gen, kap in (("C1", datasets.config1, 0), ("C3", datasets.config3, 0), ("C5", datasets.config5, 400)):
+            if key.lower() not in legs:
+                continue
+            leg = Leg(key, gen(), args, threads, dev)
+            leg.gpu(args.config_steps, 1)
+            r = leg.report()
+            log(f"{key}: {r['value']:.0f} targets/s FP32+IR, {r['fp64']['value']:.0f} FP64, "
+                f"x{r['speedup_fp32ir_over_fp64']:.2f}")
+            if threads:
+                r.update(leg.oracle_leg(args.cpu_seconds, kappa_sample=kap))
+                log(f"{key} CPU oracle {r['cpu_baseline']['value']:.0f} targets/s; neighbour mismatches "
+                    f"{r['neighbour_set_mismatches']}")
+            configs[key] = r
+            leg.close()
+            del leg
+    if args.loo_steps > 0 and "c4" in legs:
+        configs["C4"] = loo_leg(args, dev, threads)
+    mik = mik_leg(args, w, dev, threads) if args.mik_steps > 0 and "mik" in legs else None
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        d = oracle.Data(w.x, w.y, w.z, w.value, None, w.hole)
-        probe = T[:: max(m // 4000, 1)][:4000]
-        t0 = time.perf_counter()
-        oracle.krige(d, vg, search, probe, threads=threads)
-        rate = len(probe) / (time.perf_counter() - t0)
-        k = int(min(m, max(len(probe), rate * args.cpu_seconds)))
-        sample = T[:: max(m // k, 1)][:k]
-        # Whole passes over the sample until about cpu_seconds of work.
-        passes, done, t0 = 0, 0, time.perf_counter()
-        while True:
-            oracle.krige(d, vg, search, sample, threads=threads)
-            passes += 1
-            done += len(sample)
-            dt = time.perf_counter() - t0
-            if dt >= args.cpu_seconds or passes >= 64:
-                break
-        cpu = dict(value=done / dt, unit="targets/s", cores=threads, kind="port",
-                   sample=f"{passes} pass(es) over {len(sample)} of the {m} targets (every {max(m // k, 1)}th), "
-                          f"neighbour search + FP64 ordinary kriging, {dt:.1f} s")
-        # The same restatement on one thread, over a smaller strided sample.
-        one = T[:: max(m // 60000, 1)][:60000]
-        t0 = time.perf_counter()
-        oracle.krige(d, vg, search, one, threads=1)
-        dt1 = time.perf_counter() - t0
-        cpu["single_thread"] = dict(value=len(one) / dt1, unit="targets/s", cores=1,
-                                    sample=f"{len(one)} of the {m} targets (every {max(m // 60000, 1)}th), {dt1:.1f} s")
-        print(f"[bench] CPU oracle {cpu['value']:.0f} targets/s on {threads} threads, "
-              f"{cpu['single_thread']['value']:.0f} on one", file=sys.stderr, flush=True)
-        if mik_line is not None:
-            # The oracle's multiple indicator kriging on a strided sample.
-            few = T[:: max(m // 20000, 1)][:20000]
-            t0 = time.perf_counter()
-            oracle.mik_predict(d, mk, thr, [vg], search, few, threads=threads, **kw)
-            dtm = time.perf_counter() - t0
-            mik_line["cpu_baseline"] = dict(value=len(few) / dtm, unit="targets/s", cores=threads, kind="port",
-                                            sample=f"{len(few)} of the {m} targets, {dtm:.1f} s")
-
+    other = {"search_c2": rep["search"], "solve_fp64_c2": rep["solve_fp64"]}
+    for key in ("C3", "C5"):
+        if key in configs:
+            other[f"solve_fp32ir_{key.lower()}"] = dict(configs[key]["solve_fp32ir"],
+                                                     kernels=configs[key]["kernels"]["fp32ir"])
     line = {
         "metric": "kriged targets/s (FP32+IR vs FP64) at <=1e-6 rel err; LOO-CV grad steps/s",
-        "value": m / dt32,
+        "value": m / head.dt32,
         "unit": "targets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt32 * 1e3,
+        "ms_per_step": head.dt32 * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -382,20 +582,21 @@ def main():
         "config": {"workload": "C2: ordinary point kriging, 100k samples, 1M block targets, <=32 neighbours, "
                                "nested exponential + spherical with per-structure ranges",
                    "targets": m, "samples": w.n, "max_samples": 32, "parallelism": "single GPU"},
-        "fp64": {"value": m / dt64, "ms_per_step": dt64 * 1e3},
-        "speedup_fp32ir_over_fp64": dt64 / dt32,
-        "max_rel_err_vs_fp64": {"value": err_v, "variance": err_q, "floor": floor},
-        "status_equal": status_equal,
-        "stages_ms_fp32ir": st32,
-        "stages_ms_fp64": st64,
+        "fp64": rep["fp64"],
+        "speedup_fp32ir_over_fp64": rep["speedup_fp32ir_over_fp64"],
+        "max_rel_err_vs_fp64": {"value": rep["rel_err_fp32ir_vs_fp64"]["value"]["max"],
+                                "variance": rep["rel_err_fp32ir_vs_fp64"]["variance"]["max"],
+                                "floor": rep["rel_err_floor"]},
+        "status_equal": rep["status_equal_fp32ir_fp64"],
+        "stages_ms_fp32ir": rep["stages_ms_fp32ir"],
+        "stages_ms_fp64": rep["stages_ms_fp64"],
         "roofline": roof,
         "roofline_other": other,
-        "cpu_baseline": cpu,
-        "loo_cv_grad": loo_line,
-        "mik": mik_line,
-        "other_workloads": configs,
-        "autotune": tuned,
-        "graph_launches": e.stat(_abi.STAT_GRAPH_LAUNCHES),
+        "cpu_baseline": rep.get("cpu_baseline"),
+        "loo_cv_grad": configs.get("C4"),
+        "mik": mik,
+        "configs": configs,
+        "graph_launches": graph_launches,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
@@ -403,7 +604,6 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-    e.close()
 
 
 if __name__ == "__main__":

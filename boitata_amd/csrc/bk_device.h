@@ -278,6 +278,62 @@ __device__ __forceinline__ double exp_fast(double x) {
   return ldexp(p, (int)k);
 }
 
+// exp_fast's constants as opaque scalar-register values, for loops that
+// evaluate many exponentials: each Horner step is then one v_fma_f64 with a
+// scalar operand (an FP64 literal rematerialised in a VGPR pair costs two
+// v_mov_b32 before every step). Same arithmetic, same results as exp_fast.
+struct ExpConstants {
+  double c[10];  // 1/12! .. 1/3!; 0.5 and 1.0 are inline constants
+  double log2e, ln2_hi, ln2_lo;
+};
+
+__device__ __forceinline__ ExpConstants exp_constants() {
+  ExpConstants k;
+  k.c[0] = 1.0 / 479001600.0;
+  k.c[1] = 1.0 / 39916800.0;
+  k.c[2] = 1.0 / 3628800.0;
+  k.c[3] = 1.0 / 362880.0;
+  k.c[4] = 1.0 / 40320.0;
+  k.c[5] = 1.0 / 5040.0;
+  k.c[6] = 1.0 / 720.0;
+  k.c[7] = 1.0 / 120.0;
+  k.c[8] = 1.0 / 24.0;
+  k.c[9] = 1.0 / 6.0;
+  k.log2e = 1.44269504088896338700;
+  k.ln2_hi = 6.93147180369123816490e-01;
+  k.ln2_lo = 1.90821492927058770002e-10;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) asm volatile("" : "+s"(k.c[i]));
+  asm volatile("" : "+s"(k.log2e));
+  asm volatile("" : "+s"(k.ln2_hi));
+  asm volatile("" : "+s"(k.ln2_lo));
+  return k;
+}
+
+__device__ __forceinline__ double exp_fast(double x, const ExpConstants& k) {
+#pragma clang fp contract(fast)
+  if (x < -708.0) return 0.0;
+  double n = rint(x * k.log2e);
+  double r = (x - n * k.ln2_hi) - n * k.ln2_lo;
+  double p = k.c[0];
+#pragma unroll
+  for (int i = 1; i < 10; ++i) p = p * r + k.c[i];
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)n);
+}
+
+// sqrt(x) for x > 0 (x = 0 gives NaN), from the hardware reciprocal square
+// root and one Newton step.
+__device__ __forceinline__ double sqrt_pos(double x) {
+#pragma clang fp contract(fast)
+  double y = __builtin_amdgcn_rsq(x);
+  double s = x * y;
+  double e = fma(-s, s, x);  // x - s^2
+  return s + 0.5 * e * y;
+}
+
 // sqrt(x) from the hardware reciprocal square root and one Newton step.
 __device__ __forceinline__ double sqrt_fast(double x) {
 #pragma clang fp contract(fast)

@@ -704,6 +704,115 @@ __device__ __forceinline__ bool pairs_cov_fast(const DevVariogram& v, const doub
   return nearv != 0;
 }
 
+// half_pass for the systems that a near pair sends to the FP64 kernel: a pair
+// that is not near has a positive lag in every frame, so the nugget needs no
+// per-pair test, the square root no guard, and the near test one flag per lane
+// (offsets d >= n, possible for n < NMAX / 2 + 1, repeat pairs and are left out).
+template <int CH, class Shape>
+__device__ __forceinline__ void half_pass_near(const double* PJ, int SJ, int off, const double* me, int r, int n,
+                                               int d0, double sill, double ir, bool first, bool& near, double near2,
+                                               double* acc, Shape shape) {
+#pragma clang fp contract(fast)
+  const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
+#pragma unroll
+  for (int dd = 0; dd < CH; ++dd) {
+    const int d = d0 + dd;
+    int j = r + d;
+    j = j >= n ? j - n : j;
+    j = j < n ? j : 0;
+    const double* pj = PJ + j * SJ + off;
+    const double d0v = q0 - pj[0], d1 = q1 - pj[1], d2 = q2 - pj[2];
+    const double l2 = d0v * d0v + d1 * d1 + d2 * d2;
+    if (first) near |= l2 <= near2 && d < n;
+    acc[dd] += sill * shape(sqrt_pos(l2) * ir);
+    if (dd % 4 == 3) asm volatile("" ::: "memory");
+  }
+}
+
+// pairs_cov_fast for the near-pretest kernels (see half_pass_near): the
+// strict lower triangle into T, and whether one of the lane's pairs is near.
+// A near system's entries are not used.
+template <int NMAX>
+__device__ __forceinline__ bool pairs_cov_near(const DevVariogram& v, const double* PJ, int SJ, int r, int n,
+                                               double* T) {
+#pragma clang fp contract(fast)
+  if (r >= n) return false;
+  constexpr int CH = NMAX / 2 < 16 ? NMAX / 2 : 16;
+  const double* me = PJ + r * SJ;
+  bool near = false;
+  const double near2 = v.near2;
+  const double c0 = v.c0, nug = v.nugget;
+  const int half = (n - 1) / 2;
+  const bool even = (n & 1) == 0;
+  const ExpConstants k = exp_constants();
+#pragma unroll
+  for (int d0 = 1; d0 <= NMAX / 2; d0 += CH) {
+    double acc[CH];
+#pragma unroll
+    for (int dd = 0; dd < CH; ++dd) acc[dd] = 0.0;
+    for (int s = 0; s < v.S; ++s) {
+      const int off = 3 + 3 * v.frame[s];
+      const double sill = v.sill[s], ir = v.inv_range[s];
+      const bool first = s == 0;
+      switch (v.model[s]) {
+        case BK_SPHERICAL:
+          half_pass_near<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, near, near2, acc,
+                             [](double t) { return t >= 1.0 ? 1.0 : t * (1.5 - 0.5 * t * t); });
+          break;
+        case BK_EXPONENTIAL:
+          half_pass_near<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, near, near2, acc,
+                             [&k](double t) { return 1.0 - exp_fast(-3.0 * t, k); });
+          break;
+        case BK_GAUSSIAN:
+          half_pass_near<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, near, near2, acc,
+                             [&k](double t) { return 1.0 - exp_fast(-3.0 * t * t, k); });
+          break;
+        case BK_CUBIC:
+          half_pass_near<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, near, near2, acc, [](double t) {
+            const double t2 = t * t;
+            return t >= 1.0 ? 1.0 : t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2)));
+          });
+          break;
+        case BK_PENTASPHERICAL:
+          half_pass_near<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, near, near2, acc, [](double t) {
+            const double t2 = t * t;
+            return t >= 1.0 ? 1.0 : t * (1.875 + t2 * (-1.25 + 0.375 * t2));
+          });
+          break;
+        default:
+          half_pass_near<CH>(PJ, SJ, off, me, r, n, d0, sill, ir, first, near, near2, acc,
+                             [](double) { return __builtin_nan(""); });
+      }
+    }
+#pragma unroll
+    for (int dd = 0; dd < CH; ++dd) {
+      const int d = d0 + dd;
+      const bool valid = d <= half || (even && d == n / 2 && r < n / 2);
+      if (valid) {
+        int j = r + d;
+        j = j >= n ? j - n : j;
+        T[j < r ? r * (r + 1) / 2 + j : j * (j + 1) / 2 + r] = c0 - (nug + acc[dd]);
+      }
+    }
+  }
+  return near;
+}
+
+// The covariance of sample r's record to the target (cov_fast64 on the
+// record's frame coordinates, read where they lie in LDS).
+__device__ __forceinline__ double rhs_record(const DevVariogram& v, const double* me) {
+#pragma clang fp contract(fast)
+  const int nf0 = 3 + 3 * v.nug_frame;
+  if (!(norm3_fast(me[nf0], me[nf0 + 1], me[nf0 + 2]) > 0.0)) return v.c0;
+  double s = 0.0;
+  for (int k = 0; k < v.S; ++k) {
+    const int off = 3 + 3 * v.frame[k];
+    const double h = norm3_fast(me[off], me[off + 1], me[off + 2]);
+    s += v.sill[k] * shape_fast64(v.model[k], v.param[k], h, v.inv_range[k]);
+  }
+  return v.c0 - (v.nugget + s);
+}
+
 // The pair phase of the refined kernels for the system of target t (n
 // samples; lane r of group g): loads the samples as records in LDS (world
 // coordinates and the projection on every frame relative to the target,
@@ -782,8 +891,8 @@ __device__ __forceinline__ bool refined_pairs(const SolveArgs& a, int t, int n, 
   bool guard;
   c = 0.0;
   if constexpr (NEAR) {
-    if (active && r < n) c = rhs();
-    const bool near = active && pairs_cov_fast<NMAX>(a.v, PJ, SJ, r, n, T);
+    if (active && r < n) c = a.block ? rhs_fast_world(a, t0, t1, t2, wx, wy, wz) : rhs_record(a.v, PJ + r * SJ);
+    const bool near = active && pairs_cov_near<NMAX>(a.v, PJ, SJ, r, n, T);
     guard = group_any<NMAX>(near, g);
   } else {
     guard = coincident();
