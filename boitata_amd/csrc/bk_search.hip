@@ -840,14 +840,24 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn(const SearchArgs a) {
 __device__ __forceinline__ void cx_keys(uint64_t& k, uint32_t& i, int lane, int stride, bool up) {
   const uint64_t ok = shfl_xor_u64(k, stride);
   const uint32_t oi = __shfl_xor(i, stride, BK_WAVE);
-  const bool lower = (lane & stride) == 0;
-  // The lower lane keeps the smaller key when sorting up.
-  const bool other_less = key_less(ok, oi, k, i);
-  const bool take = (lower == up) ? other_less : !other_less && !(ok == k && oi == i);
-  if (take) {
-    k = ok;
-    i = oi;
+  // The lower lane keeps the smaller key when sorting up. Keys are distinct
+  // except padding entries, which are identical, so "the other is not
+  // smaller" means "the other is larger" or the same entry.
+  const bool want_small = ((lane & stride) == 0) == up;
+  const bool take = key_less(ok, oi, k, i) == want_small;
+  k = take ? ok : k;
+  i = take ? oi : i;
+}
+
+// Inclusive prefix sum over groups of G lanes (lane is the index within the group).
+template <int G>
+__device__ __forceinline__ uint32_t group_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, G);
+    v += lane >= o ? u : 0u;
   }
+  return v;
 }
 
 // Ascending bitonic sort of one key per lane over groups of G lanes (lane is
@@ -990,8 +1000,8 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
       const double bound = fmin(h2 * (double)m0, beyond) * slack;
       if (open && (bound > a.radius2 || (tk != KEY_INF && bound > __longlong_as_double((long long)tk)))) open = false;
       if (!__any(open)) break;
-      // Lane l: cell o0 + l.
-      uint32_t e = 0, e1 = 0;
+      // Lane l: cell o0 + l, whose entries are [e, e + cnt) of the CSR.
+      uint32_t e = 0, cnt = 0;
       const uint32_t o = o0 + lane;
       if (open && o < n_off) {
         const char4 f = a.tab.offsets[o];
@@ -999,16 +1009,28 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
         if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2]) {
           const int id = x + g.dims[0] * (y + g.dims[1] * z);
           e = g.start[id];
-          e1 = g.start[id + 1];
+          cnt = g.start[id + 1] - e;
         }
       }
-      while (__any(e < e1)) {
+      // The batch's entries, cell after cell, G per round: entry f of the
+      // concatenation lies in the cell of the first lane whose inclusive
+      // count exceeds f (a binary search over the group's lanes).
+      const uint32_t incl = group_incl_scan<G>(cnt, lane);
+      const uint32_t excl = incl - cnt;
+      const uint32_t total = __shfl(incl, G - 1, G);
+      for (uint32_t f = lane; __any(f - lane < total); f += G) {
+        int pos = 0;
+#pragma unroll
+        for (int step = G / 2; step > 0; step >>= 1) {
+          const uint32_t probe = __shfl(incl, pos + step - 1, G);
+          pos += probe <= f ? step : 0;
+        }
+        const uint32_t entry = __shfl(e, pos, G) + (f - __shfl(excl, pos, G));
         bool keep = false;
         uint64_t kk = 0;
         uint32_t ii = 0;
-        if (e < e1) {
-          const double4 E = g.entries[e];
-          ++e;
+        if (f < total) {
+          const double4 E = g.entries[entry];
           const double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
           const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
           ii = (uint32_t)__double_as_longlong(E.w);
@@ -1017,10 +1039,10 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
         }
         const uint64_t mask = group_ballot<G>(keep, h);
         if (__any(mask != 0)) {
-          const int pos = nbuf + __popcll(mask & ((1ull << lane) - 1ull));
+          const int slot = nbuf + __popcll(mask & ((1ull << lane) - 1ull));
           if (keep) {
-            bk[pos] = kk;
-            bi[pos] = ii;
+            bk[slot] = kk;
+            bi[slot] = ii;
           }
           nbuf += __popcll(mask);
           wave_sync();
