@@ -518,6 +518,54 @@ int prepare_search(bk_ctx* c, const bk_search* s, const bk_variogram* vg, Search
   a.min_sectors = (int)s->min_sectors;
   a.max_empty_sectors = (int)s->max_empty_sectors;
   a.min_holes = (int)std::min<int64_t>(s->min_holes, 1 << 30);
+  // The sector of a sample from the signs of its projected offset: axes F^-1
+  // must map each projected axis to one sector axis with a positive or
+  // negative scale (F = D R and axes the rows of R permuted with a sign).
+  a.sec_fast = 0;
+  if (a.balances && !a.plane) {
+    const double* f = F.m;
+    const double det = f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) +
+                       f[2] * (f[3] * f[7] - f[4] * f[6]);
+    if (std::isfinite(det) && det != 0.0) {
+      M3 inv{{(f[4] * f[8] - f[5] * f[7]) / det, (f[2] * f[7] - f[1] * f[8]) / det, (f[1] * f[5] - f[2] * f[4]) / det,
+              (f[5] * f[6] - f[3] * f[8]) / det, (f[0] * f[8] - f[2] * f[6]) / det, (f[2] * f[3] - f[0] * f[5]) / det,
+              (f[3] * f[7] - f[4] * f[6]) / det, (f[1] * f[6] - f[0] * f[7]) / det, (f[0] * f[4] - f[1] * f[3]) / det}};
+      const M3 map = mm(axes, inv);
+      int perm[3], sgn[3];
+      bool ok = true;
+      for (int k = 0; k < 3 && ok; ++k) {
+        int p = 0;
+        for (int j = 1; j < 3; ++j)
+          if (std::fabs(map.m[3 * k + j]) > std::fabs(map.m[3 * k + p])) p = j;
+        const double big = std::fabs(map.m[3 * k + p]);
+        for (int j = 0; j < 3; ++j)
+          if (j != p && !(std::fabs(map.m[3 * k + j]) <= 1e-9 * big)) ok = false;
+        perm[k] = p;
+        sgn[k] = map.m[3 * k + p] > 0.0 ? 1 : -1;
+      }
+      ok = ok && perm[0] != perm[1] && perm[1] != perm[2] && perm[0] != perm[2];
+      if (ok) {
+        double inv_norm = 0.0;
+        for (int i = 0; i < 3; ++i)
+          inv_norm = std::max(inv_norm, std::fabs(inv.m[3 * i]) + std::fabs(inv.m[3 * i + 1]) +
+                                            std::fabs(inv.m[3 * i + 2]));
+        for (int p = 0; p < 3; ++p)
+          a.sec_tol[p] = 1e-9 * (std::fabs(f[3 * p]) + std::fabs(f[3 * p + 1]) + std::fabs(f[3 * p + 2])) * inv_norm;
+        uint32_t code = 0;
+        for (int pat = 0; pat < 8; ++pat) {
+          int bit[3];
+          for (int k = 0; k < 3; ++k) {
+            const int pos = (pat >> perm[k]) & 1;  // u_perm[k] >= 0
+            bit[k] = sgn[k] > 0 ? pos : 1 - pos;
+          }
+          const int sec = a.planar ? (bit[0] << 1) | bit[1] : (bit[0] << 2) | (bit[1] << 1) | bit[2];
+          code |= (uint32_t)sec << (3 * pat);
+        }
+        a.sec_code = code;
+        a.sec_fast = 1;
+      }
+    }
+  }
   a.sx = (const double*)c->x.p;
   a.sy = (const double*)c->y.p;
   a.sz = (const double*)c->z.p;

@@ -51,6 +51,14 @@ struct SearchArgs {
   int max_per_hole;           // -1 none
   int min_sectors, max_empty_sectors, min_holes;  // -1 none
   double axes[9];             // block_frame of the metric's angles (search.rs:404-417)
+  // Octant or quadrant of a sample from its projected offset u = E - q when
+  // axes F^-1 is a signed permutation times a positive diagonal (always, for
+  // F = D R): sec_code holds the sector of each sign pattern of u (3 bits per
+  // pattern (u0 >= 0) | (u1 >= 0) << 1 | (u2 >= 0) << 2); a component with
+  // |u_k| <= sec_tol[k] (|E| + |q|) (1-norms) takes the world computation.
+  int sec_fast;
+  uint32_t sec_code;
+  double sec_tol[3];
   const double* sx;
   const double* sy;
   const double* sz;

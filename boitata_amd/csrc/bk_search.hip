@@ -474,11 +474,67 @@ struct Ctx {
   }
 };
 
-__global__ void __launch_bounds__(BK_WAVE) k_search(const SearchArgs a) {
-  __shared__ SearchLds L;
+// Selector::finish (search.rs:547-576): the status of a selection of n
+// samples chosen[0 .. n) holding per_sector[s] samples of sector s.
+__device__ int selection_status(const SearchArgs& a, int n, const int16_t* per_sector, const int* chosen) {
+  if (n < a.min_samples) return BK_SEARCH_FAILED;
+  if (a.min_sectors >= 0) {
+    int held = 0;
+    for (int s = 0; s < a.n_sectors; ++s) held += per_sector[s] > 0;
+    if (held < a.min_sectors) return BK_SEARCH_FAILED;
+  }
+  if (a.max_empty_sectors >= 0) {
+    // search.rs:445-467: longest run of empty sectors around each ring.
+    int best = 0;
+    const int rings = a.plane ? 1 : (a.planar ? 1 : 2);
+    for (int ri = 0; ri < rings; ++ri) {
+      const int len = a.plane ? a.plane : 4;
+      auto at = [&](int k) {
+        if (a.plane) return k;
+        if (a.planar) {
+          const int r0[4] = {3, 1, 0, 2};
+          return r0[k];
+        }
+        const int r1[4] = {7, 3, 1, 5};
+        const int r2[4] = {6, 2, 0, 4};
+        return ri == 0 ? r1[k] : r2[k];
+      };
+      bool all = true;
+      for (int k = 0; k < len; ++k) all = all && per_sector[at(k)] == 0;
+      if (all) {
+        best = max(best, len);
+        continue;
+      }
+      int run = 0, longest = 0;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int k = 0; k < len; ++k) {
+          run = per_sector[at(k)] == 0 ? run + 1 : 0;
+          longest = max(longest, run);
+        }
+      best = max(best, longest);
+    }
+    if (best > a.max_empty_sectors) return BK_SEARCH_FAILED;
+  }
+  if (a.min_holes >= 0) {
+    int distinct = 0;
+    for (int k = 0; k < n; ++k) {
+      const int h = a.hole ? a.hole[chosen[k]] : -1;
+      if (h < 0) {
+        distinct++;
+        continue;
+      }
+      bool seen = false;
+      for (int j = 0; j < k && !seen; ++j) seen = a.hole[chosen[j]] == h;
+      distinct += !seen;
+    }
+    if (distinct < a.min_holes) return BK_SEARCH_FAILED;
+  }
+  return BK_OK;
+}
+
+// The selection of target t by one wavefront (L: its LDS).
+__device__ void search_target(const SearchArgs& a, SearchLds& L, int64_t t) {
   const int lane = threadIdx.x;
-  const int64_t b = blockIdx.x;
-  const int64_t t = a.list ? a.list[b] : b;
   Ctx c;
   c.a = &a;
   c.L = &L;
@@ -573,66 +629,20 @@ __global__ void __launch_bounds__(BK_WAVE) k_search(const SearchArgs a) {
   int* out = a.idx + (int64_t)t * a.stride;
   for (int k = lane; k < n; k += BK_WAVE) out[k] = L.chosen[k];
   if (lane == 0) {
-    int status = BK_OK;
-    if (c.st.overflow) {
-      status = BK_ST_OVERFLOW;
-    } else if (n < a.min_samples) {
-      status = BK_SEARCH_FAILED;
-    } else {
-      if (a.min_sectors >= 0) {
-        int held = 0;
-        for (int s = 0; s < a.n_sectors; ++s) held += L.per_sector[s] > 0;
-        if (held < a.min_sectors) status = BK_SEARCH_FAILED;
-      }
-      if (status == BK_OK && a.max_empty_sectors >= 0) {
-        // search.rs:445-467: longest run of empty sectors around each ring.
-        int best = 0;
-        int rings = a.plane ? 1 : (a.planar ? 1 : 2);
-        for (int ri = 0; ri < rings; ++ri) {
-          int len = a.plane ? a.plane : 4;
-          auto at = [&](int k) {
-            if (a.plane) return k;
-            if (a.planar) {
-              const int r0[4] = {3, 1, 0, 2};
-              return r0[k];
-            }
-            const int r1[4] = {7, 3, 1, 5};
-            const int r2[4] = {6, 2, 0, 4};
-            return ri == 0 ? r1[k] : r2[k];
-          };
-          bool all = true;
-          for (int k = 0; k < len; ++k) all = all && L.per_sector[at(k)] == 0;
-          if (all) {
-            best = max(best, len);
-            continue;
-          }
-          int run = 0, longest = 0;
-          for (int pass = 0; pass < 2; ++pass)
-            for (int k = 0; k < len; ++k) {
-              run = L.per_sector[at(k)] == 0 ? run + 1 : 0;
-              longest = max(longest, run);
-            }
-          best = max(best, longest);
-        }
-        if (best > a.max_empty_sectors) status = BK_SEARCH_FAILED;
-      }
-      if (status == BK_OK && a.min_holes >= 0) {
-        int distinct = 0;
-        for (int k = 0; k < n; ++k) {
-          int h = a.hole ? a.hole[L.chosen[k]] : -1;
-          if (h < 0) {
-            distinct++;
-            continue;
-          }
-          bool seen = false;
-          for (int j = 0; j < k && !seen; ++j) seen = a.hole[L.chosen[j]] == h;
-          distinct += !seen;
-        }
-        if (distinct < a.min_holes) status = BK_SEARCH_FAILED;
-      }
-    }
+    const int status = c.st.overflow ? BK_ST_OVERFLOW : selection_status(a, n, L.per_sector, L.chosen);
     a.count[t] = n;
     a.status[t] = status;
+  }
+}
+
+// One block per target, or with a.m_dev (a list whose length is on the
+// device) blocks striding over the list.
+__global__ void __launch_bounds__(BK_WAVE) k_search(const SearchArgs a) {
+  __shared__ SearchLds L;
+  const int64_t end = a.m_dev ? (int64_t)*a.m_dev : (int64_t)blockIdx.x + 1;
+  for (int64_t b = blockIdx.x; b < end; b += gridDim.x) {
+    search_target(a, L, a.list ? a.list[b] : b);
+    wave_sync();
   }
 }
 
@@ -1086,6 +1096,207 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
     }
     a.count[t] = n;
     a.status[t] = status;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Octant (or planar quadrant) searches without a hole cap, cap <= S per
+// sector. Selector::offer (search.rs:519-543) takes a candidate, in (d2,
+// index) order, iff its sector holds fewer than cap and the selection is not
+// full; so the selection is the first max_samples entries, in that order, of
+// the union of every sector's cap smallest keys within the radius (before
+// the selection is full each sector takes exactly its first cap candidates,
+// and what the walk offers after that lies beyond all of them). One target
+// per wavefront: lanes S o .. S o + S - 1 hold sector o's smallest keys,
+// sorted; candidates below their sector's cap-th key gather in per-sector
+// LDS buffers, merged into the lists by bitonic networks over S lanes, all
+// sectors at once. The walk follows the shell table as k_knn_wave does and
+// stops once every sector holds cap keys and the shell's lower bound passes
+// the largest cap-th key (or the radius); a target still open after the
+// table goes to k_search.
+
+// The sector of projected offset u (entry E, target q) where the signs
+// decide it; -1 where a component lies within rounding of 0.
+__device__ __forceinline__ int sector_fast(const SearchArgs& a, double u0, double u1, double u2, double scale) {
+  const bool amb = fabs(u0) <= a.sec_tol[0] * scale || fabs(u1) <= a.sec_tol[1] * scale ||
+                   fabs(u2) <= a.sec_tol[2] * scale;
+  const int pat = (u0 >= 0.0 ? 1 : 0) | (u1 >= 0.0 ? 2 : 0) | (u2 >= 0.0 ? 4 : 0);
+  return amb ? -1 : (int)((a.sec_code >> (3 * pat)) & 7u);
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
+  constexpr int NG = BK_WAVE / S;    // sector groups
+  constexpr int CAPB = BK_WAVE + S;  // buffer entries per sector: < S held + one round
+  __shared__ uint64_t buf_k[4][NG][CAPB];
+  __shared__ uint32_t buf_i[4][NG][CAPB];
+  __shared__ uint32_t buf_n[4][NG];
+  __shared__ uint64_t thr_k[4][NG];
+  __shared__ uint32_t thr_i[4][NG];
+  __shared__ int16_t held[4][8];
+  __shared__ int chosen[4][BK_WAVE];
+  const int wave = threadIdx.x / BK_WAVE, lane = threadIdx.x % BK_WAVE;
+  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
+  if (b >= a.m) return;  // whole wave
+  const int64_t t = a.list ? a.list[b] : b;
+  const int64_t left_out = a.left_out ? a.left_out[t] : -1;
+  const int o = lane / S, j = lane % S;
+  const int cap = a.sector_cap;
+  const int K = a.max_samples;
+  const GridDev& g = a.g;
+  const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+  double q0, q1, q2;
+  mv3(a.F, t0 - a.origin[0], t1 - a.origin[1], t2 - a.origin[2], q0, q1, q2);
+  const double qabs = fabs(q0) + fabs(q1) + fabs(q2);
+  uint64_t lk = KEY_INF;  // this lane's list entry
+  uint32_t li = 0xffffffffu;
+  if (lane < NG) {
+    buf_n[wave][lane] = 0;
+    thr_k[wave][lane] = KEY_INF;
+    thr_i[wave][lane] = 0xffffffffu;
+  }
+  wave_sync();
+  // Every buffer into the lists, S entries of each sector per step from the
+  // top of its buffer; then the thresholds (the cap-th key of each sector).
+  auto flush = [&]() {
+    for (;;) {
+      const uint32_t nb = buf_n[wave][o];
+      if (!__any(nb > 0)) break;
+      const uint32_t take = nb < (uint32_t)S ? nb : (uint32_t)S;
+      uint64_t k = KEY_INF;
+      uint32_t i = 0xffffffffu;
+      if ((uint32_t)j < take) {
+        k = buf_k[wave][o][nb - take + j];
+        i = buf_i[wave][o][nb - take + j];
+      }
+      wave_sync();
+      if (j == 0) buf_n[wave][o] = nb - take;
+      group_sort_keys<S>(k, i, j);
+      group_merge_keys<S>(lk, li, k, i, j);
+      wave_sync();
+    }
+    if (j == cap - 1) {
+      thr_k[wave][o] = lk;
+      thr_i[wave][o] = li;
+    }
+    wave_sync();
+  };
+  bool open = g.start != nullptr && K > 0;
+  if (open) {
+    const int c0 = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
+    const int c1 = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
+    const int c2 = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+    const double h2 = g.h * g.h;
+    const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
+    const double slack = 1.0 - 1e-9;
+    const uint32_t n_off = a.tab.end[a.tab.n - 1];
+    for (uint32_t o0 = 0; o0 < n_off; o0 += BK_WAVE) {
+      // The largest cap-th key over the sectors, +inf while one holds fewer.
+      uint64_t tmax = lane < a.n_sectors ? thr_k[wave][lane] : 0ull;
+#pragma unroll
+      for (int x = 1; x < 8; x <<= 1) {
+        const uint64_t other = shfl_xor_u64(tmax, x);
+        tmax = other > tmax ? other : tmax;
+      }
+      tmax = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(tmax >> 32), 0) << 32) |
+             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)tmax, 0);
+      const char4 f0 = a.tab.offsets[o0];
+      const int m0 = max(abs(f0.x) - 1, 0) * max(abs(f0.x) - 1, 0) + max(abs(f0.y) - 1, 0) * max(abs(f0.y) - 1, 0) +
+                     max(abs(f0.z) - 1, 0) * max(abs(f0.z) - 1, 0);
+      const double bound = fmin(h2 * (double)m0, beyond) * slack;
+      if (bound > a.radius2 || (tmax != KEY_INF && bound > __longlong_as_double((long long)tmax))) {
+        open = false;
+        break;
+      }
+      // Lane l: cell o0 + l, whose entries are [e, e + cnt) of the CSR.
+      uint32_t e = 0, cnt = 0;
+      const uint32_t oc = o0 + lane;
+      if (oc < n_off) {
+        const char4 f = a.tab.offsets[oc];
+        const int x = c0 + f.x, y = c1 + f.y, z = c2 + f.z;
+        if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2]) {
+          const int id = x + g.dims[0] * (y + g.dims[1] * z);
+          e = g.start[id];
+          cnt = g.start[id + 1] - e;
+        }
+      }
+      const uint32_t incl = group_incl_scan<BK_WAVE>(cnt, lane);
+      const uint32_t excl = incl - cnt;
+      const uint32_t total = __shfl(incl, BK_WAVE - 1, BK_WAVE);
+      for (uint32_t base = 0; base < total; base += BK_WAVE) {
+        const uint32_t f = base + lane;
+        int pos = 0;
+#pragma unroll
+        for (int step = BK_WAVE / 2; step > 0; step >>= 1) {
+          const uint32_t probe = __shfl(incl, pos + step - 1, BK_WAVE);
+          pos += probe <= f ? step : 0;
+        }
+        const uint32_t entry = __shfl(e, pos, BK_WAVE) + (f - __shfl(excl, pos, BK_WAVE));
+        if (f < total) {
+          const double4 E = g.entries[entry];
+          const double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
+          const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
+          const uint32_t ii = (uint32_t)__double_as_longlong(E.w);
+          const uint64_t kk = dbits(d2);
+          bool keep = d2 <= a.radius2 && (int64_t)ii != left_out && (tmax == KEY_INF || kk <= tmax);
+          if (keep) {
+            int sec = a.sec_fast ? sector_fast(a, u0, u1, u2, fabs(E.x) + fabs(E.y) + fabs(E.z) + qabs) : -1;
+            if (sec < 0) sec = sector_at(a, t0, t1, t2, ii);
+            keep = key_less(kk, ii, thr_k[wave][sec], thr_i[wave][sec]);
+            if (keep) {
+              const uint32_t slot = atomicAdd(&buf_n[wave][sec], 1u);
+              buf_k[wave][sec][slot] = kk;
+              buf_i[wave][sec][slot] = ii;
+            }
+          }
+        }
+        wave_sync();
+        const bool full = lane < NG && buf_n[wave][lane] >= (uint32_t)S;
+        if (__any(full)) flush();
+      }
+    }
+    flush();
+  }
+  if (open) {
+    if (lane == 0) spill[atomicAdd(spill_count, 1)] = (int32_t)t;
+    return;
+  }
+  // The selection: the sectors' first cap keys, sorted, the first K of them.
+  uint64_t k = j < cap ? lk : KEY_INF;
+  uint32_t i = j < cap ? li : 0xffffffffu;
+  uint32_t sec = (uint32_t)o;
+  // Ascending sort of (k, i) across the wave with the sector alongside.
+#pragma unroll
+  for (int size = 2; size <= BK_WAVE; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const uint64_t ok = shfl_xor_u64(k, stride);
+      const uint32_t oi = __shfl_xor(i, stride, BK_WAVE);
+      const uint32_t os = __shfl_xor(sec, stride, BK_WAVE);
+      const bool want_small = ((lane & stride) == 0) == ((lane & size) == 0);
+      const bool take = key_less(ok, oi, k, i) == want_small;
+      k = take ? ok : k;
+      i = take ? oi : i;
+      sec = take ? os : sec;
+    }
+  }
+  const bool have = lane < K && k != KEY_INF;
+  const int n = __popcll(__ballot(have));
+  if (lane < 8) held[wave][lane] = 0;
+  wave_sync();
+  for (int s = 0; s < a.n_sectors; ++s) {
+    const int cs = __popcll(__ballot(have && sec == (uint32_t)s));
+    if (lane == 0) held[wave][s] = (int16_t)cs;
+  }
+  int* out = a.idx + t * a.stride;
+  if (have) {
+    out[lane] = (int)i;
+    chosen[wave][lane] = (int)i;
+  }
+  wave_sync();
+  if (lane == 0) {
+    a.count[t] = n;
+    a.status[t] = selection_status(a, n, held[wave], chosen[wave]);
   }
 }
 
@@ -1671,9 +1882,46 @@ static int launch_waves(hipStream_t s, const SearchArgs& a, int64_t m, TileBuffe
   return launch_spill(s, a, m, spill, count);
 }
 
+// Lanes per sector of k_sectors for this search, or 0 when it does not apply
+// (plane sectors, a hole cap, or more than 64 list slots).
+static int sector_lanes(const SearchArgs& a) {
+  if (!a.balances || a.plane || a.max_per_hole >= 0) return 0;
+  if (a.n_sectors == 8 && a.sector_cap <= 8) return 8;
+  if (a.n_sectors == 4 && a.sector_cap <= 16) return 16;
+  return 0;
+}
+
+// Capped searches by k_sectors, the targets it leaves open by k_search.
+static int launch_sectors(hipStream_t s, const SearchArgs& a, int64_t m, int lanes, TileBuffers& tb) {
+  if (m > 2147483647LL) return -1;
+  if (grow(tb.ids, tb.cap_ids, 3 * m + 1)) return -1;
+  int32_t* spill = tb.ids + 2 * m;
+  int32_t* count = tb.ids + 3 * m;
+  k_zero32<<<1, 64, 0, s>>>(count, 1);
+  SearchArgs c = a;
+  c.m = m;
+  const dim3 grid((unsigned)((m + 3) / 4));
+  if (lanes == 8)
+    k_sectors<8><<<grid, dim3(256), 0, s>>>(c, spill, count);
+  else
+    k_sectors<16><<<grid, dim3(256), 0, s>>>(c, spill, count);
+  HIPCHK(hipGetLastError());
+  tb.spill_count = count;
+  SearchArgs d = a;
+  d.list = spill;
+  d.m = m;
+  d.m_dev = count;
+  k_search<<<dim3((unsigned)std::min<int64_t>(m, 4096)), dim3(BK_WAVE), 0, s>>>(d);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mode, TileBuffers& tb) {
   if (n_targets <= 0) return 0;
   tb.spill_count = nullptr;
+  if (const int lanes = sector_lanes(a)) {
+    if (!(tb.fixed && 3 * n_targets + 1 > tb.cap_ids)) return launch_sectors(s, a, n_targets, lanes, tb);
+  }
   if (knn_path(a)) {
     if (mode > 0 && a.list == nullptr && !tb.fixed) return launch_tiles(s, a, n_targets, mode, tb);
     if (mode > 0 && tb.fixed) mode = 0;
