@@ -278,5 +278,6 @@ STAT_GRAPH_LAUNCHES = 1
 STAT_REFINED = 2
 STAT_REFINE_SWEEPS = 3
 STAT_REFINE_REDO = 4
+STAT_SEARCH_RETRY = 5
 
 STAGES = ("grid", "search", "bucket", "solve", "deferred", "total")

@@ -2205,13 +2205,15 @@ int bk_get_stat(bk_ctx* c, int32_t which, int64_t* value) {
       *value = which == BK_STAT_REFINED ? h[6] : (which == BK_STAT_REFINE_SWEEPS ? h[7] : h[5]);
       return 0;
     }
-    case BK_STAT_SEARCH_SPILL: {
+    case BK_STAT_SEARCH_SPILL:
+    case BK_STAT_SEARCH_RETRY: {
       // The count stays on the device during the pass; read it now.
       *value = 0;
-      if (!c->tiles.spill_count) return 0;
+      const int32_t* src = which == BK_STAT_SEARCH_SPILL ? c->tiles.spill_count : c->tiles.retry_count;
+      if (!src) return 0;
       HIPOK(c, hipSetDevice(c->device));
       int32_t h = 0;
-      HIPOK(c, hipMemcpyAsync(&h, c->tiles.spill_count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPOK(c, hipMemcpyAsync(&h, src, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
       HIPOK(c, hipStreamSynchronize(c->stream));
       *value = h;
       return 0;

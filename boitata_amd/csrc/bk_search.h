@@ -12,12 +12,14 @@
 
 #include "bk_device.h"
 
-#define BK_POOL 512       // candidates held per target between releases
+#define BK_POOL 512       // candidates held per target between releases (LDS)
+#define BK_POOL_BIG 65536 // the same in global memory, for the targets that overflow BK_POOL
+#define BK_BIG_BLOCKS 32  // blocks (pools) of the global-memory pass
 #define BK_MAX_KNN 64     // selections without caps up to this size run one target per lane
 #define BK_REACH 12
 
-// Internal per-target status: the candidate pool overflowed; the target is
-// searched again by the windowed kernel.
+// Internal per-target status: the candidate pool overflowed (BK_POOL, then
+// BK_POOL_BIG entries pending between two releases).
 #define BK_ST_OVERFLOW 100
 
 struct GridDev {
@@ -70,6 +72,8 @@ struct SearchArgs {
   const int32_t* list;        // targets to search (null: 0..m)
   int64_t m;                  // their number, or a bound on it when m_dev is set
   const int32_t* m_dev;       // the number on the device (a spill count), or null
+  int32_t* ovf;               // targets whose LDS pool overflows are added here (may be null)
+  int32_t* ovf_count;
   int32_t* idx;               // m * stride
   int32_t* count;
   int32_t* status;
@@ -111,6 +115,8 @@ struct TileBuffers {
   int64_t cap_keys = 0, cap_ids = 0;
   const int32_t* spill_count = nullptr;  // device count of the targets the last wave or tile search left to k_knn
   bool fixed = false;                    // caller-owned ids of fixed capacity: the searches allocate nothing
+  void* big = nullptr;                   // BK_BIG_BLOCKS global-memory selector pools (allocated on first use)
+  const int32_t* retry_count = nullptr;  // device count of the targets the last capped search searched again
 };
 void free_tiles(TileBuffers& tb);
 // SearchTree::balanced order of each target's selection (search.rs:909-936),

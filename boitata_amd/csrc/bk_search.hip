@@ -80,11 +80,17 @@ __device__ __forceinline__ int clamp_cell(double v, int dim) {
   return c < 0 ? 0 : (c > dim - 1 ? dim - 1 : c);
 }
 
-struct SearchLds {
-  uint64_t pk[BK_POOL];
-  uint32_t pi[BK_POOL];
-  int16_t ps[BK_POOL];  // sector (< 360)
-  int ph[BK_POOL];
+// The selector's state for one target: a pool of POOL candidates held
+// between releases, sorted by (key, idx), and the selection so far. In LDS
+// with BK_POOL entries; in global memory with BK_POOL_BIG for the targets
+// whose pool overflows the first.
+template <int POOL>
+struct SearchPool {
+  static constexpr int kPool = POOL;
+  uint64_t pk[POOL];
+  uint32_t pi[POOL];
+  int16_t ps[POOL];  // sector (< 360)
+  int ph[POOL];
   uint64_t bk[BK_WAVE];
   uint32_t bi[BK_WAVE];
   int cs[BK_WAVE];
@@ -96,6 +102,8 @@ struct SearchLds {
   uint64_t thr_sec[8];
   int scalar[4];
 };
+using SearchLds = SearchPool<BK_POOL>;
+using SearchBig = SearchPool<BK_POOL_BIG>;
 
 struct State {
   int P;          // pooled candidates
@@ -108,7 +116,8 @@ struct State {
 };
 
 // Number of pool entries with (key, idx) < (k, i).
-__device__ __forceinline__ int pool_lower(const SearchLds& L, int P, uint64_t k, uint32_t i) {
+template <class Pool>
+__device__ __forceinline__ int pool_lower(const Pool& L, int P, uint64_t k, uint32_t i) {
   int lo = 0, hi = P;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -120,7 +129,8 @@ __device__ __forceinline__ int pool_lower(const SearchLds& L, int P, uint64_t k,
   return lo;
 }
 
-__device__ __forceinline__ int batch_lower(const SearchLds& L, int nb, uint64_t k, uint32_t i) {
+template <class Pool>
+__device__ __forceinline__ int batch_lower(const Pool& L, int nb, uint64_t k, uint32_t i) {
   int lo = 0, hi = nb;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -136,8 +146,8 @@ __device__ __forceinline__ int batch_lower(const SearchLds& L, int nb, uint64_t 
 // lane from the bottom: an entry only moves down, so a chunk's destinations
 // never reach entries not yet read. keep(in, j, sector) is called once per
 // chunk by the whole wave (it may use wave-wide operations).
-template <class Keep>
-__device__ __forceinline__ int pool_compact(SearchLds& L, int P, int lane, Keep keep) {
+template <class Pool, class Keep>
+__device__ __forceinline__ int pool_compact(Pool& L, int P, int lane, Keep keep) {
   int out = 0;
   const uint64_t lt = (1ull << lane) - 1ull;
   for (int base = 0; base < P; base += BK_WAVE) {
@@ -190,9 +200,10 @@ __device__ __forceinline__ int sector_at(const SearchArgs& a, double t0, double 
 // rather than sorted across the wave.
 constexpr int kSmallBatch = 16;
 
+template <class Pool>
 struct Ctx {
   const SearchArgs* a;
-  SearchLds* L;
+  Pool* L;
   State st;
   int lane;
   double q0, q1, q2;
@@ -229,7 +240,7 @@ struct Ctx {
       for (int o = 0; o < 8; ++o) running[o] = 0;
       int sure = 0;
       const uint64_t lt = (1ull << lane) - 1ull;
-      SearchLds* Lc = L;
+      Pool* Lc = L;
       const int cap = a->sector_cap;
       // Thresholds: an octant holding all it may still take makes larger keys
       // of that octant useless; its threshold is the key of its last kept
@@ -283,7 +294,7 @@ struct Ctx {
         const uint32_t iq = (uint32_t)__builtin_amdgcn_readlane((int)idx, src);
         rank += key_less(((uint64_t)hi << 32) | lo, iq, key, idx) ? 1 : 0;
       }
-      if (st.P + nb > BK_POOL) {
+      if (st.P + nb > Pool::kPool) {
         st.overflow = 1;
         return;
       }
@@ -301,7 +312,7 @@ struct Ctx {
         hl = -1;
       }
       wave_sort(key, idx, sec, hl, lane);
-      if (st.P + nb > BK_POOL) {
+      if (st.P + nb > Pool::kPool) {
         st.overflow = 1;
         return;
       }
@@ -532,10 +543,12 @@ __device__ int selection_status(const SearchArgs& a, int n, const int16_t* per_s
   return BK_OK;
 }
 
-// The selection of target t by one wavefront (L: its LDS).
-__device__ void search_target(const SearchArgs& a, SearchLds& L, int64_t t) {
+// The selection of target t by one wavefront (L: its selector state). A
+// target whose pool overflows is added to a.ovf when that list is given.
+template <class Pool>
+__device__ void search_target(const SearchArgs& a, Pool& L, int64_t t) {
   const int lane = threadIdx.x;
-  Ctx c;
+  Ctx<Pool> c;
   c.a = &a;
   c.L = &L;
   c.lane = lane;
@@ -632,6 +645,7 @@ __device__ void search_target(const SearchArgs& a, SearchLds& L, int64_t t) {
     const int status = c.st.overflow ? BK_ST_OVERFLOW : selection_status(a, n, L.per_sector, L.chosen);
     a.count[t] = n;
     a.status[t] = status;
+    if (c.st.overflow && a.ovf) a.ovf[atomicAdd(a.ovf_count, 1)] = (int32_t)t;
   }
 }
 
@@ -642,6 +656,17 @@ __global__ void __launch_bounds__(BK_WAVE) k_search(const SearchArgs a) {
   const int64_t end = a.m_dev ? (int64_t)*a.m_dev : (int64_t)blockIdx.x + 1;
   for (int64_t b = blockIdx.x; b < end; b += gridDim.x) {
     search_target(a, L, a.list ? a.list[b] : b);
+    wave_sync();
+  }
+}
+
+// The targets whose LDS pool overflowed (a.list, length *a.m_dev), again with
+// a pool of BK_POOL_BIG entries in global memory: block b uses ws[b].
+__global__ void __launch_bounds__(BK_WAVE) k_search_big(const SearchArgs a, SearchBig* ws) {
+  SearchBig& L = ws[blockIdx.x];
+  const int64_t end = (int64_t)*a.m_dev;
+  for (int64_t b = blockIdx.x; b < end; b += gridDim.x) {
+    search_target(a, L, (int64_t)a.list[b]);
     wave_sync();
   }
 }
@@ -1859,6 +1884,7 @@ void free_tiles(TileBuffers& tb) {
   (void)hipFree(tb.keys);
   (void)hipFree(tb.ids);
   (void)hipFree(tb.tmp);
+  (void)hipFree(tb.big);
   tb = TileBuffers();
 }
 
@@ -1891,13 +1917,35 @@ static int sector_lanes(const SearchArgs& a) {
   return 0;
 }
 
-// Capped searches by k_sectors, the targets it leaves open by k_search.
+// The targets of a capped search whose LDS pool overflowed (ovf, length on
+// the device), again with global-memory pools (allocated on first use).
+static int launch_big(hipStream_t s, const SearchArgs& a, int64_t m, int32_t* ovf, const int32_t* ovf_count,
+                      TileBuffers& tb) {
+  if (!tb.big) HIPCHK(hipMalloc(&tb.big, (size_t)BK_BIG_BLOCKS * sizeof(SearchBig)));
+  SearchArgs d = a;
+  d.list = ovf;
+  d.m = m;
+  d.m_dev = ovf_count;
+  d.ovf = nullptr;
+  d.ovf_count = nullptr;
+  k_search_big<<<dim3(BK_BIG_BLOCKS), dim3(BK_WAVE), 0, s>>>(d, (SearchBig*)tb.big);
+  HIPCHK(hipGetLastError());
+  tb.retry_count = ovf_count;
+  return 0;
+}
+
+// Capped searches by k_sectors, the targets it leaves open by k_search, and
+// those whose pool overflows there by k_search_big (not with fixed buffers).
 static int launch_sectors(hipStream_t s, const SearchArgs& a, int64_t m, int lanes, TileBuffers& tb) {
   if (m > 2147483647LL) return -1;
-  if (grow(tb.ids, tb.cap_ids, 3 * m + 1)) return -1;
+  const bool big = !tb.fixed;
+  if (grow(tb.ids, tb.cap_ids, 3 * m + (big ? 2 : 1))) return -1;
   int32_t* spill = tb.ids + 2 * m;
   int32_t* count = tb.ids + 3 * m;
+  int32_t* ovf = big ? tb.ids : nullptr;
+  int32_t* ovf_count = big ? tb.ids + 3 * m + 1 : nullptr;
   k_zero32<<<1, 64, 0, s>>>(count, 1);
+  if (big) k_zero32<<<1, 64, 0, s>>>(ovf_count, 1);
   SearchArgs c = a;
   c.m = m;
   const dim3 grid((unsigned)((m + 3) / 4));
@@ -1911,14 +1959,17 @@ static int launch_sectors(hipStream_t s, const SearchArgs& a, int64_t m, int lan
   d.list = spill;
   d.m = m;
   d.m_dev = count;
+  d.ovf = ovf;
+  d.ovf_count = ovf_count;
   k_search<<<dim3((unsigned)std::min<int64_t>(m, 4096)), dim3(BK_WAVE), 0, s>>>(d);
   HIPCHK(hipGetLastError());
-  return 0;
+  return big ? launch_big(s, a, m, ovf, ovf_count, tb) : 0;
 }
 
 int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mode, TileBuffers& tb) {
   if (n_targets <= 0) return 0;
   tb.spill_count = nullptr;
+  tb.retry_count = nullptr;
   if (const int lanes = sector_lanes(a)) {
     if (!(tb.fixed && 3 * n_targets + 1 > tb.cap_ids)) return launch_sectors(s, a, n_targets, lanes, tb);
   }
@@ -1934,9 +1985,22 @@ int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mod
     HIPCHK(hipGetLastError());
     return 0;
   }
+  // Capped searches: a block per target; targets whose pool overflows are
+  // listed for k_search_big (one launch of at most 2^31 - 1 targets).
+  const bool big = !tb.fixed && n_targets <= 2147483647LL;
+  int32_t* ovf = nullptr;
+  int32_t* ovf_count = nullptr;
+  if (big) {
+    if (grow(tb.ids, tb.cap_ids, n_targets + 1)) return -1;
+    ovf = tb.ids;
+    ovf_count = tb.ids + n_targets;
+    k_zero32<<<1, 64, 0, s>>>(ovf_count, 1);
+  }
   for (int64_t b = 0; b < n_targets; b += 2147483647LL) {
     int64_t nb = std::min<int64_t>(n_targets - b, 2147483647LL);
     SearchArgs c = a;
+    c.ovf = ovf;
+    c.ovf_count = ovf_count;
     if (b) {
       if (c.list)
         c.list += b;
@@ -1946,7 +2010,7 @@ int launch_search(hipStream_t s, const SearchArgs& a, int64_t n_targets, int mod
     k_search<<<dim3((unsigned)nb), dim3(BK_WAVE), 0, s>>>(c);
   }
   HIPCHK(hipGetLastError());
-  return 0;
+  return big ? launch_big(s, a, n_targets, ovf, ovf_count, tb) : 0;
 }
 
 }  // namespace bk

@@ -40,7 +40,8 @@ enum {
   BK_SINGULAR = 2,           /* pivot <= 1e-12 * sill (kernel.rs:58-65) or LU breakdown */
   BK_INSUFFICIENT_DATA = 3,  /* no samples */
   BK_INVALID_PARAMETERS = 4,
-  BK_UNSUPPORTED = 5         /* a capped search whose pending candidates outgrew the device pool */
+  BK_UNSUPPORTED = 5         /* a capped search with more than 65,536 candidates pending at once, or with
+                                bk_krige_noalloc more than 512 (the device pools) */
 };
 
 /* Return codes of the API calls. */
@@ -399,7 +400,10 @@ int bk_autotune(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, in
 int bk_set_graphs(bk_ctx* ctx, int32_t enable);
 
 /* Counters (no counterpart in the reference): BK_STAT_SEARCH_SPILL, targets
- * the last wave or tiled search handed back to the lane-per-target search;
+ * the last wave, tiled or sector search handed back to the walk without a
+ * shell table; BK_STAT_SEARCH_RETRY, targets of the last capped search whose
+ * candidate pool outgrew its 512 LDS entries and which were searched again
+ * with a pool of 65,536 entries in device memory (beyond that, BK_UNSUPPORTED);
  * BK_STAT_GRAPH_LAUNCHES, bk_krige calls run as a graph so far. Of the last
  * kriging pass: BK_STAT_REFINED, systems solved by FP32 elimination and FP64
  * refinement; BK_STAT_REFINE_SWEEPS, their refinement sweeps in total;
@@ -410,6 +414,7 @@ int bk_set_graphs(bk_ctx* ctx, int32_t enable);
 #define BK_STAT_REFINED 2
 #define BK_STAT_REFINE_SWEEPS 3
 #define BK_STAT_REFINE_REDO 4
+#define BK_STAT_SEARCH_RETRY 5
 int bk_get_stat(bk_ctx* ctx, int32_t which, int64_t* value);
 
 /* Stage timing with HIP events recorded on the context's stream (no
