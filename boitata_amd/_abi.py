@@ -244,6 +244,7 @@ SIGNATURES = [
      [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, C.POINTER(Block), I32, I64, P, P, P, P]),
     ("bk_krige_weights", C.c_int,
      [P, C.POINTER(Variogram), C.POINTER(Search), I32, D, P, I32, I64, P, P, P, P, P, P, P]),
+    ("bk_sgs", C.c_int, [P, C.POINTER(Variogram), C.POINTER(Search), I32, I64, P, P, P, P, P, P, P]),
     ("bk_loo_prepare", C.c_int, [P, C.POINTER(Search), C.POINTER(Variogram)]),
     ("bk_loo_loss_grad", C.c_int, [P, C.POINTER(Variogram), I32, P, P, P]),
     ("bk_krige_diagnostics", C.c_int,
@@ -279,5 +280,6 @@ STAT_REFINED = 2
 STAT_REFINE_SWEEPS = 3
 STAT_REFINE_REDO = 4
 STAT_SEARCH_RETRY = 5
+STAT_SGS_PASSES = 6
 
 STAGES = ("grid", "search", "bucket", "solve", "deferred", "total")

@@ -14,6 +14,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/boitata_gpu.h"
@@ -156,6 +157,11 @@ struct bk_ctx {
   std::vector<int64_t> mik_bounds;
   double mik_lower = 0, mik_upper = 0;
   Buf mik_gv, mik_gs, mik_par, mik_cls, mik_tab, mik_w, mik_raw, mik_st, mik_cdf, mik_out;
+  // Sequential simulation (bk_sgs): a context of its own over the data and
+  // the simulated nodes, and the pass's arrays.
+  bk_ctx* sgs = nullptr;
+  int sgs_passes = 0;  // draw passes of the last bk_sgs
+  Buf sgs_x, sgs_y, sgs_z, sgs_e, sgs_h, sgs_v, sgs_s, sgs_done, sgs_limit, sgs_datum, sgs_pos, sgs_ctr, sgs_in, sgs_w;
   // Stage timing (bk_profile): events on the context's stream.
   bool prof = false;
   hipEvent_t ev[7] = {};
@@ -467,6 +473,60 @@ __global__ void k_map_overflow(int64_t m, int32_t* status) {
 
 unsigned blocks_of(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>((n + t - 1) / t, 1); }
 
+// The sector of a sample from the signs of its projected offset (octants and
+// planar quadrants): axes F^-1 must map each projected axis to one sector axis
+// with a positive or negative scale (F = D R, axes the rows of R permuted with
+// a sign). Depends on a.F, a.axes, a.balances, a.plane and a.planar.
+void sector_map(SearchArgs& a) {
+  a.sec_fast = 0;
+  if (a.balances && !a.plane) {
+    const double* f = a.F;
+    const double det = f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) +
+                       f[2] * (f[3] * f[7] - f[4] * f[6]);
+    if (std::isfinite(det) && det != 0.0) {
+      M3 inv{{(f[4] * f[8] - f[5] * f[7]) / det, (f[2] * f[7] - f[1] * f[8]) / det, (f[1] * f[5] - f[2] * f[4]) / det,
+              (f[5] * f[6] - f[3] * f[8]) / det, (f[0] * f[8] - f[2] * f[6]) / det, (f[2] * f[3] - f[0] * f[5]) / det,
+              (f[3] * f[7] - f[4] * f[6]) / det, (f[1] * f[6] - f[0] * f[7]) / det, (f[0] * f[4] - f[1] * f[3]) / det}};
+      M3 axes;
+      std::memcpy(axes.m, a.axes, sizeof(axes.m));
+      const M3 map = mm(axes, inv);
+      int perm[3], sgn[3];
+      bool ok = true;
+      for (int k = 0; k < 3 && ok; ++k) {
+        int p = 0;
+        for (int j = 1; j < 3; ++j)
+          if (std::fabs(map.m[3 * k + j]) > std::fabs(map.m[3 * k + p])) p = j;
+        const double big = std::fabs(map.m[3 * k + p]);
+        for (int j = 0; j < 3; ++j)
+          if (j != p && !(std::fabs(map.m[3 * k + j]) <= 1e-9 * big)) ok = false;
+        perm[k] = p;
+        sgn[k] = map.m[3 * k + p] > 0.0 ? 1 : -1;
+      }
+      ok = ok && perm[0] != perm[1] && perm[1] != perm[2] && perm[0] != perm[2];
+      if (ok) {
+        double inv_norm = 0.0;
+        for (int i = 0; i < 3; ++i)
+          inv_norm = std::max(inv_norm, std::fabs(inv.m[3 * i]) + std::fabs(inv.m[3 * i + 1]) +
+                                            std::fabs(inv.m[3 * i + 2]));
+        for (int p = 0; p < 3; ++p)
+          a.sec_tol[p] = 1e-9 * (std::fabs(f[3 * p]) + std::fabs(f[3 * p + 1]) + std::fabs(f[3 * p + 2])) * inv_norm;
+        uint32_t code = 0;
+        for (int pat = 0; pat < 8; ++pat) {
+          int bit[3];
+          for (int k = 0; k < 3; ++k) {
+            const int pos = (pat >> perm[k]) & 1;  // u_perm[k] >= 0
+            bit[k] = sgn[k] > 0 ? pos : 1 - pos;
+          }
+          const int sec = a.planar ? (bit[0] << 1) | bit[1] : (bit[0] << 2) | (bit[1] << 1) | bit[2];
+          code |= (uint32_t)sec << (3 * pat);
+        }
+        a.sec_code = code;
+        a.sec_fast = 1;
+      }
+    }
+  }
+}
+
 // Search parameters for the kernels, and the grid for the search's frame.
 int prepare_search(bk_ctx* c, const bk_search* s, const bk_variogram* vg, SearchArgs& a, int64_t& stride) {
   if (const char* m = check_search(s)) return fail(c, BK_E_INVALID, m);
@@ -518,54 +578,7 @@ int prepare_search(bk_ctx* c, const bk_search* s, const bk_variogram* vg, Search
   a.min_sectors = (int)s->min_sectors;
   a.max_empty_sectors = (int)s->max_empty_sectors;
   a.min_holes = (int)std::min<int64_t>(s->min_holes, 1 << 30);
-  // The sector of a sample from the signs of its projected offset: axes F^-1
-  // must map each projected axis to one sector axis with a positive or
-  // negative scale (F = D R and axes the rows of R permuted with a sign).
-  a.sec_fast = 0;
-  if (a.balances && !a.plane) {
-    const double* f = F.m;
-    const double det = f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) +
-                       f[2] * (f[3] * f[7] - f[4] * f[6]);
-    if (std::isfinite(det) && det != 0.0) {
-      M3 inv{{(f[4] * f[8] - f[5] * f[7]) / det, (f[2] * f[7] - f[1] * f[8]) / det, (f[1] * f[5] - f[2] * f[4]) / det,
-              (f[5] * f[6] - f[3] * f[8]) / det, (f[0] * f[8] - f[2] * f[6]) / det, (f[2] * f[3] - f[0] * f[5]) / det,
-              (f[3] * f[7] - f[4] * f[6]) / det, (f[1] * f[6] - f[0] * f[7]) / det, (f[0] * f[4] - f[1] * f[3]) / det}};
-      const M3 map = mm(axes, inv);
-      int perm[3], sgn[3];
-      bool ok = true;
-      for (int k = 0; k < 3 && ok; ++k) {
-        int p = 0;
-        for (int j = 1; j < 3; ++j)
-          if (std::fabs(map.m[3 * k + j]) > std::fabs(map.m[3 * k + p])) p = j;
-        const double big = std::fabs(map.m[3 * k + p]);
-        for (int j = 0; j < 3; ++j)
-          if (j != p && !(std::fabs(map.m[3 * k + j]) <= 1e-9 * big)) ok = false;
-        perm[k] = p;
-        sgn[k] = map.m[3 * k + p] > 0.0 ? 1 : -1;
-      }
-      ok = ok && perm[0] != perm[1] && perm[1] != perm[2] && perm[0] != perm[2];
-      if (ok) {
-        double inv_norm = 0.0;
-        for (int i = 0; i < 3; ++i)
-          inv_norm = std::max(inv_norm, std::fabs(inv.m[3 * i]) + std::fabs(inv.m[3 * i + 1]) +
-                                            std::fabs(inv.m[3 * i + 2]));
-        for (int p = 0; p < 3; ++p)
-          a.sec_tol[p] = 1e-9 * (std::fabs(f[3 * p]) + std::fabs(f[3 * p + 1]) + std::fabs(f[3 * p + 2])) * inv_norm;
-        uint32_t code = 0;
-        for (int pat = 0; pat < 8; ++pat) {
-          int bit[3];
-          for (int k = 0; k < 3; ++k) {
-            const int pos = (pat >> perm[k]) & 1;  // u_perm[k] >= 0
-            bit[k] = sgn[k] > 0 ? pos : 1 - pos;
-          }
-          const int sec = a.planar ? (bit[0] << 1) | bit[1] : (bit[0] << 2) | (bit[1] << 1) | bit[2];
-          code |= (uint32_t)sec << (3 * pat);
-        }
-        a.sec_code = code;
-        a.sec_fast = 1;
-      }
-    }
-  }
+  sector_map(a);
   a.sx = (const double*)c->x.p;
   a.sy = (const double*)c->y.p;
   a.sz = (const double*)c->z.p;
@@ -961,8 +974,11 @@ void bk_destroy(bk_ctx* c) {
                  &c->sstatus, &c->lists, &c->counts, &c->status, &c->value, &c->variance, &c->lagrange, &c->support,
                  &c->n_used, &c->guard, &c->redo, &c->offs, &c->which, &c->iters, &c->work, &c->loo_idx, &c->loo_count,
                  &c->loo_status, &c->loo_lists, &c->mik_gv, &c->mik_gs, &c->mik_par, &c->mik_cls, &c->mik_tab,
-                 &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out, &c->lu_big})
+                 &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out, &c->lu_big, &c->sgs_x, &c->sgs_y,
+                 &c->sgs_z, &c->sgs_e, &c->sgs_h, &c->sgs_v, &c->sgs_s, &c->sgs_done, &c->sgs_limit, &c->sgs_datum,
+                 &c->sgs_pos, &c->sgs_ctr, &c->sgs_in, &c->sgs_w})
     release(*b);
+  if (c->sgs) bk_destroy(c->sgs);
   bk::free_grid(c->gb);
   bk::free_tiles(c->tiles);
   for (auto& g : c->graph_cache)
@@ -2192,6 +2208,9 @@ int bk_get_stat(bk_ctx* c, int32_t which, int64_t* value) {
     case BK_STAT_GRAPH_LAUNCHES:
       *value = c->graph_launches;
       return 0;
+    case BK_STAT_SGS_PASSES:
+      *value = c->sgs_passes;
+      return 0;
     case BK_STAT_REFINED:
     case BK_STAT_REFINE_SWEEPS:
     case BK_STAT_REFINE_REDO: {
@@ -2348,3 +2367,342 @@ int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t k
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Sequential Gaussian simulation in normal scores (crates/simulation/src/
+// sgs.rs:600-750 with one search pass and no domains, trend, high-grade caps,
+// local anisotropy or secondary variable; the plan of each node, constant
+// and node weights, as crates/simulation/src/shared.rs:104-143 keeps it).
+// Every node's position is known before any is drawn, so node k of the
+// visiting order that is not on a datum searches the samples below index
+// n + k of "the data, then the nodes in that order": exactly the data and the
+// nodes the sequential loop has added by then. All nodes are searched and
+// their simple-kriging (mean 0) weights solved at once; only the draws are
+// sequential. Node k's score, sum_j w_j s_j + sqrt(max(var, 0)) z_k over its
+// neighbours' scores (z_k for a node without neighbours), is evaluated in
+// passes: a node is drawn once every simulated neighbour was drawn in an
+// earlier pass (70-80 passes for 10^4 to 10^6 nodes of a random path).
+namespace {
+
+__global__ void k_sgs_limits(int64_t m2, int64_t n, int32_t* limit) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < m2) limit[k] = (int32_t)(n + k);
+}
+
+// One pass of the draws over the nodes in visiting order (k < m2). s: the
+// scores of the data (first n) and of the nodes drawn so far; done[k]: the
+// pass that drew node k, 0 while it waits.
+__global__ void k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx, const int32_t* count,
+                           const int32_t* status, const double* w, const double* variance, const double* z, double* s,
+                           int32_t* done, int32_t pass, unsigned long long* drawn) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= m2 || done[k] != 0) return;
+  const int st = status[k];
+  const int cnt = st == BK_OK ? count[k] : 0;
+  const int32_t* id = nidx + k * (int64_t)stride;
+  // A neighbour drawn in this pass may not be visible yet: wait for the next.
+  for (int j = 0; j < cnt; ++j) {
+    const int64_t q = (int64_t)id[j] - n;
+    if (q >= 0) {
+      const int32_t d = done[q];
+      if (d == 0 || d >= pass) return;
+    }
+  }
+  double v;
+  if (cnt > 0) {
+    const double* wk = w + k * (int64_t)stride;
+    double mean = 0.0;
+    for (int j = 0; j < cnt; ++j) mean += wk[j] * s[id[j]];
+    v = mean + sqrt(fmax(variance[k], 0.0)) * z[k];
+  } else if (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA) {
+    v = z[k];  // no neighbours: the marginal (sgs.rs:688-691, 725-728)
+  } else {
+    v = __longlong_as_double(0x7ff8000000000000LL);
+  }
+  s[n + k] = v;
+  done[k] = pass;
+  atomicAdd(drawn, 1ull);
+}
+
+// Scores and sources in grid order: a node on a datum takes the datum's score
+// (source = its index); others their draw (source -1 kriged, -2 marginal,
+// -3 the kriging failed: score NaN).
+__global__ void k_sgs_out(int64_t m, int64_t n, const int64_t* datum, const int32_t* pos, const double* s,
+                          const int32_t* status, const int32_t* count, double* score, int32_t* source) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t d = datum[i];
+  if (d >= 0) {
+    score[i] = s[d];
+    source[i] = (int32_t)d;
+    return;
+  }
+  const int32_t k = pos[i];
+  const int st = status[k];
+  score[i] = s[n + k];
+  source[i] = st == BK_OK && count[k] > 0 ? -1
+              : (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA ? -2 : -3);
+}
+
+struct LocKey {
+  uint64_t b[3];
+  bool operator==(const LocKey& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
+};
+struct LocHash {
+  size_t operator()(const LocKey& k) const {
+    uint64_t h = k.b[0] * 0x9E3779B97F4A7C15ull;
+    h ^= k.b[1] + 0x7F4A7C15ull + (h << 6) + (h >> 2);
+    h ^= k.b[2] + 0x94D049BBull + (h << 6) + (h >> 2);
+    return (size_t)h;
+  }
+};
+// sgs.rs:862-865: equal for 0.0 and -0.0.
+LocKey loc_key(double x, double y, double z) {
+  const double v[3] = {x + 0.0, y + 0.0, z + 0.0};
+  LocKey k;
+  std::memcpy(k.b, v, sizeof(v));
+  return k;
+}
+
+}  // namespace
+
+int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precision, int64_t m, const double* gx,
+           const double* gy, const double* gz, const int64_t* path, const double* z, double* score, int32_t* source) {
+  if (!c) return BK_E_INVALID;
+  if (!vg || !s) return fail(c, BK_E_INVALID, "null argument");
+  if (m < 0) return fail(c, BK_E_INVALID, "m < 0");
+  if (m > 0 && (!gx || !gy || !gz || !path || !z || !score || !source)) return fail(c, BK_E_INVALID, "null argument");
+  if (precision != BK_FP64 && precision != BK_FP32_IR) return fail(c, BK_E_INVALID, "unknown precision");
+  if (const char* msg = check_variogram(vg)) return fail(c, BK_E_INVALID, msg);
+  if (const char* msg = check_search(s)) return fail(c, BK_E_INVALID, msg);
+  for (int k = 0; k < vg->n_structures; ++k)
+    if (vg->structures[k].model == BK_POWER)
+      return fail(c, BK_E_INVALID, "simple kriging of scores needs a variogram with a sill (no power structure)");
+  if (s->calibration) return fail(c, BK_E_INVALID, "the simulation takes searches without calibration");
+  if (s->max_per_hole == 0 || (s->octant && s->max_per_octant == 0))
+    return fail(c, BK_E_INVALID, "caps of 0 are not supported by the simulation");
+  if (c->n <= 0) return fail(c, BK_E_INVALID, "no conditioning data");
+  HIPOK(c, hipSetDevice(c->device));
+  HIPOK(c, hipStreamSynchronize(c->stream));
+  // sgs.rs:881-886: the scores' variogram with a total sill of 1.
+  bk_variogram vu = *vg;
+  {
+    double total = vu.nugget;
+    for (int k = 0; k < vu.n_structures; ++k) total += vu.structures[k].sill;
+    if (total > 0.0) {
+      const double f = 1.0 / total;
+      vu.nugget *= f;
+      for (int k = 0; k < vu.n_structures; ++k) vu.structures[k].sill *= f;
+    }
+  }
+  const int64_t n = c->n;
+  // The data's and the nodes' locations and the path on the host: the datum
+  // under each node, the order in which the others join the samples.
+  std::vector<double> X(n), Y(n), Z(n), GX(m), GY(m), GZ(m);
+  std::vector<int64_t> P(m);
+  HIPOK(c, hipMemcpy(X.data(), c->x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIPOK(c, hipMemcpy(Y.data(), c->y.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIPOK(c, hipMemcpy(Z.data(), c->z.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (m > 0) {
+    HIPOK(c, hipMemcpy(GX.data(), gx, m * sizeof(double), hipMemcpyDefault));
+    HIPOK(c, hipMemcpy(GY.data(), gy, m * sizeof(double), hipMemcpyDefault));
+    HIPOK(c, hipMemcpy(GZ.data(), gz, m * sizeof(double), hipMemcpyDefault));
+    HIPOK(c, hipMemcpy(P.data(), path, m * sizeof(int64_t), hipMemcpyDefault));
+  }
+  std::vector<int32_t> pos(m, -1);
+  for (int64_t k = 0; k < m; ++k) {
+    if (P[k] < 0 || P[k] >= m || pos[P[k]] != -1) return fail(c, BK_E_INVALID, "path must be a permutation of the nodes");
+    pos[P[k]] = 0;
+  }
+  std::unordered_map<LocKey, int64_t, LocHash> first;
+  first.reserve((size_t)n * 2);
+  for (int64_t i = 0; i < n; ++i) first.emplace(loc_key(X[i], Y[i], Z[i]), i);  // keeps the first (datum_index)
+  std::vector<int64_t> datum(m, -1);
+  {
+    std::unordered_map<LocKey, int64_t, LocHash> nodes;
+    nodes.reserve((size_t)m * 2);
+    for (int64_t i = 0; i < m; ++i) {
+      const LocKey key = loc_key(GX[i], GY[i], GZ[i]);
+      if (!nodes.emplace(key, i).second) return fail(c, BK_E_INVALID, "grid nodes must be distinct");
+      auto it = first.find(key);
+      if (it != first.end()) datum[i] = it->second;
+    }
+  }
+  std::vector<int64_t> app;  // nodes not on a datum, in visiting order
+  app.reserve(m);
+  for (int64_t k = 0; k < m; ++k)
+    if (datum[P[k]] < 0) {
+      pos[P[k]] = (int32_t)app.size();
+      app.push_back(P[k]);
+    }
+  const int64_t m2 = (int64_t)app.size();
+  const int64_t N = n + m2;
+  if (N > 2147483647LL) return fail(c, BK_E_INVALID, "data and nodes beyond 2^31 samples");
+  // search.rs:785-797: the sectors stay planar while every sample added lies
+  // at the first sample's elevation. Nodes [0, n_planar) search planar.
+  bool planar_data = true;
+  for (int64_t i = 1; i < n; ++i) planar_data = planar_data && Z[i] == Z[0];
+  int64_t n_planar = 0;
+  if (planar_data) {
+    while (n_planar < m2 && GZ[app[n_planar]] == Z[0]) ++n_planar;
+    n_planar = std::min<int64_t>(n_planar + 1, m2);
+  }
+  // The samples of the search: the data, then the nodes in visiting order.
+  for (Buf* b : {&c->sgs_x, &c->sgs_y, &c->sgs_z, &c->sgs_e, &c->sgs_v, &c->sgs_s})
+    if (int r = ensure(c, *b, std::max<int64_t>(N, 1) * sizeof(double))) return r;
+  if (int r = ensure(c, c->sgs_h, std::max<int64_t>(N, 1) * sizeof(int32_t))) return r;
+  {
+    std::vector<double> ax(m2), ay(m2), az(m2);
+    for (int64_t k = 0; k < m2; ++k) {
+      ax[k] = GX[app[k]];
+      ay[k] = GY[app[k]];
+      az[k] = GZ[app[k]];
+    }
+    auto put = [&](Buf& dst, const Buf& data, const std::vector<double>* nodes, double fill) -> int {
+      HIPOK(c, hipMemcpyAsync(dst.p, data.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      if (m2 > 0) {
+        if (nodes) {
+          HIPOK(c, hipMemcpyAsync((double*)dst.p + n, nodes->data(), m2 * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+        } else {
+          std::vector<double> f(m2, fill);
+          HIPOK(c, hipMemcpyAsync((double*)dst.p + n, f.data(), m2 * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+          HIPOK(c, hipStreamSynchronize(c->stream));
+        }
+      }
+      return 0;
+    };
+    if (int r = put(c->sgs_x, c->x, &ax, 0.0)) return r;
+    if (int r = put(c->sgs_y, c->y, &ay, 0.0)) return r;
+    if (int r = put(c->sgs_z, c->z, &az, 0.0)) return r;
+    if (int r = put(c->sgs_v, c->v, nullptr, 0.0)) return r;
+    if (c->has_err)
+      if (int r = put(c->sgs_e, c->e, nullptr, 0.0)) return r;
+    std::vector<int32_t> none(c->has_hole ? m2 : 0, -1);
+    if (c->has_hole) {
+      HIPOK(c, hipMemcpyAsync(c->sgs_h.p, c->hole.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+      if (m2 > 0)
+        HIPOK(c, hipMemcpyAsync((int32_t*)c->sgs_h.p + n, none.data(), m2 * sizeof(int32_t), hipMemcpyHostToDevice,
+                                c->stream));
+    }
+    HIPOK(c, hipStreamSynchronize(c->stream));
+  }
+  if (!c->sgs)
+    if (int r = bk_create(&c->sgs, c->device)) return fail(c, r, std::string("simulation context: ") + g_err);
+  bk_ctx* k = c->sgs;
+  k->cell_factor = c->cell_factor;
+  k->cell_factor_capped = c->cell_factor_capped;
+  k->search_mode = c->search_mode;
+  k->solve_waves = c->solve_waves;
+  k->refine_max = c->refine_max;
+  k->max_iter = c->max_iter;
+  k->tol = c->tol;
+  for (int b = 0; b < 3; ++b) k->edges[b] = c->edges[b];
+  if (int r = bk_set_samples(k, N, (const double*)c->sgs_x.p, (const double*)c->sgs_y.p, (const double*)c->sgs_z.p,
+                             (const double*)c->sgs_v.p, c->has_err ? (const double*)c->sgs_e.p : nullptr,
+                             c->has_hole ? (const int32_t*)c->sgs_h.p : nullptr))
+    return fail(c, r, std::string("simulation samples: ") + k->err);
+  int64_t stride = 1;
+  const double* tx = (const double*)k->x.p + n;
+  const double* ty = (const double*)k->y.p + n;
+  const double* tz = (const double*)k->z.p + n;
+  if (m2 > 0) {
+    // Each node's neighbours among the samples before it.
+    SearchArgs a;
+    if (int r = prepare_search(k, s, &vu, a, stride)) return fail(c, r, k->err);
+    if (int r = ensure(k, k->nidx, m2 * stride * sizeof(int32_t))) return fail(c, r, k->err);
+    if (int r = ensure(k, k->ncount, m2 * sizeof(int32_t))) return fail(c, r, k->err);
+    if (int r = ensure(k, k->sstatus, m2 * sizeof(int32_t))) return fail(c, r, k->err);
+    if (int r = ensure(c, c->sgs_limit, m2 * sizeof(int32_t))) return r;
+    int32_t* limit = (int32_t*)c->sgs_limit.p;
+    k_sgs_limits<<<blocks_of(m2), 256, 0, k->stream>>>(m2, n, limit);
+    const int64_t cut[3] = {0, n_planar, m2};
+    for (int part = 0; part < 2; ++part) {
+      const int64_t r0 = cut[part], r1 = cut[part + 1];
+      if (r1 <= r0) continue;
+      SearchArgs b = a;
+      b.planar = part == 0;
+      b.n_sectors = b.plane ? b.plane : (b.planar ? 4 : 8);
+      if (s->octant)
+        b.sector_cap = (int)std::min<int64_t>(
+            s->max_per_octant >= 0 ? s->max_per_octant : (s->max_samples + b.n_sectors - 1) / b.n_sectors,
+            std::numeric_limits<int>::max());
+      sector_map(b);
+      b.tx = tx + r0;
+      b.ty = ty + r0;
+      b.tz = tz + r0;
+      b.m = r1 - r0;
+      b.list = nullptr;
+      b.left_out = nullptr;
+      b.limit = limit + r0;
+      b.idx = (int32_t*)k->nidx.p + r0 * stride;
+      b.count = (int32_t*)k->ncount.p + r0;
+      b.status = (int32_t*)k->sstatus.p + r0;
+      if (bk::launch_search(k->stream, b, r1 - r0, k->search_mode, k->tiles)) return fail(c, BK_E_HIP, "search launch failed");
+    }
+    k_map_overflow<<<blocks_of(m2), 256, 0, k->stream>>>(m2, (int32_t*)k->sstatus.p);
+    // Simple kriging (mean 0) weights and variances from positions alone.
+    for (Buf* b : {&k->value, &k->variance})
+      if (int r = ensure(k, *b, m2 * sizeof(double))) return fail(c, r, k->err);
+    if (int r = ensure(k, k->n_used, m2 * sizeof(int32_t))) return fail(c, r, k->err);
+    if (int r = ensure(k, k->status, m2 * sizeof(int32_t))) return fail(c, r, k->err);
+    if (int r = ensure(c, c->sgs_w, m2 * stride * sizeof(double))) return r;
+    if (int r = run_solve(k, &vu, s, BK_SIMPLE, 0.0, precision, m2, tx, ty, tz, (int)stride, nullptr,
+                          (double*)k->value.p, (double*)k->variance.p, nullptr, nullptr, (int32_t*)k->n_used.p,
+                          (int32_t*)k->status.p, (double*)c->sgs_w.p))
+      return fail(c, r, k->err);
+  }
+  // The draws, in passes.
+  if (int r = ensure(c, c->sgs_done, std::max<int64_t>(m2, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->sgs_ctr, sizeof(unsigned long long))) return r;
+  if (int r = ensure(c, c->sgs_in, std::max<int64_t>(m, 1) * sizeof(double))) return r;
+  if (int r = ensure(c, c->sgs_datum, std::max<int64_t>(m, 1) * sizeof(int64_t))) return r;
+  if (int r = ensure(c, c->sgs_pos, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
+  double* sc = (double*)c->sgs_s.p;
+  HIPOK(c, hipMemcpyAsync(sc, c->v.p, n * sizeof(double), hipMemcpyDeviceToDevice, k->stream));
+  if (m > 0) {
+    HIPOK(c, hipMemcpyAsync(c->sgs_in.p, z, m * sizeof(double), hipMemcpyDefault, k->stream));
+    HIPOK(c, hipMemcpyAsync(c->sgs_datum.p, datum.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, k->stream));
+    HIPOK(c, hipMemcpyAsync(c->sgs_pos.p, pos.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, k->stream));
+  }
+  if (m2 > 0) {
+    if (bk::zero32(k->stream, (int32_t*)c->sgs_done.p, m2) || bk::zero32(k->stream, (int32_t*)c->sgs_ctr.p, 2))
+      return fail(c, BK_E_HIP, "fill launch failed");
+    unsigned long long drawn = 0;
+    int32_t pass = 0;
+    while (drawn < (unsigned long long)m2) {
+      // Each pass draws at least the first node still waiting (its simulated
+      // neighbours come before it), so m2 passes always end.
+      if (pass > m2) return fail(c, BK_E_HIP, "simulation passes did not finish");
+      for (int q = 0; q < 8; ++q) {
+        ++pass;
+        k_sgs_pass<<<blocks_of(m2), 256, 0, k->stream>>>(m2, n, (int)stride, (const int32_t*)k->nidx.p,
+                                                          (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
+                                                          (const double*)c->sgs_w.p, (const double*)k->variance.p,
+                                                          (const double*)c->sgs_in.p, sc, (int32_t*)c->sgs_done.p,
+                                                          pass, (unsigned long long*)c->sgs_ctr.p);
+      }
+      HIPOK(c, hipGetLastError());
+      HIPOK(c, hipMemcpyAsync(&drawn, c->sgs_ctr.p, sizeof(drawn), hipMemcpyDeviceToHost, k->stream));
+      HIPOK(c, hipStreamSynchronize(k->stream));
+    }
+    c->sgs_passes = pass;
+  }
+  if (m > 0) {
+    double* osc;
+    int32_t* osrc;
+    if (int r = out_ptr(c, score, m, c->value, &osc)) return r;
+    if (int r = out_ptr(c, source, m, c->n_used, &osrc)) return r;
+    const int32_t* kst = m2 > 0 ? (const int32_t*)k->status.p : nullptr;
+    const int32_t* kcnt = m2 > 0 ? (const int32_t*)k->n_used.p : nullptr;
+    k_sgs_out<<<blocks_of(m), 256, 0, k->stream>>>(m, n, (const int64_t*)c->sgs_datum.p, (const int32_t*)c->sgs_pos.p,
+                                                   sc, kst, kcnt, osc, osrc);
+    HIPOK(c, hipGetLastError());
+    HIPOK(c, hipStreamSynchronize(k->stream));
+    if (int r = stage_out(c, score, osc, m)) return r;
+    if (int r = stage_out(c, source, osrc, m)) return r;
+    HIPOK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}

@@ -69,6 +69,7 @@ struct SearchArgs {
   const double* ty;
   const double* tz;
   const int64_t* left_out;    // may be null
+  const int32_t* limit;       // per target: only samples below this index are candidates (may be null)
   const int32_t* list;        // targets to search (null: 0..m)
   int64_t m;                  // their number, or a bound on it when m_dev is set
   const int32_t* m_dev;       // the number on the device (a spill count), or null

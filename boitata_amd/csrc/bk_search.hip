@@ -209,6 +209,7 @@ struct Ctx {
   double q0, q1, q2;
   double t0, t1, t2;
   int64_t left_out;
+  uint32_t limit;  // samples from this index on are not candidates
   int capped;
   __device__ bool full() const { return st.n >= a->max_samples; }
 
@@ -389,7 +390,7 @@ struct Ctx {
         double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
         double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
         idx = (uint32_t)__double_as_longlong(E.w);
-        if (d2 <= a->radius2 && (int64_t)idx != left_out) {
+        if (d2 <= a->radius2 && (int64_t)idx != left_out && idx < limit) {
           key = dbits(d2);
           have = key_less(key, idx, st.thr, st.thr_i);
           if (have && a->balances) {
@@ -555,6 +556,7 @@ __device__ void search_target(const SearchArgs& a, Pool& L, int64_t t) {
   c.st = State{0, 0, 0, 0, 0, ~0ull, 0xffffffffu};
   c.capped = a.balances || a.max_per_hole >= 0;
   c.left_out = a.left_out ? a.left_out[t] : -1;
+  c.limit = a.limit ? (uint32_t)a.limit[t] : 0xffffffffu;
   c.t0 = a.tx[t];
   c.t1 = a.ty[t];
   c.t2 = a.tz[t];
@@ -736,6 +738,7 @@ __device__ __forceinline__ void knn_lane(const SearchArgs& a, int64_t b, int64_t
   const bool live = b < mm;
   const int64_t t = live ? (a.list ? a.list[b] : b) : 0;
   const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
+  const uint32_t limit = (live && a.limit) ? (uint32_t)a.limit[t] : 0xffffffffu;
   double q0 = 0, q1 = 0, q2 = 0;
   if (live) mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
   int size = 0;
@@ -761,7 +764,7 @@ __device__ __forceinline__ void knn_lane(const SearchArgs& a, int64_t b, int64_t
       double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
       double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
       uint32_t ii = (uint32_t)__double_as_longlong(E.w);
-      if (!(d2 <= a.radius2) || (int64_t)ii == left_out) continue;
+      if (!(d2 <= a.radius2) || (int64_t)ii == left_out || ii >= limit) continue;
       uint64_t kk = dbits(d2);
       if (size < K) {
         H.push(size, kk, ii);
@@ -948,6 +951,7 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
   const bool live = b < a.m;
   const int64_t t = live ? (a.list ? a.list[b] : b) : 0;
   const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
+  const uint32_t limit = (live && a.limit) ? (uint32_t)a.limit[t] : 0xffffffffu;
   const int K = a.max_samples;
   const GridDev& g = a.g;
   double q0, q1, q2;
@@ -1070,7 +1074,7 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
           const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
           ii = (uint32_t)__double_as_longlong(E.w);
           kk = dbits(d2);
-          keep = d2 <= a.radius2 && (int64_t)ii != left_out && key_less(kk, ii, tk, ti);
+          keep = d2 <= a.radius2 && (int64_t)ii != left_out && ii < limit && key_less(kk, ii, tk, ti);
         }
         const uint64_t mask = group_ballot<G>(keep, h);
         if (__any(mask != 0)) {
@@ -1165,6 +1169,7 @@ __global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* sp
   if (b >= a.m) return;  // whole wave
   const int64_t t = a.list ? a.list[b] : b;
   const int64_t left_out = a.left_out ? a.left_out[t] : -1;
+  const uint32_t limit = a.limit ? (uint32_t)a.limit[t] : 0xffffffffu;
   const int o = lane / S, j = lane % S;
   const int cap = a.sector_cap;
   const int K = a.max_samples;
@@ -1263,7 +1268,7 @@ __global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* sp
           const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
           const uint32_t ii = (uint32_t)__double_as_longlong(E.w);
           const uint64_t kk = dbits(d2);
-          bool keep = d2 <= a.radius2 && (int64_t)ii != left_out && (tmax == KEY_INF || kk <= tmax);
+          bool keep = d2 <= a.radius2 && (int64_t)ii != left_out && ii < limit && (tmax == KEY_INF || kk <= tmax);
           if (keep) {
             int sec = a.sec_fast ? sector_fast(a, u0, u1, u2, fabs(E.x) + fabs(E.y) + fabs(E.z) + qabs) : -1;
             if (sec < 0) sec = sector_at(a, t0, t1, t2, ii);
@@ -1387,6 +1392,7 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn_tile(const SearchArgs a, int re
   const bool live = b < a.m;
   const int64_t t = live ? a.list[b] : 0;
   const int64_t left_out = (live && a.left_out) ? a.left_out[t] : -1;
+  const uint32_t limit = (live && a.limit) ? (uint32_t)a.limit[t] : 0xffffffffu;
   const GridDev& g = a.g;
   double q0 = 0, q1 = 0, q2 = 0;
   if (live) mv3(a.F, a.tx[t] - a.origin[0], a.ty[t] - a.origin[1], a.tz[t] - a.origin[2], q0, q1, q2);
@@ -1454,7 +1460,7 @@ __global__ void __launch_bounds__(BK_WAVE) k_knn_tile(const SearchArgs a, int re
         const double u0 = E.x - q0, u1 = E.y - q1, u2 = E.z - q2;
         const double d2 = (u0 * u0 + u1 * u1) + u2 * u2;
         const uint32_t ii = (uint32_t)__double_as_longlong(E.w);
-        if (!(d2 <= a.radius2) || (int64_t)ii == left_out) continue;
+        if (!(d2 <= a.radius2) || (int64_t)ii == left_out || ii >= limit) continue;
         const uint64_t kk = dbits(d2);
         if (size < K) {
           H.push(size, kk, ii);

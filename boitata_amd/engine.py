@@ -278,6 +278,30 @@ class Engine:
                                                _ptr(status)))
         return dict(idx=idx, weights=weights, count=count, status=status)
 
+    def sgs(self, variogram, search, gx, gy, gz, path, z, precision=_abi.FP64):
+        """bk_sgs: sequential Gaussian simulation of normal scores at the grid
+        nodes (gx, gy, gz), visited in the order `path`, the k-th node of the
+        path not on a datum drawing z[k]. The samples set on the engine are the
+        conditioning data with their scores as values. Returns per node the
+        score and its source (>= 0 the datum it lies on, -1 kriged, -2 drawn
+        from the marginal, -3 failed)."""
+        gx, gy, gz = _f64(gx), _f64(gy), _f64(gz)
+        path, z = _i64(path), _f64(z)
+        m = int(len(gx))
+        if len(gy) != m or len(gz) != m or len(path) != m or len(z) != m:
+            raise ValueError("one coordinate triple, path entry and draw per node")
+        if hasattr(gx, "data_ptr") and gx.is_cuda:
+            import torch
+
+            score = torch.empty(m, dtype=torch.float64, device=gx.device)
+            source = torch.empty(m, dtype=torch.int32, device=gx.device)
+        else:
+            score = np.empty(m, dtype=np.float64)
+            source = np.empty(m, dtype=np.int32)
+        self._check(self._lib.bk_sgs(self._h, C.byref(variogram), C.byref(search), int(precision), m, _ptr(gx),
+                                     _ptr(gy), _ptr(gz), _ptr(path), _ptr(z), _ptr(score), _ptr(source)))
+        return dict(score=score, source=source)
+
     def _mik_outputs(self, m, k, nc, nq, like):
         """Output arrays of multiple indicator kriging: entry-major (rows, m)."""
         if like is not None and hasattr(like, "data_ptr") and like.is_cuda:

@@ -252,6 +252,24 @@ int bk_krige_weights(bk_ctx* ctx, const bk_variogram* vg, const bk_search* searc
                      const bk_block* block, int32_t precision, int64_t m, const double* tx, const double* ty,
                      const double* tz, int32_t* idx, double* weights, int32_t* count, int32_t* status);
 
+/* Sequential Gaussian simulation in normal scores: the node loop of
+ * crates/simulation/src/sgs.rs:600-750 (sequential, with its search tree grown
+ * by SearchTree::add, search.rs:785-797) for one search pass, without domains,
+ * trend, high-grade caps, local anisotropy or a secondary variable. The
+ * context's samples are the conditioning data with their normal scores as
+ * values; vg is the scores' variogram, scaled to a total sill of 1 as sgs.rs
+ * does. The m grid nodes (distinct) are visited in the order `path` (a
+ * permutation of 0 .. m-1, the reference's shuffled path); z holds m standard
+ * normal draws, the k-th node of the path that is not on a datum taking z[k]
+ * (the reference draws from its generator in that order). Per node: score[i],
+ * and source[i] >= 0 the datum it lies on (score = that datum's score), -1
+ * simple-kriged from its neighbours, -2 drawn from the marginal (no
+ * neighbours), -3 its kriging failed (score NaN; the reference returns an
+ * error). Searches with calibration or caps of 0 are rejected. */
+int bk_sgs(bk_ctx* ctx, const bk_variogram* vg, const bk_search* search, int32_t precision, int64_t m,
+           const double* gx, const double* gy, const double* gz, const int64_t* path, const double* z, double* score,
+           int32_t* source);
+
 /* Leave-one-out cross-validation loss and its gradient (no counterpart in the
  * reference). The neighbour sets are those of bk_loo with `search`, fixed before
  * the call by bk_loo_prepare, so they do not move with the parameters.
@@ -404,7 +422,8 @@ int bk_set_graphs(bk_ctx* ctx, int32_t enable);
  * shell table; BK_STAT_SEARCH_RETRY, targets of the last capped search whose
  * candidate pool outgrew its 512 LDS entries and which were searched again
  * with a pool of 65,536 entries in device memory (beyond that, BK_UNSUPPORTED);
- * BK_STAT_GRAPH_LAUNCHES, bk_krige calls run as a graph so far. Of the last
+ * BK_STAT_GRAPH_LAUNCHES, bk_krige calls run as a graph so far;
+ * BK_STAT_SGS_PASSES, the draw passes of the last bk_sgs. Of the last
  * kriging pass: BK_STAT_REFINED, systems solved by FP32 elimination and FP64
  * refinement; BK_STAT_REFINE_SWEEPS, their refinement sweeps in total;
  * BK_STAT_REFINE_REDO, systems the refined form handed to the FP64 kernel
@@ -415,6 +434,7 @@ int bk_set_graphs(bk_ctx* ctx, int32_t enable);
 #define BK_STAT_REFINE_SWEEPS 3
 #define BK_STAT_REFINE_REDO 4
 #define BK_STAT_SEARCH_RETRY 5
+#define BK_STAT_SGS_PASSES 6
 int bk_get_stat(bk_ctx* ctx, int32_t which, int64_t* value);
 
 /* Stage timing with HIP events recorded on the context's stream (no

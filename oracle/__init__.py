@@ -72,6 +72,8 @@ def lib() -> C.CDLL:
             "or_block_krige_lists": (I32, [SP, VP, I32, D, BP, I64, P, P, P, P, P, I64, I32, P, P, P, P, P, P,
                                            I32]),
             "or_system_matrix": (None, [SP, VP, I32, P, P]),
+            "or_sgs": (I32, [SP, VP, SR, I64, P, P, P, P, P, P, P]),
+            "or_sgs_plan": (I32, [SP, VP, SR, I64, P, P, P, P, P, P, P, I32]),
             "or_prefix_quality": (I32, [SP, VP, I32, P, BP, I32, P, P, P]),
             "or_calibrated_count": (I32, [I32, D, I64, I32, P, P, P]),
             "or_balanced": (I32, [SP, SR, VP, P, I32, P, P]),
@@ -301,6 +303,30 @@ def block_krige_lists(data: Data, vg, blk, targets, idx, count, solver=0, lanes=
                                _p(out["variance"]), _p(out["lagrange"]), _p(out["support"]), _p(status), threads)
     out["status"] = status
     return out
+
+
+def sgs(data: Data, vg, search, grid, path, z, plan=False, threads=0):
+    """Sequential Gaussian simulation of scores at the grid nodes (m, 3),
+    node by node along `path` (or_sgs; the data's values are their scores),
+    or with plan=True as per-node plans over one grid (or_sgs_plan, parallel
+    searches and solves). Returns (score, source) per node."""
+    g = np.ascontiguousarray(grid, dtype=np.float64).reshape(-1, 3)
+    gx, gy, gz = (np.ascontiguousarray(g[:, k]) for k in range(3))
+    m = len(g)
+    path = np.ascontiguousarray(path, dtype=np.int64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    assert len(path) == m and len(z) >= m
+    score = np.full(m, np.nan)
+    source = np.zeros(m, np.int32)
+    if plan:
+        rc = lib().or_sgs_plan(C.byref(data.s), C.byref(vg), C.byref(search), m, _p(gx), _p(gy), _p(gz), _p(path),
+                               _p(z), _p(score), _p(source), threads)
+    else:
+        rc = lib().or_sgs(C.byref(data.s), C.byref(vg), C.byref(search), m, _p(gx), _p(gy), _p(gz), _p(path),
+                          _p(z), _p(score), _p(source))
+    if rc != 0:
+        raise ValueError("invalid search or no data")
+    return score, source
 
 
 def system_matrix(data: Data, vg, idx):

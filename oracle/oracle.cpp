@@ -46,6 +46,7 @@
 #include <limits>
 #include <queue>
 #include <utility>
+#include <map>
 #include <vector>
 
 namespace orc {
@@ -1406,9 +1407,11 @@ struct SearchTree {
     for (auto& r : ranked) out.push_back(r.second);
     return out;
   }
-  // search.rs:951-982
-  int neighbors_around(const P3& target, int64_t left_out, std::vector<int64_t>& out) const {
-    Selector sel(&target, &p, &sectors);
+  // search.rs:951-982. limit >= 0: only samples below that index are
+  // candidates; sec: sectors in place of the tree's.
+  int neighbors_around(const P3& target, int64_t left_out, std::vector<int64_t>& out, int64_t limit = -1,
+                       const Sectors* sec = nullptr) const {
+    Selector sel(&target, &p, sec ? sec : &sectors);
     if (s->empty() || p.max_samples == 0) {
       out = sel.chosen;
       return sel.finish();
@@ -1420,7 +1423,7 @@ struct SearchTree {
     auto offer = [&]() {
       bool f = false;
       for (int64_t i : group) {
-        if (i == left_out) continue;
+        if (i == left_out || (limit >= 0 && i >= limit)) continue;
         if (sel.offer(i, (*s)[i].loc, (*s)[i].hole)) {
           f = true;
           break;
@@ -2037,6 +2040,194 @@ int or_neighbors(const or_samples* sm, const bk_search* sp, const bk_variogram* 
     for (size_t k = 0; k < out.size(); ++k) idx[t * stride + k] = (int32_t)out[k];
   }
   delete tree;
+  return 0;
+}
+
+// Sequential Gaussian simulation in normal scores, the node loop of
+// crates/simulation/src/sgs.rs:600-750 run one node at a time (the reference's
+// batches reproduce this sequential realization for any batch size): one
+// search pass, no domains, trend, high-grade caps, local anisotropy or
+// secondary variable. The samples are the data with their scores as values;
+// vg is scaled to a total sill of 1 (unit_sill, sgs.rs:881-886). Node path[p]
+// searches the data and the nodes added before it (the scan order of
+// search.rs:647-687; SearchTree::add, search.rs:785-797, grows the set and its
+// planar flag as the scan's planar test over the current set does); a datum at
+// its location gives the node that datum's score (sgs.rs:665-674, the first
+// datum at the location when the search fails: datum_index, sgs.rs:868-875);
+// without neighbours the node takes the draw itself (marginal, sgs.rs:688-691);
+// otherwise krige(Simple { mean: 0 }) and score = value + sqrt(max(variance, 0))
+// z (sgs.rs:693-705, 466-469). Every node not on a datum takes the next draw
+// from z and joins the samples with its score, no hole and no error variance.
+// source[i]: the datum index, -1 kriged, -2 marginal, -3 the kriging failed
+// (score NaN).
+int or_sgs(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int64_t m, const double* gx,
+           const double* gy, const double* gz, const int64_t* path, const double* z, double* score, int32_t* source) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  if (s.empty()) return BK_E_INVALID;
+  bk_variogram vu = *v;
+  double total = vu.nugget;
+  for (int k = 0; k < vu.n_structures; ++k) total += vu.structures[k].sill;
+  if (total > 0.0) {
+    const double f = 1.0 / total;
+    vu.nugget *= f;
+    for (int k = 0; k < vu.n_structures; ++k) vu.structures[k].sill *= f;
+  }
+  Variogram vg = make_variogram(&vu);
+  const int64_t n = (int64_t)s.size();
+  auto same = [](const P3& a, const P3& b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; };
+  int64_t draw = 0;
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t node = path[k];
+    const P3 target{gx[node], gy[node], gz[node]};
+    std::vector<int64_t> idx;
+    const int st = neighbors_scan(target, s, p, &vg, -1, idx);
+    int64_t datum = -1;
+    if (st == BK_OK) {
+      for (int64_t i : idx)
+        if (same(s[i].loc, target)) {
+          datum = i;
+          break;
+        }
+    } else {
+      for (int64_t i = 0; i < n; ++i)
+        if (same(s[i].loc, target)) {
+          datum = i;
+          break;
+        }
+    }
+    if (datum >= 0) {
+      score[node] = s[datum].value;
+      source[node] = (int32_t)datum;
+      continue;
+    }
+    double sc;
+    if (st != BK_OK || idx.empty()) {
+      sc = z[draw];
+      source[node] = -2;
+    } else {
+      std::vector<Sample> sel;
+      for (int64_t i : idx) sel.push_back(s[i]);
+      Estimate e;
+      if (krige(SIMPLE, 0.0, target, sel, vg, e) == BK_OK) {
+        sc = e.value + std::sqrt(std::max(e.variance, 0.0)) * z[draw];
+        source[node] = -1;
+      } else {
+        sc = NAN;
+        source[node] = -3;
+      }
+    }
+    draw++;
+    score[node] = sc;
+    s.push_back(Sample{target, sc, -1, 0.0});
+  }
+  return 0;
+}
+
+// The realization of or_sgs computed as a plan per node (the constant and
+// node weights of crates/simulation/src/shared.rs:104-143): node k of the path
+// that is not on a datum searches, over one grid of the data followed by every
+// such node in path order, the samples below index n + k -- those or_sgs has
+// by then -- with the sectors planar while the data and those nodes share the
+// first datum's elevation (search.rs:785-797); the searches and simple-kriging
+// weights run in parallel (OpenMP), the draws in path order. A datum lies
+// under a node where a datum has the node's location (or_sgs takes it then,
+// with caps of at least 1). Used as the CPU baseline; checked against or_sgs.
+int or_sgs_plan(const or_samples* sm, const bk_variogram* v, const bk_search* sp, int64_t m, const double* gx,
+                const double* gy, const double* gz, const int64_t* path, const double* z, double* score,
+                int32_t* source, int nthreads) {
+  Search p = make_search(sp);
+  if (!check(p)) return BK_E_INVALID;
+  auto s = samples_of(sm);
+  if (s.empty()) return BK_E_INVALID;
+  bk_variogram vu = *v;
+  double total = vu.nugget;
+  for (int k = 0; k < vu.n_structures; ++k) total += vu.structures[k].sill;
+  if (total > 0.0) {
+    const double f = 1.0 / total;
+    vu.nugget *= f;
+    for (int k = 0; k < vu.n_structures; ++k) vu.structures[k].sill *= f;
+  }
+  Variogram vg = make_variogram(&vu);
+  const int64_t n = (int64_t)s.size();
+  auto key = [](const P3& q) {
+    std::array<uint64_t, 3> k;
+    for (int d = 0; d < 3; ++d) {
+      const double x = q[d] + 0.0;
+      std::memcpy(&k[d], &x, 8);
+    }
+    return k;
+  };
+  std::map<std::array<uint64_t, 3>, int64_t> first;
+  for (int64_t i = 0; i < n; ++i) first.emplace(key(s[i].loc), i);
+  std::vector<int64_t> app, datum(m, -1);
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t node = path[k];
+    auto it = first.find(key(P3{gx[node], gy[node], gz[node]}));
+    if (it != first.end())
+      datum[node] = it->second;
+    else
+      app.push_back(node);
+  }
+  const int64_t m2 = (int64_t)app.size();
+  for (int64_t k = 0; k < m2; ++k) s.push_back(Sample{P3{gx[app[k]], gy[app[k]], gz[app[k]]}, 0.0, -1, 0.0});
+  bool planar = true;
+  for (int64_t i = 1; i < n; ++i) planar = planar && s[i].loc[2] == s[0].loc[2];
+  int64_t n_planar = 0;
+  if (planar) {
+    while (n_planar < m2 && s[n + n_planar].loc[2] == s[0].loc[2]) ++n_planar;
+    n_planar = std::min(n_planar + 1, m2);
+  }
+  SearchTree tree(s, p, &vg);
+  const Sectors flat = make_sectors(metric(p, &vg), true, p), solid = make_sectors(metric(p, &vg), false, p);
+  std::vector<std::vector<int64_t>> lists(m2);
+  std::vector<std::vector<double>> weights(m2);
+  std::vector<double> var(m2, 0.0);
+  std::vector<int> st(m2, BK_OK);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+  for (int64_t k = 0; k < m2; ++k) {
+    const P3 target = s[n + k].loc;
+    std::vector<int64_t> idx;
+    int r = tree.neighbors_around(target, -1, idx, n + k, k < n_planar ? &flat : &solid);
+    if (r != BK_OK || idx.empty()) {
+      st[k] = BK_SEARCH_FAILED;
+      continue;
+    }
+    std::vector<Sample> sel;
+    for (int64_t i : idx) sel.push_back(s[i]);
+    Estimate e;
+    r = krige(SIMPLE, 0.0, target, sel, vg, e);
+    if (r != BK_OK) {
+      st[k] = r;
+      continue;
+    }
+    lists[k] = idx;
+    weights[k] = e.weights;
+    var[k] = e.variance;
+  }
+  for (int64_t k = 0; k < m2; ++k) {
+    double sc;
+    if (st[k] == BK_SEARCH_FAILED) {
+      sc = z[k];
+      source[app[k]] = -2;
+    } else if (st[k] != BK_OK) {
+      sc = NAN;
+      source[app[k]] = -3;
+    } else {
+      double mean = 0.0;
+      for (size_t j = 0; j < lists[k].size(); ++j) mean += weights[k][j] * s[lists[k][j]].value;
+      sc = mean + std::sqrt(std::max(var[k], 0.0)) * z[k];
+      source[app[k]] = -1;
+    }
+    s[n + k].value = sc;
+    score[app[k]] = sc;
+  }
+  for (int64_t i = 0; i < m; ++i)
+    if (datum[i] >= 0) {
+      score[i] = s[datum[i]].value;
+      source[i] = (int32_t)datum[i];
+    }
   return 0;
 }
 
