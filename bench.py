@@ -487,6 +487,76 @@ def mik_leg(args, w, dev, threads):
     return line
 
 
+def sgs_leg(args, dev, threads):
+    """Sequential Gaussian simulation of normal scores: the 100,000 samples of
+    the headline workload (rank normal scores of their values), 1,000,000
+    nodes of a 100 x 100 x 100 grid over the same volume, the headline's
+    ellipsoid search (at most 32 neighbours) and nested model scaled to a unit
+    sill, a random path and standard normal draws (numpy, seed 5)."""
+    import torch
+    from scipy.stats import norm
+
+    from boitata_amd import _abi, datasets
+    from boitata_amd.engine import Engine
+
+    w = datasets.config2(targets=False)
+    r = np.argsort(np.argsort(w.value, kind="stable"), kind="stable")
+    scores = norm.ppf((r + 0.5) / w.n)
+    n_side = args.sgs_side
+    axes = [np.linspace(lo + (hi - lo) / (2 * n_side), hi - (hi - lo) / (2 * n_side), n_side)
+            for lo, hi in ((0.0, 1000.0), (0.0, 1000.0), (-500.0, 0.0))]
+    G = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    m = len(G)
+    rng = np.random.default_rng(5)
+    path = rng.permutation(m)
+    z = rng.normal(size=m)
+    tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    e = Engine(0)
+    e.set_samples(tt(w.x), tt(w.y), tt(w.z), tt(scores), None, tt(w.hole.astype(np.int32)))
+    gx, gy, gz, dpath, dz = tt(G[:, 0]), tt(G[:, 1]), tt(G[:, 2]), tt(path), tt(z)
+    res = {}
+    for prec in (_abi.FP32_IR, _abi.FP64):
+        e.sgs(w.variogram, w.search, gx, gy, gz, dpath, dz, precision=prec)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.sgs_steps):
+            out = e.sgs(w.variogram, w.search, gx, gy, gz, dpath, dz, precision=prec)
+        torch.cuda.synchronize()
+        res[prec] = ((time.perf_counter() - t0) / args.sgs_steps, out, e.stat(_abi.STAT_SGS_PASSES))
+    (dt32, o32, passes), (dt64, o64, _) = res[_abi.FP32_IR], res[_abi.FP64]
+    s32, s64 = host(o32["score"]), host(o64["score"])
+    src = host(o64["source"])
+    line = {"workload": f"SGS in normal scores: {w.n} samples, {m} nodes, <=32 neighbours",
+            "value": m / dt32, "unit": "nodes/s", "ms_per_realization": dt32 * 1e3,
+            "fp64": {"value": m / dt64, "ms_per_realization": dt64 * 1e3},
+            "speedup_fp32ir_over_fp64": dt64 / dt32, "draw_passes": passes,
+            "rel_err_fp32ir_vs_fp64": rel_stats(s32, s64, 1e-3),
+            "nodes": {"kriged": int((src == -1).sum()), "marginal": int((src == -2).sum()),
+                      "on_data": int((src >= 0).sum())},
+            "steps": args.sgs_steps}
+    log(f"SGS: {line['value']:.0f} nodes/s FP32+IR, {line['fp64']['value']:.0f} FP64, {passes} draw passes")
+    e.close()
+    if threads:
+        import oracle
+
+        # The oracle's plan form (parallel searches and solves, draws in path
+        # order) on the same data and a coarser grid over the same volume.
+        k = 30
+        ax = [np.linspace(lo + (hi - lo) / (2 * k), hi - (hi - lo) / (2 * k), k)
+              for lo, hi in ((0.0, 1000.0), (0.0, 1000.0), (-500.0, 0.0))]
+        Gs = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+        ps = rng.permutation(len(Gs))
+        zs = rng.normal(size=len(Gs))
+        d = oracle.Data(w.x, w.y, w.z, scores, None, w.hole)
+        t0 = time.perf_counter()
+        oracle.sgs(d, w.variogram, w.search, Gs, ps, zs, plan=True, threads=threads)
+        dtc = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": len(Gs) / dtc, "unit": "nodes/s", "cores": threads, "kind": "port",
+                                "sample": f"one realization of {len(Gs)} nodes (30 x 30 x 30 over the same volume) "
+                                          f"from the same data, per-node plans, {dtc:.1f} s"}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,7 +570,9 @@ def main():
     ap.add_argument("--mik-steps", type=int, default=3, help="timed passes of the MIK leg (0: skip)")
     ap.add_argument("--config-steps", type=int, default=2,
                     help="timed passes of the C1, C3 and C5 legs (0: skip them)")
-    ap.add_argument("--legs", default="c1,c3,c4,c5,mik", help="comma-separated legs beside the headline")
+    ap.add_argument("--legs", default="c1,c3,c4,c5,mik,sgs", help="comma-separated legs beside the headline")
+    ap.add_argument("--sgs-side", type=int, default=100, help="SGS leg: nodes per grid side")
+    ap.add_argument("--sgs-steps", type=int, default=2, help="SGS leg: timed realizations per form")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="finish with a device-to-device copy of this many MiB (byte calibration of PMC runs)")
     args = ap.parse_args()
@@ -560,6 +632,7 @@ def main():
     if args.loo_steps > 0 and "c4" in legs:
         configs["C4"] = loo_leg(args, dev, threads)
     mik = mik_leg(args, w, dev, threads) if args.mik_steps > 0 and "mik" in legs else None
+    sgs = sgs_leg(args, dev, threads) if args.sgs_steps > 0 and "sgs" in legs else None
 
     other = {"search_c2": rep["search"], "solve_fp64_c2": rep["solve_fp64"]}
     for key in ("C3", "C5"):
@@ -595,6 +668,7 @@ def main():
         "cpu_baseline": rep.get("cpu_baseline"),
         "loo_cv_grad": configs.get("C4"),
         "mik": mik,
+        "sgs": sgs,
         "configs": configs,
         "graph_launches": graph_launches,
     }

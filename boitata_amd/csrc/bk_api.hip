@@ -14,7 +14,6 @@
 #include <limits>
 #include <mutex>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/boitata_gpu.h"
@@ -161,7 +160,10 @@ struct bk_ctx {
   // the simulated nodes, and the pass's arrays.
   bk_ctx* sgs = nullptr;
   int sgs_passes = 0;  // draw passes of the last bk_sgs
-  Buf sgs_x, sgs_y, sgs_z, sgs_e, sgs_h, sgs_v, sgs_s, sgs_done, sgs_limit, sgs_datum, sgs_pos, sgs_ctr, sgs_in, sgs_w;
+  Buf sgs_x, sgs_y, sgs_z, sgs_e, sgs_h, sgs_v, sgs_s, sgs_done, sgs_limit, sgs_datum, sgs_pos, sgs_ctr, sgs_in, sgs_w,
+      sgs_path, sgs_app, sgs_work;
+  void* sgs_tmp = nullptr;  // scan storage
+  size_t sgs_tmp_bytes = 0;
   // Stage timing (bk_profile): events on the context's stream.
   bool prof = false;
   hipEvent_t ev[7] = {};
@@ -976,9 +978,10 @@ void bk_destroy(bk_ctx* c) {
                  &c->loo_status, &c->loo_lists, &c->mik_gv, &c->mik_gs, &c->mik_par, &c->mik_cls, &c->mik_tab,
                  &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out, &c->lu_big, &c->sgs_x, &c->sgs_y,
                  &c->sgs_z, &c->sgs_e, &c->sgs_h, &c->sgs_v, &c->sgs_s, &c->sgs_done, &c->sgs_limit, &c->sgs_datum,
-                 &c->sgs_pos, &c->sgs_ctr, &c->sgs_in, &c->sgs_w})
+                 &c->sgs_pos, &c->sgs_ctr, &c->sgs_in, &c->sgs_w, &c->sgs_path, &c->sgs_app, &c->sgs_work})
     release(*b);
   if (c->sgs) bk_destroy(c->sgs);
+  if (c->sgs_tmp) (void)hipFree(c->sgs_tmp);
   bk::free_grid(c->gb);
   bk::free_tiles(c->tiles);
   for (auto& g : c->graph_cache)
@@ -2384,6 +2387,26 @@ int bk_autotune(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t k
 // earlier pass (70-80 passes for 10^4 to 10^6 nodes of a random path).
 namespace {
 
+// The nodes in visiting order after the data: coordinates, score 0 (drawn
+// later), no error variance and no hole; first_off[0] becomes the first
+// rank whose elevation differs from the first sample's (m2 if none).
+__global__ void k_sgs_gather(int64_t m2, int64_t n, const int64_t* app, const double* gx, const double* gy,
+                             const double* gz, double* x, double* y, double* z, double* v, double* e, int32_t* hole,
+                             int32_t* first_off) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= m2) return;
+  const int64_t node = app[k];
+  x[n + k] = gx[node];
+  y[n + k] = gy[node];
+  z[n + k] = gz[node];
+  v[n + k] = 0.0;
+  e[n + k] = 0.0;
+  if (hole) hole[n + k] = -1;
+  if (!(gz[node] == z[0])) atomicMin(first_off, (int32_t)k);
+}
+
+__global__ void k_sgs_set(int32_t* p, int32_t v) { p[0] = v; }
+
 __global__ void k_sgs_limits(int64_t m2, int64_t n, int32_t* limit) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k < m2) limit[k] = (int32_t)(n + k);
@@ -2393,13 +2416,17 @@ __global__ void k_sgs_limits(int64_t m2, int64_t n, int32_t* limit) {
 // scores of the data (first n) and of the nodes drawn so far; done[k]: the
 // pass that drew node k, 0 while it waits.
 __global__ void k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx, const int32_t* count,
-                           const int32_t* status, const double* w, const double* variance, const double* z, double* s,
-                           int32_t* done, int32_t pass, unsigned long long* drawn) {
+                           const int32_t* status, const double* w, const double* variance, const double* z,
+                           const double* x, const double* y, const double* zc, double* s, int32_t* done, int32_t pass,
+                           unsigned long long* drawn, int32_t* dup) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= m2 || done[k] != 0) return;
   const int st = status[k];
   const int cnt = st == BK_OK ? count[k] : 0;
   const int32_t* id = nidx + k * (int64_t)stride;
+  // A node at an earlier node's location: that node is the first neighbour
+  // (distance 0); the grid's nodes must be distinct.
+  if (cnt > 0 && id[0] >= n && x[id[0]] == x[n + k] && y[id[0]] == y[n + k] && zc[id[0]] == zc[n + k]) dup[0] = 1;
   // A neighbour drawn in this pass may not be visible yet: wait for the next.
   for (int j = 0; j < cnt; ++j) {
     const int64_t q = (int64_t)id[j] - n;
@@ -2444,26 +2471,6 @@ __global__ void k_sgs_out(int64_t m, int64_t n, const int64_t* datum, const int3
               : (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA ? -2 : -3);
 }
 
-struct LocKey {
-  uint64_t b[3];
-  bool operator==(const LocKey& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
-};
-struct LocHash {
-  size_t operator()(const LocKey& k) const {
-    uint64_t h = k.b[0] * 0x9E3779B97F4A7C15ull;
-    h ^= k.b[1] + 0x7F4A7C15ull + (h << 6) + (h >> 2);
-    h ^= k.b[2] + 0x94D049BBull + (h << 6) + (h >> 2);
-    return (size_t)h;
-  }
-};
-// sgs.rs:862-865: equal for 0.0 and -0.0.
-LocKey loc_key(double x, double y, double z) {
-  const double v[3] = {x + 0.0, y + 0.0, z + 0.0};
-  LocKey k;
-  std::memcpy(k.b, v, sizeof(v));
-  return k;
-}
-
 }  // namespace
 
 int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precision, int64_t m, const double* gx,
@@ -2496,98 +2503,66 @@ int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precis
     }
   }
   const int64_t n = c->n;
-  // The data's and the nodes' locations and the path on the host: the datum
-  // under each node, the order in which the others join the samples.
-  std::vector<double> X(n), Y(n), Z(n), GX(m), GY(m), GZ(m);
-  std::vector<int64_t> P(m);
-  HIPOK(c, hipMemcpy(X.data(), c->x.p, n * sizeof(double), hipMemcpyDeviceToHost));
-  HIPOK(c, hipMemcpy(Y.data(), c->y.p, n * sizeof(double), hipMemcpyDeviceToHost));
-  HIPOK(c, hipMemcpy(Z.data(), c->z.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  // Inputs on the device.
+  const double *dgx = nullptr, *dgy = nullptr, *dgz = nullptr, *dzin = nullptr;
+  const int64_t* dpath = nullptr;
+  if (int r = stage_in(c, gx, m, c->tx, &dgx)) return r;
+  if (int r = stage_in(c, gy, m, c->ty, &dgy)) return r;
+  if (int r = stage_in(c, gz, m, c->tz, &dgz)) return r;
+  if (int r = stage_in(c, path, m, c->sgs_path, &dpath)) return r;
+  if (int r = stage_in(c, z, m, c->sgs_in, &dzin)) return r;
+  const int64_t M = std::max<int64_t>(m, 1);
+  if (int r = ensure(c, c->sgs_datum, M * sizeof(int64_t))) return r;
+  if (int r = ensure(c, c->sgs_pos, M * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->sgs_app, M * sizeof(int64_t))) return r;
+  if (int r = ensure(c, c->sgs_work, (3 * M + 2) * sizeof(int32_t))) return r;
+  // The datum under each node (the data's grid in the search frame), and the
+  // order in which the other nodes join the samples: their ranks along the path.
+  int64_t m2 = 0;
   if (m > 0) {
-    HIPOK(c, hipMemcpy(GX.data(), gx, m * sizeof(double), hipMemcpyDefault));
-    HIPOK(c, hipMemcpy(GY.data(), gy, m * sizeof(double), hipMemcpyDefault));
-    HIPOK(c, hipMemcpy(GZ.data(), gz, m * sizeof(double), hipMemcpyDefault));
-    HIPOK(c, hipMemcpy(P.data(), path, m * sizeof(int64_t), hipMemcpyDefault));
+    SearchArgs a0;
+    int64_t stride0;
+    if (int r = prepare_search(c, s, &vu, a0, stride0)) return r;
+    a0.sx = (const double*)c->x.p;
+    a0.sy = (const double*)c->y.p;
+    a0.sz = (const double*)c->z.p;
+    int32_t out[2] = {0, 0};
+    if (bk::sgs_order(c->stream, a0, m, dgx, dgy, dgz, dpath, (int64_t*)c->sgs_datum.p, (int32_t*)c->sgs_pos.p,
+                      (int64_t*)c->sgs_app.p, (int32_t*)c->sgs_work.p, &c->sgs_tmp, &c->sgs_tmp_bytes, out))
+      return fail(c, BK_E_HIP, "simulation order failed");
+    if (out[0]) return fail(c, BK_E_INVALID, "path must be a permutation of the nodes");
+    m2 = out[1];
   }
-  std::vector<int32_t> pos(m, -1);
-  for (int64_t k = 0; k < m; ++k) {
-    if (P[k] < 0 || P[k] >= m || pos[P[k]] != -1) return fail(c, BK_E_INVALID, "path must be a permutation of the nodes");
-    pos[P[k]] = 0;
-  }
-  std::unordered_map<LocKey, int64_t, LocHash> first;
-  first.reserve((size_t)n * 2);
-  for (int64_t i = 0; i < n; ++i) first.emplace(loc_key(X[i], Y[i], Z[i]), i);  // keeps the first (datum_index)
-  std::vector<int64_t> datum(m, -1);
-  {
-    std::unordered_map<LocKey, int64_t, LocHash> nodes;
-    nodes.reserve((size_t)m * 2);
-    for (int64_t i = 0; i < m; ++i) {
-      const LocKey key = loc_key(GX[i], GY[i], GZ[i]);
-      if (!nodes.emplace(key, i).second) return fail(c, BK_E_INVALID, "grid nodes must be distinct");
-      auto it = first.find(key);
-      if (it != first.end()) datum[i] = it->second;
-    }
-  }
-  std::vector<int64_t> app;  // nodes not on a datum, in visiting order
-  app.reserve(m);
-  for (int64_t k = 0; k < m; ++k)
-    if (datum[P[k]] < 0) {
-      pos[P[k]] = (int32_t)app.size();
-      app.push_back(P[k]);
-    }
-  const int64_t m2 = (int64_t)app.size();
   const int64_t N = n + m2;
   if (N > 2147483647LL) return fail(c, BK_E_INVALID, "data and nodes beyond 2^31 samples");
-  // search.rs:785-797: the sectors stay planar while every sample added lies
-  // at the first sample's elevation. Nodes [0, n_planar) search planar.
-  bool planar_data = true;
-  for (int64_t i = 1; i < n; ++i) planar_data = planar_data && Z[i] == Z[0];
-  int64_t n_planar = 0;
-  if (planar_data) {
-    while (n_planar < m2 && GZ[app[n_planar]] == Z[0]) ++n_planar;
-    n_planar = std::min<int64_t>(n_planar + 1, m2);
-  }
   // The samples of the search: the data, then the nodes in visiting order.
   for (Buf* b : {&c->sgs_x, &c->sgs_y, &c->sgs_z, &c->sgs_e, &c->sgs_v, &c->sgs_s})
     if (int r = ensure(c, *b, std::max<int64_t>(N, 1) * sizeof(double))) return r;
   if (int r = ensure(c, c->sgs_h, std::max<int64_t>(N, 1) * sizeof(int32_t))) return r;
-  {
-    std::vector<double> ax(m2), ay(m2), az(m2);
-    for (int64_t k = 0; k < m2; ++k) {
-      ax[k] = GX[app[k]];
-      ay[k] = GY[app[k]];
-      az[k] = GZ[app[k]];
-    }
-    auto put = [&](Buf& dst, const Buf& data, const std::vector<double>* nodes, double fill) -> int {
-      HIPOK(c, hipMemcpyAsync(dst.p, data.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      if (m2 > 0) {
-        if (nodes) {
-          HIPOK(c, hipMemcpyAsync((double*)dst.p + n, nodes->data(), m2 * sizeof(double), hipMemcpyHostToDevice,
-                                  c->stream));
-        } else {
-          std::vector<double> f(m2, fill);
-          HIPOK(c, hipMemcpyAsync((double*)dst.p + n, f.data(), m2 * sizeof(double), hipMemcpyHostToDevice,
-                                  c->stream));
-          HIPOK(c, hipStreamSynchronize(c->stream));
-        }
-      }
-      return 0;
-    };
-    if (int r = put(c->sgs_x, c->x, &ax, 0.0)) return r;
-    if (int r = put(c->sgs_y, c->y, &ay, 0.0)) return r;
-    if (int r = put(c->sgs_z, c->z, &az, 0.0)) return r;
-    if (int r = put(c->sgs_v, c->v, nullptr, 0.0)) return r;
-    if (c->has_err)
-      if (int r = put(c->sgs_e, c->e, nullptr, 0.0)) return r;
-    std::vector<int32_t> none(c->has_hole ? m2 : 0, -1);
-    if (c->has_hole) {
-      HIPOK(c, hipMemcpyAsync(c->sgs_h.p, c->hole.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
-      if (m2 > 0)
-        HIPOK(c, hipMemcpyAsync((int32_t*)c->sgs_h.p + n, none.data(), m2 * sizeof(int32_t), hipMemcpyHostToDevice,
-                                c->stream));
-    }
+  if (int r = ensure(c, c->sgs_ctr, 4 * sizeof(int32_t))) return r;
+  for (auto pr : {std::make_pair(&c->sgs_x, &c->x), std::make_pair(&c->sgs_y, &c->y), std::make_pair(&c->sgs_z, &c->z),
+                  std::make_pair(&c->sgs_v, &c->v), std::make_pair(&c->sgs_e, &c->e)})
+    if (pr.first != &c->sgs_e || c->has_err)
+      HIPOK(c, hipMemcpyAsync(pr.first->p, pr.second->p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (c->has_hole)
+    HIPOK(c, hipMemcpyAsync(c->sgs_h.p, c->hole.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+  // search.rs:785-797: the sectors stay planar while every sample added lies
+  // at the first sample's elevation. Nodes [0, n_planar) search planar.
+  int64_t n_planar = 0;
+  if (m2 > 0) {
+    int32_t* first_off = (int32_t*)c->sgs_ctr.p + 3;
+    k_sgs_set<<<1, 1, 0, c->stream>>>(first_off, (int32_t)m2);
+    k_sgs_gather<<<blocks_of(m2), 256, 0, c->stream>>>(m2, n, (const int64_t*)c->sgs_app.p, dgx, dgy, dgz,
+                                                        (double*)c->sgs_x.p, (double*)c->sgs_y.p, (double*)c->sgs_z.p,
+                                                        (double*)c->sgs_v.p, (double*)c->sgs_e.p,
+                                                        c->has_hole ? (int32_t*)c->sgs_h.p : nullptr, first_off);
+    HIPOK(c, hipGetLastError());
+    int32_t first = 0;
+    HIPOK(c, hipMemcpyAsync(&first, first_off, sizeof(first), hipMemcpyDeviceToHost, c->stream));
     HIPOK(c, hipStreamSynchronize(c->stream));
+    if (c->planar) n_planar = std::min<int64_t>((int64_t)first + 1, m2);
   }
+  HIPOK(c, hipStreamSynchronize(c->stream));
   if (!c->sgs)
     if (int r = bk_create(&c->sgs, c->device)) return fail(c, r, std::string("simulation context: ") + g_err);
   bk_ctx* k = c->sgs;
@@ -2655,19 +2630,11 @@ int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precis
   }
   // The draws, in passes.
   if (int r = ensure(c, c->sgs_done, std::max<int64_t>(m2, 1) * sizeof(int32_t))) return r;
-  if (int r = ensure(c, c->sgs_ctr, sizeof(unsigned long long))) return r;
-  if (int r = ensure(c, c->sgs_in, std::max<int64_t>(m, 1) * sizeof(double))) return r;
-  if (int r = ensure(c, c->sgs_datum, std::max<int64_t>(m, 1) * sizeof(int64_t))) return r;
-  if (int r = ensure(c, c->sgs_pos, std::max<int64_t>(m, 1) * sizeof(int32_t))) return r;
   double* sc = (double*)c->sgs_s.p;
   HIPOK(c, hipMemcpyAsync(sc, c->v.p, n * sizeof(double), hipMemcpyDeviceToDevice, k->stream));
-  if (m > 0) {
-    HIPOK(c, hipMemcpyAsync(c->sgs_in.p, z, m * sizeof(double), hipMemcpyDefault, k->stream));
-    HIPOK(c, hipMemcpyAsync(c->sgs_datum.p, datum.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, k->stream));
-    HIPOK(c, hipMemcpyAsync(c->sgs_pos.p, pos.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, k->stream));
-  }
+  int32_t* dup = (int32_t*)c->sgs_ctr.p + 2;
   if (m2 > 0) {
-    if (bk::zero32(k->stream, (int32_t*)c->sgs_done.p, m2) || bk::zero32(k->stream, (int32_t*)c->sgs_ctr.p, 2))
+    if (bk::zero32(k->stream, (int32_t*)c->sgs_done.p, m2) || bk::zero32(k->stream, (int32_t*)c->sgs_ctr.p, 4))
       return fail(c, BK_E_HIP, "fill launch failed");
     unsigned long long drawn = 0;
     int32_t pass = 0;
@@ -2680,14 +2647,19 @@ int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precis
         k_sgs_pass<<<blocks_of(m2), 256, 0, k->stream>>>(m2, n, (int)stride, (const int32_t*)k->nidx.p,
                                                           (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
                                                           (const double*)c->sgs_w.p, (const double*)k->variance.p,
-                                                          (const double*)c->sgs_in.p, sc, (int32_t*)c->sgs_done.p,
-                                                          pass, (unsigned long long*)c->sgs_ctr.p);
+                                                          dzin, (const double*)k->x.p, (const double*)k->y.p,
+                                                          (const double*)k->z.p, sc, (int32_t*)c->sgs_done.p, pass,
+                                                          (unsigned long long*)c->sgs_ctr.p, dup);
       }
       HIPOK(c, hipGetLastError());
       HIPOK(c, hipMemcpyAsync(&drawn, c->sgs_ctr.p, sizeof(drawn), hipMemcpyDeviceToHost, k->stream));
       HIPOK(c, hipStreamSynchronize(k->stream));
     }
     c->sgs_passes = pass;
+    int32_t d = 0;
+    HIPOK(c, hipMemcpyAsync(&d, dup, sizeof(d), hipMemcpyDeviceToHost, k->stream));
+    HIPOK(c, hipStreamSynchronize(k->stream));
+    if (d) return fail(c, BK_E_INVALID, "grid nodes must be distinct");
   }
   if (m > 0) {
     double* osc;

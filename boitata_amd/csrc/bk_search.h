@@ -125,6 +125,14 @@ void free_tiles(TileBuffers& tb);
 int launch_balance(hipStream_t s, const SearchArgs& a, int64_t m);
 // p[0 .. n) = 0 by a kernel (graph-safe).
 int zero32(hipStream_t s, int32_t* p, int64_t n);
+// Sequential simulation bookkeeping over m nodes visited in `path`: datum[i],
+// the first datum at node i's exact location (-1 none; a's grid holds the
+// data); for the nodes not on a datum, pos[node] their rank along the path
+// and app[rank] the node. work: 3 m + 2 ints. out[0]: 1 when the path is not
+// a permutation of 0 .. m-1; out[1]: the nodes not on a datum.
+int sgs_order(hipStream_t s, const SearchArgs& a, int64_t m, const double* gx, const double* gy, const double* gz,
+              const int64_t* path, int64_t* datum, int32_t* pos, int64_t* app, int32_t* work, void** tmp,
+              size_t* tmp_bytes, int32_t out[2]);
 // Searches without caps by `mode`: 0 one wavefront per target (k_knn_wave),
 // 1..8 sorted tiles staging the cells within `mode` cells (k_knn_tile), -1
 // one lane per target (k_knn). Capped searches always run k_search.

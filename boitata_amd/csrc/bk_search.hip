@@ -1880,6 +1880,88 @@ int launch_balance(hipStream_t s, const SearchArgs& a, int64_t m) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// The first datum (lowest index) at each node's exact location: the node
+// projects to the same cell as that datum (the same projection), whose
+// entries are compared in world coordinates.
+__global__ void k_sgs_datum(const SearchArgs a, int64_t m, const double* gx, const double* gy, const double* gz,
+                            int64_t* datum) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double x = gx[i], y = gy[i], z = gz[i];
+  int64_t best = -1;
+  const GridDev& g = a.g;
+  if (g.start) {
+    double q0, q1, q2;
+    mv3(a.F, x - a.origin[0], y - a.origin[1], z - a.origin[2], q0, q1, q2);
+    const int c0 = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
+    const int c1 = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
+    const int c2 = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+    const int id = c0 + g.dims[0] * (c1 + g.dims[1] * c2);
+    for (uint32_t e = g.start[id]; e < g.start[id + 1]; ++e) {
+      const uint32_t k = (uint32_t)__double_as_longlong(g.entries[e].w);
+      if (a.sx[k] == x && a.sy[k] == y && a.sz[k] == z && (best < 0 || (int64_t)k < best)) best = k;
+    }
+  }
+  datum[i] = best;
+}
+
+// flag[k]: path[k] is a node not on a datum; seen[i]: times node i is on
+// the path (bad[0] set when the path is not a permutation).
+__global__ void k_sgs_flags(int64_t m, const int64_t* path, const int64_t* datum, int32_t* flag, int32_t* seen,
+                            int32_t* bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int64_t node = path[k];
+  if (node < 0 || node >= m) {
+    bad[0] = 1;
+    flag[k] = 0;
+    return;
+  }
+  flag[k] = datum[node] < 0 ? 1 : 0;
+  atomicAdd(&seen[node], 1);
+}
+
+__global__ void k_sgs_place(int64_t m, const int64_t* path, const int32_t* flag, const int32_t* scan,
+                            const int32_t* seen, int32_t* pos, int64_t* app, int32_t* bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  if (seen[k] != 1) bad[0] = 1;
+  if (k == m - 1) bad[1] = scan[k] + flag[k];
+  const int64_t node = path[k];
+  if (flag[k] && node >= 0 && node < m) {
+    pos[node] = scan[k];
+    app[scan[k]] = node;
+  }
+}
+
+int sgs_order(hipStream_t s, const SearchArgs& a, int64_t m, const double* gx, const double* gy, const double* gz,
+              const int64_t* path, int64_t* datum, int32_t* pos, int64_t* app, int32_t* work, void** tmp,
+              size_t* tmp_bytes, int32_t out[2]) {
+  if (m <= 0 || m > 2147483647LL) return -1;
+  int32_t* flag = work;
+  int32_t* scan = work + m;
+  int32_t* seen = work + 2 * m;
+  int32_t* bad = work + 3 * m;
+  const unsigned blocks = (unsigned)((m + 255) / 256);
+  if (zero32(s, seen, m) || zero32(s, bad, 2)) return -1;
+  k_sgs_datum<<<dim3(blocks), dim3(256), 0, s>>>(a, m, gx, gy, gz, datum);
+  k_sgs_flags<<<dim3(blocks), dim3(256), 0, s>>>(m, path, datum, flag, seen, bad);
+  size_t need = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, scan, (int)m, s));
+  if (need > *tmp_bytes) {
+    (void)hipFree(*tmp);
+    *tmp = nullptr;
+    *tmp_bytes = 0;
+    HIPCHK(hipMalloc(tmp, need));
+    *tmp_bytes = need;
+  }
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(*tmp, need, flag, scan, (int)m, s));
+  k_sgs_place<<<dim3(blocks), dim3(256), 0, s>>>(m, path, flag, scan, seen, pos, app, bad);
+  HIPCHK(hipMemcpyAsync(out, bad, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 int zero32(hipStream_t s, int32_t* p, int64_t n) {
   if (n <= 0) return 0;
   k_zero32<<<dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s>>>(p, n);
