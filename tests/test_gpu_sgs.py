@@ -121,3 +121,25 @@ def test_invalid_inputs(engine):
     G[3] = G[4]
     with pytest.raises(EngineError):
         engine.sgs(vg, s, G[:, 0], G[:, 1], G[:, 2], np.arange(20), np.zeros(20))
+
+
+def test_host_class_realizations():
+    from boitata_amd.estimation import Search, Variogram
+    from boitata_amd.simulation import SGS
+
+    rng = np.random.default_rng(10)
+    D = rng.uniform(0, 200, (800, 3))
+    s = rng.normal(size=len(D))
+    G = block_grid((16, 16, 6), (5, 5, 5), (195, 195, 195))
+    vg = Variogram([("spherical", 0.8, 90.0)], nugget=0.2)
+    search = Search(ranges=(90.0, 90.0, 90.0), min_samples=2, max_samples=16)
+    sim = SGS(vg, search).fit(D, s)
+    got = sim.simulate(G, n=2, seed=3)
+    d = oracle.Data(D[:, 0], D[:, 1], D[:, 2], s)
+    for r in range(2):
+        g = np.random.default_rng(3 + r)
+        path, z = g.permutation(len(G)), g.normal(size=len(G))
+        want, _ = oracle.sgs(d, vg._c, search._c, G, path, z)
+        assert np.max(np.abs(got[r] - want) / np.maximum(np.abs(want), 1e-3)) <= 1e-9
+    assert not np.array_equal(got[0], got[1])
+    sim.close()
