@@ -161,7 +161,7 @@ struct bk_ctx {
   bk_ctx* sgs = nullptr;
   int sgs_passes = 0;  // draw passes of the last bk_sgs
   Buf sgs_x, sgs_y, sgs_z, sgs_e, sgs_h, sgs_v, sgs_s, sgs_done, sgs_limit, sgs_datum, sgs_pos, sgs_ctr, sgs_in, sgs_w,
-      sgs_path, sgs_app, sgs_work;
+      sgs_path, sgs_app, sgs_work, sgs_prog;
   void* sgs_tmp = nullptr;  // scan storage
   size_t sgs_tmp_bytes = 0;
   // Stage timing (bk_profile): events on the context's stream.
@@ -978,7 +978,7 @@ void bk_destroy(bk_ctx* c) {
                  &c->loo_status, &c->loo_lists, &c->mik_gv, &c->mik_gs, &c->mik_par, &c->mik_cls, &c->mik_tab,
                  &c->mik_w, &c->mik_raw, &c->mik_st, &c->mik_cdf, &c->mik_out, &c->lu_big, &c->sgs_x, &c->sgs_y,
                  &c->sgs_z, &c->sgs_e, &c->sgs_h, &c->sgs_v, &c->sgs_s, &c->sgs_done, &c->sgs_limit, &c->sgs_datum,
-                 &c->sgs_pos, &c->sgs_ctr, &c->sgs_in, &c->sgs_w, &c->sgs_path, &c->sgs_app, &c->sgs_work})
+                 &c->sgs_pos, &c->sgs_ctr, &c->sgs_in, &c->sgs_w, &c->sgs_path, &c->sgs_app, &c->sgs_work, &c->sgs_prog})
     release(*b);
   if (c->sgs) bk_destroy(c->sgs);
   if (c->sgs_tmp) (void)hipFree(c->sgs_tmp);
@@ -2412,43 +2412,68 @@ __global__ void k_sgs_limits(int64_t m2, int64_t n, int32_t* limit) {
   if (k < m2) limit[k] = (int32_t)(n + k);
 }
 
-// One pass of the draws over the nodes in visiting order (k < m2). s: the
-// scores of the data (first n) and of the nodes drawn so far; done[k]: the
-// pass that drew node k, 0 while it waits.
+// One pass of the draws over the nodes of ranks [0, m2) in visiting order.
+// s: the scores of the data (first n) and of the nodes drawn so far; done[k]:
+// the pass that drew node k, 0 while it waits; prog[k]: the neighbour it last
+// waited on (the earlier ones were drawn). A neighbour drawn in this pass may
+// not be visible yet: the node waits for the next pass.
 __global__ void k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx, const int32_t* count,
-                           const int32_t* status, const double* w, const double* variance, const double* z,
-                           const double* x, const double* y, const double* zc, double* s, int32_t* done, int32_t pass,
-                           unsigned long long* drawn, int32_t* dup) {
+                           const int32_t* status, const double* w, const double* variance, const double* z, double* s,
+                           int32_t* done, int32_t* prog, int32_t pass) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= m2 || done[k] != 0) return;
-  const int st = status[k];
-  const int cnt = st == BK_OK ? count[k] : 0;
-  const int32_t* id = nidx + k * (int64_t)stride;
-  // A node at an earlier node's location: that node is the first neighbour
-  // (distance 0); the grid's nodes must be distinct.
-  if (cnt > 0 && id[0] >= n && x[id[0]] == x[n + k] && y[id[0]] == y[n + k] && zc[id[0]] == zc[n + k]) dup[0] = 1;
-  // A neighbour drawn in this pass may not be visible yet: wait for the next.
-  for (int j = 0; j < cnt; ++j) {
-    const int64_t q = (int64_t)id[j] - n;
-    if (q >= 0) {
-      const int32_t d = done[q];
-      if (d == 0 || d >= pass) return;
+  if (k < m2 && done[k] == 0) {
+    const int st = status[k];
+    const int cnt = st == BK_OK ? count[k] : 0;
+    const int32_t* id = nidx + k * (int64_t)stride;
+    int j = prog[k];
+    for (; j < cnt; ++j) {
+      const int64_t q = (int64_t)id[j] - n;
+      if (q >= 0) {
+        const int32_t d = done[q];
+        if (d == 0 || d >= pass) break;
+      }
+    }
+    if (j < cnt) {
+      prog[k] = j;
+    } else {
+      double v;
+      if (cnt > 0) {
+        const double* wk = w + k * (int64_t)stride;
+        double mean = 0.0;
+        for (int q = 0; q < cnt; ++q) mean += wk[q] * s[id[q]];
+        v = mean + sqrt(fmax(variance[k], 0.0)) * z[k];
+      } else if (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA) {
+        v = z[k];  // no neighbours: the marginal (sgs.rs:688-691, 725-728)
+      } else {
+        v = __longlong_as_double(0x7ff8000000000000LL);
+      }
+      s[n + k] = v;
+      done[k] = pass;
     }
   }
-  double v;
-  if (cnt > 0) {
-    const double* wk = w + k * (int64_t)stride;
-    double mean = 0.0;
-    for (int j = 0; j < cnt; ++j) mean += wk[j] * s[id[j]];
-    v = mean + sqrt(fmax(variance[k], 0.0)) * z[k];
-  } else if (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA) {
-    v = z[k];  // no neighbours: the marginal (sgs.rs:688-691, 725-728)
-  } else {
-    v = __longlong_as_double(0x7ff8000000000000LL);
+}
+
+// left[0] += the nodes still waiting (one atomic per block of 256).
+__global__ void __launch_bounds__(256) k_sgs_left(int64_t m2, const int32_t* done, int32_t* left) {
+  __shared__ int32_t part[4];
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int c = __popcll(__ballot(k < m2 && done[k] == 0));
+  if ((threadIdx.x & (BK_WAVE - 1)) == 0) part[threadIdx.x / BK_WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(left, t);
   }
-  s[n + k] = v;
-  done[k] = pass;
-  atomicAdd(drawn, 1ull);
+}
+
+// Node k at an earlier node's location: that node is its first neighbour
+// (distance 0). The grid's nodes must be distinct.
+__global__ void k_sgs_dup(int64_t m2, int64_t n, int stride, const int32_t* nidx, const int32_t* count,
+                          const int32_t* status, const double* x, const double* y, const double* z, int32_t* dup) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= m2 || status[k] != BK_OK || count[k] == 0) return;
+  const int32_t j = nidx[k * (int64_t)stride];
+  if (j >= n && x[j] == x[n + k] && y[j] == y[n + k] && z[j] == z[n + k]) dup[0] = 1;
 }
 
 // Scores and sources in grid order: a node on a datum takes the datum's score
@@ -2630,29 +2655,35 @@ int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precis
   }
   // The draws, in passes.
   if (int r = ensure(c, c->sgs_done, std::max<int64_t>(m2, 1) * sizeof(int32_t))) return r;
+  if (int r = ensure(c, c->sgs_prog, std::max<int64_t>(m2, 1) * sizeof(int32_t))) return r;
   double* sc = (double*)c->sgs_s.p;
   HIPOK(c, hipMemcpyAsync(sc, c->v.p, n * sizeof(double), hipMemcpyDeviceToDevice, k->stream));
   int32_t* dup = (int32_t*)c->sgs_ctr.p + 2;
   if (m2 > 0) {
-    if (bk::zero32(k->stream, (int32_t*)c->sgs_done.p, m2) || bk::zero32(k->stream, (int32_t*)c->sgs_ctr.p, 4))
+    int32_t* ctr = (int32_t*)c->sgs_ctr.p;  // [0]: nodes still waiting after the last pass; [2]: duplicate
+    if (bk::zero32(k->stream, (int32_t*)c->sgs_done.p, m2) || bk::zero32(k->stream, (int32_t*)c->sgs_prog.p, m2) ||
+        bk::zero32(k->stream, ctr, 4))
       return fail(c, BK_E_HIP, "fill launch failed");
-    unsigned long long drawn = 0;
-    int32_t pass = 0;
-    while (drawn < (unsigned long long)m2) {
+    k_sgs_dup<<<blocks_of(m2), 256, 0, k->stream>>>(m2, n, (int)stride, (const int32_t*)k->nidx.p,
+                                                     (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
+                                                     (const double*)k->x.p, (const double*)k->y.p,
+                                                     (const double*)k->z.p, dup);
+    int32_t pass = 0, left = 1;
+    while (left > 0) {
       // Each pass draws at least the first node still waiting (its simulated
       // neighbours come before it), so m2 passes always end.
       if (pass > m2) return fail(c, BK_E_HIP, "simulation passes did not finish");
       for (int q = 0; q < 8; ++q) {
         ++pass;
-        k_sgs_pass<<<blocks_of(m2), 256, 0, k->stream>>>(m2, n, (int)stride, (const int32_t*)k->nidx.p,
-                                                          (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
-                                                          (const double*)c->sgs_w.p, (const double*)k->variance.p,
-                                                          dzin, (const double*)k->x.p, (const double*)k->y.p,
-                                                          (const double*)k->z.p, sc, (int32_t*)c->sgs_done.p, pass,
-                                                          (unsigned long long*)c->sgs_ctr.p, dup);
+        k_sgs_pass<<<blocks_of(m2), 256, 0, k->stream>>>(
+            m2, n, (int)stride, (const int32_t*)k->nidx.p, (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
+            (const double*)c->sgs_w.p, (const double*)k->variance.p, dzin, sc, (int32_t*)c->sgs_done.p,
+            (int32_t*)c->sgs_prog.p, pass);
       }
+      if (bk::zero32(k->stream, ctr, 1)) return fail(c, BK_E_HIP, "fill launch failed");
+      k_sgs_left<<<blocks_of(m2), 256, 0, k->stream>>>(m2, (const int32_t*)c->sgs_done.p, ctr);
       HIPOK(c, hipGetLastError());
-      HIPOK(c, hipMemcpyAsync(&drawn, c->sgs_ctr.p, sizeof(drawn), hipMemcpyDeviceToHost, k->stream));
+      HIPOK(c, hipMemcpyAsync(&left, ctr, sizeof(left), hipMemcpyDeviceToHost, k->stream));
       HIPOK(c, hipStreamSynchronize(k->stream));
     }
     c->sgs_passes = pass;
