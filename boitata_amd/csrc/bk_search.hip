@@ -936,9 +936,13 @@ __device__ __forceinline__ uint64_t group_ballot(bool p, int h) {
   return G == BK_WAVE ? m : (m >> (h * G)) & ((1ull << G) - 1ull);
 }
 
+// Consecutive targets a group of lanes takes in turn. A target usually has
+// its predecessor close by (block models list their blocks row by row), and
+// the predecessor's K-th distance bounds its own.
 // One target per group of G lanes (G = 64, or 32 when max_samples <= 32: two
 // targets per wavefront, whose networks, ballots and broadcasts then serve
-// both at once).
+// both at once). Once K keys are held, a cell whose box lies beyond the K-th
+// key is skipped whole: every entry in it would be rejected one by one.
 template <int G>
 __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
   constexpr int PER = BK_WAVE / G;  // targets per wavefront
@@ -969,6 +973,17 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
     const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
     const double slack = 1.0 - 1e-9;
     const uint32_t n_off = a.tab.end[a.tab.n - 1];
+    // The target's place in its cell, in cells (outside [0, 1) for a target
+    // beyond the grid). A cell dx > 0 cells away along an axis has its near
+    // face dx - fr cells away, one dx < 0 away |dx| - 1 + fr; up and dn are
+    // fr and 1 - fr plus 1e-5 cells for the rounding of the cell assignment
+    // and of these FP32 values, so the box distance is never overestimated.
+    const double ih2 = 1.0 / h2;
+    const float fr0 = (float)((q0 - g.lo[0]) / g.h - (double)c0);
+    const float fr1 = (float)((q1 - g.lo[1]) / g.h - (double)c1);
+    const float fr2 = (float)((q2 - g.lo[2]) / g.h - (double)c2);
+    const float up0 = fr0 + 1e-5f, up1 = fr1 + 1e-5f, up2 = fr2 + 1e-5f;
+    const float dn0 = (1.0f - fr0) + 1e-5f, dn1 = (1.0f - fr1) + 1e-5f, dn2 = (1.0f - fr2) + 1e-5f;
     int nbuf = 0;  // the same in the group's lanes
     auto merge = [&]() {
       if (nbuf <= kSmallBatch) {
@@ -1032,20 +1047,28 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
       const uint32_t thi = __shfl((unsigned)(ak >> 32), K - 1, G);
       const uint64_t tk = ((uint64_t)thi << 32) | tlo;
       const uint32_t ti = __shfl(ai, K - 1, G);
+      const double tcap = tk != KEY_INF ? fmin(a.radius2, __longlong_as_double((long long)tk)) : a.radius2;
       // Lower bound of every offset from o0 on (sorted by m).
       const char4 f0 = a.tab.offsets[o0];
       const int m0 = max(abs(f0.x) - 1, 0) * max(abs(f0.x) - 1, 0) + max(abs(f0.y) - 1, 0) * max(abs(f0.y) - 1, 0) +
                      max(abs(f0.z) - 1, 0) * max(abs(f0.z) - 1, 0);
       const double bound = fmin(h2 * (double)m0, beyond) * slack;
-      if (open && (bound > a.radius2 || (tk != KEY_INF && bound > __longlong_as_double((long long)tk)))) open = false;
+      if (open && bound > tcap) open = false;
       if (!__any(open)) break;
-      // Lane l: cell o0 + l, whose entries are [e, e + cnt) of the CSR.
+      // The threshold in squared cells, rounded up.
+      const float tcc = (float)(tcap * ih2) * (1.0f + 1e-6f);
+      // Lane l: cell o0 + l, whose entries are [e, e + cnt) of the CSR,
+      // unless its box lies beyond the threshold.
       uint32_t e = 0, cnt = 0;
       const uint32_t o = o0 + lane;
       if (open && o < n_off) {
         const char4 f = a.tab.offsets[o];
         const int x = c0 + f.x, y = c1 + f.y, z = c2 + f.z;
-        if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2]) {
+        const float gx = fmaxf((float)abs(f.x) - (f.x > 0 ? up0 : dn0), 0.0f);
+        const float gy = fmaxf((float)abs(f.y) - (f.y > 0 ? up1 : dn1), 0.0f);
+        const float gz = fmaxf((float)abs(f.z) - (f.z > 0 ? up2 : dn2), 0.0f);
+        const bool far = (gx * gx + gy * gy) + gz * gz > tcc;
+        if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2] && !far) {
           const int id = x + g.dims[0] * (y + g.dims[1] * z);
           e = g.start[id];
           cnt = g.start[id + 1] - e;

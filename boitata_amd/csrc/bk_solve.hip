@@ -2608,48 +2608,71 @@ __global__ void __launch_bounds__(64) k_guarded_big(const SolveArgs a, double* l
   }
 }
 
-__global__ void k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus, int32_t* status, double* value,
-                         double* variance, int32_t* lists, int32_t* counts, int64_t list_stride, int e0, int e1,
-                         int e2, int e3, int n_buckets, int stationary) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x % BK_WAVE;
-  const bool live = t < m;
-  int st = live ? sstatus[t] : BK_SEARCH_FAILED;
-  int n = live ? count[t] : 0;
-  int b = -1;
-  if (live && st == BK_OK) {
-    if (n == 0) {
-      st = BK_INSUFFICIENT_DATA;
-    } else if (!stationary || n > BK_MAX_CHOL) {
-      b = n_buckets;  // the LU list
-    } else {
-      int edges[4] = {e0, e1, e2, e3};
-      for (int k = 0; k < n_buckets; ++k)
-        if (n <= edges[k]) {
-          b = k;
-          break;
-        }
+// Targets per thread of k_bucket: a workgroup takes 256 * 8 consecutive
+// targets and adds to each list's global count once, so the counts see a few
+// hundred atomics per million targets instead of one per wavefront.
+constexpr int BUCKET_PER = 8;
+
+__global__ void __launch_bounds__(256) k_bucket(int64_t m, const int32_t* count, const int32_t* sstatus,
+                                                int32_t* status, double* value, double* variance, int32_t* lists,
+                                                int32_t* counts, int64_t list_stride, int e0, int e1, int e2, int e3,
+                                                int n_buckets, int stationary) {
+  __shared__ int s_cnt[5], s_base[5];
+  const int tid = threadIdx.x, lane = tid % BK_WAVE;
+  if (tid < 5) s_cnt[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * 256 * BUCKET_PER;
+  int bk[BUCKET_PER], pos[BUCKET_PER];
+#pragma unroll
+  for (int r = 0; r < BUCKET_PER; ++r) {
+    const int64_t t = base + (int64_t)r * 256 + tid;
+    const bool live = t < m;
+    int st = live ? sstatus[t] : BK_SEARCH_FAILED;
+    const int n = live ? count[t] : 0;
+    int b = -1;
+    if (live && st == BK_OK) {
+      if (n == 0) {
+        st = BK_INSUFFICIENT_DATA;
+      } else if (!stationary || n > BK_MAX_CHOL) {
+        b = n_buckets;  // the LU list
+      } else {
+        const int edges[4] = {e0, e1, e2, e3};
+        for (int k = 0; k < n_buckets; ++k)
+          if (n <= edges[k]) {
+            b = k;
+            break;
+          }
+      }
     }
+    if (live && b < 0) {
+      status[t] = st;
+      value[t] = __longlong_as_double(0x7ff8000000000000LL);
+      variance[t] = __longlong_as_double(0x7ff8000000000000LL);
+    }
+    // Places within the workgroup: one LDS atomic per wavefront and list.
+    int p = -1;
+    for (int k = 0; k <= n_buckets; ++k) {
+      const uint64_t mask = __ballot(b == k);
+      if (mask == 0) continue;
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      int w = 0;
+      if (lane == leader) w = atomicAdd(&s_cnt[k], __popcll(mask));
+      w = __shfl(w, leader, BK_WAVE);
+      if (b == k) p = w + __popcll(mask & ((1ull << lane) - 1ull));
+    }
+    bk[r] = b;
+    pos[r] = p;
   }
-  // One atomic per wave and bucket: the lanes of a bucket take consecutive slots.
-  int pos = -1;
-  for (int k = 0; k <= n_buckets; ++k) {
-    uint64_t mask = __ballot(b == k);
-    if (mask == 0) continue;
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&counts[k], __popcll(mask));
-    base = __shfl(base, leader, BK_WAVE);
-    if (b == k) pos = base + __popcll(mask & ((1ull << lane) - 1ull));
-  }
-  if (!live) return;
-  if (b >= 0) {
-    lists[(int64_t)b * list_stride + pos] = (int32_t)t;
-    status[t] = BK_OK;
-  } else {
-    status[t] = st;
-    value[t] = __longlong_as_double(0x7ff8000000000000LL);
-    variance[t] = __longlong_as_double(0x7ff8000000000000LL);
+  __syncthreads();
+  if (tid <= n_buckets) s_base[tid] = s_cnt[tid] > 0 ? atomicAdd(&counts[tid], s_cnt[tid]) : 0;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < BUCKET_PER; ++r) {
+    const int64_t t = base + (int64_t)r * 256 + tid;
+    if (bk[r] >= 0) {
+      lists[(int64_t)bk[r] * list_stride + s_base[bk[r]] + pos[r]] = (int32_t)t;
+      status[t] = BK_OK;
+    }
   }
 }
 #endif
@@ -2895,9 +2918,10 @@ int launch_bucket(hipStream_t s, int64_t m, const int32_t* count, const int32_t*
   if (m <= 0) return 0;
   int e[4] = {0, 0, 0, 0};
   for (int k = 0; k < n_buckets && k < 4; ++k) e[k] = edges[k];
-  k_bucket<<<dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s>>>(m, count, sstatus, status, value, variance, lists,
-                                                                     counts, list_stride, e[0], e[1], e[2], e[3],
-                                                                     n_buckets, stationary);
+  const int64_t per_block = 256 * BUCKET_PER;
+  k_bucket<<<dim3((unsigned)((m + per_block - 1) / per_block)), dim3(256), 0, s>>>(
+      m, count, sstatus, status, value, variance, lists, counts, list_stride, e[0], e[1], e[2], e[3], n_buckets,
+      stationary);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 #endif
