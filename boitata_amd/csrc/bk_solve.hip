@@ -521,6 +521,34 @@ __device__ __forceinline__ float group_maxf(float v) {
   return v;
 }
 
+// Convergence of the refinement. Sweep k adds a correction d_k of relative
+// size delta_k = max|d_k| / max|x_0|; the corrections still to come add up to
+// about delta_k * rho, rho the contraction per sweep, ~ ||I - X C|| for the
+// FP32 inverse X of C / c0. The first sweep estimates rho as the larger of
+// delta_1 (the first correction is (I - X C) applied to the solution) and
+// n u / d_min (u = 2^-24, d_min the smallest pivot of the elimination, which
+// bounds the smallest eigenvalue of C / c0 from above), and converges when
+// delta_1 * rho <= tol: a well-conditioned system (C2: d_min >= 0.1 with its
+// 0.1 nugget, delta_1 ~ 1e-5) stops after one sweep with a predicted
+// remainder below 1e-9, an ill-conditioned one (d_min ~ 1e-4, rho ~ 0.1)
+// sweeps on. Later sweeps converge when delta_k <= tol, as a contraction
+// measured from two corrections of an ill-conditioned system can understate
+// the slow modes.
+constexpr float kUnitRoundoff32 = 5.9604645e-8f;
+
+// A lane's correction relative to max|x_0| (0 where x_0 = 0); delta_k is
+// the group's largest over the right-hand sides.
+__device__ __forceinline__ float rel_corr(float d, float m) {
+  return m > 0.0f ? fabsf(d) * __builtin_amdgcn_rcpf(m) : 0.0f;
+}
+
+// The first sweep's test for a system of at most nmax samples (rho_p = nmax
+// u / d_min; false for a NaN or a non-positive pivot).
+__device__ __forceinline__ bool first_sweep_converged(float delta, float nmax, float dmin, float tol) {
+  const float rho_p = nmax * kUnitRoundoff32 / dmin;
+  return dmin > 0.0f && delta * fmaxf(delta, rho_p) <= tol;
+}
+
 // LDS layout of one system of the refined form: per sample j a record of SJ
 // doubles (rec_doubles), iterates (NMAX double2), pivot rows (2 x NMAX
 // floats), product vectors (NMAX float2), the FP64 matrix as a packed strict
@@ -916,7 +944,7 @@ template <int NMAX>
 __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, int r, int g, int lane, bool active,
                                               bool report_guard, double* PJ, int SJ, float* P2, double* T,
                                               double* RowL, double& z, double& e, double& c, float (&x)[NMAX],
-                                              double (&crow)[NMAX <= 32 ? NMAX : 1], bool& bad) {
+                                              double (&crow)[NMAX <= 32 ? NMAX : 1], bool& bad, float& dmin) {
   constexpr bool ROWREG = NMAX <= 32;
   const bool guard = refined_pairs<NMAX>(a, t, n, r, g, active, report_guard, PJ, SJ, T, z, e, c);
   const bool go = active && !guard;
@@ -946,6 +974,7 @@ __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, 
   // the right-hand-side buffer after it (4 NMAX floats), which the solves use
   // only after the elimination.
   bad = false;
+  dmin = 3.4e38f;
   float* rows = P2;
 #pragma unroll
   for (int k = 0; k < NMAX; k += 4) {
@@ -971,6 +1000,7 @@ __device__ __forceinline__ bool refined_front(const SolveArgs& a, int t, int n, 
     for (int p = 0; p < 4; ++p) {
       const float pv = P[p][p];
       bad |= !(pv > 1e-5f);
+      dmin = fminf(dmin, pv);
       const float ip = __builtin_amdgcn_rcpf(pv);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
@@ -1047,8 +1077,9 @@ __device__ __forceinline__ void krige32_blk(const SolveArgs& a, int64_t nl, int 
   double crow[ROWREG ? NMAX : 1];
   float x[NMAX];
   bool bad;
+  float dmin;
   const bool guard =
-      refined_front<NMAX>(a, t, n, r, g, lane, active, true, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
+      refined_front<NMAX>(a, t, n, r, g, lane, active, true, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad, dmin);
   const bool go = active && !guard;
   const bool live = go && r < n;
   const double c0 = a.v.c0;
@@ -1103,7 +1134,15 @@ __device__ __forceinline__ void krige32_blk(const SolveArgs& a, int64_t nl, int 
     apply((float)(ra * ic0), (float)(rb * ic0), da, db);
     xa += (double)da;
     xb += (double)db;
-    if (group_all<NMAX>(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb, g)) {
+    bool done;
+    if (it == 0) {
+      const float delta = group_maxf<NMAX>(fmaxf(rel_corr(da, ma), rel_corr(db, mb)));
+      done = group_all<NMAX>(!__builtin_isnan(da) && !__builtin_isnan(db), g) &&
+             first_sweep_converged(delta, (float)NMAX, dmin, tolf);
+    } else {
+      done = group_all<NMAX>(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb, g);
+    }
+    if (done) {
       conv = true;
       ++it;
       break;
@@ -1268,9 +1307,14 @@ __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int
     dmin1 = min(dmin1, b1);
     const float inv = __builtin_amdgcn_rcpf(__int_as_float(g ? b1 : b0));
     // Operand entry i of lane 32 b + i: -u_i (from row k by the sign rule)
-    // and v_i.
-    const float u = r == k ? inv - 1.0f : (r < k ? row * inv : -row * inv);
-    const float v = r == k ? row + 1.0f : row;
+    // and v_i. The lane's offset from the pivot is opaque, so that these are
+    // selects rather than branches on 32 pivots' lane masks.
+    int dk = r - k;
+    asm volatile("" : "+v"(dk));
+    const float sg = dk < 0 ? inv : -inv;
+    float u = row * sg;
+    u = dk == 0 ? inv - 1.0f : u;
+    const float v = dk == 0 ? row + 1.0f : row;
     X = __builtin_amdgcn_mfma_f32_32x32x1f32(u, v, X, 0, 0, 0);
   }
   const bool bad0 = !(dmin0 > __float_as_int(1e-5f)), bad1 = !(dmin1 > __float_as_int(1e-5f));
@@ -1351,8 +1395,21 @@ __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int
       xa[b] += (double)da[b];
       xb[b] += (double)db[b];
     }
-    const bool small0 = __ballot(!(fabsf(da[0]) <= tolf * ma[0] && fabsf(db[0]) <= tolf * mb[0])) == 0;
-    const bool small1 = __ballot(!(fabsf(da[1]) <= tolf * ma[1] && fabsf(db[1]) <= tolf * mb[1])) == 0;
+    // Every lane holds entry r of both systems' corrections (rows 0..31 twice).
+    bool small0, small1;
+    if (it == 0) {
+      // Half g reduces system g's largest corrections.
+      const float delta = group_maxf<32>(fmaxf(rel_corr(g ? da[1] : da[0], g ? ma[1] : ma[0]),
+                                               rel_corr(g ? db[1] : db[0], g ? mb[1] : mb[0])));
+      const bool done = first_sweep_converged(delta, 32.0f, __int_as_float(g ? dmin1 : dmin0), tolf);
+      const bool nan0 = __ballot(__builtin_isnan(da[0]) || __builtin_isnan(db[0])) != 0;
+      const bool nan1 = __ballot(__builtin_isnan(da[1]) || __builtin_isnan(db[1])) != 0;
+      small0 = !nan0 && __builtin_amdgcn_readlane(done ? 1 : 0, 0) != 0;
+      small1 = !nan1 && __builtin_amdgcn_readlane(done ? 1 : 0, 32) != 0;
+    } else {
+      small0 = __ballot(!(fabsf(da[0]) <= tolf * ma[0] && fabsf(db[0]) <= tolf * mb[0])) == 0;
+      small1 = __ballot(!(fabsf(da[1]) <= tolf * ma[1] && fabsf(db[1]) <= tolf * mb[1])) == 0;
+    }
     if (more0) {
       it0 = it + 1;
       conv0 = small0;
@@ -1550,7 +1607,15 @@ __device__ __forceinline__ int2 krige64m_blk(const SolveArgs& a, int64_t nl, int
     apply((float)((ba - ca) * ic0), (float)((bb - cb) * ic0), da, db);
     xa += (double)da;
     xb += (double)db;
-    if (__ballot(!(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb)) == 0) {
+    bool done;
+    if (it == 0) {
+      const float delta = group_maxf<64>(fmaxf(rel_corr(da, ma), rel_corr(db, mb)));
+      done = __ballot(__builtin_isnan(da) || __builtin_isnan(db)) == 0 &&
+             first_sweep_converged(delta, 64.0f, __int_as_float(dmin), tolf);
+    } else {
+      done = __ballot(!(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb)) == 0;
+    }
+    if (done) {
       conv = true;
       ++it;
       break;
@@ -2035,8 +2100,9 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
   double crow[ROWREG ? NMAX : 1];
   float x[NMAX];
   bool bad;
+  float dmin;
   const bool guard =
-      refined_front<NMAX>(a, t, n, r, g, lane, active, false, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad);
+      refined_front<NMAX>(a, t, n, r, g, lane, active, false, PJ, SJ, P2, T, RowL, z, e, c, x, crow, bad, dmin);
   const bool go = active && !guard;
   const bool live = go && r < n;
   const double c0 = a.v.c0;
@@ -2091,6 +2157,8 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
     xa += (double)da;
     xb += (double)db;
     xz += (double)dz;
+    // The gradient's forward solve keeps the plain test, delta_k <= tol (its
+    // registers are at the kernel's limit).
     if (group_all<NMAX>(fabsf(da) <= tolf * ma && fabsf(db) <= tolf * mb && fabsf(dz) <= tolf * mz, g)) {
       conv = true;
       break;

@@ -64,8 +64,12 @@ enum { BK_ORDINARY = 0, BK_SIMPLE = 1, BK_INDICATOR = 2 };
  *   systems above 64 samples and power models take the bordered LU.
  * BK_FP32_IR: covariances in FP64 (a fast exp and square root), an FP32
  *   Gauss-Jordan inverse of C / c0 on the matrix cores (MFMA), FP64 residual
- *   refinement until the correction is below the tolerance, and the Schur
- *   scalar and weights in FP64. Size buckets above the refine limit
+ *   refinement, and the Schur scalar and weights in FP64. The refinement stops
+ *   when the correction still to come is predicted below the tolerance
+ *   (relative to the first iterate; 1e-9 by default): after the first sweep
+ *   from its correction delta and the contraction estimated as
+ *   max(delta, n 2^-24 / smallest pivot), after later sweeps from their own
+ *   correction. Size buckets above the refine limit
  *   (bk_set_refine_limit), variograms with a circular, sine-hole, Matern or
  *   power structure, and systems whose FP32 pivot falls below 1e-5 of the
  *   scaled sill or that do not converge take the FP64 path. */
