@@ -310,9 +310,11 @@ __device__ __forceinline__ ExpConstants exp_constants() {
   return k;
 }
 
-__device__ __forceinline__ double exp_fast(double x, const ExpConstants& k) {
+// exp(x) for finite x >= -745 (below that ldexp returns 0 or the smallest
+// subnormal): no range test, so a caller that clamps its argument (a
+// covariance's exp of minus a finite lag) has no branch.
+__device__ __forceinline__ double exp_core(double x, const ExpConstants& k) {
 #pragma clang fp contract(fast)
-  if (x < -708.0) return 0.0;
   double n = rint(x * k.log2e);
   double r = (x - n * k.ln2_hi) - n * k.ln2_lo;
   double p = k.c[0];
@@ -322,6 +324,11 @@ __device__ __forceinline__ double exp_fast(double x, const ExpConstants& k) {
   p = p * r + 1.0;
   p = p * r + 1.0;
   return ldexp(p, (int)n);
+}
+
+__device__ __forceinline__ double exp_fast(double x, const ExpConstants& k) {
+  if (x < -708.0) return 0.0;
+  return exp_core(x, k);
 }
 
 // sqrt(x) for x > 0 (x = 0 gives NaN), from the hardware reciprocal square

@@ -1204,11 +1204,31 @@ k_krige32(const SolveArgs a) {
 // i > k. A pivot costs about a dozen VALU instructions besides the matrix
 // instruction, where a lane-per-row sweep issues 32 fused multiply-adds per
 // row.
-__host__ __device__ constexpr int sysm_doubles(int nf) {
-  // Records of both systems (later the iterates and product vectors: 160
-  // doubles), then the two packed triangles.
-  return (2 * 32 * rec_doubles(nf) > 160 ? 2 * 32 * rec_doubles(nf) : 160) + 2 * 528;
+//
+// The pair phase (pairs_q32) keeps each sample's projections on the frames,
+// relative to the target, as a record of rec_q(nf) doubles (an odd count, so
+// that lanes reading consecutive records spread over the banks). Lane r takes
+// the pairs (r, (r + d) mod 32), d = 1 .. 16: the same 32-sample circulant in
+// every system whatever its n. The FP64 matrix of a system is Q[d][k] = C(k,
+// (k + d) mod 32) with row stride QS (Q[0][k] the diagonal): a lane writes
+// row d at its own column, so the stores have an immediate offset per d and
+// no bank conflicts. Entry (i, j) with delta = (j - i) mod 32 lies at
+// Q[delta][i] for delta < 16 and at Q[32 - delta][j] otherwise (Q[16][k] and
+// Q[16][k + 16] hold the same pair). Samples beyond n, and every sample of an
+// inactive system, get sentinel records 1e30 (lane + 1) in every coordinate:
+// their pairs are finite and never near, and their entries are reset to the
+// identity's (0 off the diagonal, c0 on it) after the pairs.
+__host__ __device__ constexpr int rec_q(int nf) { return (3 * nf) | 1; }
+constexpr int kQS = 35;           // row stride of Q in doubles (odd: column reads spread over the banks)
+constexpr int kQSys = 17 * kQS;   // doubles of one system's Q (rows d = 0 .. 16)
+
+__host__ __device__ constexpr int qrec_doubles(int nf) {
+  // Records of both systems; later the iterates and product vectors (192
+  // doubles).
+  return 2 * 32 * rec_q(nf) > 192 ? 2 * 32 * rec_q(nf) : 192;
 }
+
+__host__ __device__ constexpr int sysm_doubles(int nf) { return qrec_doubles(nf) + 2 * kQSys; }
 
 typedef float bk_floatx32 __attribute__((ext_vector_type(32)));
 
@@ -1236,62 +1256,205 @@ __device__ __forceinline__ double other_half(double v, int h) {
   return __longlong_as_double((long long)(((unsigned long long)u << 32) | l));
 }
 
+// One structure's contributions to eight of the lane's pairs (d = D0 + 1 ..
+// D0 + 8) in pairs_q32: acc[dd] += f(l2), l2 the squared lag in the
+// structure's frame (at `off` in the records); with `first`, the near test on
+// that frame.
+template <class Contrib>
+__device__ __forceinline__ void q_pass(const double* me, const uint32_t (&pk)[4], int off, bool first,
+                                       bool& near, double near2, double (&acc)[8], Contrib f) {
+#pragma clang fp contract(fast)
+  extern __shared__ double dyn[];
+  const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
+#pragma unroll
+  for (int dd = 0; dd < 8; ++dd) {
+    // The partner's record: its offset in dynamic LDS, unpacked from pk in an
+    // instruction of its own per structure, so that the offsets do not stay
+    // live across the structures (pk holds two per register).
+    uint32_t o;
+    if (dd % 2 == 0)
+      asm volatile("v_and_b32 %0, 0xffff, %1" : "=v"(o) : "v"(pk[dd / 2]));
+    else
+      asm volatile("v_lshrrev_b32 %0, 16, %1" : "=v"(o) : "v"(pk[dd / 2]));
+    const double* p = reinterpret_cast<const double*>(reinterpret_cast<const char*>(dyn) + o) + off;
+    const double d0 = q0 - p[0], d1 = q1 - p[1], d2 = q2 - p[2];
+    const double l2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (first) near |= l2 <= near2;
+    acc[dd] += f(l2);
+    if (dd % 4 == 3) asm volatile("" ::: "memory");
+  }
+}
+
+// Pairs d = D0 + 1 .. D0 + 8 of one system of at most 32 samples (records at
+// R, lane r on sample r) into Q rows D0 + 1 .. D0 + 8: C = c0 - (nugget + sum
+// of sill * shape), in the fast FP64 forms with the constants folded per
+// structure (exponential and Gaussian: c0 - nugget - sill + sill e^x, x
+// clamped at -745; spherical: sill h (1.5 / a - 0.5 l2 / a^3) below the
+// range). Returns whether a pair of the lane lies within near2 in the frame
+// of structure 0.
+template <int D0>
+__device__ __forceinline__ bool pairs_q32_part(const DevVariogram& v, const double* R, int SQ, int r, double* Qg) {
+#pragma clang fp contract(fast)
+  extern __shared__ double dyn[];
+  const double* me = R + r * SQ;
+  // Byte offsets of the partners' records in dynamic LDS (below 64 KiB), two
+  // per register.
+  uint32_t pk[4];
+  {
+    const uint32_t base = (uint32_t)(reinterpret_cast<const char*>(R) - reinterpret_cast<const char*>(dyn));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t lo = base + (uint32_t)(((r + D0 + 2 * i + 1) & 31) * SQ) * 8u;
+      const uint32_t hi = base + (uint32_t)(((r + D0 + 2 * i + 2) & 31) * SQ) * 8u;
+      pk[i] = lo | (hi << 16);
+    }
+  }
+  double acc[8];
+#pragma unroll
+  for (int dd = 0; dd < 8; ++dd) acc[dd] = 0.0;
+  bool near = false;
+  const double near2 = v.near2;
+  double K = v.c0 - v.nugget;
+  const ExpConstants k = exp_constants();
+  for (int st = 0; st < v.S; ++st) {
+    const int off = 3 * v.frame[st];
+    const double sill = v.sill[st], ir = v.inv_range[st];
+    const bool first = st == 0;
+    switch (v.model[st]) {
+      case BK_SPHERICAL: {
+        const double A = 1.5 * sill * ir, B = 0.5 * sill * (ir * ir * ir), a2 = 1.0 / (ir * ir);
+        q_pass(me, pk, off, first, near, near2, acc, [=](double l2) {
+          const double h = sqrt_pos(l2);
+          return l2 >= a2 ? -sill : -(h * (A - B * l2));
+        });
+        break;
+      }
+      case BK_EXPONENTIAL: {
+        K -= sill;
+        const double m = -3.0 * ir;
+        q_pass(me, pk, off, first, near, near2, acc,
+               [=, &k](double l2) { return sill * exp_core(fmax(sqrt_pos(l2) * m, -745.0), k); });
+        break;
+      }
+      case BK_GAUSSIAN: {
+        K -= sill;
+        const double m = -3.0 * ir * ir;
+        q_pass(me, pk, off, first, near, near2, acc,
+               [=, &k](double l2) { return sill * exp_core(fmax(l2 * m, -745.0), k); });
+        break;
+      }
+      case BK_CUBIC:
+        q_pass(me, pk, off, first, near, near2, acc, [=](double l2) {
+          const double t = sqrt_pos(l2) * ir, t2 = t * t;
+          return t >= 1.0 ? -sill : -sill * (t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2))));
+        });
+        break;
+      case BK_PENTASPHERICAL:
+        q_pass(me, pk, off, first, near, near2, acc, [=](double l2) {
+          const double t = sqrt_pos(l2) * ir, t2 = t * t;
+          return t >= 1.0 ? -sill : -sill * (t * (1.875 + t2 * (-1.25 + 0.375 * t2)));
+        });
+        break;
+      default:
+        // No fast form (the host sends such variograms to the FP64 kernel).
+        q_pass(me, pk, off, first, near, near2, acc, [](double) { return __builtin_nan(""); });
+    }
+  }
+#pragma unroll
+  for (int dd = 0; dd < 8; ++dd) Qg[(D0 + dd + 1) * kQS + r] = K + acc[dd];
+  return near;
+}
+
+// The pairs of one system (see pairs_q32_part), in two halves of eight
+// offsets to bound the registers.
+__device__ __forceinline__ bool pairs_q32(const DevVariogram& v, const double* R, int SQ, int r, double* Qg) {
+  const bool n0 = pairs_q32_part<0>(v, R, SQ, r, Qg);
+  const bool n1 = pairs_q32_part<8>(v, R, SQ, r, Qg);
+  return n0 || n1;
+}
+
+// Offset in a system's Q of entry (i, j) (see pairs_q32).
+__device__ __forceinline__ int q_entry(int i, int j) {
+  const int dl = (j - i) & 31;
+  return dl >= 1 && dl <= 15 ? dl * kQS + i : ((32 - dl) & 31) * kQS + j;
+}
+
 // Returns the systems this unit solved by refinement and their sweeps (x, y),
 // the same in every lane.
 __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int unit) {
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
-  const int SJ = rec_doubles(nf);
+  const int SQ = rec_q(nf);
   const int tid = opaque_tid();
   const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
   const int g = lane >> 5, r = lane & 31;
   double* S = dyn + wave * sysm_doubles(nf);
-  double* T = S + 2 * 32 * SJ;  // two packed strict lower triangles of 528 doubles
+  double* Q = S + qrec_doubles(nf);  // the two systems' matrices, kQSys doubles each
+  double* Qg = Q + g * kQSys;
   const int64_t sys = (int64_t)unit * 2 + g;
   const bool active = sys < nl;
   const int t = active ? a.list[sys] : 0;
   const int n = active ? a.ncount[t] : 0;
-  double z, e, c;
-  // A system with a near pair goes to the FP64 kernel (exact coincidence test).
-  const bool near =
-      refined_pairs<32, true>(a, t, n, r, g, active, false, S + g * 32 * SJ, SJ, T + g * 528, z, e, c);
-  const bool guard = false;
-  const bool go = active && !near;
+  const bool mine = active && r < n;
   const double c0 = a.v.c0;
   const double ic0 = a.v.inv_c0;
-  const bool go0 = __builtin_amdgcn_readlane(go ? 1 : 0, 0) != 0, go1 = __builtin_amdgcn_readlane(go ? 1 : 0, 32) != 0;
-  // The diagonal, and identity rows beyond n (all rows of a system not solved
-  // here), complete the packed triangle of each system.
+  double z = 0.0, e = 0.0, c = 0.0;
   {
-    double* Tg = T + g * 528;
-    const int tr = r * (r + 1) / 2;
-    if (go && r < n) {
-      Tg[tr + r] = c0 + e;
-    } else {
-#pragma unroll
-      for (int j = 0; j < 32; ++j)
-        if (j <= r) Tg[tr + j] = j == r ? c0 : 0.0;
+#pragma clang fp contract(fast)
+    // Sample r: its value, error variance, and record (kernel.rs:36-42).
+    const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    if (mine) {
+      const int id = a.nidx[(int64_t)t * a.stride + r];
+      wx = a.sx[id];
+      wy = a.sy[id];
+      wz = a.sz[id];
+      z = a.sv[id];
+      e = a.serr ? a.serr[id] : 0.0;
     }
+    const double d0 = wx - t0, d1 = wy - t1, d2 = wz - t2;
+    const double sentinel = 1e30 * (double)(lane + 1);
+    double* me = S + (g * 32 + r) * SQ;
+    double h[BK_MAX_FRAMES];
+    for (int f = 0; f < nf; ++f) {
+      const double* F = a.v.F[f];
+      const bool id = a.v.ident[f];
+      const double p0 = id ? d0 : F[0] * d0 + F[1] * d1 + F[2] * d2;
+      const double p1 = id ? d1 : F[3] * d0 + F[4] * d1 + F[5] * d2;
+      const double p2 = id ? d2 : F[6] * d0 + F[7] * d1 + F[8] * d2;
+      me[3 * f] = mine ? p0 : sentinel;
+      me[3 * f + 1] = mine ? p1 : sentinel;
+      me[3 * f + 2] = mine ? p2 : sentinel;
+      h[f] = norm3_fast(p0, p1, p2);
+    }
+    // The right-hand side: the covariance to the target (or to the block).
+    if (mine) c = a.block ? rhs_fast_world(a, t0, t1, t2, wx, wy, wz) : cov_fast64(a.v, h);
   }
   wave_sync();
-  // Column r of both systems' matrices at the lane's rows i = i0 + 4 g
-  // (i0 = 8 q + p): entry (i, r) of the packed triangle, at tri(i) + r for
-  // i >= r and tri(r) + i otherwise, with tri(i0 + 4 g) = tri(i0) + g (4 i0 + 10).
-  // The triangles stay in LDS for the residuals, read through these offsets.
+  // A system with a near pair goes to the FP64 kernel (exact coincidence test).
+  const bool near = group_any<32>(pairs_q32(a.v, S + g * 32 * SQ, SQ, r, Qg), g);
+  const bool go = active && !near;
+  const bool go0 = __builtin_amdgcn_readlane(go ? 1 : 0, 0) != 0, go1 = __builtin_amdgcn_readlane(go ? 1 : 0, 32) != 0;
+  // The diagonal; entries of samples beyond n take the identity's.
+  Qg[r] = mine ? c0 + e : c0;
+  if (__builtin_amdgcn_readlane(n, 0) < 32 || __builtin_amdgcn_readlane(n, 32) < 32) {
+#pragma unroll
+    for (int dd = 1; dd <= 16; ++dd)
+      if (!(r < n && ((r + dd) & 31) < n)) Qg[dd * kQS + r] = 0.0;
+  }
+  wave_sync();
+  // Column r of both systems' matrices at the lane's rows i = 8 q + 4 g + p:
+  // entry (i, r) of Q, whose offsets the residuals reuse.
   int idx[16];
   bk_floatx32 X;
-  {
-    const int tr = r * (r + 1) / 2 + 4 * g, rg = r - 4 * g;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int p = 0; p < 4; ++p) {
-        const int i0 = 8 * q + p;
-        idx[4 * q + p] = i0 >= rg ? r + g * (4 * i0 + 10) + i0 * (i0 + 1) / 2 : tr + i0;
+    for (int p = 0; p < 4; ++p) {
+      idx[4 * q + p] = q_entry(8 * q + 4 * g + p, r);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) X[16 * b + 4 * q + p] = (float)(T[b * 528 + idx[4 * q + p]] * ic0);
-      }
-  }
+      for (int b = 0; b < 2; ++b) X[16 * b + 4 * q + p] = (float)(Q[b * kQSys + idx[4 * q + p]] * ic0);
+    }
   // The smallest pivot of each system as an integer over the float bits: for
   // floats >= 0 the order is the same, a negative pivot is a negative integer,
   // and a NaN pivot fails the convergence test later.
@@ -1380,7 +1543,7 @@ __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           const double2 xv = reinterpret_cast<const double2*>(X2)[32 * b + 8 * q + 4 * g + p];
-          const double cv = T[b * 528 + idx[4 * q + p]];
+          const double cv = Q[b * kQSys + idx[4 * q + p]];
           sa = fma(cv, xv.x, sa);
           sb = fma(cv, xv.y, sb);
         }
