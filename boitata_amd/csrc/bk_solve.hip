@@ -1379,9 +1379,53 @@ __device__ __forceinline__ int q_entry(int i, int j) {
   return dl >= 1 && dl <= 15 ? dl * kQS + i : ((32 - dl) & 31) * kQS + j;
 }
 
+// Gauss-Jordan pivots K .. 31 of both systems in the accumulator of
+// v_mfma_f32_32x32x1_2b_f32 (see krige32m_blk), one rank-one update each;
+// dmin takes the smallest pivot of the lane's system (float bits as an
+// integer).
+template <int K>
+__device__ __forceinline__ void gj_pivots32(bk_floatx32& X, int& dmin) {
+  constexpr int rk = 4 * (K / 8) + (K % 4);
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[rk]), __float_as_uint(X[16 + rk]), false, false);
+  const float row = __uint_as_float(((K / 4) % 2) ? p[1] : p[0]);
+  // The pivot of the lane's system, lane K of its half (a swizzle broadcast
+  // within 32 lanes: and-mask 0, or-mask K).
+  const int pb = __builtin_amdgcn_ds_swizzle(__float_as_int(row), K << 5);
+  dmin = min(dmin, pb);
+  const float inv = __builtin_amdgcn_rcpf(__int_as_float(pb));
+  // Operand entry i of lane 32 b + i: -u_i (from row K by the sign rule) and
+  // v_i. The lane's offset from the pivot is opaque, so that these are
+  // selects rather than branches on the pivots' lane masks.
+  int dk = (int)(threadIdx.x & 31) - K;
+  asm volatile("" : "+v"(dk));
+  const float sg = dk < 0 ? inv : -inv;
+  float u = row * sg;
+  u = dk == 0 ? inv - 1.0f : u;
+  const float v = dk == 0 ? row + 1.0f : row;
+  X = __builtin_amdgcn_mfma_f32_32x32x1f32(u, v, X, 0, 0, 0);
+  if constexpr (K + 1 < 32) gj_pivots32<K + 1>(X, dmin);
+}
+
 // Returns the systems this unit solved by refinement and their sweeps (x, y),
 // the same in every lane.
-__device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int unit) {
+// Byte offset in dynamic LDS of entry k of the lane's packed offsets (two
+// 16-bit offsets per register), unpacked by an instruction of its own at each
+// use so that the sixteen offsets stay packed across the loop over units.
+__device__ __forceinline__ uint32_t unpack_off(const uint32_t (&pk)[8], int k) {
+  uint32_t o;
+  if (k % 2 == 0)
+    asm volatile("v_and_b32 %0, 0xffff, %1" : "=v"(o) : "v"(pk[k / 2]));
+  else
+    asm volatile("v_lshrrev_b32 %0, 16, %1" : "=v"(o) : "v"(pk[k / 2]));
+  return o;
+}
+
+__device__ __forceinline__ const double& lds_at(uint32_t byte_off) {
+  extern __shared__ double dyn[];
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(dyn) + byte_off);
+}
+
+__device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int unit, const uint32_t (&ipk)[8]) {
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
   const int SQ = rec_q(nf);
@@ -1444,42 +1488,21 @@ __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int
   }
   wave_sync();
   // Column r of both systems' matrices at the lane's rows i = 8 q + 4 g + p:
-  // entry (i, r) of Q, whose offsets the residuals reuse.
-  int idx[16];
+  // entry (i, r) of Q at idx (the same in every unit), also read by the
+  // residuals.
   bk_floatx32 X;
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int k = 0; k < 16; ++k) {
+    const double* e0 = &lds_at(unpack_off(ipk, k));
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      idx[4 * q + p] = q_entry(8 * q + 4 * g + p, r);
-#pragma unroll
-      for (int b = 0; b < 2; ++b) X[16 * b + 4 * q + p] = (float)(Q[b * kQSys + idx[4 * q + p]] * ic0);
-    }
+    for (int b = 0; b < 2; ++b) X[16 * b + k] = (float)(e0[b * kQSys] * ic0);
+  }
   // The smallest pivot of each system as an integer over the float bits: for
   // floats >= 0 the order is the same, a negative pivot is a negative integer,
   // and a NaN pivot fails the convergence test later.
-  int dmin0 = 0x7f7fffff, dmin1 = 0x7f7fffff;
-#pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    const int rk = 4 * (k / 8) + (k % 4);
-    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[rk]), __float_as_uint(X[16 + rk]), false, false);
-    const float row = __uint_as_float(((k / 4) % 2) ? p[1] : p[0]);
-    const int b0 = __builtin_amdgcn_readlane(__float_as_int(row), k);
-    const int b1 = __builtin_amdgcn_readlane(__float_as_int(row), 32 + k);
-    dmin0 = min(dmin0, b0);
-    dmin1 = min(dmin1, b1);
-    const float inv = __builtin_amdgcn_rcpf(__int_as_float(g ? b1 : b0));
-    // Operand entry i of lane 32 b + i: -u_i (from row k by the sign rule)
-    // and v_i. The lane's offset from the pivot is opaque, so that these are
-    // selects rather than branches on 32 pivots' lane masks.
-    int dk = r - k;
-    asm volatile("" : "+v"(dk));
-    const float sg = dk < 0 ? inv : -inv;
-    float u = row * sg;
-    u = dk == 0 ? inv - 1.0f : u;
-    const float v = dk == 0 ? row + 1.0f : row;
-    X = __builtin_amdgcn_mfma_f32_32x32x1f32(u, v, X, 0, 0, 0);
-  }
+  int dmin = 0x7f7fffff;  // of the lane's system
+  gj_pivots32<0>(X, dmin);
+  const int dmin0 = __builtin_amdgcn_readlane(dmin, 0), dmin1 = __builtin_amdgcn_readlane(dmin, 32);
   const bool bad0 = !(dmin0 > __float_as_int(1e-5f)), bad1 = !(dmin1 > __float_as_int(1e-5f));
   // Solves. Lane (r, g) publishes its own system g's entry r of a vector; the
   // products take the lane's 16 rows of column r of both systems and the two
@@ -1543,7 +1566,7 @@ __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           const double2 xv = reinterpret_cast<const double2*>(X2)[32 * b + 8 * q + 4 * g + p];
-          const double cv = Q[b * kQSys + idx[4 * q + p]];
+          const double cv = (&lds_at(unpack_off(ipk, 4 * q + p)))[b * kQSys];
           sa = fma(cv, xv.x, sa);
           sb = fma(cv, xv.y, sb);
         }
@@ -1822,10 +1845,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int64_t nl = list_len(a);
   UnitLoop q = unit_loop((nl + 1) / 2, a.work);
   int2 st = make_int2(0, 0);
+  // Byte offsets in dynamic LDS of the lane's matrix entries (i, r) of the
+  // first system, i = 8 q + 4 g + p (entry 4 q + p; the second system's lie
+  // kQSys doubles on), the same in every unit.
+  uint32_t ipk[8];
+  {
+    const int lane = threadIdx.x % BK_WAVE, g = lane >> 5, r = lane & 31;
+    const uint32_t qb = (uint32_t)((threadIdx.x / BK_WAVE) * sysm_doubles(a.v.nf) + qrec_doubles(a.v.nf)) * 8u;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+      const uint32_t lo = qb + 8u * (uint32_t)q_entry(8 * (k / 4) + 4 * g + k % 4, r);
+      const uint32_t hi = qb + 8u * (uint32_t)q_entry(8 * ((k + 1) / 4) + 4 * g + (k + 1) % 4, r);
+      ipk[k / 2] = lo | (hi << 16);
+    }
+  }
   while (q.unit < q.end) {
     // The next unit is taken before this one is solved, hiding the atomic.
     const int next = unit_next(q);
-    const int2 u = krige32m_blk(kernarg_first<SolveArgs>(), nl, q.unit);
+    const int2 u = krige32m_blk(kernarg_first<SolveArgs>(), nl, q.unit, ipk);
     st.x += u.x;
     st.y += u.y;
     wave_sync();

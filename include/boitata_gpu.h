@@ -402,7 +402,11 @@ int bk_set_refine_limit(bk_ctx* ctx, int32_t max_n);
  * targets given (a sample of a workload) with HIP events and keeps, in the
  * context, the fastest grid cell factor and uncapped search mode (by the
  * search stage) and the fastest solve workgroup size and bucket edges (by
- * the bucket, solve and deferred stages). None of them changes results.
+ * the bucket, solve and deferred stages). The cell factor, search mode and
+ * workgroup size do not change results. The bucket edges and the refine
+ * limit choose the kernel that solves a size class: in BK_FP64 its results
+ * are the same bit for bit; in BK_FP32_IR the refined kernels agree with each
+ * other and with FP64 to the refinement's tolerance, not bit for bit.
  * report (may be NULL) receives BK_AUTOTUNE_REPORT doubles: cell factor,
  * search mode, waves, edges[3], then search and solve milliseconds per pass
  * before and after tuning, then the refine limit (bk_set_refine_limit; for
