@@ -1041,12 +1041,17 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
       wave_sync();
       nbuf = rest;
     };
-    for (uint32_t o0 = 0; o0 < n_off; o0 += G) {
-      // The threshold: the K-th key (rank K - 1), +inf until K are held.
+    // The threshold: the K-th key held (rank K - 1), +inf until K are held.
+    uint64_t tk;
+    uint32_t ti;
+    auto threshold = [&]() {
       const uint32_t tlo = __shfl((unsigned)(ak & 0xffffffffu), K - 1, G);
       const uint32_t thi = __shfl((unsigned)(ak >> 32), K - 1, G);
-      const uint64_t tk = ((uint64_t)thi << 32) | tlo;
-      const uint32_t ti = __shfl(ai, K - 1, G);
+      tk = ((uint64_t)thi << 32) | tlo;
+      ti = __shfl(ai, K - 1, G);
+    };
+    for (uint32_t o0 = 0; o0 < n_off; o0 += G) {
+      threshold();
       const double tcap = tk != KEY_INF ? fmin(a.radius2, __longlong_as_double((long long)tk)) : a.radius2;
       // Lower bound of every offset from o0 on (sorted by m).
       const char4 f0 = a.tab.offsets[o0];
@@ -1110,6 +1115,10 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
           wave_sync();
           if (__any(nbuf >= G)) {
             if (nbuf >= G) merge();
+            // Entries of the rest of the batch must beat the K-th key held
+            // now: the first batch, whose threshold starts at +inf, then
+            // keeps only what can still enter the list.
+            threshold();
           }
         }
       }
