@@ -683,11 +683,24 @@ int run_solve(bk_ctx* c, const bk_variogram* vg, const bk_search* srch, int32_t 
             off[3 * k + d] = (t0 + t1) + t2;
           }
         }
+    // Then each offset projected on each frame, for the refined kernels'
+    // right-hand sides (a record minus F_f o_p is the lag to point p).
+    const int nf = a.v.nf;
+    off.resize(3 * (size_t)npts * (1 + nf));
+    double* po = off.data() + 3 * npts;
+    for (int k2 = 0; k2 < npts; ++k2)
+      for (int fr = 0; fr < nf; ++fr) {
+        const double* F = a.v.F[fr];
+        const double* o = off.data() + 3 * k2;
+        for (int d = 0; d < 3; ++d)
+          po[(k2 * nf + fr) * 3 + d] = a.v.ident[fr] ? o[d] : (F[3 * d] * o[0] + F[3 * d + 1] * o[1]) + F[3 * d + 2] * o[2];
+      }
     if (int r = ensure(c, c->offs, off.size() * sizeof(double))) return r;
     HIPOK(c, hipMemcpyAsync(c->offs.p, off.data(), off.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     a.block = 1;
     a.npts = npts;
     a.offs = (const double*)c->offs.p;
+    a.poffs = (const double*)c->offs.p + 3 * npts;
     a.cbb = 0.0;  // set per system: C(B,B) does not depend on the centre, computed once on the host below
     // composite.rs:120-129 on the discretisation centred at the origin.
     {

@@ -31,6 +31,7 @@ struct SolveArgs {
   int block;            // block support: RHS averaged over the discretisation
   int npts;
   const double* offs;   // npts * 3 offsets from the block centre (world axes)
+  const double* poffs;  // the offsets projected on each frame: F_f o_p at (p * nf + f) * 3 (refined kernels)
   double cbb;           // C(B, B) of the discretisation (block.rs:100-118)
   const double* sx;
   const double* sy;

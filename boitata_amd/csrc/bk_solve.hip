@@ -1256,10 +1256,9 @@ __device__ __forceinline__ double other_half(double v, int h) {
   return __longlong_as_double((long long)(((unsigned long long)u << 32) | l));
 }
 
-// One structure's contributions to eight of the lane's pairs (d = D0 + 1 ..
-// D0 + 8) in pairs_q32: acc[dd] += f(l2), l2 the squared lag in the
-// structure's frame (at `off` in the records); with `first`, the near test on
-// that frame.
+// One structure's contributions to eight of the lane's pairs in pairs_circ:
+// acc[dd] += f(l2), l2 the squared lag in the structure's frame (at `off` in
+// the records); with `first`, the near test on that frame.
 template <class Contrib>
 __device__ __forceinline__ void q_pass(const double* me, const uint32_t (&pk)[4], int off, bool first,
                                        bool& near, double near2, double (&acc)[8], Contrib f) {
@@ -1285,15 +1284,70 @@ __device__ __forceinline__ void q_pass(const double* me, const uint32_t (&pk)[4]
   }
 }
 
-// Pairs d = D0 + 1 .. D0 + 8 of one system of at most 32 samples (records at
-// R, lane r on sample r) into Q rows D0 + 1 .. D0 + 8: C = c0 - (nugget + sum
-// of sill * shape), in the fast FP64 forms with the constants folded per
-// structure (exponential and Gaussian: c0 - nugget - sill + sill e^x, x
-// clamped at -745; spherical: sill h (1.5 / a - 0.5 l2 / a^3) below the
-// range). Returns whether a pair of the lane lies within near2 in the frame
-// of structure 0.
-template <int D0>
-__device__ __forceinline__ bool pairs_q32_part(const DevVariogram& v, const double* R, int SQ, int r, double* Qg) {
+// Calls f with each structure's contribution to a covariance, as a function
+// of the squared lag in the structure's frame, in the fast FP64 forms with
+// the constants folded: C = K + sum of the contributions, K = c0 - nugget -
+// (the sills of the exponential and Gaussian structures); exponential and
+// Gaussian contribute sill e^x (x clamped at -745), spherical -sill h (1.5 /
+// a - 0.5 l2 / a^3) below the range and -sill beyond, cubic and
+// pentaspherical -sill shape(h / a). Every contribution is 0 at l2 = 0, so K
+// is the covariance of distinct coincident points (nugget left out). Models
+// without a fast form contribute NaN (the host sends such variograms to the
+// FP64 kernel).
+template <class Each>
+__device__ __forceinline__ double fast_structures(const DevVariogram& v, const ExpConstants& k, Each each) {
+#pragma clang fp contract(fast)
+  double K = v.c0 - v.nugget;
+  for (int st = 0; st < v.S; ++st) {
+    const int off = 3 * v.frame[st];
+    const double sill = v.sill[st], ir = v.inv_range[st];
+    const bool first = st == 0;
+    switch (v.model[st]) {
+      case BK_SPHERICAL: {
+        const double A = 1.5 * sill * ir, B = 0.5 * sill * (ir * ir * ir), a2 = 1.0 / (ir * ir);
+        each(off, first, [=](double l2) {
+          const double h = sqrt_pos(l2);
+          return l2 >= a2 ? -sill : -(h * (A - B * l2));
+        });
+        break;
+      }
+      case BK_EXPONENTIAL: {
+        K -= sill;
+        const double m = -3.0 * ir;
+        each(off, first, [=, &k](double l2) { return sill * exp_core(fmax(sqrt_pos(l2) * m, -745.0), k); });
+        break;
+      }
+      case BK_GAUSSIAN: {
+        K -= sill;
+        const double m = -3.0 * ir * ir;
+        each(off, first, [=, &k](double l2) { return sill * exp_core(fmax(l2 * m, -745.0), k); });
+        break;
+      }
+      case BK_CUBIC:
+        each(off, first, [=](double l2) {
+          const double t = sqrt_pos(l2) * ir, t2 = t * t;
+          return t >= 1.0 ? -sill : -sill * (t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2))));
+        });
+        break;
+      case BK_PENTASPHERICAL:
+        each(off, first, [=](double l2) {
+          const double t = sqrt_pos(l2) * ir, t2 = t * t;
+          return t >= 1.0 ? -sill : -sill * (t * (1.875 + t2 * (-1.25 + 0.375 * t2)));
+        });
+        break;
+      default:
+        each(off, first, [](double) { return __builtin_nan(""); });
+    }
+  }
+  return K;
+}
+
+// Pairs (r, (r + d) mod N), d = D0 + 1 .. D0 + 8, of one system of at most N
+// samples (records at R, lane r on sample r): out[dd] = C of pair D0 + dd + 1
+// (fast_structures). Returns whether a pair of the lane lies within near2 in
+// the frame of structure 0.
+template <int N, int D0>
+__device__ __forceinline__ bool pairs_circ(const DevVariogram& v, const double* R, int SQ, int r, double (&out)[8]) {
 #pragma clang fp contract(fast)
   extern __shared__ double dyn[];
   const double* me = R + r * SQ;
@@ -1304,65 +1358,100 @@ __device__ __forceinline__ bool pairs_q32_part(const DevVariogram& v, const doub
     const uint32_t base = (uint32_t)(reinterpret_cast<const char*>(R) - reinterpret_cast<const char*>(dyn));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t lo = base + (uint32_t)(((r + D0 + 2 * i + 1) & 31) * SQ) * 8u;
-      const uint32_t hi = base + (uint32_t)(((r + D0 + 2 * i + 2) & 31) * SQ) * 8u;
+      const uint32_t lo = base + (uint32_t)(((r + D0 + 2 * i + 1) & (N - 1)) * SQ) * 8u;
+      const uint32_t hi = base + (uint32_t)(((r + D0 + 2 * i + 2) & (N - 1)) * SQ) * 8u;
       pk[i] = lo | (hi << 16);
     }
   }
-  double acc[8];
 #pragma unroll
-  for (int dd = 0; dd < 8; ++dd) acc[dd] = 0.0;
+  for (int dd = 0; dd < 8; ++dd) out[dd] = 0.0;
   bool near = false;
   const double near2 = v.near2;
-  double K = v.c0 - v.nugget;
   const ExpConstants k = exp_constants();
-  for (int st = 0; st < v.S; ++st) {
-    const int off = 3 * v.frame[st];
-    const double sill = v.sill[st], ir = v.inv_range[st];
-    const bool first = st == 0;
-    switch (v.model[st]) {
-      case BK_SPHERICAL: {
-        const double A = 1.5 * sill * ir, B = 0.5 * sill * (ir * ir * ir), a2 = 1.0 / (ir * ir);
-        q_pass(me, pk, off, first, near, near2, acc, [=](double l2) {
-          const double h = sqrt_pos(l2);
-          return l2 >= a2 ? -sill : -(h * (A - B * l2));
-        });
-        break;
-      }
-      case BK_EXPONENTIAL: {
-        K -= sill;
-        const double m = -3.0 * ir;
-        q_pass(me, pk, off, first, near, near2, acc,
-               [=, &k](double l2) { return sill * exp_core(fmax(sqrt_pos(l2) * m, -745.0), k); });
-        break;
-      }
-      case BK_GAUSSIAN: {
-        K -= sill;
-        const double m = -3.0 * ir * ir;
-        q_pass(me, pk, off, first, near, near2, acc,
-               [=, &k](double l2) { return sill * exp_core(fmax(l2 * m, -745.0), k); });
-        break;
-      }
-      case BK_CUBIC:
-        q_pass(me, pk, off, first, near, near2, acc, [=](double l2) {
-          const double t = sqrt_pos(l2) * ir, t2 = t * t;
-          return t >= 1.0 ? -sill : -sill * (t2 * (7.0 + t * (-8.75 + t2 * (3.5 - 0.75 * t2))));
-        });
-        break;
-      case BK_PENTASPHERICAL:
-        q_pass(me, pk, off, first, near, near2, acc, [=](double l2) {
-          const double t = sqrt_pos(l2) * ir, t2 = t * t;
-          return t >= 1.0 ? -sill : -sill * (t * (1.875 + t2 * (-1.25 + 0.375 * t2)));
-        });
-        break;
-      default:
-        // No fast form (the host sends such variograms to the FP64 kernel).
-        q_pass(me, pk, off, first, near, near2, acc, [](double) { return __builtin_nan(""); });
-    }
-  }
+  const double K = fast_structures(v, k, [&](int off, bool first, auto f) {
+    q_pass(me, pk, off, first, near, near2, out, f);
+  });
 #pragma unroll
-  for (int dd = 0; dd < 8; ++dd) Qg[(D0 + dd + 1) * kQS + r] = K + acc[dd];
+  for (int dd = 0; dd < 8; ++dd) out[dd] += K;
   return near;
+}
+
+// Pairs d = D0 + 1 .. D0 + 8 of one system of at most 32 samples into Q rows
+// D0 + 1 .. D0 + 8.
+template <int D0>
+__device__ __forceinline__ bool pairs_q32_part(const DevVariogram& v, const double* R, int SQ, int r, double* Qg) {
+  double c[8];
+  const bool near = pairs_circ<32, D0>(v, R, SQ, r, c);
+#pragma unroll
+  for (int dd = 0; dd < 8; ++dd) Qg[(D0 + dd + 1) * kQS + r] = c[dd];
+  return near;
+}
+
+// Mean covariance between a sample (record me: F_f (w - t) on each frame,
+// relative to the block centre t) and the block's discretisation points t +
+// o_p (F_f o_p in a.poffs), in the fast forms of fast_structures; the nugget
+// is left out for every point (npts > 1, block.rs:100-118).
+__device__ __forceinline__ double rhs_block_fast(const SolveArgs& a, const double* me) {
+#pragma clang fp contract(fast)
+  const DevVariogram& v = a.v;
+  const int nf = v.nf, P = a.npts;
+  const ExpConstants k = exp_constants();
+  double acc = 0.0;
+  const double K = fast_structures(v, k, [&](int off, bool, auto f) {
+    const double q0 = me[off], q1 = me[off + 1], q2 = me[off + 2];
+    const double* o = a.poffs + off;
+    for (int p = 0; p < P; ++p, o += 3 * nf) {
+      const double d0 = q0 - o[0], d1 = q1 - o[1], d2 = q2 - o[2];
+      acc += f(d0 * d0 + d1 * d1 + d2 * d2);
+    }
+  });
+  return K + acc / (double)P;
+}
+
+// The record of sample r of the system of target t (n samples) for the
+// circulant pair phases: the projection on each frame relative to the target
+// (kernel.rs:36-42), or the sentinel 1e30 (lane + 1) in every coordinate for
+// r >= n; and the sample's value z, error variance e and right-hand side c
+// (the covariance to the target, or the mean over the block's points).
+__device__ __forceinline__ void circ_record(const SolveArgs& a, int t, bool mine, int idx_r, int lane, double* me,
+                                            double& z, double& e, double& c) {
+#pragma clang fp contract(fast)
+  const int nf = a.v.nf;
+  const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  z = 0.0;
+  e = 0.0;
+  c = 0.0;
+  if (mine) {
+    const int id = a.nidx[(int64_t)t * a.stride + idx_r];
+    wx = a.sx[id];
+    wy = a.sy[id];
+    wz = a.sz[id];
+    z = a.sv[id];
+    e = a.serr ? a.serr[id] : 0.0;
+  }
+  const double d0 = wx - t0, d1 = wy - t1, d2 = wz - t2;
+  const double sentinel = 1e30 * (double)(lane + 1);
+  double h[BK_MAX_FRAMES];
+  for (int f = 0; f < nf; ++f) {
+    const double* F = a.v.F[f];
+    const bool id = a.v.ident[f];
+    const double p0 = id ? d0 : F[0] * d0 + F[1] * d1 + F[2] * d2;
+    const double p1 = id ? d1 : F[3] * d0 + F[4] * d1 + F[5] * d2;
+    const double p2 = id ? d2 : F[6] * d0 + F[7] * d1 + F[8] * d2;
+    me[3 * f] = mine ? p0 : sentinel;
+    me[3 * f + 1] = mine ? p1 : sentinel;
+    me[3 * f + 2] = mine ? p2 : sentinel;
+    h[f] = norm3_fast(p0, p1, p2);
+  }
+  if (mine) {
+    if (!a.block)
+      c = cov_fast64(a.v, h);
+    else if (a.npts > 1)
+      c = rhs_block_fast(a, me);
+    else
+      c = rhs_fast_world(a, t0, t1, t2, wx, wy, wz);
+  }
 }
 
 // The pairs of one system (see pairs_q32_part), in two halves of eight
@@ -1442,38 +1531,8 @@ __device__ __forceinline__ int2 krige32m_blk(const SolveArgs& a, int64_t nl, int
   const bool mine = active && r < n;
   const double c0 = a.v.c0;
   const double ic0 = a.v.inv_c0;
-  double z = 0.0, e = 0.0, c = 0.0;
-  {
-#pragma clang fp contract(fast)
-    // Sample r: its value, error variance, and record (kernel.rs:36-42).
-    const double t0 = a.tx[t], t1 = a.ty[t], t2 = a.tz[t];
-    double wx = 0.0, wy = 0.0, wz = 0.0;
-    if (mine) {
-      const int id = a.nidx[(int64_t)t * a.stride + r];
-      wx = a.sx[id];
-      wy = a.sy[id];
-      wz = a.sz[id];
-      z = a.sv[id];
-      e = a.serr ? a.serr[id] : 0.0;
-    }
-    const double d0 = wx - t0, d1 = wy - t1, d2 = wz - t2;
-    const double sentinel = 1e30 * (double)(lane + 1);
-    double* me = S + (g * 32 + r) * SQ;
-    double h[BK_MAX_FRAMES];
-    for (int f = 0; f < nf; ++f) {
-      const double* F = a.v.F[f];
-      const bool id = a.v.ident[f];
-      const double p0 = id ? d0 : F[0] * d0 + F[1] * d1 + F[2] * d2;
-      const double p1 = id ? d1 : F[3] * d0 + F[4] * d1 + F[5] * d2;
-      const double p2 = id ? d2 : F[6] * d0 + F[7] * d1 + F[8] * d2;
-      me[3 * f] = mine ? p0 : sentinel;
-      me[3 * f + 1] = mine ? p1 : sentinel;
-      me[3 * f + 2] = mine ? p2 : sentinel;
-      h[f] = norm3_fast(p0, p1, p2);
-    }
-    // The right-hand side: the covariance to the target (or to the block).
-    if (mine) c = a.block ? rhs_fast_world(a, t0, t1, t2, wx, wy, wz) : cov_fast64(a.v, h);
-  }
+  double z, e, c;
+  circ_record(a, t, mine, r, lane, S + (g * 32 + r) * SQ, z, e, c);
   wave_sync();
   // A system with a near pair goes to the FP64 kernel (exact coincidence test).
   const bool near = group_any<32>(pairs_q32(a.v, S + g * 32 * SQ, SQ, r, Qg), g);
@@ -1645,33 +1704,59 @@ __device__ __forceinline__ void add_refine_stats(const SolveArgs& a, int2 st) {
 // (entry l ^ 32 of v from lane l). Row k lies in the registers of row block
 // k / 32 at the half-wave (k / 4) % 2, one register per column block; a
 // permlane32 swap of the two gives v, another the swapped v.
+__host__ __device__ constexpr int qrec64_doubles(int nf) {
+  // Records (later the iterates and product vectors: 192 doubles).
+  return 64 * rec_q(nf) > 192 ? 64 * rec_q(nf) : 192;
+}
 __host__ __device__ constexpr int sysm64_doubles(int nf) {
-  // Records (later the iterates and product vectors: 192 doubles), then the
-  // packed triangle with the diagonal.
-  return (64 * rec_doubles(nf) > 192 ? 64 * rec_doubles(nf) : 192) + 2080;
+  // Records, then the packed triangle with the diagonal.
+  return qrec64_doubles(nf) + 2080;
 }
 
 __device__ __forceinline__ int2 krige64m_blk(const SolveArgs& a, int64_t nl, int unit) {
   extern __shared__ double dyn[];
   const int nf = a.v.nf;
-  const int SJ = rec_doubles(nf);
+  const int SQ = rec_q(nf);
   const int tid = opaque_tid();
   const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
   const int j = lane & 31, h = lane >> 5;
   double* S = dyn + wave * sysm64_doubles(nf);
-  double* T = S + 64 * SJ;  // packed lower triangle with the diagonal, 2080 doubles
+  double* T = S + qrec64_doubles(nf);  // packed lower triangle with the diagonal, 2080 doubles
   const bool active = unit < nl;
   const int t = active ? a.list[unit] : 0;
   const int n = active ? a.ncount[t] : 0;
+  const bool mine = active && lane < n;
   double z, e, c;
-  // A system with a near pair goes to the FP64 kernel (exact coincidence test).
-  const bool near = refined_pairs<64, true>(a, t, n, lane, 0, active, false, S, SJ, T, z, e, c);
+  circ_record(a, t, mine, lane, lane, S + lane * SQ, z, e, c);
+  wave_sync();
+  // Pairs (lane, (lane + d) mod 64), d = 1 .. 32, in four parts of eight, into
+  // the packed triangle; a system with a near pair goes to the FP64 kernel
+  // (exact coincidence test).
+  bool nearl = false;
+  auto part = [&](auto d0) {
+    constexpr int D0 = decltype(d0)::value;
+    double cv[8];
+    nearl |= pairs_circ<64, D0>(a.v, S, SQ, lane, cv);
+#pragma unroll
+    for (int dd = 0; dd < 8; ++dd) {
+      const int jj = (lane + D0 + dd + 1) & 63;
+      const int hi = max(lane, jj), lo = min(lane, jj);
+      T[((hi * (hi + 1)) >> 1) + lo] = cv[dd];
+    }
+  };
+  part(std::integral_constant<int, 0>());
+  part(std::integral_constant<int, 8>());
+  part(std::integral_constant<int, 16>());
+  part(std::integral_constant<int, 24>());
+  const bool near = __any(nearl);
   const bool go = active && !near;
   const double c0 = a.v.c0;
   const double ic0 = a.v.inv_c0;
+  wave_sync();
+  // The diagonal; rows beyond n take the identity's (lane r writes row r).
   {
     const int tr = lane * (lane + 1) / 2;
-    if (go && lane < n) {
+    if (mine) {
       T[tr + lane] = c0 + e;
     } else {
 #pragma unroll
