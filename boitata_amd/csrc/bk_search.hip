@@ -1185,6 +1185,11 @@ __device__ __forceinline__ int sector_fast(const SearchArgs& a, double u0, doubl
   return amb ? -1 : (int)((a.sec_code >> (3 * pat)) & 7u);
 }
 
+// Cells within this many cell sizes of a coordinate plane through the target
+// count as straddling it (far above the rounding of the cell assignment, of
+// the FP32 cell coordinates and of the sign tests).
+constexpr float kSideMargin = 1e-3f;
+
 template <int S>
 __global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
   constexpr int NG = BK_WAVE / S;    // sector groups
@@ -1252,6 +1257,10 @@ __global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* sp
     const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
     const double slack = 1.0 - 1e-9;
     const uint32_t n_off = a.tab.end[a.tab.n - 1];
+    // The target's place in cells (outside [0, dims) beyond the grid), for
+    // the cells' boxes relative to it.
+    const double qc0 = (q0 - g.lo[0]) / g.h, qc1 = (q1 - g.lo[1]) / g.h, qc2 = (q2 - g.lo[2]) / g.h;
+    const double ih2 = 1.0 / h2;
     for (uint32_t o0 = 0; o0 < n_off; o0 += BK_WAVE) {
       // The largest cap-th key over the sectors, +inf while one holds fewer.
       uint64_t tmax = lane < a.n_sectors ? thr_k[wave][lane] : 0ull;
@@ -1270,16 +1279,45 @@ __global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* sp
         open = false;
         break;
       }
-      // Lane l: cell o0 + l, whose entries are [e, e + cnt) of the CSR.
+      // Lane l: cell o0 + l, whose entries are [e, e + cnt) of the CSR,
+      // unless every sector its entries can fall in already holds cap keys
+      // nearer than the cell's box: where the sectors are sign patterns of
+      // the projected offset (sec_fast), a cell at least kSideMargin cells
+      // from a coordinate plane through the target lies on one side of it,
+      // and a cell straddling a plane can hold both signs.
       uint32_t e = 0, cnt = 0;
       const uint32_t oc = o0 + lane;
       if (oc < n_off) {
         const char4 f = a.tab.offsets[oc];
         const int x = c0 + f.x, y = c1 + f.y, z = c2 + f.z;
         if (x >= 0 && x < g.dims[0] && y >= 0 && y < g.dims[1] && z >= 0 && z < g.dims[2]) {
-          const int id = x + g.dims[0] * (y + g.dims[1] * z);
-          e = g.start[id];
-          cnt = g.start[id + 1] - e;
+          // Cells on one side of every plane (most cells once the near
+          // ones are done) lie in one sector.
+          bool skip = false;
+          if (a.sec_fast) {
+            const float l0 = (float)((double)x - qc0), l1 = (float)((double)y - qc1), l2 = (float)((double)z - qc2);
+            const bool p0 = l0 > kSideMargin, p1 = l1 > kSideMargin, p2 = l2 > kSideMargin;
+            const bool n0 = l0 + 1.0f < -kSideMargin, n1 = l1 + 1.0f < -kSideMargin, n2 = l2 + 1.0f < -kSideMargin;
+            if ((p0 || n0) && (p1 || n1) && (p2 || n2)) {
+              // The gap to the box along each axis, less the margin: a
+              // lower bound of every entry's squared distance, in cells.
+              const float g0 = fmaxf((p0 ? l0 : -(l0 + 1.0f)) - kSideMargin, 0.0f);
+              const float g1 = fmaxf((p1 ? l1 : -(l1 + 1.0f)) - kSideMargin, 0.0f);
+              const float g2 = fmaxf((p2 ? l2 : -(l2 + 1.0f)) - kSideMargin, 0.0f);
+              const int pat = (p0 ? 1 : 0) | (p1 ? 2 : 0) | (p2 ? 4 : 0);
+              const int sec = (int)((a.sec_code >> (3 * pat)) & 7u);
+              const uint64_t th = thr_k[wave][sec];
+              // The sector's cap-th distance in squared cells, rounded up.
+              skip = th != KEY_INF &&
+                     ((g0 * g0 + g1 * g1) + g2 * g2) * (1.0f - 1e-5f) >
+                         (float)(__longlong_as_double((long long)th) * ih2) * (1.0f + 1e-5f);
+            }
+          }
+          if (!skip) {
+            const int id = x + g.dims[0] * (y + g.dims[1] * z);
+            e = g.start[id];
+            cnt = g.start[id + 1] - e;
+          }
         }
       }
       const uint32_t incl = group_incl_scan<BK_WAVE>(cnt, lane);
