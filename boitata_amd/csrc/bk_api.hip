@@ -1584,8 +1584,9 @@ int bk_loo_loss_grad(bk_ctx* c, const bk_variogram* vg, int32_t precision, doubl
     a.n_list = c->loo_hc[b];
     a.work = (int32_t*)c->work.p + b * BK_CTR_INTS;
     const bool ref_b = precision == BK_FP32_IR && c->loo_edges[b] <= c->refine_max;
-    const int r = ref_b ? bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b])
-                        : bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b]);
+    const int r = !ref_b                   ? bk::launch_loo_grad(c->stream, a, gp, c->loo_edges[b])
+                  : c->loo_edges[b] == 32 ? bk::launch_loo_grad32m(c->stream, a, gp, g32)
+                                          : bk::launch_loo_grad32(c->stream, a, gp, g32, c->loo_edges[b]);
     if (r) return fail(c, BK_E_HIP, "gradient launch failed");
   }
   // Systems the refined kernel handed back: the FP64 gradient over the redo

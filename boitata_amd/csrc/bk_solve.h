@@ -128,6 +128,9 @@ int launch_loo_grad(hipStream_t s, const SolveArgs& a, const GradArgs& gp, int n
 // The refined form: FP32 Gauss-Jordan forward with FP64 refinement, FP32
 // adjoint accumulation.
 int launch_loo_grad32(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32, int nmax);
+// The refined form for systems of at most 32 samples with the elimination on
+// the matrix cores (two systems per wavefront, persistent over a.work).
+int launch_loo_grad32m(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32);
 // Column sums of part (cols columns of m values each, column-major) into
 // out[0..cols), in an order fixed by m alone; work holds
 // cols * reduce_blocks(m) doubles.

@@ -1230,6 +1230,14 @@ __host__ __device__ constexpr int qrec_doubles(int nf) {
 
 __host__ __device__ constexpr int sysm_doubles(int nf) { return qrec_doubles(nf) + 2 * kQSys; }
 
+// The same for the refined gradient (k_loo_grad32m): the record area later
+// holds three iterates and product inputs per row (384 doubles), then each
+// sample's position and (lambda, v) (256 doubles).
+__host__ __device__ constexpr int qrecl_doubles(int nf) {
+  return 2 * 32 * rec_q(nf) > 384 ? 2 * 32 * rec_q(nf) : 384;
+}
+__host__ __device__ constexpr int sysl_doubles(int nf) { return qrecl_doubles(nf) + 2 * kQSys; }
+
 typedef float bk_floatx32 __attribute__((ext_vector_type(32)));
 
 // v of lane l plus v of lane l ^ 32, the same bits in both lanes.
@@ -2334,6 +2342,52 @@ __device__ __forceinline__ void cov_grad32(const GradArgs32& gp, float d0, float
   }
 }
 
+// cov_grad32 with the nugget and sill terms in FP64 (coef in FP64, the
+// shapes in FP32): those terms sum correlations with large weights of both
+// signs where samples nearly coincide, and carry most of the cancellation.
+__device__ __forceinline__ void cov_grad_mixed(const GradArgs32& gp, float d0, float d1, float d2, double coef,
+                                               double* acc) {
+#pragma clang fp contract(fast)
+  const int S = gp.S;
+  if (d0 == 0.0f && d1 == 0.0f && d2 == 0.0f) {
+    acc[0] += coef;
+#pragma unroll
+    for (int s = 0; s < BK_MAX_STRUCTURES; ++s)
+      if (s < S) acc[G32_SILL + s] += coef;
+    return;
+  }
+  const float cf = (float)coef;
+  const float r0 = gp.R[0] * d0 + gp.R[1] * d1 + gp.R[2] * d2;
+  const float r1 = gp.R[3] * d0 + gp.R[4] * d1 + gp.R[5] * d2;
+  const float r2 = gp.R[6] * d0 + gp.R[7] * d1 + gp.R[8] * d2;
+  float q[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    q[b][0] = gp.dR[b][0] * d0 + gp.dR[b][1] * d1 + gp.dR[b][2] * d2;
+    q[b][1] = gp.dR[b][3] * d0 + gp.dR[b][4] * d1 + gp.dR[b][5] * d2;
+    q[b][2] = gp.dR[b][6] * d0 + gp.dR[b][7] * d1 + gp.dR[b][8] * d2;
+  }
+#pragma unroll
+  for (int s = 0; s < BK_MAX_STRUCTURES; ++s) {
+    if (s >= S) break;
+    const float i0 = gp.inv_r[s][0], i1 = gp.inv_r[s][1], i2 = gp.inv_r[s][2];
+    const float u0 = r0 * i0, u1 = r1 * i1, u2 = r2 * i2;
+    const float tt = u0 * u0 + u1 * u1 + u2 * u2;
+    const float rs = __builtin_amdgcn_rsqf(tt);
+    const float t = tt * rs;
+    float g, gd;
+    shape_d32(gp.model[s], t, g, gd);
+    acc[G32_SILL + s] += coef * (double)(1.0f - g);
+    const float k = cf * gp.sill[s] * gd * rs;
+    acc[G32_RANGE + 3 * s + 0] += k * u0 * u0 * i0;
+    acc[G32_RANGE + 3 * s + 1] += k * u1 * u1 * i1;
+    acc[G32_RANGE + 3 * s + 2] += k * u2 * u2 * i2;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      acc[G32_ANGLE + b] -= k * (u0 * q[b][0] * i0 + u1 * q[b][1] * i1 + u2 * q[b][2] * i2);
+  }
+}
+
 // Position in the C ABI's packed layout of fixed-layout index p (S structures).
 __device__ __forceinline__ int g32_packed(int p, int S) {
   if (p < G32_RANGE) return p;  // nugget, sills
@@ -2527,6 +2581,269 @@ k_loo_grad32(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
     a.status[t] = ok ? BK_OK : (guard ? BK_UNSUPPORTED : BK_SINGULAR);
   }
 }
+
+#if BK_PART != 2
+// ---------------------------------------------------------------------------
+// The refined leave-one-out gradient for systems of at most 32 samples, two
+// per wavefront, on the matrix cores: the front of k_krige32m (circulant
+// pairs, MFMA Gauss-Jordan) with three right-hand sides, c, the ones and the
+// neighbours' values z, refined in FP64 with the same stopping rule; then
+// the forward quantities and the adjoint v = K^-1 [z; 0] in FP64 and the
+// pair gradient of k_loo_grad32 (cov_grad32 over the pairs (r, (r + d) mod
+// 32), d = 1 .. 16, the pairs of d = 16 taken once). Systems with a near pair,
+// a small pivot or no convergence go to the FP64 gradient kernel through the
+// redo list (it applies the exact coincidence test, so coincident systems get
+// its status).
+__device__ __forceinline__ void loo32m_blk(const SolveArgs& a, const GradArgs32& g32, int64_t gm, int64_t nl,
+                                           int unit, const uint32_t (&ipk)[8]) {
+  extern __shared__ double dyn[];
+  const int nf = a.v.nf;
+  const int SQ = rec_q(nf);
+  const int tid = opaque_tid();
+  const int wave = tid / BK_WAVE, lane = tid % BK_WAVE;
+  const int g = lane >> 5, r = lane & 31;
+  double* S = dyn + wave * sysl_doubles(nf);
+  double* Q = S + qrecl_doubles(nf);
+  double* Qg = Q + g * kQSys;
+  const int64_t sys = (int64_t)unit * 2 + g;
+  const bool active = sys < nl;
+  const int t = active ? a.list[sys] : 0;
+  const int n = active ? a.ncount[t] : 0;
+  const bool mine = active && r < n;
+  const double c0 = a.v.c0;
+  const double ic0 = a.v.inv_c0;
+  double z, e, c;
+  circ_record(a, t, mine, r, lane, S + (g * 32 + r) * SQ, z, e, c);
+  wave_sync();
+  const bool near = group_any<32>(pairs_q32(a.v, S + g * 32 * SQ, SQ, r, Qg), g);
+  const bool go = active && !near;
+  const bool go0 = __builtin_amdgcn_readlane(go ? 1 : 0, 0) != 0, go1 = __builtin_amdgcn_readlane(go ? 1 : 0, 32) != 0;
+  Qg[r] = mine ? c0 + e : c0;
+  if (__builtin_amdgcn_readlane(n, 0) < 32 || __builtin_amdgcn_readlane(n, 32) < 32) {
+#pragma unroll
+    for (int dd = 1; dd <= 16; ++dd)
+      if (!(r < n && ((r + dd) & 31) < n)) Qg[dd * kQS + r] = 0.0;
+  }
+  wave_sync();
+  bk_floatx32 X;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const double* e0 = &lds_at(unpack_off(ipk, k));
+#pragma unroll
+    for (int b = 0; b < 2; ++b) X[16 * b + k] = (float)(e0[b * kQSys] * ic0);
+  }
+  int dmin = 0x7f7fffff;
+  gj_pivots32<0>(X, dmin);
+  const int dmin0 = __builtin_amdgcn_readlane(dmin, 0), dmin1 = __builtin_amdgcn_readlane(dmin, 32);
+  const bool bad0 = !(dmin0 > __float_as_int(1e-5f)), bad1 = !(dmin1 > __float_as_int(1e-5f));
+  // Solves for the three right-hand sides (see krige32m_blk): iterates
+  // double4 [b][j], product inputs float4 [b][j] (the records are no longer
+  // read).
+  double* X3 = S;
+  float* V3 = (float*)(S + 4 * 64);
+  auto apply = [&](float va, float vb, float vz, float (&ya)[2], float (&yb)[2], float (&yz)[2]) {
+    reinterpret_cast<float4*>(V3)[g * 32 + r] = make_float4(va, vb, vz, 0.0f);
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      float sa = 0.0f, sb = 0.0f, sz = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const float4 w = reinterpret_cast<const float4*>(V3)[32 * b + 8 * q + 4 * g + p];
+          const float xv = X[16 * b + 4 * q + p];
+          sa = fmaf(xv, w.x, sa);
+          sb = fmaf(xv, w.y, sb);
+          sz = fmaf(xv, w.z, sz);
+        }
+      ya[b] = halves_sum(sa);
+      yb[b] = halves_sum(sb);
+      yz[b] = halves_sum(sz);
+    }
+    wave_sync();
+  };
+  const bool live = go && r < n;
+  const double ba = live ? c : 0.0, bb = live ? 1.0 : 0.0, bz = live ? z : 0.0;
+  const double ba_o = other_half(ba, g), bb_o = other_half(bb, g), bz_o = other_half(bz, g);
+  const double rha[2] = {g ? ba_o : ba, g ? ba : ba_o}, rhb[2] = {g ? bb_o : bb, g ? bb : bb_o},
+               rhz[2] = {g ? bz_o : bz, g ? bz : bz_o};
+  float fa[2], fb[2], fz[2];
+  apply((float)(ba * ic0), (float)(bb * ic0), (float)(bz * ic0), fa, fb, fz);
+  double xa[2] = {fa[0], fa[1]}, xb[2] = {fb[0], fb[1]}, xz[2] = {fz[0], fz[1]};
+  const float tolf = (float)a.tol;
+  float ma[2], mb[2], mz[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    ma[b] = group_maxf<32>(fabsf(fa[b]));
+    mb[b] = group_maxf<32>(fabsf(fb[b]));
+    mz[b] = group_maxf<32>(fabsf(fz[b]));
+  }
+  bool conv0 = false, conv1 = false;
+  for (int it = 0; it < a.max_iter; ++it) {
+    const bool more0 = go0 && !bad0 && !conv0, more1 = go1 && !bad1 && !conv1;
+    if (!more0 && !more1) break;
+    reinterpret_cast<double4*>(X3)[g * 32 + r] =
+        make_double4(g ? xa[1] : xa[0], g ? xb[1] : xb[0], g ? xz[1] : xz[0], 0.0);
+    wave_sync();
+    double ra[2], rb[2], rz[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      double sa = 0.0, sb = 0.0, sz = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const double4 xv = reinterpret_cast<const double4*>(X3)[32 * b + 8 * q + 4 * g + p];
+          const double cv = (&lds_at(unpack_off(ipk, 4 * q + p)))[b * kQSys];
+          sa = fma(cv, xv.x, sa);
+          sb = fma(cv, xv.y, sb);
+          sz = fma(cv, xv.z, sz);
+        }
+      ra[b] = rha[b] - halves_sum(sa);
+      rb[b] = rhb[b] - halves_sum(sb);
+      rz[b] = rhz[b] - halves_sum(sz);
+    }
+    wave_sync();
+    float da[2], db[2], dz[2];
+    apply((float)((g ? ra[1] : ra[0]) * ic0), (float)((g ? rb[1] : rb[0]) * ic0), (float)((g ? rz[1] : rz[0]) * ic0),
+          da, db, dz);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      xa[b] += (double)da[b];
+      xb[b] += (double)db[b];
+      xz[b] += (double)dz[b];
+    }
+    bool small0, small1;
+    if (it == 0) {
+      const float delta = group_maxf<32>(fmaxf(fmaxf(rel_corr(g ? da[1] : da[0], g ? ma[1] : ma[0]),
+                                                     rel_corr(g ? db[1] : db[0], g ? mb[1] : mb[0])),
+                                               rel_corr(g ? dz[1] : dz[0], g ? mz[1] : mz[0])));
+      const bool done = first_sweep_converged(delta, 32.0f, __int_as_float(g ? dmin1 : dmin0), tolf);
+      const bool nan0 = __ballot(__builtin_isnan(da[0]) || __builtin_isnan(db[0]) || __builtin_isnan(dz[0])) != 0;
+      const bool nan1 = __ballot(__builtin_isnan(da[1]) || __builtin_isnan(db[1]) || __builtin_isnan(dz[1])) != 0;
+      small0 = !nan0 && __builtin_amdgcn_readlane(done ? 1 : 0, 0) != 0;
+      small1 = !nan1 && __builtin_amdgcn_readlane(done ? 1 : 0, 32) != 0;
+    } else {
+      small0 = __ballot(!(fabsf(da[0]) <= tolf * ma[0] && fabsf(db[0]) <= tolf * mb[0] &&
+                          fabsf(dz[0]) <= tolf * mz[0])) == 0;
+      small1 = __ballot(!(fabsf(da[1]) <= tolf * ma[1] && fabsf(db[1]) <= tolf * mb[1] &&
+                          fabsf(dz[1]) <= tolf * mz[1])) == 0;
+    }
+    if (more0) conv0 = small0;
+    if (more1) conv1 = small1;
+  }
+  const bool bad = g ? bad1 : bad0, conv = g ? conv1 : conv0;
+  const bool okf = go && !bad && conv;
+  const bool redo = active && !okf && a.redo;
+  if (redo && r == 0) a.redo[atomicAdd(a.redo_count, 1)] = t;
+  // Forward quantities in FP64 (kernel.rs:86-97), adjoint vector v = K^-1 [z; 0].
+  const double xag = g ? xa[1] : xa[0], xbg = g ? xb[1] : xb[0], xzg = g ? xz[1] : xz[0];
+  const bool in = okf && r < n;
+  const double sa = group_sum<32>(in ? xag : 0.0);
+  const double sb = group_sum<32>(in ? xbg : 0.0);
+  const double sz = group_sum<32>(in ? xzg : 0.0);
+  const double mu = (sa - 1.0) / sb;
+  const double vmu = sz / sb;
+  const double lam = xag - mu * xbg;
+  const double vl = xzg - vmu * xbg;
+  const double zhat = group_sum<32>(in ? lam * z : 0.0);
+  const double swc = group_sum<32>(in ? lam * c : 0.0);
+  const double sigma2 = c0 - swc - mu;
+  const int64_t held = a.left_out_ids ? a.left_out_ids[t] : t;
+  const double zi = active ? a.sv[held] : 0.0;
+  const bool ok = okf && sigma2 > 0.0 && n > 0;
+  const double alpha = -2.0 * (zi - zhat), beta = 1.0 / sigma2;
+  // The gradient over the pairs, from each sample's world position and its
+  // (lambda, v) in LDS (the iterates are no longer read).
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  if (mine) {
+    const int id = a.nidx[(int64_t)t * a.stride + r];
+    wx = a.sx[id];
+    wy = a.sy[id];
+    wz = a.sz[id];
+  }
+  // Position and (lambda, v) of each sample: 5 doubles (an odd stride).
+  double* GW = S + g * 32 * 5;
+  wave_sync();
+  GW[5 * r] = wx;
+  GW[5 * r + 1] = wy;
+  GW[5 * r + 2] = wz;
+  GW[5 * r + 3] = ok && r < n ? lam : 0.0;
+  GW[5 * r + 4] = ok && r < n ? vl : 0.0;
+  wave_sync();
+  // Per-lane sums in FP64: near-coincident samples make the terms large and
+  // of both signs (FP32 increments, FP64 accumulation).
+  double acc[BK_MAX_GRAD];
+#pragma unroll
+  for (int p = 0; p < BK_MAX_GRAD; ++p) acc[p] = 0.0;
+  if (ok && r < n) {
+    for (int d = 1; d <= 16; ++d) {
+      const int j = (r + d) & 31;
+      if (j >= n || (d == 16 && r >= 16)) continue;
+      const double* pj = GW + 5 * j;
+      const double coef = -alpha * (vl * pj[3] + pj[4] * lam) + 2.0 * beta * lam * pj[3];
+      cov_grad_mixed(g32, (float)(wx - pj[0]), (float)(wy - pj[1]), (float)(wz - pj[2]), coef, acc);
+    }
+    const double coefd = -alpha * vl * lam + beta * lam * lam;
+    acc[0] += coefd;
+#pragma unroll
+    for (int s = 0; s < BK_MAX_STRUCTURES; ++s)
+      if (s < g32.S) acc[G32_SILL + s] += coefd;
+    const double gr = alpha * vl - 2.0 * beta * lam;
+    cov_grad_mixed(g32, (float)(a.tx[t] - wx), (float)(a.ty[t] - wy), (float)(a.tz[t] - wz), gr, acc);
+  }
+  if (ok && r == 0) {
+    acc[0] += beta;
+#pragma unroll
+    for (int s = 0; s < BK_MAX_STRUCTURES; ++s)
+      if (s < g32.S) acc[G32_SILL + s] += beta;
+  }
+  double tot[BK_MAX_GRAD];
+#pragma unroll
+  for (int p = 0; p < BK_MAX_GRAD; ++p) tot[p] = g32_used(p, g32.S) ? group_sum<32>(acc[p]) : 0.0;
+  if (active && r == 0 && !redo) {
+    double* out = a.grad_part + t;
+    const int64_t m = gm;
+    if (ok) {
+      out[0] = (zi - zhat) * (zi - zhat) + log(sigma2);
+      out[m] = 1.0;
+#pragma unroll
+      for (int p = 0; p < BK_MAX_GRAD; ++p)
+        if (g32_used(p, g32.S)) out[(2 + g32_packed(p, g32.S)) * m] = tot[p];
+    } else {
+      for (int p = 0; p < g32.P + 2; ++p) out[p * m] = 0.0;
+    }
+    a.value[t] = ok ? zhat : __longlong_as_double(0x7ff8000000000000LL);
+    a.variance[t] = ok ? sigma2 : __longlong_as_double(0x7ff8000000000000LL);
+    a.status[t] = ok ? BK_OK : BK_SINGULAR;
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_loo_grad32m(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
+  const int64_t nl = list_len(a);
+  UnitLoop q = unit_loop((nl + 1) / 2, a.work);
+  uint32_t ipk[8];
+  {
+    const int lane = threadIdx.x % BK_WAVE, g = lane >> 5, r = lane & 31;
+    const uint32_t qb = (uint32_t)((threadIdx.x / BK_WAVE) * sysl_doubles(a.v.nf) + qrecl_doubles(a.v.nf)) * 8u;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+      const uint32_t lo = qb + 8u * (uint32_t)q_entry(8 * (k / 4) + 4 * g + k % 4, r);
+      const uint32_t hi = qb + 8u * (uint32_t)q_entry(8 * ((k + 1) / 4) + 4 * g + (k + 1) % 4, r);
+      ipk[k / 2] = lo | (hi << 16);
+    }
+  }
+  while (q.unit < q.end) {
+    const int next = unit_next(q);
+    loo32m_blk(kernarg_first<SolveArgs>(), g32, gp.m, nl, q.unit, ipk);
+    wave_sync();
+    q.unit = next;
+  }
+}
+
+#endif
 
 // Deterministic column sums of a column-major m x cols array: block
 // (b, col) sums rows [b*chunk, (b+1)*chunk) of its column (coalesced, fixed
@@ -3127,6 +3444,16 @@ static int run64m(hipStream_t s, const SolveArgs& a) {
   const void* fn = (const void*)k_krige64m;
   allow_lds(fn, q.sh);
   k_krige64m<<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_loo_grad32m(hipStream_t s, const SolveArgs& a, const GradArgs& gp, const GradArgs32& g32) {
+  const SolveShape q = solve_shape<32>(a, sysl_doubles(a.v.nf) / 2);
+  if (q.work == 0) return 0;
+  if (!a.work) return -1;
+  const void* fn = (const void*)k_loo_grad32m;
+  allow_lds(fn, q.sh);
+  k_loo_grad32m<<<dim3(persistent_grid(fn, 64 * q.waves, q.sh, q.work)), dim3(64 * q.waves), q.sh, s>>>(a, gp, g32);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
