@@ -16,10 +16,11 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libboitata_gpu.so"
 # (source, object, extra flags). bk_solve.hip is built as two objects: the
-# kriging pass with its persistent solve loops (BK_PART 1), where machine LICM
-# would hoist FP64 constants (polynomial coefficients, zeroed accumulators)
-# out of the loop over work units and spill them, so it is off there; and the
-# refined gradient (BK_PART 2), which has no such loop and keeps it.
+# kriging pass and the persistent refined gradient with their loops over work
+# units (BK_PART 1), where machine LICM would hoist FP64 constants (polynomial
+# coefficients, zeroed accumulators) out of the loop and spill them, so it is
+# off there; and the other refined gradients (BK_PART 2), which have no such
+# loop and keep it.
 UNITS = [
     ("bk_search.hip", "bk_search.o", []),
     ("bk_solve.hip", "bk_solve_pass.o", ["-DBK_PART=1", "-mllvm", "-disable-machine-licm"]),

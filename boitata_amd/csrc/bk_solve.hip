@@ -10,8 +10,9 @@
 #include "bk_solve.h"
 
 // Built as two objects: BK_PART 1 holds the kriging pass (persistent solve
-// kernels, guarded systems, bucketing) and the FP64 gradient, BK_PART 2 the
-// refined gradient and the reductions; the build compiles them with different
+// kernels, guarded systems, bucketing), the FP64 gradient and the persistent
+// refined gradient of systems up to 32 samples, BK_PART 2 the other refined
+// gradients and the reductions; the build compiles them with different
 // backend options. Without BK_PART the file holds everything.
 #ifndef BK_PART
 #define BK_PART 0
