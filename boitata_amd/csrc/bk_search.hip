@@ -929,6 +929,10 @@ __device__ __forceinline__ void group_merge_keys(uint64_t& ak, uint32_t& ai, uin
 
 constexpr uint64_t KEY_INF = ~0ull;
 
+// k_knn_wave ranks batches up to this size entry by entry (about 24 VALU per
+// entry) and sorts larger ones into the list by networks (about 130 VALU).
+constexpr int kSmallMerge = 6;
+
 // The bits of a wave ballot that belong to this lane's group of G lanes.
 template <int G>
 __device__ __forceinline__ uint64_t group_ballot(bool p, int h) {
@@ -936,15 +940,15 @@ __device__ __forceinline__ uint64_t group_ballot(bool p, int h) {
   return G == BK_WAVE ? m : (m >> (h * G)) & ((1ull << G) - 1ull);
 }
 
-// Consecutive targets a group of lanes takes in turn. A target usually has
-// its predecessor close by (block models list their blocks row by row), and
-// the predecessor's K-th distance bounds its own.
 // One target per group of G lanes (G = 64, or 32 when max_samples <= 32: two
 // targets per wavefront, whose networks, ballots and broadcasts then serve
 // both at once). Once K keys are held, a cell whose box lies beyond the K-th
 // key is skipped whole: every entry in it would be rejected one by one.
+// The walk waits on shuffles and gathers in turn, so it runs at seven
+// wavefronts per SIMD (at most 72 registers).
 template <int G>
-__global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_knn_wave(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
   constexpr int PER = BK_WAVE / G;  // targets per wavefront
   __shared__ uint64_t buf_k[4][2 * BK_WAVE];
   __shared__ uint32_t buf_i[4][2 * BK_WAVE];
@@ -966,9 +970,13 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
   uint32_t* bi = buf_i[wave] + h * 2 * G;
   bool open = live && g.start != nullptr && K > 0;
   if (__any(open)) {
-    const int c0 = clamp_cell((q0 - g.lo[0]) / g.h, g.dims[0]);
-    const int c1 = clamp_cell((q1 - g.lo[1]) / g.h, g.dims[1]);
-    const int c2 = clamp_cell((q2 - g.lo[2]) / g.h, g.dims[2]);
+    // The target in cells, by one reciprocal: the cell only starts the walk,
+    // and a rounding of it is within the slack of the bounds below.
+    const double ihc = 1.0 / g.h;
+    const double s0 = (q0 - g.lo[0]) * ihc, s1 = (q1 - g.lo[1]) * ihc, s2 = (q2 - g.lo[2]) * ihc;
+    const int c0 = clamp_cell(s0, g.dims[0]);
+    const int c1 = clamp_cell(s1, g.dims[1]);
+    const int c2 = clamp_cell(s2, g.dims[2]);
     const double h2 = g.h * g.h;
     const double beyond = h2 * (double)((BK_REACH - 1) * (BK_REACH - 1));
     const double slack = 1.0 - 1e-9;
@@ -978,15 +986,15 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
     // face dx - fr cells away, one dx < 0 away |dx| - 1 + fr; up and dn are
     // fr and 1 - fr plus 1e-5 cells for the rounding of the cell assignment
     // and of these FP32 values, so the box distance is never overestimated.
-    const double ih2 = 1.0 / h2;
-    const float fr0 = (float)((q0 - g.lo[0]) / g.h - (double)c0);
-    const float fr1 = (float)((q1 - g.lo[1]) / g.h - (double)c1);
-    const float fr2 = (float)((q2 - g.lo[2]) / g.h - (double)c2);
+    const double ih2 = ihc * ihc;
+    const float fr0 = (float)(s0 - (double)c0);
+    const float fr1 = (float)(s1 - (double)c1);
+    const float fr2 = (float)(s2 - (double)c2);
     const float up0 = fr0 + 1e-5f, up1 = fr1 + 1e-5f, up2 = fr2 + 1e-5f;
     const float dn0 = (1.0f - fr0) + 1e-5f, dn1 = (1.0f - fr1) + 1e-5f, dn2 = (1.0f - fr2) + 1e-5f;
     int nbuf = 0;  // the same in the group's lanes
     auto merge = [&]() {
-      if (nbuf <= kSmallBatch) {
+      if (nbuf <= kSmallMerge) {
         // Few entries: every list entry and every batch entry counts the
         // entries below it (the batch's keys read from the group's lanes)
         // and is written at its rank in the merged list, in the buffer's
@@ -1023,8 +1031,19 @@ __global__ void __launch_bounds__(256) k_knn_wave(const SearchArgs a, int32_t* s
       }
       uint64_t k = lane < nbuf ? bk[lane] : KEY_INF;
       uint32_t i = lane < nbuf ? bi[lane] : 0xffffffffu;
-      group_sort_keys<G>(k, i, lane);
-      group_merge_keys<G>(ak, ai, k, i, lane);
+      // At most G / 2 entries in every group: sorting groups of G / 2 lanes
+      // leaves the padding above them, so the G lanes are sorted too.
+      if (__any(nbuf > G / 2))
+        group_sort_keys<G>(k, i, lane);
+      else
+        group_sort_keys<G / 2>(k, i, lane & (G / 2 - 1));
+      if (group_ballot<G>(ak != KEY_INF, h) == 0) {
+        // Nothing held yet (the first merge): the sorted batch is the list.
+        ak = k;
+        ai = i;
+      } else {
+        group_merge_keys<G>(ak, ai, k, i, lane);
+      }
       // Keep what did not fit in this merge.
       const int rest = nbuf > G ? nbuf - G : 0;
       uint64_t kk = 0;
