@@ -1209,8 +1209,11 @@ __device__ __forceinline__ int sector_fast(const SearchArgs& a, double u0, doubl
 // the FP32 cell coordinates and of the sign tests).
 constexpr float kSideMargin = 1e-3f;
 
+// Four wavefronts per SIMD (at most 128 registers): the walk waits on
+// shuffles, LDS and gathers in turn.
 template <int S>
-__global__ void __launch_bounds__(256) k_sectors(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_sectors(const SearchArgs a, int32_t* spill, int32_t* spill_count) {
   constexpr int NG = BK_WAVE / S;    // sector groups
   constexpr int CAPB = BK_WAVE + S;  // buffer entries per sector: < S held + one round
   __shared__ uint64_t buf_k[4][NG][CAPB];

@@ -2821,7 +2821,7 @@ __device__ __forceinline__ void loo32m_blk(const SolveArgs& a, const GradArgs32&
   }
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_loo_grad32m(const SolveArgs a, const GradArgs gp, const GradArgs32 g32) {
   const int64_t nl = list_len(a);
   UnitLoop q = unit_loop((nl + 1) / 2, a.work);
