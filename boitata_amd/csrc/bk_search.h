@@ -16,7 +16,12 @@
 #define BK_POOL_BIG 65536 // the same in global memory, for the targets that overflow BK_POOL
 #define BK_BIG_BLOCKS 32  // blocks (pools) of the global-memory pass
 #define BK_MAX_KNN 64     // selections without caps up to this size run one target per lane
-#define BK_REACH 12
+// Cells of the shell table on each side of the target's cell; a walk goes on
+// by rings beyond it. The reference's table reaches 12 (grid.rs:11); the
+// selections do not depend on the reach, and 24 keeps the walks of sparse
+// neighbourhoods (simulation nodes early on the path) in the wave searches
+// instead of the lane-per-target ring walk.
+#define BK_REACH 24
 
 // Internal per-target status: the candidate pool overflowed (BK_POOL, then
 // BK_POOL_BIG entries pending between two releases).

@@ -1840,7 +1840,7 @@ size_t grid_scan_bytes(int64_t cells) {
   return need;
 }
 
-// grid.rs:39-64: offsets within REACH sorted by m = sum max(0, |o| - 1)^2.
+// grid.rs:39-64: offsets within BK_REACH sorted by m = sum max(0, |o| - 1)^2.
 int upload_shell_table(ShellTable& tab, void** storage) {
   struct K {
     uint32_t m;
