@@ -420,13 +420,20 @@ void dev_variogram(const bk_variogram* v, DevVariogram& d, DevVariogram32& f) {
   }
 }
 
-__global__ void k_first_finite(int64_t n, const double* x, const double* y, const double* z, unsigned long long* out,
-                               unsigned long long* nonplanar) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (isfinite(x[i]) && isfinite(y[i]) && isfinite(z[i])) atomicMin(out, (unsigned long long)i);
+// One atomic of each kind per wavefront: the first finite point of the
+// wavefront, and its count of points off the first point's elevation.
+__global__ void __launch_bounds__(256) k_first_finite(int64_t n, const double* x, const double* y, const double* z,
+                                                      unsigned long long* out, unsigned long long* nonplanar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < n;
+  const uint64_t fin = __ballot(in && isfinite(x[i]) && isfinite(y[i]) && isfinite(z[i]));
   // search.rs:470-473: every point at the first point's elevation.
-  if (!(z[i] == z[0])) atomicAdd(nonplanar, 1ull);
+  const uint64_t off = __ballot(in && !(z[i] == z[0]));
+  if ((threadIdx.x & (BK_WAVE - 1)) == 0) {
+    const int64_t w0 = i;  // the wavefront's first point
+    if (fin) atomicMin(out, (unsigned long long)(w0 + __builtin_ctzll(fin)));
+    if (off) atomicAdd(nonplanar, (unsigned long long)__popcll(off));
+  }
 }
 
 __global__ void k_copy_int(int64_t n, const int32_t* src, int32_t* dst) {
@@ -2436,15 +2443,13 @@ __global__ void k_sgs_limits(int64_t m2, int64_t n, int32_t* limit) {
 // next pass.
 __device__ __forceinline__ bool sgs_drawn_before(int32_t state, int32_t pass) { return state > 0 && state < pass; }
 
-__global__ void __launch_bounds__(256) k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx,
-                                                  const int32_t* count, const int32_t* status, const double* w,
-                                                  const double* variance, const double* z, double* s, int32_t* state,
-                                                  int32_t* prog, int32_t pass) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= m2) return;
+// Node k in pass `pass`: drawn, or still waiting (true).
+__device__ __forceinline__ bool sgs_step(int64_t k, int64_t n, int stride, const int32_t* nidx, const int32_t* count,
+                                         const int32_t* status, const double* w, const double* variance,
+                                         const double* z, double* s, int32_t* state, int32_t* prog, int32_t pass) {
   const int32_t sk = state[k];
-  if (sk > 0) return;
-  if (sk <= -2 && !sgs_drawn_before(state[-2 - sk], pass)) return;
+  if (sk > 0) return false;
+  if (sk <= -2 && !sgs_drawn_before(state[-2 - sk], pass)) return true;
   const int st = status[k];
   const int cnt = st == BK_OK ? count[k] : 0;
   const int32_t* id = nidx + k * (int64_t)stride;
@@ -2457,7 +2462,7 @@ __global__ void __launch_bounds__(256) k_sgs_pass(int64_t m2, int64_t n, int str
   if (j < cnt) {
     prog[k] = j;
     state[k] = -2 - (int32_t)q;
-    return;
+    return true;
   }
   double v;
   if (cnt > 0) {
@@ -2472,13 +2477,21 @@ __global__ void __launch_bounds__(256) k_sgs_pass(int64_t m2, int64_t n, int str
   }
   s[n + k] = v;
   state[k] = pass;
+  return false;
 }
 
-// left[0] += the nodes still waiting (one atomic per block of 256).
-__global__ void __launch_bounds__(256) k_sgs_left(int64_t m2, const int32_t* state, int32_t* left) {
+// One pass over every node. With `clear`, thread 0 zeroes *left; with `left`,
+// the pass adds the nodes still waiting after it (one atomic per block).
+__global__ void __launch_bounds__(256) k_sgs_pass(int64_t m2, int64_t n, int stride, const int32_t* nidx,
+                                                  const int32_t* count, const int32_t* status, const double* w,
+                                                  const double* variance, const double* z, double* s, int32_t* state,
+                                                  int32_t* prog, int32_t pass, int32_t* clear, int32_t* left) {
   __shared__ int32_t part[4];
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int c = __popcll(__ballot(k < m2 && state[k] <= 0));
+  if (clear && k == 0) *clear = 0;
+  const bool waits = k < m2 && sgs_step(k, n, stride, nidx, count, status, w, variance, z, s, state, prog, pass);
+  if (!left) return;
+  const int c = __popcll(__ballot(waits));
   if ((threadIdx.x & (BK_WAVE - 1)) == 0) part[threadIdx.x / BK_WAVE] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -2694,15 +2707,14 @@ int bk_sgs(bk_ctx* c, const bk_variogram* vg, const bk_search* s, int32_t precis
       // Each pass draws at least the first node still waiting (its simulated
       // neighbours come before it), so m2 passes always end.
       if (pass > m2) return fail(c, BK_E_HIP, "simulation passes did not finish");
+      // The first pass of eight clears the count, the last one counts.
       for (int q = 0; q < 8; ++q) {
         ++pass;
         k_sgs_pass<<<blocks_of(m2), 256, 0, k->stream>>>(
             m2, n, (int)stride, (const int32_t*)k->nidx.p, (const int32_t*)k->ncount.p, (const int32_t*)k->status.p,
             (const double*)c->sgs_w.p, (const double*)k->variance.p, dzin, sc, (int32_t*)c->sgs_done.p,
-            (int32_t*)c->sgs_prog.p, pass);
+            (int32_t*)c->sgs_prog.p, pass, q == 0 ? ctr : nullptr, q == 7 ? ctr : nullptr);
       }
-      if (bk::zero32(k->stream, ctr, 1)) return fail(c, BK_E_HIP, "fill launch failed");
-      k_sgs_left<<<blocks_of(m2), 256, 0, k->stream>>>(m2, (const int32_t*)c->sgs_done.p, ctr);
       HIPOK(c, hipGetLastError());
       HIPOK(c, hipMemcpyAsync(&left, ctr, sizeof(left), hipMemcpyDeviceToHost, k->stream));
       HIPOK(c, hipStreamSynchronize(k->stream));
