@@ -1794,8 +1794,13 @@ __device__ __forceinline__ int2 krige64m_blk(const SolveArgs& a, int64_t nl, int
       A2[16 + 4 * q + p] = (float)(entry(1, 0, q, p) * ic0);
     }
   int dmin = 0x7f7fffff;
+  // Rows and columns from n on are the identity's, whose pivots (1) change
+  // nothing: they are skipped, and dmin takes 1 in their place.
+  const int nmax = __builtin_amdgcn_readfirstlane(n);
+  if (nmax < 64) dmin = __float_as_int(1.0f);
 #pragma unroll
   for (int k = 0; k < 64; ++k) {
+    if (k >= nmax) continue;
     const int K = k / 32, kk = k % 32;
     const int rk = 4 * (kk / 8) + (kk % 4);
     const float R0 = K == 0 ? A1[rk] : A2[16 + rk];  // row k, columns 0..31
